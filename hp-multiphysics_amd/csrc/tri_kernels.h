@@ -1,0 +1,48 @@
+// Device-side data structures shared by the tri_hp kernels and the host context.
+#pragma once
+
+namespace hpg {
+
+constexpr int NV = 3;  // tri_hp_ins unknowns (u, v, p)
+
+template <int P>
+struct Dims {
+    static constexpr int TM = (P + 1) * (P + 2) / 2;
+    static constexpr int SM = P - 1;
+    static constexpr int IM = (P - 1) * (P - 2) / 2;
+    static constexpr int BM = 3 * P;
+    static constexpr int G = (P + 1) * (P + 1);
+    static constexpr int GT = (G + 15) / 16;  // 16-row tiles over Gauss points
+    static constexpr int KT = (TM + 3) / 4;   // K-steps of 4 over modes
+    // lane-ready operator pack (doubles): P, Pr, Ps | Dr, Ds | Ir, Is
+    static constexpr int PROJ = GT * KT * 64;
+    static constexpr int DER = GT * GT * 4 * 64;
+    static constexpr int INT = GT * 4 * 64;
+    static constexpr int OFF_P = 0, OFF_PR = PROJ, OFF_PS = 2 * PROJ;
+    static constexpr int OFF_DR = 3 * PROJ, OFF_DS = 3 * PROJ + DER;
+    static constexpr int OFF_IR = 3 * PROJ + 2 * DER, OFF_IS = OFF_IR + INT;
+    static constexpr int PACK = OFF_IS + INT;
+};
+
+struct ElemArgs {
+    const double* pnts;
+    const int* tri_pnt;
+    const int* tri_seg;
+    const int* tri_sgn;
+    const int* order;
+    int begin, end;
+    const double* ugv;
+    const double* ugs;
+    const double* ugi;
+    double* resv;  // imaginary part (lf_im) target
+    double* ress;
+    double* resi;
+    double* rrv;   // real part (lf_re) target
+    double* rrs;
+    double* rri;
+    const double* ops;
+    int sm0, im0;
+    double rho, mu, adis, beta;
+};
+
+}  // namespace hpg

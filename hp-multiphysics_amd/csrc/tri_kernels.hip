@@ -1,0 +1,712 @@
+// HIP kernels of the tri_hp / tri_hp_ins residual and relaxation path (gfx950).
+//
+// Element residual: 16 elements per wave64, every basis transform a dense
+// contraction on v_mfma_f64_16x16x4f64.  Rows of an MFMA tile are Gauss points
+// (or modes), columns are the 16 elements of the wave.  Operands: A (16x4) lane l
+// holds A[l%16][l/16], B (4x16) lane l holds B[l/16][l%16]; the f64 accumulator
+// register i of lane l holds D[4*i + l/16][l%16] (checked on the device by
+// hpg_selftest_mfma).  Accumulators are fed straight back as the B operand of the
+// next contraction: the operator matrices are stored with their K index permuted
+// so that K-step (tile t, register i) reads row 16t + 4*i + l/16, which is exactly
+// what lane l holds.  No data moves between lanes from the modal gather to the
+// modal scatter.
+#include <hip/hip_runtime.h>
+#include "tri_kernels.h"
+
+namespace hpg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------------
+// tri_hp::ugtouht / lftog index map for mode k of element t (tri_hp/gtol.cpp:14-55,
+// 364-402): returns the offset of the NV-vector and the side sign sgn^m.
+template <int P>
+__device__ __forceinline__ int mode_offset(int k, int t, const int* vtx, const int* seg, const int* sgn,
+                                           int sm0, int im0, int& kind, double& sign) {
+    using D = Dims<P>;
+    sign = 1.0;
+    if (k < 3) { kind = 0; return vtx[k] * NV; }
+    if (k < D::BM) {
+        const int i = (k - 3) / D::SM, m = (k - 3) - i * D::SM;
+        kind = 1;
+        sign = ((m & 1) && sgn[i] < 0) ? -1.0 : 1.0;
+        return (seg[i] * sm0 + m) * NV;
+    }
+    kind = 2;
+    return (t * im0 + (k - D::BM)) * NV;
+}
+
+// tri_hp_ins::element_rsdl, straight branch (tri_hp/ins/rsdl.cpp:259-433), steady,
+// for the elements order[begin:end) (one colour: no two share a vertex or side).
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
+    using D = Dims<P>;
+    constexpr int GT = D::GT, KT = D::KT;
+    __shared__ double sops[D::PACK];
+    // XCD-aware block order: hardware deals consecutive blocks round-robin to the
+    // 8 XCDs; give each XCD a contiguous run of elements so neighbours share L2.
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nlogical) return;
+    for (int k = threadIdx.x; k < D::PACK; k += blockDim.x) sops[k] = a.ops[k];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = lane & 15, q = lane >> 4;
+    const int idx = a.begin + (lb * 4 + w) * 16 + e;
+    const bool valid = idx < a.end;
+    if (__builtin_amdgcn_ballot_w64(valid) == 0) return;  // whole wave past the colour end
+    const int t = valid ? a.order[idx] : a.order[a.begin];
+
+    int vtx[3], seg[3], sgn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        vtx[k] = a.tri_pnt[3 * t + k];
+        seg[k] = a.tri_seg[3 * t + k];
+        sgn[k] = a.tri_sgn[3 * t + k];
+    }
+    const double x0 = a.pnts[2 * vtx[0]], y0 = a.pnts[2 * vtx[0] + 1];
+    const double x1 = a.pnts[2 * vtx[1]], y1 = a.pnts[2 * vtx[1] + 1];
+    const double x2 = a.pnts[2 * vtx[2]], y2 = a.pnts[2 * vtx[2] + 1];
+    // ldcrd(n,0) = dx_n/dr, ldcrd(n,1) = dx_n/ds (rsdl.cpp:44-47)
+    const double l00 = 0.5 * (x2 - x1), l01 = 0.5 * (x0 - x1);
+    const double l10 = 0.5 * (y2 - y1), l11 = 0.5 * (y0 - y1);
+
+    // gather the B operands: K-step s of lane l holds mode 4s + l/16 of element l%16
+    double b[NV][KT];
+#pragma unroll
+    for (int s = 0; s < KT; ++s) {
+        const int k = 4 * s + q;
+        int kind;
+        double sg;
+        const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg);
+        const double* src = kind == 0 ? a.ugv : (kind == 1 ? a.ugs : a.ugi);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) b[n][s] = 0.0;
+        if (valid && k < D::TM) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) b[n][s] = sg * src[off + n];
+        }
+    }
+
+    const d4 z = {0.0, 0.0, 0.0, 0.0};
+    d4 U[GT], V[GT], Pp[GT];
+#pragma unroll
+    for (int mt = 0; mt < GT; ++mt) {
+        d4 cu = z, cv = z, cp = z;
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+            const double av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
+            cu = mfma(av, b[0][s], cu);
+            cv = mfma(av, b[1][s], cv);
+            cp = mfma(av, b[2][s], cp);
+        }
+        U[mt] = cu; V[mt] = cv; Pp[mt] = cp;
+    }
+
+    // inviscid fluxes (rsdl.cpp:262-291): continuity C, momentum A
+    d4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
+    const double rho = a.rho;
+#pragma unroll
+    for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
+            const double fx = rho * u, fy = rho * v;
+            const double c0 = +l11 * fx - l01 * fy;
+            const double c1 = -l10 * fx + l00 * fy;
+            C0[mt][i] = c0; C1[mt][i] = c1;
+            A00[mt][i] = u * c0 + l11 * pr;
+            A01[mt][i] = u * c1 - l10 * pr;
+            A10[mt][i] = v * c0 - l01 * pr;
+            A11[mt][i] = v * c1 + l00 * pr;
+        }
+
+    // lf_im = intgrtrs of the inviscid fluxes (rsdl.cpp:292-294)
+    d4 LI0 = z, LI1 = z, LI2 = z;
+#pragma unroll
+    for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int kk = kt * 4 + i;
+            const double ar = sops[D::OFF_IR + kk * 64 + lane];
+            const double as = sops[D::OFF_IS + kk * 64 + lane];
+            LI0 = mfma(ar, A00[kt][i], LI0); LI0 = mfma(as, A01[kt][i], LI0);
+            LI1 = mfma(ar, A10[kt][i], LI1); LI1 = mfma(as, A11[kt][i], LI1);
+            LI2 = mfma(ar, C0[kt][i], LI2);  LI2 = mfma(as, C1[kt][i], LI2);
+        }
+
+    d4 LR0 = z, LR1 = z, LR2 = z;
+    if constexpr (FULL) {
+        // derivatives of u, v at the Gauss points (proj with r/s derivatives)
+        d4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
+#pragma unroll
+        for (int mt = 0; mt < GT; ++mt) {
+            d4 c1 = z, c2 = z, c3 = z, c4 = z;
+#pragma unroll
+            for (int s = 0; s < KT; ++s) {
+                const double ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
+                const double as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
+                c1 = mfma(ar, b[0][s], c1); c2 = mfma(as, b[0][s], c2);
+                c3 = mfma(ar, b[1][s], c3); c4 = mfma(as, b[1][s], c4);
+            }
+            Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
+        }
+        // viscous stress tensor (rsdl.cpp:303-326); steady so rhobd0 = 0
+        const double cjcb = l00 * l11 - l10 * l01;
+        const double cjcbi = a.mu / cjcb;
+        const double v0000 = -cjcbi * (2. * l11 * l11 + l01 * l01);
+        const double v0011 = -cjcbi * (2. * l10 * l10 + l00 * l00);
+        const double v0001 = cjcbi * (2. * l11 * l10 + l01 * l00);
+        const double v1100 = -cjcbi * (l11 * l11 + 2. * l01 * l01);
+        const double v1111 = -cjcbi * (l10 * l10 + 2. * l00 * l00);
+        const double v1101 = cjcbi * (l11 * l10 + 2. * l01 * l00);
+        const double v0100 = cjcbi * l01 * l11;
+        const double v0111 = cjcbi * l00 * l10;
+        const double v0101 = -cjcbi * l01 * l10;
+        const double v0110 = -cjcbi * l00 * l11;
+        d4 F00[GT], F01[GT], F10[GT], F11[GT];
+#pragma unroll
+        for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
+                const double f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
+                const double f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
+                const double f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
+                const double f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
+                F00[mt][i] = f00; F01[mt][i] = f01; F10[mt][i] = f10; F11[mt][i] = f11;
+                A00[mt][i] += f00; A01[mt][i] += f01; A10[mt][i] += f10; A11[mt][i] += f11;
+            }
+        // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:369-375)
+        d4 R0[GT], R1[GT], R2[GT];
+#pragma unroll
+        for (int mt = 0; mt < GT; ++mt) {
+            d4 r0 = z, r1 = z, r2 = z;
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = (mt * GT + kt) * 4 + i;
+                    const double dr = sops[D::OFF_DR + kk * 64 + lane];
+                    const double ds = sops[D::OFF_DS + kk * 64 + lane];
+                    r0 = mfma(dr, A00[kt][i], r0); r0 = mfma(ds, A01[kt][i], r0);
+                    r1 = mfma(dr, A10[kt][i], r1); r1 = mfma(ds, A11[kt][i], r1);
+                    r2 = mfma(dr, C0[kt][i], r2);  r2 = mfma(ds, C1[kt][i], r2);
+                }
+            R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
+        }
+        // per-point tau (CALC_TAU2, rsdl.cpp:377-395) and the stabilisation (:397-421)
+        const double dx1 = x0 - x2, dy1 = y0 - y2, dx2 = x1 - x0, dy2 = y1 - y0;
+        const double ar2 = dx1 * dy2 - dy1 * dx2;
+        const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
+                             sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
+        const double h = (ar2 / perim) / (0.25 * (P + 1) * (P + 1));
+        const double nu = a.mu / rho;
+        const double gv = 2. * nu / h;
+        const double gvis = gv * gv;
+        const double tfac = a.adis * h / cjcb;
+#pragma unroll
+        for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double u = U[mt][i], v = V[mt][i];
+                const double qq = u * u + v * v;
+                double gam = 3.0 * qq + gvis;
+                if (a.mu == 0.0) gam = fmax(gam, 0.1);
+                const double tau0 = tfac / sqrt(gam);
+                const double tau2 = sqrt(qq) * tau0;
+                const double t0 = tau0 * R0[mt][i], t1 = tau0 * R1[mt][i], t2 = tau2 * R2[mt][i];
+                F00[mt][i] -= (l11 * (2 * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
+                F01[mt][i] -= (-l10 * (2 * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
+                F10[mt][i] -= +l11 * v * t0 + (l11 * u - l01 * (2. * v)) * t1 - l01 * t2;
+                F11[mt][i] -= -l10 * v * t0 + (-l10 * u + l00 * (2. * v)) * t1 + l00 * t2;
+                C0[mt][i] = -(l11 * t0 - l01 * t1);
+                C1[mt][i] = -(-l10 * t0 + l00 * t1);
+            }
+        // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
+#pragma unroll
+        for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kk = kt * 4 + i;
+                const double ar = sops[D::OFF_IR + kk * 64 + lane];
+                const double as = sops[D::OFF_IS + kk * 64 + lane];
+                LR0 = mfma(ar, F00[kt][i], LR0); LR0 = mfma(as, F01[kt][i], LR0);
+                LR1 = mfma(ar, F10[kt][i], LR1); LR1 = mfma(as, F11[kt][i], LR1);
+                LR2 = mfma(ar, C0[kt][i], LR2);  LR2 = mfma(as, C1[kt][i], LR2);
+            }
+    }
+
+    // lftog: register i of lane l holds mode 4*i + l/16 of element l%16
+    if (!valid) return;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * i + q;
+        if (k >= D::TM) continue;
+        int kind;
+        double sg;
+        const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg);
+        double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
+        di[off + 0] += sg * LI0[i];
+        di[off + 1] += sg * LI1[i];
+        di[off + 2] += sg * LI2[i];
+        if constexpr (FULL) {
+            double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+            const double bt = a.beta;
+            dr[off + 0] += sg * (LR0[i] * bt);
+            dr[off + 1] += sg * (LR1[i] * bt);
+            dr[off + 2] += sg * (LR2[i] * bt);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Mode-array helpers over v(nent, mstride, NV) with the first nm modes active.
+// op 0: x = 0, y *= c           (rsdl start, tri_hp/nstage.cpp:21-49)
+// op 1: x += y                  (rsdl end, nstage.cpp:76-82; mg_source later visits)
+// op 2: y = c*z - x, x += y     (mg_source first visit, nstage.cpp:113-128)
+// op 3: y = x                   (mg_restrict, movco.cpp:18-37)
+// op 4: x = y - c*z             (update, nstage.cpp:387-409)
+__global__ void k_modes(int op, long nent, int nm, int mstride, double* __restrict__ x, double* __restrict__ y,
+                        const double* __restrict__ zz, double c) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long span = (long)nm * NV;
+    if (k >= nent * span) return;
+    const long ent = k / span, r = k - ent * span;
+    const long ix = ent * (long)mstride * NV + r;
+    switch (op) {
+        case 0: x[ix] = 0.0; y[ix] *= c; break;
+        case 1: x[ix] += y[ix]; break;
+        case 2: { const double d = c * zz[ix] - x[ix]; y[ix] = d; x[ix] += d; } break;
+        case 3: y[ix] = x[ix]; break;
+        case 4: x[ix] = y[ix] - c * zz[ix]; break;
+    }
+}
+
+// edge-boundary flux residual, tri_hp/hp_boundary.cpp:973-1049 with bdry_ins::generic::flux
+struct EdgeArgs {
+    const int* eseg;
+    int begin, end;
+    const int* seg_pnt;
+    const double* pnts;
+    const double* ugv;
+    const double* ugs;
+    double* resv;
+    double* ress;
+    const double* g1;   // gpx x (sm+2)
+    const double* dg1;
+    const double* w1;
+    int gpx, sm, sm0;
+    double rho, bc_pressure;
+};
+
+__global__ void k_edge_ins(EdgeArgs a) {
+    const int idx = a.begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.end) return;
+    const int sind = a.eseg[idx];
+    const int v0 = a.seg_pnt[2 * sind], v1 = a.seg_pnt[2 * sind + 1];
+    const int n1 = a.sm + 2;
+    double uh[NV][6];
+    for (int n = 0; n < NV; ++n) {
+        uh[n][0] = a.ugv[v0 * NV + n];
+        uh[n][1] = a.ugv[v1 * NV + n];
+        for (int m = 0; m < a.sm; ++m) uh[n][m + 2] = a.ugs[(sind * a.sm0 + m) * NV + n];
+    }
+    const double X0 = a.pnts[2 * v0], Y0 = a.pnts[2 * v0 + 1], X1 = a.pnts[2 * v1], Y1 = a.pnts[2 * v1 + 1];
+    double lf[NV][6];
+    for (int n = 0; n < NV; ++n)
+        for (int k = 0; k < n1; ++k) lf[n][k] = 0.0;
+    for (int ip = 0; ip < a.gpx; ++ip) {
+        const double* g = a.g1 + ip * n1;
+        const double* dg = a.dg1 + ip * n1;
+        const double dxc = X0 * dg[0] + X1 * dg[1];
+        const double dyc = Y0 * dg[0] + Y1 * dg[1];
+        double u[NV];
+        for (int n = 0; n < NV; ++n) {
+            double s = 0.0;
+            for (int k = 0; k < n1; ++k) s += uh[n][k] * g[k];
+            u[n] = s;
+        }
+        double nx = dyc, ny = -dxc;
+        const double jcb = sqrt(nx * nx + ny * ny);
+        nx /= jcb; ny /= jcb;
+        const double f2 = a.rho * (u[0] * nx + u[1] * ny);
+        const double fl[NV] = {f2 * u[0] + a.bc_pressure * nx, f2 * u[1] + a.bc_pressure * ny, f2};
+        for (int n = 0; n < NV; ++n) {
+            const double r = fl[n] * jcb * a.w1[ip];
+            for (int k = 0; k < n1; ++k) lf[n][k] += g[k] * r;
+        }
+    }
+    for (int n = 0; n < NV; ++n) {
+        a.resv[v0 * NV + n] += lf[n][0];
+        a.resv[v1 * NV + n] += lf[n][1];
+        for (int m = 0; m < a.sm; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += lf[n][m + 2];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// setup_preconditioner, element part (tri_hp/ins/setup_preconditioner.cpp:33-219,
+// diagonal branch, REFINED_WAY, straight elements, steady)
+struct SetupArgs {
+    const double* pnts;
+    const int* tri_pnt;
+    const int* tri_seg;
+    const int* tri_sgn;
+    const double* ugv;
+    const double* ugs;
+    const double* ugi;
+    const double* Pmat;  // G x tm
+    double* tprcn;
+    int* err;
+    int ntri, p, sm, im, tm, G, sm0, im0;
+    double rho, mu, adis;
+};
+
+__global__ void k_setup_elem(SetupArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntri) return;
+    int vtx[3];
+    for (int k = 0; k < 3; ++k) vtx[k] = a.tri_pnt[3 * t + k];
+    double uh[2][15];
+    for (int k = 0; k < a.tm; ++k) {
+        int off;
+        double sg = 1.0;
+        const double* src;
+        if (k < 3) { off = vtx[k] * NV; src = a.ugv; }
+        else if (k < 3 * a.p) {
+            const int i = (k - 3) / a.sm, m = (k - 3) - i * a.sm;
+            const int sgn = a.tri_sgn[3 * t + i];
+            sg = ((m & 1) && sgn < 0) ? -1.0 : 1.0;
+            off = (a.tri_seg[3 * t + i] * a.sm0 + m) * NV;
+            src = a.ugs;
+        } else { off = (t * a.im0 + (k - 3 * a.p)) * NV; src = a.ugi; }
+        uh[0][k] = sg * src[off];
+        uh[1][k] = sg * src[off + 1];
+    }
+    double qmax = 0.0;
+    for (int g = 0; g < a.G; ++g) {
+        double u = 0.0, v = 0.0;
+        for (int k = 0; k < a.tm; ++k) {
+            const double pv = a.Pmat[g * a.tm + k];
+            u += pv * uh[0][k];
+            v += pv * uh[1][k];
+        }
+        const double q = u * u + v * v;
+        qmax = (qmax > q) ? qmax : q;  // MAX of sys/param.h, as the reference
+    }
+    const double X[3] = {a.pnts[2 * vtx[0]], a.pnts[2 * vtx[1]], a.pnts[2 * vtx[2]]};
+    const double Y[3] = {a.pnts[2 * vtx[0] + 1], a.pnts[2 * vtx[1] + 1], a.pnts[2 * vtx[2] + 1]};
+    const double dx1 = X[0] - X[2], dy1 = Y[0] - Y[2], dx2 = X[1] - X[0], dy2 = Y[1] - Y[0];
+    double jcb = 0.25 * (dx1 * dy2 - dy1 * dx2);
+    double hmax = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        const int j1 = (j + 1) % 3;
+        const double hh = (X[j] - X[j1]) * (X[j] - X[j1]) + (Y[j] - Y[j1]) * (Y[j] - Y[j1]);
+        hmax = hh > hmax ? hh : hmax;
+    }
+    hmax = sqrt(hmax);
+    const double pf = 0.25 * (a.p + 1) * (a.p + 1);
+    const double h = 4. * jcb / (pf * hmax);
+    hmax = hmax / pf;
+    if (!(h > 0.0) || qmax != qmax) {
+        atomicOr(a.err, 1);
+        a.tprcn[3 * t + 0] = a.tprcn[3 * t + 1] = a.tprcn[3 * t + 2] = 1.0;
+        return;
+    }
+    const double nu = a.mu / a.rho;
+    double gam = 3.0 * qmax + (2. * nu / hmax) * (2. * nu / hmax);
+    if (a.mu == 0.0) gam = fmax(gam, 0.1);
+    const double qs = sqrt(qmax);
+    const double lam1 = qs + sqrt(qmax + gam);
+    jcb *= 2. * nu * (1. / (hmax * hmax) + 1. / (h * h)) + 3 * lam1 / h;
+    a.tprcn[3 * t + 0] = a.rho * jcb;
+    a.tprcn[3 * t + 1] = a.rho * jcb;
+    a.tprcn[3 * t + 2] = jcb / gam;
+}
+
+// vprcn = 1/(vdiag * sum of incident tprcn) (setup_preconditioner.cpp:213-219 and
+// tri_hp/setup_preconditioner.cpp:68-76); incident elements in ascending order
+__global__ void k_setup_vrtx(int npnt, const int* ptr, const int* tri, const double* tprcn, double* vprcn, double vdiag) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= npnt) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = ptr[v]; k < ptr[v + 1]; ++k) {
+        const int t = tri[k];
+        s0 += tprcn[3 * t]; s1 += tprcn[3 * t + 1]; s2 += tprcn[3 * t + 2];
+    }
+    vprcn[3 * v] = 1.0 / (vdiag * s0);
+    vprcn[3 * v + 1] = 1.0 / (vdiag * s1);
+    vprcn[3 * v + 2] = 1.0 / (vdiag * s2);
+}
+
+__global__ void k_setup_side(int nseg, const int* st_tri, const double* tprcn, double* sprcn) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        const int t = st_tri[2 * s + k];
+        if (t < 0) continue;
+        a0 += tprcn[3 * t]; a1 += tprcn[3 * t + 1]; a2 += tprcn[3 * t + 2];
+    }
+    sprcn[3 * s] = 1.0 / a0; sprcn[3 * s + 1] = 1.0 / a1; sprcn[3 * s + 2] = 1.0 / a2;
+}
+
+// ---------------------------------------------------------------------------------
+// minvrt, tri_hp/setup_preconditioner.cpp:102-373 (diagonal preconditioner)
+struct MinvArgs {
+    const int* tri_pnt;
+    const int* tri_seg;
+    const int* tri_sgn;
+    const int* vs_ptr;
+    const int* vs_side;
+    const int* vs_end;
+    const int* vt_ptr;
+    const int* vt_tri;
+    const int* vt_loc;
+    const int* st_tri;
+    const int* st_loc;
+    const unsigned char* vess;
+    const unsigned char* sess;
+    double* resv;
+    double* ress;
+    double* resi;
+    const double* vprcn;
+    const double* sprcn;
+    const double* tprcn;
+    const double* lump;  // sfmv | ifmb | vfms | sdiag | sfms | bfmi | idiag
+    int npnt, nseg, ntri, sm, im, bm, sm0, im0;
+    int o_sfmv, o_ifmb, o_vfms, o_sdiag, o_sfms, o_bfmi, o_idiag;
+};
+
+// vertices: subtract side and interior couplings (:110-135), scale (:151-152),
+// vertex Dirichlet (:179-183)
+__global__ void k_minv_vrtx(MinvArgs a) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.npnt) return;
+    double r[NV];
+    for (int n = 0; n < NV; ++n) r[n] = a.resv[v * NV + n];
+    if (a.sm > 0) {
+        for (int k = a.vs_ptr[v]; k < a.vs_ptr[v + 1]; ++k) {
+            const int s = a.vs_side[k], i = a.vs_end[k];
+            for (int m = 0; m < a.sm; ++m) {
+                const double c = a.lump[a.o_sfmv + i * a.sm + m];
+                for (int n = 0; n < NV; ++n) r[n] -= c * a.ress[(s * a.sm0 + m) * NV + n];
+            }
+        }
+        if (a.im > 0)
+            for (int k = a.vt_ptr[v]; k < a.vt_ptr[v + 1]; ++k) {
+                const int t = a.vt_tri[k], i = a.vt_loc[k];
+                for (int m = 0; m < a.im; ++m) {
+                    const double c = a.lump[a.o_ifmb + i * a.im + m];
+                    for (int n = 0; n < NV; ++n) r[n] -= c * a.resi[(t * a.im0 + m) * NV + n];
+                }
+            }
+    }
+    const unsigned msk = a.vess[v];
+    for (int n = 0; n < NV; ++n) {
+        const double x = r[n] * a.vprcn[v * NV + n];
+        a.resv[v * NV + n] = (msk >> n) & 1 ? 0.0 : x;
+    }
+}
+
+// sides: subtract interior couplings (:137-148), then the vertex solution (:192-214)
+__global__ void k_minv_side(MinvArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nseg) return;
+    double r[5][NV];
+    for (int m = 0; m < a.sm; ++m)
+        for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
+    if (a.im > 0)
+        for (int k = 0; k < 2; ++k) {
+            const int t = a.st_tri[2 * s + k];
+            if (t < 0) continue;
+            const int i = a.st_loc[2 * s + k];
+            const int sgn = a.tri_sgn[3 * t + i];
+            double ms = 1.0;
+            for (int j = 0; j < a.sm; ++j) {
+                for (int m = 0; m < a.im; ++m) {
+                    const double c = ms * a.lump[a.o_ifmb + (3 + i * a.sm + j) * a.im + m];
+                    for (int n = 0; n < NV; ++n) r[j][n] -= c * a.resi[(t * a.im0 + m) * NV + n];
+                }
+                ms *= sgn;
+            }
+        }
+    for (int k = 0; k < 2; ++k) {
+        const int t = a.st_tri[2 * s + k];
+        if (t < 0) continue;
+        const int i = a.st_loc[2 * s + k];
+        const int sgn = a.tri_sgn[3 * t + i];
+        double uh[3][NV];
+        for (int j = 0; j < 3; ++j) {
+            const int w = a.tri_pnt[3 * t + j];
+            for (int n = 0; n < NV; ++n) uh[j][n] = a.resv[w * NV + n] * a.tprcn[3 * t + n];
+        }
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + j) % 3;
+            double ms = 1.0;
+            for (int m = 0; m < a.sm; ++m) {
+                const double c = a.lump[a.o_vfms + j * a.sm + m];
+                for (int n = 0; n < NV; ++n) r[m][n] -= ms * c * uh[i1][n];
+                ms *= sgn;
+            }
+        }
+    }
+    for (int m = 0; m < a.sm; ++m)
+        for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
+}
+
+__device__ __forceinline__ double side_scaled(const MinvArgs& a, int s, int mode, int n) {
+    if ((a.sess[s] >> n) & 1) return 0.0;
+    return a.ress[(s * a.sm0 + mode) * NV + n] * (a.sprcn[s * NV + n] * a.lump[a.o_sdiag + mode]);
+}
+
+// remove solved side mode `mode` from the higher modes (:244-328); the solved value
+// sprcn*sdiag*res with sdirichlet is formed on the fly, stored later by k_minv_sscale
+__global__ void k_minv_sub(MinvArgs a, int mode) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nseg) return;
+    double r[5][NV];
+    for (int m = mode + 1; m < a.sm; ++m)
+        for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
+    for (int k = 0; k < 2; ++k) {
+        const int t = a.st_tri[2 * s + k];
+        if (t < 0) continue;
+        const int i = a.st_loc[2 * s + k];
+        double uh[3][NV];
+        int sign[3];
+        for (int j = 0; j < 3; ++j) {
+            const int sj = a.tri_seg[3 * t + j];
+            sign[j] = a.tri_sgn[3 * t + j];
+            const double sg = (mode % 2) ? sign[j] : 1.0;
+            for (int n = 0; n < NV; ++n) uh[j][n] = sg * side_scaled(a, sj, mode, n) * a.tprcn[3 * t + n];
+        }
+        double ms = ((mode + 1) % 2) ? sign[i] : 1.0;
+        for (int m = mode + 1; m < a.sm; ++m) {
+            for (int j = 0; j < 3; ++j) {
+                const int ix = (i + j) % 3;
+                const double c = ms * a.lump[a.o_sfms + (mode * a.sm + m) * 3 + j];
+                for (int n = 0; n < NV; ++n) r[m][n] -= c * uh[ix][n];
+            }
+            ms *= sign[i];
+        }
+    }
+    for (int m = mode + 1; m < a.sm; ++m)
+        for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
+}
+
+// store the solved side modes: res.s *= sprcn*sdiag(mode), sdirichlet (:245-267, :331-343)
+__global__ void k_minv_sscale(MinvArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nseg) return;
+    double out[5][NV];
+    for (int m = 0; m < a.sm; ++m)
+        for (int n = 0; n < NV; ++n) out[m][n] = side_scaled(a, s, m, n);
+    for (int m = 0; m < a.sm; ++m)
+        for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = out[m][n];
+}
+
+// interiors: banded solve with the literal DPBTRSNU2 indexing (utilities/myblas.c:91-122),
+// divide by tprcn, subtract the boundary solution through bfmi (:347-370)
+__global__ void k_minv_int(MinvArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntri) return;
+    const int im = a.im;
+    double b[3][NV];
+    for (int k = 0; k < im; ++k)
+        for (int n = 0; n < NV; ++n) b[k][n] = a.resi[(t * a.im0 + k) * NV + n];
+    const double* ab = a.lump + a.o_idiag;  // stride 2, ofdg 1
+    for (int i = 0; i < im; ++i) {
+        const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
+        for (int j = frst; j < 1; ++j)
+            for (int n = 0; n < NV; ++n) b[i][n] -= ab[i * 2 + j] * b[ba + j][n];
+        for (int n = 0; n < NV; ++n) b[i][n] /= ab[i * 2 + 1];
+    }
+    for (int i = im - 1; i >= 0; --i) {
+        for (int n = 0; n < NV; ++n) b[i][n] /= ab[i * 2 + 1];
+        const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
+        for (int j = frst; j < 1; ++j)
+            for (int n = 0; n < NV; ++n) b[ba + j][n] -= ab[i * 2 + j] * b[i][n];
+    }
+    double uh[12][NV];
+    for (int i = 0; i < 3; ++i) {
+        const int w = a.tri_pnt[3 * t + i];
+        for (int n = 0; n < NV; ++n) uh[i][n] = a.resv[w * NV + n];
+    }
+    int cnt = 3;
+    for (int i = 0; i < 3; ++i) {
+        const int sd = a.tri_seg[3 * t + i], sgn = a.tri_sgn[3 * t + i];
+        double ms = 1.0;
+        for (int m = 0; m < a.sm; ++m) {
+            for (int n = 0; n < NV; ++n) uh[cnt][n] = ms * a.ress[(sd * a.sm0 + m) * NV + n];
+            ms *= sgn;
+            ++cnt;
+        }
+    }
+    for (int k = 0; k < im; ++k) {
+        for (int n = 0; n < NV; ++n) b[k][n] /= a.tprcn[3 * t + n];
+        for (int i = 0; i < a.bm; ++i) {
+            const double c = a.lump[a.o_bfmi + i * im + k];
+            for (int n = 0; n < NV; ++n) b[k][n] -= c * uh[i][n];
+        }
+    }
+    for (int k = 0; k < im; ++k)
+        for (int n = 0; n < NV; ++n) a.resi[(t * a.im0 + k) * NV + n] = b[k][n];
+}
+
+// ---------------------------------------------------------------------------------
+// max |x| over (nent, nm active modes of mstride, NV), per variable, into out[NV] as
+// ordered bit patterns of non-negative doubles; mask (optional) skips essential entries
+__global__ void k_absmax(long nent, int nm, int mstride, const double* __restrict__ x,
+                         const unsigned char* __restrict__ mask, unsigned long long* out) {
+    __shared__ unsigned long long red[NV][256];
+    unsigned long long mx[NV] = {0ull, 0ull, 0ull};
+    const long span = (long)nm;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nent * span; k += (long)gridDim.x * blockDim.x) {
+        const long ent = k / span, m = k - ent * span;
+        const unsigned msk = mask ? mask[ent] : 0u;
+        for (int n = 0; n < NV; ++n) {
+            if ((msk >> n) & 1) continue;
+            const double v = fabs(x[(ent * mstride + m) * NV + n]);
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+            mx[n] = bits > mx[n] ? bits : mx[n];
+        }
+    }
+    for (int n = 0; n < NV; ++n) red[n][threadIdx.x] = mx[n];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int n = 0; n < NV; ++n) {
+                const unsigned long long o = red[n][threadIdx.x + st];
+                red[n][threadIdx.x] = o > red[n][threadIdx.x] ? o : red[n][threadIdx.x];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int n = 0; n < NV; ++n) atomicMax(&out[n], red[n][0]);
+}
+
+// MFMA lane-layout self test: D = A(16x4) * B(4x16) with the layout the element
+// kernel assumes; out[lane*4+i] = D element held by (lane, register i)
+__global__ void k_mfma_probe(const double* A, const double* B, double* out) {
+    const int l = threadIdx.x;
+    const double av = A[(l & 15) * 4 + (l >> 4)];
+    const double bv = B[(l >> 4) * 16 + (l & 15)];
+    d4 c = {0.0, 0.0, 0.0, 0.0};
+    c = mfma(av, bv, c);
+    for (int i = 0; i < 4; ++i) out[l * 4 + i] = c[i];
+}
+
+// explicit instantiations used by the host context
+template __global__ void k_elem_ins<1, true>(ElemArgs, int);
+template __global__ void k_elem_ins<1, false>(ElemArgs, int);
+template __global__ void k_elem_ins<2, true>(ElemArgs, int);
+template __global__ void k_elem_ins<2, false>(ElemArgs, int);
+template __global__ void k_elem_ins<4, true>(ElemArgs, int);
+template __global__ void k_elem_ins<4, false>(ElemArgs, int);
+
+}  // namespace hpg

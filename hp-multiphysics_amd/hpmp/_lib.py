@@ -1,0 +1,95 @@
+"""ctypes bindings of libhpgpu.so (declarations in include/hpgpu.h).
+
+The product path has no CPU fallback: if the library is missing, loading fails
+with an error that says how to build it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libhpgpu.so")
+
+HPG_PHYSICS_INS = 0
+HPG_FP64 = 0
+HPG_FP32_DEFECT = 1
+HPG_PRCN_ANALYTIC = 0
+HPG_PRCN_LEARNED = 1
+
+# every entry point include/hpgpu.h declares
+EXPORTS = (
+    "hpg_create", "hpg_destroy", "hpg_set_state", "hpg_get_state", "hpg_get_residual", "hpg_set_residual",
+    "hpg_get_preconditioner", "hpg_set_tprcn", "hpg_rsdl", "hpg_setup_preconditioner", "hpg_minvrt",
+    "hpg_update", "hpg_maxres", "hpg_resnorm", "hpg_mg_restrict", "hpg_mg_prolongate", "hpg_get_log2p",
+    "hpg_set_precision", "hpg_set_preconditioner", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
+    "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read",
+)
+
+
+class TriMeshC(C.Structure):
+    _fields_ = [
+        ("npnt", C.c_int), ("nseg", C.c_int), ("ntri", C.c_int),
+        ("pnts", C.c_void_p), ("seg_pnt", C.c_void_p), ("tri_pnt", C.c_void_p),
+        ("tri_seg", C.c_void_p), ("tri_sgn", C.c_void_p),
+        ("nebd", C.c_int), ("ebdry_start", C.c_void_p), ("ebdry_seg", C.c_void_p),
+        ("ebdry_essential", C.c_void_p),
+    ]
+
+
+class ParamsC(C.Structure):
+    _fields_ = [
+        ("physics", C.c_int), ("log2p", C.c_int),
+        ("rho", C.c_double), ("mu", C.c_double), ("adis", C.c_double),
+        ("bc_pressure", C.c_double), ("fadd", C.c_double),
+        ("nstage", C.c_int), ("alpha", C.c_double * 8), ("beta", C.c_double * 8), ("cfl", C.c_double * 4),
+    ]
+
+
+_lib = None
+
+
+class HpgError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libhpgpu.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HpgError(f"{LIB_PATH} is missing: run `python hp-multiphysics_amd/build.py` (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.c_void_p
+    L.hpg_create.argtypes = [C.POINTER(TriMeshC), C.POINTER(ParamsC), C.POINTER(vp)]
+    L.hpg_destroy.argtypes = [vp]
+    for nm in ("hpg_set_state", "hpg_get_state", "hpg_get_residual", "hpg_set_residual", "hpg_get_preconditioner"):
+        getattr(L, nm).argtypes = [vp, dp, dp, dp]
+    L.hpg_set_tprcn.argtypes = [vp, dp]
+    L.hpg_rsdl.argtypes = [vp, C.c_int]
+    for nm in ("hpg_setup_preconditioner", "hpg_minvrt", "hpg_update", "hpg_mg_restrict", "hpg_mg_prolongate",
+               "hpg_synchronize"):
+        getattr(L, nm).argtypes = [vp]
+    L.hpg_maxres.argtypes = [vp, dp, dp]
+    L.hpg_resnorm.argtypes = [vp, dp]
+    L.hpg_get_log2p.argtypes = [vp, C.POINTER(C.c_int)]
+    L.hpg_set_precision.argtypes = [vp, C.c_int]
+    L.hpg_set_preconditioner.argtypes = [vp, C.c_int]
+    L.hpg_last_error.argtypes = [vp]
+    L.hpg_last_error.restype = C.c_char_p
+    L.hpg_selftest_mfma.argtypes = [dp, dp, dp]
+    L.hpg_basis_operator.argtypes = [C.c_int, C.c_char_p, dp]
+    L.hpg_basis_operator.restype = C.c_int
+    L.hpg_kernel_timing.argtypes = [vp, C.c_int]
+    L.hpg_kernel_timing_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                         C.POINTER(C.c_longlong)]
+    _lib = L
+    return L
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else C.c_void_p(0)

@@ -1,0 +1,193 @@
+"""tri_hp_ins on the GPU: the reference's per-block interface over libhpgpu.so.
+
+``TriHpIns`` keeps the method names and meanings of tri_hp / tri_hp_ins
+(tri_hp/tri_hp.h:207-252, tri_hp/ins/tri_hp_ins.h:43-46): ``rsdl(stage)``,
+``setup_preconditioner() -> int``, ``update()``, ``minvrt()``, ``mg_restrict()``,
+``mg_prolongate()``, ``maxres()``; ``cycle`` restates block::cycle
+(tri_mesh/blocks.cpp:1267-1348) for one grid with ``extra_finest_levels``
+p-levels.  State lives on the device between ``set_state`` and ``get_state``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .input_map import InputMap
+from .mesh import TriMesh
+
+NSTAGE_ALPHA = (0.25, 1.0 / 6.0, 0.375, 0.5, 1.0)   # tri_mesh/blocks.cpp:1133
+NSTAGE_BETA = (1.0, 0.0, 5.0 / 9.0, 0.0, 4.0 / 9.0)  # tri_mesh/blocks.cpp:1137
+CFL_DEFAULT = (2.5, 1.5, 1.0, 0.5)                   # tri_hp/allocate.cpp:275
+
+
+class TriHpIns:
+    NV = 3
+
+    def __init__(self, mesh: TriMesh, log2p: int = 2, rho: float = 1.0, mu: float = 0.0, adis: float = 1.0,
+                 bc_pressure: float = 0.0, fadd: float = 1.0, alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA,
+                 cfl=CFL_DEFAULT):
+        self.L = _lib.load()
+        self.mesh = mesh
+        self.log2pmax = int(log2p)
+        self.p = 1 << self.log2pmax
+        self.sm0 = self.p - 1
+        self.im0 = (self.p - 1) * (self.p - 2) // 2
+        self.nstage = len(alpha)
+        m = mesh
+        self._keep = [np.ascontiguousarray(a) for a in (
+            m.pnts.astype(np.float64), m.seg_pnt.astype(np.int32), m.tri_pnt.astype(np.int32),
+            m.tri_seg.astype(np.int32), m.tri_sgn.astype(np.int32), m.ebdry_start.astype(np.int32),
+            m.ebdry_seg.astype(np.int32), m.ebdry_essential.astype(np.int32))]
+        k = self._keep
+        cm = _lib.TriMeshC(m.npnt, m.nseg, m.ntri, *(C.c_void_p(a.ctypes.data) for a in k[:5]), m.nebd,
+                           *(C.c_void_p(a.ctypes.data) for a in k[5:]))
+        prm = _lib.ParamsC()
+        prm.physics = _lib.HPG_PHYSICS_INS
+        prm.log2p = self.log2pmax
+        prm.rho, prm.mu, prm.adis, prm.bc_pressure, prm.fadd = rho, mu, adis, bc_pressure, fadd
+        prm.nstage = self.nstage
+        for s in range(self.nstage):
+            prm.alpha[s] = alpha[s]
+            prm.beta[s] = beta[s]
+        for lvl in range(4):
+            prm.cfl[lvl] = cfl[lvl] if lvl < len(cfl) else 0.5
+        self.params = dict(rho=rho, mu=mu, adis=adis, bc_pressure=bc_pressure, fadd=fadd,
+                           alpha=tuple(alpha), beta=tuple(beta), cfl=tuple(cfl))
+        h = C.c_void_p()
+        rc = self.L.hpg_create(C.byref(cm), C.byref(prm), C.byref(h))
+        if rc != 0 or not h.value:
+            raise _lib.HpgError(f"hpg_create failed ({rc})")
+        self.h = h
+
+    @classmethod
+    def from_input_map(cls, mesh: TriMesh, inmap: InputMap, prefix: str = "b0") -> "TriHpIns":
+        log2p = inmap.geti("log2p", 0, prefix)
+        nstage = inmap.geti("nstage", 5, prefix)
+        alpha = inmap.getlist("alpha", NSTAGE_ALPHA, nstage, prefix)
+        beta = inmap.getlist("beta", NSTAGE_BETA, nstage, prefix)
+        cfl = inmap.getlist("cfl", CFL_DEFAULT, log2p + 1, prefix)
+        return cls(mesh, log2p=log2p, rho=inmap.getf("rho", 1.0, prefix), mu=inmap.getf("mu", 0.0, prefix),
+                   adis=inmap.getf("dissipation", 1.0, prefix), fadd=inmap.getf("hp_fadd", 1.0, prefix),
+                   alpha=alpha, beta=beta, cfl=list(cfl) + list(CFL_DEFAULT[len(cfl):]))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.hpg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = self.L.hpg_last_error(self.h).decode()
+            raise _lib.HpgError(f"{what} failed ({rc}): {msg}")
+        return rc
+
+    # --- array exchange (reference layouts) ------------------------------------------
+    def shapes(self):
+        m = self.mesh
+        return ((m.npnt, self.NV), (m.nseg, self.sm0, self.NV), (m.ntri, self.im0, self.NV))
+
+    def zeros(self):
+        return tuple(np.zeros(s) for s in self.shapes())
+
+    def set_state(self, v, s, i):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (v, s, i)]
+        self._check(self.L.hpg_set_state(self.h, *(_lib.ptr(x) for x in a)), "set_state")
+
+    def get_state(self):
+        out = self.zeros()
+        self._check(self.L.hpg_get_state(self.h, *(_lib.ptr(x) for x in out)), "get_state")
+        return out
+
+    def get_residual(self):
+        out = self.zeros()
+        self._check(self.L.hpg_get_residual(self.h, *(_lib.ptr(x) for x in out)), "get_residual")
+        return out
+
+    def set_residual(self, v, s, i):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (v, s, i)]
+        self._check(self.L.hpg_set_residual(self.h, *(_lib.ptr(x) for x in a)), "set_residual")
+
+    def get_preconditioner(self):
+        m = self.mesh
+        vp, sp, tp = np.zeros((m.npnt, 3)), np.zeros((m.nseg, 3)), np.zeros((m.ntri, 3))
+        self._check(self.L.hpg_get_preconditioner(self.h, _lib.ptr(vp), _lib.ptr(sp), _lib.ptr(tp)), "get_prcn")
+        return vp, sp, tp
+
+    def set_tprcn(self, tp):
+        tp = np.ascontiguousarray(tp, dtype=np.float64)
+        self._check(self.L.hpg_set_tprcn(self.h, _lib.ptr(tp)), "set_tprcn")
+
+    # --- the reference's virtual interface ----------------------------------------------
+    def rsdl(self, stage: int | None = None):
+        self._check(self.L.hpg_rsdl(self.h, self.nstage if stage is None else int(stage)), "rsdl")
+
+    def setup_preconditioner(self) -> int:
+        return self._check(self.L.hpg_setup_preconditioner(self.h), "setup_preconditioner")
+
+    def minvrt(self):
+        self._check(self.L.hpg_minvrt(self.h), "minvrt")
+
+    def update(self):
+        self._check(self.L.hpg_update(self.h), "update")
+
+    def maxres(self):
+        pv = np.zeros(3)
+        e = np.zeros(1)
+        self._check(self.L.hpg_maxres(self.h, _lib.ptr(pv), _lib.ptr(e)), "maxres")
+        return float(e[0]), pv
+
+    def resnorm(self) -> float:
+        e = np.zeros(1)
+        self._check(self.L.hpg_resnorm(self.h, _lib.ptr(e)), "resnorm")
+        return float(e[0])
+
+    def mg_restrict(self):
+        self._check(self.L.hpg_mg_restrict(self.h), "mg_restrict")
+
+    def mg_prolongate(self):
+        self._check(self.L.hpg_mg_prolongate(self.h), "mg_prolongate")
+
+    @property
+    def log2p(self) -> int:
+        v = C.c_int()
+        self._check(self.L.hpg_get_log2p(self.h, C.byref(v)), "get_log2p")
+        return v.value
+
+    def set_precision(self, mode: int):
+        self._check(self.L.hpg_set_precision(self.h, mode), "set_precision")
+
+    def synchronize(self):
+        self._check(self.L.hpg_synchronize(self.h), "synchronize")
+
+    def cycle(self, vw: int = 1, mglvls: int = 1, itercrsn: int = 1, iterrfne: int = 0, lvl: int = 0) -> float:
+        """block::cycle (tri_mesh/blocks.cpp:1267-1348) on one grid; p-levels below."""
+        error = 0.0
+        for vcount in range(vw):
+            if self.setup_preconditioner():
+                raise FloatingPointError("setup_preconditioner reported an error")
+            for _ in range(itercrsn):
+                self.update()
+            if lvl + vcount == 0:
+                error, _ = self.maxres()
+                if not error > 0.0:
+                    raise FloatingPointError("maxres is not positive (NaN or exact zero)")
+            if lvl == mglvls - 1:
+                return error
+            self.rsdl()
+            self.mg_restrict()
+            self.cycle(vw, mglvls, itercrsn, iterrfne, lvl + 1)
+            self.mg_prolongate()
+            if iterrfne:
+                if self.setup_preconditioner():
+                    raise FloatingPointError("setup_preconditioner reported an error")
+            for _ in range(iterrfne):
+                self.update()
+        return error

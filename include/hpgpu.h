@@ -1,0 +1,128 @@
+/*
+ * hpgpu.h -- C-ABI of the MI355X residual/relaxation library for tri_hp physics.
+ *
+ * One context per solution block.  It mirrors the reference's per-block virtual
+ * interface (tri_hp/tri_hp.h:207-252, reached from block::cycle through
+ * multigrid_interface, tri_mesh/block.h:233-247).  All arrays passed in or out use
+ * the reference layouts (tri_hp/allocate.cpp:105-107): vertex v(npnt,NV), side
+ * s(nseg,sm0,NV), interior i(ntri,im0,NV), row-major, variable index fastest.
+ *
+ * Return codes: 0 ok, >0 numerical error count (the reference's int error of
+ * setup_preconditioner), <0 runtime/HIP error.  Nothing throws across the ABI.
+ * A context is used by one host thread at a time (one block per thread, as in
+ * tri_mesh/blocks.cpp:220-240); distinct contexts are independent.
+ */
+#ifndef HPGPU_H
+#define HPGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpg_ctx hpg_ctx;
+
+/* Mesh arrays of tri_mesh (tri_mesh/tri_mesh.h:34-98): side j of a triangle runs
+ * from vertex (j+1)%3 to (j+2)%3 and tri_sgn is +1 when the segment is stored in
+ * that direction (tri_mesh/input.cpp:363-381).  Edge boundaries list their
+ * segments in order (edge_bdry::seg); ebdry_essential[b] is a bit mask of the
+ * variables that boundary holds fixed (hp_edge_bdry::essential_indices,
+ * tri_hp/hp_boundary.h:36-38). */
+typedef struct {
+    int npnt, nseg, ntri;
+    const double *pnts;      /* npnt x 2 */
+    const int *seg_pnt;      /* nseg x 2 */
+    const int *tri_pnt;      /* ntri x 3 */
+    const int *tri_seg;      /* ntri x 3 */
+    const int *tri_sgn;      /* ntri x 3 */
+    int nebd;
+    const int *ebdry_start;  /* nebd + 1 offsets into ebdry_seg */
+    const int *ebdry_seg;
+    const int *ebdry_essential;
+} hpg_tri_mesh;
+
+enum { HPG_PHYSICS_INS = 0 };
+enum { HPG_FP64 = 0, HPG_FP32_DEFECT = 1 };
+enum { HPG_PRCN_ANALYTIC = 0, HPG_PRCN_LEARNED = 1 };
+
+/* Input-map values the hot path reads (tri_hp/ins/allocate.cpp:13-44,
+ * tri_mesh/blocks.cpp:1126-1141, tri_hp/allocate.cpp:275-276, :198). */
+typedef struct {
+    int physics;             /* HPG_PHYSICS_INS */
+    int log2p;               /* 0,1,2 -> p = 1,2,4 */
+    double rho, mu, adis;    /* b0_rho, b0_mu, b0_dissipation */
+    double bc_pressure;      /* pressure of the freestream ibc on edge fluxes */
+    double fadd;             /* hp_fadd */
+    int nstage;              /* <= 7 */
+    double alpha[8], beta[8];/* stage coefficients; entry nstage is set to 1 */
+    double cfl[4];           /* per log2p */
+} hpg_params;
+
+/* replaces tri_hp::init / tri_hp_ins::init (tri_hp/allocate.cpp, tri_hp/ins/allocate.cpp:13-44) */
+int hpg_create(const hpg_tri_mesh *mesh, const hpg_params *prm, hpg_ctx **out);
+int hpg_destroy(hpg_ctx *ctx);
+
+/* state exchange with the host-owned ug arrays (tri_hp::ug, tri_hp/tri_hp.h:71-75) */
+int hpg_set_state(hpg_ctx *ctx, const double *ugv, const double *ugs, const double *ugi);
+int hpg_get_state(hpg_ctx *ctx, double *ugv, double *ugs, double *ugi);
+/* residual exchange (tri_hp::hp_global::res, tri_hp/tri_hp.h:118-121) */
+int hpg_get_residual(hpg_ctx *ctx, double *resv, double *ress, double *resi);
+int hpg_set_residual(hpg_ctx *ctx, const double *resv, const double *ress, const double *resi);
+/* diagonal preconditioner (vprcn, sprcn, tprcn of tri_hp::hp_global, tri_hp.h:138-139) */
+int hpg_get_preconditioner(hpg_ctx *ctx, double *vprcn, double *sprcn, double *tprcn);
+int hpg_set_tprcn(hpg_ctx *ctx, const double *tprcn);
+
+/* replaces tri_hp::rsdl(int stage) (tri_hp/nstage.cpp:12-110) with
+ * tri_hp_ins::element_rsdl (tri_hp/ins/rsdl.cpp:17-436) and the edge-boundary
+ * fluxes (tri_hp/hp_boundary.cpp:973-1049); stage == nstage is the full residual */
+int hpg_rsdl(hpg_ctx *ctx, int stage);
+/* replaces tri_hp_ins::setup_preconditioner (tri_hp/ins/setup_preconditioner.cpp:9-310)
+ * + tri_hp::setup_preconditioner (tri_hp/setup_preconditioner.cpp:9-96); returns the
+ * error count (non-positive element size or NaN in the solution) */
+int hpg_setup_preconditioner(hpg_ctx *ctx);
+/* replaces tri_hp::minvrt (tri_hp/setup_preconditioner.cpp:102-373) */
+int hpg_minvrt(hpg_ctx *ctx);
+/* replaces tri_hp::update (tri_hp/nstage.cpp:256-427) */
+int hpg_update(hpg_ctx *ctx);
+/* replaces tri_hp::maxres (tri_hp/allocate.cpp:496-525); per_var may be NULL */
+int hpg_maxres(hpg_ctx *ctx, double *per_var, double *out);
+/* raw-residual judge: max |res| over every vertex, side and interior coefficient
+ * that is not held by an essential boundary condition (no reference counterpart;
+ * the convergence measure the benchmarks use) */
+int hpg_resnorm(hpg_ctx *ctx, double *out);
+/* replace tri_hp::mg_restrict / mg_prolongate for p-levels (tri_hp/movco.cpp:4-40,
+ * tri_hp/movfin.cpp:7-14) */
+int hpg_mg_restrict(hpg_ctx *ctx);
+int hpg_mg_prolongate(hpg_ctx *ctx);
+int hpg_get_log2p(hpg_ctx *ctx, int *log2p);
+
+/* precision and preconditioner selection (no reference counterpart: north-star
+ * additions behind the same update() entry point) */
+int hpg_set_precision(hpg_ctx *ctx, int mode);
+int hpg_set_preconditioner(hpg_ctx *ctx, int mode);
+
+/* blocks until the context's stream is idle */
+int hpg_synchronize(hpg_ctx *ctx);
+/* Measurement support, no reference counterpart: bracket every element-residual
+ * kernel launch with HIP events on the context's stream (enable != 0; clears the
+ * record), and read back the summed duration, launch count and elements processed
+ * of the full (beta > 0) or inviscid-only (full = 0) launches since enabling. */
+int hpg_kernel_timing(hpg_ctx *ctx, int enable);
+int hpg_kernel_timing_read(hpg_ctx *ctx, int full, double *total_ms, long long *nlaunch, long long *nelem);
+/* text of the last error on this context (static storage, never NULL) */
+const char *hpg_last_error(hpg_ctx *ctx);
+
+/* Test support, no reference counterpart.
+ * hpg_basis_operator: host-built operator of the p basis by name ("P", "Pr", "Ps",
+ *   "I", "Ir", "Is", "Dr", "Ds", "xg", "wx", "eg", "we", "g1", "dg1", "vdiag", "sfmv",
+ *   "ifmb", "vfms", "sdiag", "sfms", "bfmi", "idiag", "mass"); returns the length,
+ *   copies into out when non-NULL; no device work (tri_basis tables,
+ *   tri_basis/initialize.cpp:161-958).
+ * hpg_selftest_mfma: one v_mfma_f64_16x16x4f64 on A (16x4) and B (4x16), row-major;
+ *   out[lane*4 + i] is accumulator register i of lane `lane`. */
+int hpg_basis_operator(int p, const char *name, double *out);
+int hpg_selftest_mfma(const double *A, const double *B, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,276 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module.  It drives oracle/_build/liboracle.so, the C++ restatement of the
+reference's tri_basis and tri_hp/tri_hp_ins residual/relaxation path (see the
+headers of oracle/otri_basis.h and oracle/otri_hp_ins.h for the file:line map).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.oracle_recur.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.oracle_gauss.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.obasis_info.argtypes = [C.c_int, _ip]
+        L.obasis_get.argtypes = [C.c_int, C.c_char_p, C.c_void_p]
+        L.obasis_get.restype = C.c_int
+        L.obasis_ptvalues_deriv.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        for nm in ("obasis_proj",):
+            getattr(L, nm).argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.obasis_proj_val.argtypes = [C.c_int, _dp, _dp]
+        L.obasis_proj_lin.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+        L.obasis_derivr.argtypes = [C.c_int, _dp, _dp]
+        L.obasis_derivs.argtypes = [C.c_int, _dp, _dp]
+        L.obasis_intgrt.argtypes = [C.c_int, _dp, _dp]
+        L.obasis_intgrtrs.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.obasis_proj1d.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.obasis_intgrt1d.argtypes = [C.c_int, _dp, _dp]
+        L.oracle_dpbtrsnu2.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int]
+        L.otri_create.restype = C.c_void_p
+        L.otri_create.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, C.c_int, _ip, _ip, _ip,
+                                  _dp, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.otri_destroy.argtypes = [C.c_void_p]
+        for nm in ("otri_set_state", "otri_get_state", "otri_get_res", "otri_set_res", "otri_get_prcn"):
+            getattr(L, nm).argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.otri_set_tprcn.argtypes = [C.c_void_p, _dp]
+        L.otri_assemble_prcn.argtypes = [C.c_void_p]
+        L.otri_rsdl.argtypes = [C.c_void_p, C.c_int]
+        L.otri_setup_preconditioner.argtypes = [C.c_void_p]
+        L.otri_setup_preconditioner.restype = C.c_int
+        for nm in ("otri_minvrt", "otri_update", "otri_mg_restrict", "otri_mg_prolongate"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+        L.otri_maxres.argtypes = [C.c_void_p, _dp]
+        L.otri_maxres.restype = C.c_double
+        L.otri_log2p.argtypes = [C.c_void_p]
+        L.otri_cycle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.otri_cycle.restype = C.c_double
+        L.otri_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.otri_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+class Basis:
+    """Tables and transforms of the oracle's tri_basis<p,0>."""
+
+    def __init__(self, p: int):
+        self.p = p
+        info = np.zeros(7, dtype=np.int32)
+        lib().obasis_info(p, info)
+        self.sm, self.im, self.tm, self.bm, self.nmodx, self.gpx, self.gpn = (int(v) for v in info)
+
+    def table(self, name: str) -> np.ndarray:
+        n = lib().obasis_get(self.p, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        lib().obasis_get(self.p, name.encode(), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def ptvalues_deriv(self, x, eta):
+        a = np.zeros(self.nmodx); b = np.zeros(self.nmodx); c = np.zeros(self.tm); d = np.zeros(self.tm)
+        lib().obasis_ptvalues_deriv(self.p, x, eta, a, b, c, d)
+        return a, b, c, d
+
+    def proj(self, lin):
+        f = np.zeros((self.gpx, self.gpn)); dx = np.zeros_like(f); dy = np.zeros_like(f)
+        lib().obasis_proj(self.p, np.ascontiguousarray(lin, dtype=np.float64), f, dx, dy)
+        return f, dx, dy
+
+    def proj_val(self, lin):
+        f = np.zeros((self.gpx, self.gpn))
+        lib().obasis_proj_val(self.p, np.ascontiguousarray(lin, dtype=np.float64), f)
+        return f
+
+    def proj_lin(self, u1, u2, u3):
+        f = np.zeros((self.gpx, self.gpn))
+        lib().obasis_proj_lin(self.p, u1, u2, u3, f)
+        return f
+
+    def derivr(self, f):
+        out = np.zeros((self.gpx, self.gpn))
+        lib().obasis_derivr(self.p, np.ascontiguousarray(f, dtype=np.float64), out)
+        return out
+
+    def derivs(self, f):
+        out = np.zeros((self.gpx, self.gpn))
+        lib().obasis_derivs(self.p, np.ascontiguousarray(f, dtype=np.float64), out)
+        return out
+
+    def intgrt(self, f):
+        out = np.zeros(self.tm)
+        lib().obasis_intgrt(self.p, out, np.ascontiguousarray(f, dtype=np.float64))
+        return out
+
+    def intgrtrs(self, dx, dy, init=None):
+        out = np.zeros(self.tm) if init is None else np.array(init, dtype=np.float64)
+        lib().obasis_intgrtrs(self.p, out, np.ascontiguousarray(dx, dtype=np.float64),
+                              np.ascontiguousarray(dy, dtype=np.float64))
+        return out
+
+    def proj1d(self, lin):
+        f = np.zeros(self.gpx); dx = np.zeros(self.gpx)
+        lib().obasis_proj1d(self.p, np.ascontiguousarray(lin, dtype=np.float64), f, dx)
+        return f, dx
+
+    def intgrt1d(self, f):
+        out = np.zeros(self.sm + 2)
+        lib().obasis_intgrt1d(self.p, out, np.ascontiguousarray(f, dtype=np.float64))
+        return out
+
+
+def recur(n, ipoly, al=0.0, be=0.0):
+    a = np.zeros(n); b = np.zeros(n)
+    lib().oracle_recur(n, ipoly, al, be, a, b)
+    return a, b
+
+
+def gauss(n, a, b):
+    x = np.zeros(n); w = np.zeros(n)
+    lib().oracle_gauss(n, np.ascontiguousarray(a[:n]), np.ascontiguousarray(b[:n]), x, w)
+    return x, w
+
+
+NSTAGE_ALPHA = (0.25, 1.0 / 6.0, 0.375, 0.5, 1.0)
+NSTAGE_BETA = (1.0, 0.0, 5.0 / 9.0, 0.0, 4.0 / 9.0)
+
+
+class TriHpIns:
+    """Oracle tri_hp_ins on a TriMesh (see hpmp.mesh); reference array layouts."""
+
+    NV = 3
+
+    def __init__(self, mesh, log2p=2, rho=1.0, mu=0.0, adis=1.0, bc_pressure=0.0, fadd=1.0,
+                 alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA, cfl=(2.5, 1.5, 1.0)):
+        self.mesh = mesh
+        self.log2pmax = log2p
+        self.p = 1 << log2p
+        self.sm0 = self.p - 1
+        self.im0 = (self.p - 1) * (self.p - 2) // 2
+        prm = np.array([rho, mu, adis, bc_pressure, fadd], dtype=np.float64)
+        al = np.array(alpha, dtype=np.float64)
+        be = np.array(beta, dtype=np.float64)
+        cf = np.array(list(cfl) + [0.5] * 4, dtype=np.float64)[:4]
+        m = mesh
+        self._keep = [np.ascontiguousarray(x) for x in (
+            m.pnts.astype(np.float64), m.seg_pnt.astype(np.int32), m.tri_pnt.astype(np.int32),
+            m.tri_seg.astype(np.int32), m.tri_sgn.astype(np.int32), m.ebdry_start.astype(np.int32),
+            m.ebdry_seg.astype(np.int32), m.ebdry_essential.astype(np.int32))]
+        k = self._keep
+        self.h = lib().otri_create(m.npnt, k[0], m.nseg, k[1], m.ntri, k[2], k[3], k[4], m.nebd, k[5], k[6],
+                                   k[7], prm, log2p, len(al), al, be, cf)
+        if not self.h:
+            raise RuntimeError("otri_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().otri_destroy(self.h)
+            self.h = None
+
+    def shapes(self):
+        m = self.mesh
+        return ((m.npnt, self.NV), (m.nseg, self.sm0, self.NV), (m.ntri, self.im0, self.NV))
+
+    def _alloc(self):
+        return [np.zeros(s) for s in self.shapes()]
+
+    @staticmethod
+    def _ptr(a):
+        return a if a.size else np.zeros(1)
+
+    def set_state(self, v, s, i):
+        lib().otri_set_state(self.h, np.ascontiguousarray(v, dtype=np.float64),
+                             self._ptr(np.ascontiguousarray(s, dtype=np.float64)),
+                             self._ptr(np.ascontiguousarray(i, dtype=np.float64)))
+
+    def get_state(self):
+        v, s, i = self._alloc()
+        lib().otri_get_state(self.h, v, self._ptr(s), self._ptr(i))
+        return v, s, i
+
+    def get_res(self):
+        v, s, i = self._alloc()
+        lib().otri_get_res(self.h, v, self._ptr(s), self._ptr(i))
+        return v, s, i
+
+    def set_res(self, v, s, i):
+        lib().otri_set_res(self.h, np.ascontiguousarray(v, dtype=np.float64),
+                           self._ptr(np.ascontiguousarray(s, dtype=np.float64)),
+                           self._ptr(np.ascontiguousarray(i, dtype=np.float64)))
+
+    def get_prcn(self):
+        m = self.mesh
+        vp = np.zeros((m.npnt, 3)); sp = np.zeros((m.nseg, 3)); tp = np.zeros((m.ntri, 3))
+        lib().otri_get_prcn(self.h, vp, sp, tp)
+        return vp, sp, tp
+
+    def set_tprcn(self, tp):
+        lib().otri_set_tprcn(self.h, np.ascontiguousarray(tp, dtype=np.float64))
+
+    def assemble_prcn(self):
+        lib().otri_assemble_prcn(self.h)
+
+    def rsdl(self, stage=None):
+        lib().otri_rsdl(self.h, len(NSTAGE_ALPHA) if stage is None else stage)
+
+    def setup_preconditioner(self) -> int:
+        return lib().otri_setup_preconditioner(self.h)
+
+    def minvrt(self):
+        lib().otri_minvrt(self.h)
+
+    def update(self):
+        lib().otri_update(self.h)
+
+    def maxres(self):
+        pv = np.zeros(3)
+        e = lib().otri_maxres(self.h, pv)
+        return e, pv
+
+    def mg_restrict(self):
+        lib().otri_mg_restrict(self.h)
+
+    def mg_prolongate(self):
+        lib().otri_mg_prolongate(self.h)
+
+    @property
+    def log2p(self):
+        return lib().otri_log2p(self.h)
+
+    def cycle(self, vw=1, mglvls=1, itercrsn=1, iterrfne=0):
+        return lib().otri_cycle(self.h, vw, mglvls, itercrsn, iterrfne)
+
+    def element_rsdl(self, tind, stage, uht):
+        tm = (self.p + 1) * (self.p + 2) // 2 if self.log2p == self.log2pmax else None
+        tm = uht.shape[1]
+        lr = np.zeros((3, tm)); li = np.zeros((3, tm))
+        lib().otri_element_rsdl(self.h, tind, stage, np.ascontiguousarray(uht, dtype=np.float64), lr, li)
+        return lr, li
+
+    def element_jacobian(self, tind, eps_r=1e-8, eps_a=1e-8):
+        tm = (self.p + 1) * (self.p + 2) // 2
+        K = np.zeros((tm * 3, tm * 3))
+        lib().otri_element_jacobian(self.h, tind, eps_r, eps_a, K)
+        return K

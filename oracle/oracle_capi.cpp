@@ -1,0 +1,161 @@
+// Oracle (test infrastructure only): C entry points over the CPU restatement so
+// that tests/ (ctypes) and bench.py's cpu_baseline leg can drive it.  The product
+// library never links this file.
+#include "otri_basis.h"
+#include "otri_hp_ins.h"
+#include "olinalg.h"
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace oracle;
+
+static const TriBasis& basis_for(int p) {
+    static std::map<int, std::unique_ptr<TriBasis>> cache;
+    auto it = cache.find(p);
+    if (it == cache.end()) it = cache.emplace(p, std::unique_ptr<TriBasis>(new TriBasis(p))).first;
+    return *it->second;
+}
+
+extern "C" {
+
+void oracle_recur(int n, int ipoly, double al, double be, double* a, double* b) { recur_coeffs(n, ipoly, al, be, a, b); }
+void oracle_gauss(int n, const double* a, const double* b, double* x, double* w) { gauss_rule(n, a, b, x, w); }
+
+int obasis_info(int p, int* out) {
+    const TriBasis& B = basis_for(p);
+    out[0] = B.sm; out[1] = B.im; out[2] = B.tm; out[3] = B.bm; out[4] = B.nmodx; out[5] = B.gpx; out[6] = B.gpn;
+    return 0;
+}
+
+// Copies a named table; returns its length (or -1 for an unknown name).
+int obasis_get(int p, const char* name, double* out) {
+    const TriBasis& B = basis_for(p);
+    const std::map<std::string, const std::vector<double>*> tabs = {
+        {"gx", &B.gx}, {"dgx", &B.dgx}, {"wtx", &B.wtx}, {"xp", &B.xp}, {"x0", &B.x0},
+        {"gxwtx", &B.gxwtx}, {"dgxwtx", &B.dgxwtx}, {"dltx", &B.dltx}, {"dltx1", &B.dltx1},
+        {"dltn1", &B.dltn1}, {"gn", &B.gn}, {"dgn", &B.dgn}, {"wtn", &B.wtn}, {"np", &B.np},
+        {"n0", &B.n0}, {"gnwtn", &B.gnwtn}, {"gnwtnn0", &B.gnwtnn0}, {"dgnwtn", &B.dgnwtn},
+        {"dltn", &B.dltn}, {"dltn2", &B.dltn2}, {"norm", &B.norm}, {"sfmv", &B.sfmv},
+        {"ifmb", &B.ifmb}, {"vfms", &B.vfms}, {"sdiag", &B.sdiag}, {"sfms", &B.sfms},
+        {"bfmi", &B.bfmi}, {"idiag", &B.idiag}, {"a0", &B.a0}, {"b0", &B.b0}};
+    if (std::string(name) == "vdiag") { if (out) out[0] = B.vdiag; return 1; }
+    auto it = tabs.find(name);
+    if (it == tabs.end()) return -1;
+    if (out) std::memcpy(out, it->second->data(), it->second->size() * sizeof(double));
+    return int(it->second->size());
+}
+
+void obasis_ptvalues_deriv(int p, double x, double eta, double* pgx, double* dpgx, double* pgn, double* dpgn) {
+    const TriBasis& B = basis_for(p);
+    B.ptvalues_deriv(x, eta);
+    std::memcpy(pgx, B.pgx.data(), B.nmodx * sizeof(double));
+    std::memcpy(dpgx, B.dpgx.data(), B.nmodx * sizeof(double));
+    std::memcpy(pgn, B.pgn.data(), B.tm * sizeof(double));
+    std::memcpy(dpgn, B.dpgn.data(), B.tm * sizeof(double));
+}
+
+// Transforms on gpx x gpn point arrays (stride gpn), argument conventions of tri_basis_interface.
+void obasis_proj(int p, const double* lin, double* f, double* dx, double* dy) { const TriBasis& B = basis_for(p); B.proj(lin, f, dx, dy, B.gpn); }
+void obasis_proj_val(int p, const double* lin, double* f) { const TriBasis& B = basis_for(p); B.proj(lin, f, B.gpn); }
+void obasis_proj_lin(int p, double u1, double u2, double u3, double* f) { const TriBasis& B = basis_for(p); B.proj(u1, u2, u3, f, B.gpn); }
+void obasis_derivr(int p, const double* f, double* dr) { const TriBasis& B = basis_for(p); B.derivr(f, dr, B.gpn); }
+void obasis_derivs(int p, const double* f, double* ds) { const TriBasis& B = basis_for(p); B.derivs(f, ds, B.gpn); }
+void obasis_intgrt(int p, double* rslt, const double* f) { const TriBasis& B = basis_for(p); B.intgrt(rslt, f, B.gpn); }
+void obasis_intgrtrs(int p, double* rslt, const double* dx, const double* dy) { const TriBasis& B = basis_for(p); B.intgrtrs(rslt, dx, dy, B.gpn); }
+void obasis_proj1d(int p, const double* lin, double* f, double* dx) { basis_for(p).proj1d(lin, f, dx); }
+void obasis_intgrt1d(int p, double* rslt, const double* f) { basis_for(p).intgrt1d(rslt, f); }
+void oracle_dpbtrsnu2(const double* abd, int stride, int ordr, int ofdg, double* b, int rhs) { dpbtrsnu2_literal(abd, stride, ordr, ofdg, b, rhs); }
+
+// ---- tri_hp_ins --------------------------------------------------------------
+// prm: rho, mu, adis, bc_pressure, fadd
+void* otri_create(int npnt, const double* pnts, int nseg, const int* seg_pnt, int ntri, const int* tri_pnt,
+                  const int* tri_seg, const int* tri_sgn, int nebd, const int* ebdry_start, const int* ebdry_seg,
+                  const int* ebdry_essential, const double* prm5, int log2p, int nstage, const double* alpha,
+                  const double* beta, const double* cfl) {
+    try {
+        Mesh m;
+        m.npnt = npnt; m.nseg = nseg; m.ntri = ntri;
+        m.pnts.assign(pnts, pnts + 2 * npnt);
+        m.seg_pnt.assign(seg_pnt, seg_pnt + 2 * nseg);
+        m.tri_pnt.assign(tri_pnt, tri_pnt + 3 * ntri);
+        m.tri_seg.assign(tri_seg, tri_seg + 3 * ntri);
+        m.tri_sgn.assign(tri_sgn, tri_sgn + 3 * ntri);
+        m.ebdry_start.assign(ebdry_start, ebdry_start + nebd + 1);
+        m.ebdry_seg.assign(ebdry_seg, ebdry_seg + ebdry_start[nebd]);
+        m.ebdry_essential.assign(ebdry_essential, ebdry_essential + nebd);
+        Params p;
+        p.rho = prm5[0]; p.mu = prm5[1]; p.adis = prm5[2]; p.bc_pressure = prm5[3]; p.fadd = prm5[4];
+        p.log2pmax = log2p;
+        p.nstage = nstage;
+        for (int s = 0; s < nstage && s < 7; ++s) { p.alpha[s] = alpha[s]; p.beta[s] = beta[s]; }
+        p.alpha[nstage] = 1.0; p.beta[nstage] = 1.0;
+        for (int l = 0; l <= log2p; ++l) p.cfl[l] = cfl[l];
+        return new TriHpIns(m, p);
+    } catch (...) {
+        return nullptr;
+    }
+}
+void otri_destroy(void* h) { delete static_cast<TriHpIns*>(h); }
+
+static TriHpIns& T(void* h) { return *static_cast<TriHpIns*>(h); }
+
+void otri_set_state(void* h, const double* v, const double* s, const double* i) {
+    TriHpIns& x = T(h);
+    std::memcpy(x.ugv.data(), v, x.ugv.size() * 8);
+    if (!x.ugs.empty()) std::memcpy(x.ugs.data(), s, x.ugs.size() * 8);
+    if (!x.ugi.empty()) std::memcpy(x.ugi.data(), i, x.ugi.size() * 8);
+}
+void otri_get_state(void* h, double* v, double* s, double* i) {
+    TriHpIns& x = T(h);
+    std::memcpy(v, x.ugv.data(), x.ugv.size() * 8);
+    if (!x.ugs.empty()) std::memcpy(s, x.ugs.data(), x.ugs.size() * 8);
+    if (!x.ugi.empty()) std::memcpy(i, x.ugi.data(), x.ugi.size() * 8);
+}
+void otri_get_res(void* h, double* v, double* s, double* i) {
+    TriHpIns& x = T(h);
+    std::memcpy(v, x.resv.data(), x.resv.size() * 8);
+    if (!x.ress.empty()) std::memcpy(s, x.ress.data(), x.ress.size() * 8);
+    if (!x.resi.empty()) std::memcpy(i, x.resi.data(), x.resi.size() * 8);
+}
+void otri_set_res(void* h, const double* v, const double* s, const double* i) {
+    TriHpIns& x = T(h);
+    std::memcpy(x.resv.data(), v, x.resv.size() * 8);
+    if (!x.ress.empty()) std::memcpy(x.ress.data(), s, x.ress.size() * 8);
+    if (!x.resi.empty()) std::memcpy(x.resi.data(), i, x.resi.size() * 8);
+}
+void otri_get_prcn(void* h, double* vp, double* sp, double* tp) {
+    TriHpIns& x = T(h);
+    std::memcpy(vp, x.vprcn.data(), x.vprcn.size() * 8);
+    std::memcpy(sp, x.sprcn.data(), x.sprcn.size() * 8);
+    std::memcpy(tp, x.tprcn.data(), x.tprcn.size() * 8);
+}
+void otri_set_tprcn(void* h, const double* tp) {
+    TriHpIns& x = T(h);
+    std::memcpy(x.tprcn.data(), tp, x.tprcn.size() * 8);
+}
+void otri_assemble_prcn(void* h) { T(h).assemble_prcn(T(h).m.ntri); }
+void otri_rsdl(void* h, int stage) { T(h).rsdl(stage); }
+int otri_setup_preconditioner(void* h) { return T(h).setup_preconditioner(); }
+void otri_minvrt(void* h) { T(h).minvrt(); }
+void otri_update(void* h) { T(h).update(); }
+double otri_maxres(void* h, double* per_var) { return T(h).maxres(per_var); }
+void otri_mg_restrict(void* h) { T(h).mg_restrict(); }
+void otri_mg_prolongate(void* h) { T(h).mg_prolongate(); }
+int otri_log2p(void* h) { return T(h).log2p; }
+double otri_cycle(void* h, int vw, int mglvls, int itercrsn, int iterrfne) {
+    try { return T(h).cycle(vw, 0, mglvls, itercrsn, iterrfne); } catch (...) { return -1.0; }
+}
+void otri_element_rsdl(void* h, int tind, int stage, const double* uh, double* lf_re, double* lf_im) {
+    TriHpIns& x = T(h);
+    double u[3][15], lr[3][15], li[3][15];
+    const int tm = x.b().tm;
+    for (int n = 0; n < 3; ++n) for (int k = 0; k < tm; ++k) u[n][k] = uh[n * tm + k];
+    x.element_rsdl(tind, stage, u, lr, li);
+    for (int n = 0; n < 3; ++n) for (int k = 0; k < tm; ++k) { lf_re[n * tm + k] = lr[n][k]; lf_im[n * tm + k] = li[n][k]; }
+}
+void otri_element_jacobian(void* h, int tind, double eps_r, double eps_a, double* K) { T(h).element_jacobian(tind, eps_r, eps_a, K); }
+
+}  // extern "C"
