@@ -15,6 +15,7 @@
 
 #include "../../include/hpgpu.h"
 #include "tri_kernels.hip"
+#include "tri_kernels_lin.hip"
 #include "tri_ops.h"
 
 using namespace hpg;
@@ -31,8 +32,11 @@ struct LevelDims {
     }
 };
 
-// Lane-ready operator pack: A operand of every MFMA K-step, 64 doubles per step.
-std::vector<double> build_pack(const TriOps& o) {
+// Lane-ready operator pack: A operand of every MFMA K-step, 64 values per step.
+// `row(i, q)` is the accumulator row held by register i of lane group q (FP64:
+// 4i + q; FP32: f32_row), which fixes the K permutation of the chained operators.
+template <class RowFn>
+std::vector<double> build_pack(const TriOps& o, RowFn row) {
     const LevelDims d(o.p);
     std::vector<double> pk(d.PACK, 0.0);
     const std::vector<double>* proj[3] = {&o.P, &o.Pr, &o.Ps};
@@ -49,7 +53,7 @@ std::vector<double> build_pack(const TriOps& o) {
             for (int kt = 0; kt < d.GT; ++kt)
                 for (int i = 0; i < 4; ++i)
                     for (int l = 0; l < 64; ++l) {
-                        const int g = 16 * mt + (l & 15), k = 16 * kt + 4 * i + (l >> 4);
+                        const int g = 16 * mt + (l & 15), k = 16 * kt + row(i, l >> 4);
                         pk[3 * d.PROJ + op * d.DER + ((mt * d.GT + kt) * 4 + i) * 64 + l] =
                             (g < d.G && k < d.G) ? (*der[op])[g * d.G + k] : 0.0;
                     }
@@ -58,7 +62,7 @@ std::vector<double> build_pack(const TriOps& o) {
         for (int kt = 0; kt < d.GT; ++kt)
             for (int i = 0; i < 4; ++i)
                 for (int l = 0; l < 64; ++l) {
-                    const int m = l & 15, k = 16 * kt + 4 * i + (l >> 4);
+                    const int m = l & 15, k = 16 * kt + row(i, l >> 4);
                     pk[3 * d.PROJ + 2 * d.DER + op * d.INT + (kt * 4 + i) * 64 + l] =
                         (m < d.tm && k < d.G) ? (*in[op])[m * d.G + k] : 0.0;
                 }
@@ -67,9 +71,10 @@ std::vector<double> build_pack(const TriOps& o) {
 
 // Greedy colouring: items whose vertex sets intersect get different colours.
 // Returns the item order grouped by colour (ascending inside a colour) and offsets.
-void colour(int nitem, int per, const int* verts, int npnt, std::vector<int>& order, std::vector<int>& off) {
+void colour(int nitem, int per, const int* verts, int npnt, std::vector<int>& order, std::vector<int>& off,
+            std::vector<int>& col) {
     std::vector<uint64_t> used(npnt, 0);
-    std::vector<int> col(nitem);
+    col.assign(nitem, 0);
     int ncol = 0;
     for (int t = 0; t < nitem; ++t) {
         uint64_t m = 0;
@@ -96,6 +101,8 @@ struct Level {
     double* Pmat = nullptr;
     double* lump = nullptr;
     double* edge = nullptr;  // g1 | dg1 | w1
+    float* pack32 = nullptr;
+    float* lump32 = nullptr;
     int o_sfmv, o_ifmb, o_vfms, o_sdiag, o_sfms, o_bfmi, o_idiag;
 };
 
@@ -117,11 +124,20 @@ struct hpg_ctx {
     std::vector<int> eoff;
     int *vs_ptr, *vs_side, *vs_end, *vt_ptr, *vt_tri, *vt_loc, *st_tri, *st_loc;
     unsigned char *vess, *sess;
+    unsigned char *fcv, *fcs;  // first colour touching each vertex / side
     double *ugv, *ugs, *ugi, *ug0v, *ug0s, *ug0i;
     double *resv, *ress, *resi, *rrv, *rrs, *rri, *r0v, *r0s, *r0i;
     double *dresv[3], *dress[3], *dresi[3];
     double *vprcn, *sprcn, *tprcn;
     int* d_err;
+    // mixed precision: per-colour group offsets, linearisation data, FP32 inner arrays
+    std::vector<int> gbase;
+    int ngroups = 0;
+    int ninner = 2;
+    float* lin = nullptr;
+    float *dlv = nullptr, *dls = nullptr, *dli = nullptr, *d0v = nullptr, *d0s = nullptr, *d0i = nullptr;
+    float *r32v = nullptr, *r32s = nullptr, *r32i = nullptr, *rr32v = nullptr, *rr32s = nullptr, *rr32i = nullptr;
+    float *vp32 = nullptr, *sp32 = nullptr, *tp32 = nullptr;
     unsigned long long* d_max;
     Level lv[3];
     std::string err;
@@ -199,8 +215,10 @@ template <int P>
 void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full) {
     for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
         ElemArgs a = a0;
+        a.colour = (int)col;
         a.begin = c->coff[col];
         a.end = c->coff[col + 1];
+        a.gbase = c->gbase[col];
         const int nlogical = (a.end - a.begin + 63) / 64;
         if (nlogical == 0) continue;
         const int nb = ((nlogical + 7) / 8) * 8;
@@ -217,30 +235,64 @@ void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full) {
     }
 }
 
-MinvArgs minv_args(hpg_ctx* c) {
+template <int P>
+void launch_lin(hpg_ctx* c, const LinArgs& a0, bool full) {
+    for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+        LinArgs a = a0;
+        a.colour = (int)col;
+        a.begin = c->coff[col];
+        a.end = c->coff[col + 1];
+        a.gbase = c->gbase[col];
+        const int nlogical = (a.end - a.begin + 63) / 64;
+        if (nlogical == 0) continue;
+        const int nb = ((nlogical + 7) / 8) * 8;
+        const size_t r = c->recs.size();
+        if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+        if (full)
+            hipLaunchKernelGGL((k_elem_lin<P, true>), dim3(nb), dim3(256), 0, c->stream, a, nlogical);
+        else
+            hipLaunchKernelGGL((k_elem_lin<P, false>), dim3(nb), dim3(256), 0, c->stream, a, nlogical);
+        if (c->timing) {
+            check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+            c->recs.push_back({full ? 3 : 2, a.end - a.begin});
+        }
+    }
+}
+
+// minvrt arguments over the FP64 residual (f32 = false) or the FP32 inner residual
+template <typename T>
+MinvArgsT<T> minv_args(hpg_ctx* c) {
     const Level& L = c->lv[c->log2p];
     const LevelDims d(L.p);
-    MinvArgs a;
+    MinvArgsT<T> a;
     a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
     a.vs_ptr = c->vs_ptr; a.vs_side = c->vs_side; a.vs_end = c->vs_end;
     a.vt_ptr = c->vt_ptr; a.vt_tri = c->vt_tri; a.vt_loc = c->vt_loc;
     a.st_tri = c->st_tri; a.st_loc = c->st_loc;
     a.vess = c->vess; a.sess = c->sess;
-    a.resv = c->resv; a.ress = c->ress; a.resi = c->resi;
-    a.vprcn = c->vprcn; a.sprcn = c->sprcn; a.tprcn = c->tprcn;
-    a.lump = L.lump;
+    if constexpr (sizeof(T) == 8) {
+        a.resv = c->resv; a.ress = c->ress; a.resi = c->resi;
+        a.vprcn = c->vprcn; a.sprcn = c->sprcn; a.tprcn = c->tprcn;
+        a.lump = L.lump;
+    } else {
+        a.resv = c->r32v; a.ress = c->r32s; a.resi = c->r32i;
+        a.vprcn = c->vp32; a.sprcn = c->sp32; a.tprcn = c->tp32;
+        a.lump = L.lump32;
+    }
     a.npnt = c->npnt; a.nseg = c->nseg; a.ntri = c->ntri;
     a.sm = d.sm; a.im = d.im; a.bm = d.bm; a.sm0 = c->sm0; a.im0 = c->im0;
     a.o_sfmv = L.o_sfmv; a.o_ifmb = L.o_ifmb; a.o_vfms = L.o_vfms; a.o_sdiag = L.o_sdiag;
     a.o_sfms = L.o_sfms; a.o_bfmi = L.o_bfmi; a.o_idiag = L.o_idiag;
+    a.ugv = a.ugs = a.ugi = nullptr;
+    a.ug0v = a.ug0s = a.ug0i = nullptr;
+    a.cfa = T(0);
     return a;
 }
 
-void do_rsdl(hpg_ctx* c, int stage) {
+void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     const double f = stage < nst ? 1.0 - beta : 0.0;
-    modes_vsi(c, 0, c->resv, c->ress, c->resi, c->rrv, c->rrs, c->rri, nullptr, nullptr, nullptr, f);
     const Level& L = c->lv[c->log2p];
     ElemArgs a;
     a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
@@ -250,6 +302,9 @@ void do_rsdl(hpg_ctx* c, int stage) {
     a.rrv = c->rrv; a.rrs = c->rrs; a.rri = c->rri;
     a.ops = L.pack; a.sm0 = c->sm0; a.im0 = c->im0;
     a.rho = c->prm.rho; a.mu = c->prm.mu; a.adis = c->prm.adis; a.beta = beta;
+    a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0; a.keep = f;
+    a.lin = (store_lin && beta > 0.0) ? c->lin : nullptr;
+    a.gbase = 0;
     const bool full = beta > 0.0;
     switch (L.p) {
         case 1: launch_elem<1>(c, a, full); break;
@@ -257,7 +312,6 @@ void do_rsdl(hpg_ctx* c, int stage) {
         case 4: launch_elem<4>(c, a, full); break;
         default: throw std::runtime_error("unsupported p");
     }
-    modes_vsi(c, 1, c->resv, c->ress, c->resi, c->rrv, c->rrs, c->rri, nullptr, nullptr, nullptr, 0.0);
     const LevelDims d(L.p);
     EdgeArgs e;
     e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
@@ -283,17 +337,78 @@ void do_rsdl(hpg_ctx* c, int stage) {
     check(hipGetLastError(), "rsdl launch");
 }
 
-void do_minvrt(hpg_ctx* c) {
-    MinvArgs a = minv_args(c);
-    hipLaunchKernelGGL(k_minv_vrtx, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, a);
+// FP32 linearised residual of the inner solve: r32 = src + J(u0) delta (multistage blended)
+void do_lin_rsdl(hpg_ctx* c, int stage, bool with_source) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const double f = stage < nst ? 1.0 - beta : 0.0;
+    const Level& L = c->lv[c->log2p];
+    LinArgs a;
+    a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+    a.order = c->order; a.begin = 0; a.end = 0;
+    a.dv = c->dlv; a.ds = c->dls; a.di = c->dli;
+    a.resv = c->r32v; a.ress = c->r32s; a.resi = c->r32i;
+    a.rrv = c->rr32v; a.rrs = c->rr32s; a.rri = c->rr32i;
+    a.srcv = with_source ? c->resv : nullptr;
+    a.srcs = with_source ? c->ress : nullptr;
+    a.srci = with_source ? c->resi : nullptr;
+    a.ops = L.pack32; a.lin = c->lin; a.gbase = 0; a.sm0 = c->sm0; a.im0 = c->im0;
+    a.rho = (float)c->prm.rho; a.mu = (float)c->prm.mu; a.adis = (float)c->prm.adis;
+    a.beta = (float)beta; a.keep = (float)f;
+    a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0;
+    const bool full = beta > 0.0;
+    switch (L.p) {
+        case 1: launch_lin<1>(c, a, full); break;
+        case 2: launch_lin<2>(c, a, full); break;
+        case 4: launch_lin<4>(c, a, full); break;
+        default: throw std::runtime_error("unsupported p");
+    }
+    const LevelDims d(L.p);
+    EdgeLinArgs e;
+    e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
+    e.dv = c->dlv; e.ds = c->dls; e.resv = c->r32v; e.ress = c->r32s;
+    e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+    e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho;
+    for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
+        e.begin = c->eoff[col];
+        e.end = c->eoff[col + 1];
+        if (e.end > e.begin) hipLaunchKernelGGL(k_edge_lin, dim3(nblk(e.end - e.begin, 64)), dim3(64), 0, c->stream, e);
+    }
+    check(hipGetLastError(), "lin rsdl launch");
+}
+
+template <typename T>
+void launch_minv(hpg_ctx* c, const MinvArgsT<T>& a) {
+    hipLaunchKernelGGL(k_minv_vrtx<T>, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, a);
     if (a.sm > 0) {
-        hipLaunchKernelGGL(k_minv_side, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a);
+        hipLaunchKernelGGL(k_minv_side<T>, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a);
         for (int mode = 0; mode < a.sm - 1; ++mode)
-            hipLaunchKernelGGL(k_minv_sub, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a, mode);
-        hipLaunchKernelGGL(k_minv_sscale, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a);
-        if (a.im > 0) hipLaunchKernelGGL(k_minv_int, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, a);
+            hipLaunchKernelGGL(k_minv_sub<T>, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a, mode);
+        hipLaunchKernelGGL(k_minv_sscale<T>, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a);
+        if (a.im > 0) hipLaunchKernelGGL(k_minv_int<T>, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, a);
     }
     check(hipGetLastError(), "minvrt launch");
+}
+
+// minvrt; with update_cfa >= 0 the stage update ug = ug0 - update_cfa*res is fused
+// into the sweeps that produce the final vertex, side and interior values
+void do_minvrt(hpg_ctx* c, double update_cfa = -1.0) {
+    MinvArgsT<double> a = minv_args<double>(c);
+    if (update_cfa >= 0.0) {
+        a.ugv = c->ugv; a.ug0v = c->ug0v; a.cfa = update_cfa;
+        if (a.sm > 0) { a.ugs = c->ugs; a.ug0s = c->ug0s; }
+        if (a.im > 0) { a.ugi = c->ugi; a.ug0i = c->ug0i; }
+    }
+    launch_minv<double>(c, a);
+}
+
+// FP32 minvrt of the inner residual with the fused delta update
+void do_minvrt32(hpg_ctx* c, float cfa) {
+    MinvArgsT<float> a = minv_args<float>(c);
+    a.ugv = c->dlv; a.ug0v = c->d0v; a.cfa = cfa;
+    if (a.sm > 0) { a.ugs = c->dls; a.ug0s = c->d0s; }
+    if (a.im > 0) { a.ugi = c->dli; a.ug0i = c->d0i; }
+    launch_minv<float>(c, a);
 }
 
 int do_setup(hpg_ctx* c) {
@@ -363,8 +478,18 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->tri_sgn = c->upload(m->tri_sgn, 3 * (size_t)nt);
         c->seg_pnt = c->upload(m->seg_pnt, 2 * (size_t)ns);
 
-        std::vector<int> order, off;
-        colour(nt, 3, m->tri_pnt, np, order, off);
+        std::vector<int> order, off, tcol;
+        colour(nt, 3, m->tri_pnt, np, order, off, tcol);
+        std::vector<unsigned char> fcv(np, 255), fcs(ns, 255);
+        for (int t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k) {
+                unsigned char& a = fcv[m->tri_pnt[3 * t + k]];
+                unsigned char& b = fcs[m->tri_seg[3 * t + k]];
+                a = std::min<unsigned char>(a, (unsigned char)tcol[t]);
+                b = std::min<unsigned char>(b, (unsigned char)tcol[t]);
+            }
+        c->fcv = c->upload(fcv.data(), fcv.size());
+        c->fcs = c->upload(fcs.data(), fcs.size());
         c->order = c->upload(order.data(), order.size());
         c->coff = off;
         const int neseg = m->ebdry_start[m->nebd];
@@ -373,8 +498,8 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             ev[2 * j] = m->seg_pnt[2 * m->ebdry_seg[j]];
             ev[2 * j + 1] = m->seg_pnt[2 * m->ebdry_seg[j] + 1];
         }
-        std::vector<int> eorder, eoffv, eseg(neseg);
-        colour(neseg, 2, ev.data(), np, eorder, eoffv);
+        std::vector<int> eorder, eoffv, ecol, eseg(neseg);
+        colour(neseg, 2, ev.data(), np, eorder, eoffv, ecol);
         for (int j = 0; j < neseg; ++j) eseg[j] = m->ebdry_seg[eorder[j]];
         c->eseg = c->upload(eseg.data(), eseg.size());
         c->eoff = eoffv;
@@ -446,19 +571,41 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->d_err = c->alloc<int>(1);
         c->d_max = c->alloc<unsigned long long>(NV);
 
+        // mixed precision: 16-element groups per colour, FP32 inner arrays
+        c->gbase.assign(c->coff.size(), 0);
+        for (size_t col = 0; col + 1 < c->coff.size(); ++col)
+            c->gbase[col + 1] = c->gbase[col] + ((c->coff[col + 1] - c->coff[col] + 63) / 64) * 4;
+        c->ngroups = c->gbase.back();
+        const LevelDims dmax(p0);
+        c->lin = c->alloc<float>((size_t)c->ngroups * 5 * dmax.GT * 4 * 64);
+        float** f32v[] = {&c->dlv, &c->d0v, &c->r32v, &c->rr32v};
+        float** f32s[] = {&c->dls, &c->d0s, &c->r32s, &c->rr32s};
+        float** f32i[] = {&c->dli, &c->d0i, &c->r32i, &c->rr32i};
+        for (auto p : f32v) *p = c->alloc<float>(nv);
+        for (auto p : f32s) *p = c->alloc<float>(nsz);
+        for (auto p : f32i) *p = c->alloc<float>(ni);
+        c->vp32 = c->alloc<float>(nv);
+        c->sp32 = c->alloc<float>((size_t)ns * NV);
+        c->tp32 = c->alloc<float>((size_t)nt * NV);
+
         for (int l = 0; l <= c->log2pmax; ++l) {
             Level& L = c->lv[l];
             L.p = 1 << l;
             L.ops.reset(new TriOps(L.p));
             const TriOps& o = *L.ops;
-            const std::vector<double> pk = build_pack(o);
+            const std::vector<double> pk = build_pack(o, [](int i, int q) { return 4 * i + q; });
             L.pack = c->upload(pk.data(), pk.size());
+            const std::vector<double> pk32d = build_pack(o, [](int i, int q) { return f32_row(i, q); });
+            const std::vector<float> pk32(pk32d.begin(), pk32d.end());
+            L.pack32 = c->upload(pk32.data(), pk32.size());
             L.Pmat = c->upload(o.P.data(), o.P.size());
             std::vector<double> lump;
             auto app = [&](const std::vector<double>& v) { const int o2 = (int)lump.size(); lump.insert(lump.end(), v.begin(), v.end()); return o2; };
             L.o_sfmv = app(o.sfmv); L.o_ifmb = app(o.ifmb); L.o_vfms = app(o.vfms); L.o_sdiag = app(o.sdiag);
             L.o_sfms = app(o.sfms); L.o_bfmi = app(o.bfmi); L.o_idiag = app(o.idiag);
             L.lump = c->upload(lump.data(), lump.size());
+            const std::vector<float> lump32(lump.begin(), lump.end());
+            L.lump32 = c->upload(lump32.data(), lump32.size());
             std::vector<double> ed;
             ed.insert(ed.end(), o.g1.begin(), o.g1.end());
             ed.insert(ed.end(), o.dg1.begin(), o.dg1.end());
@@ -551,9 +698,81 @@ int hpg_minvrt(hpg_ctx* c) {
     return 0;
 }
 
+namespace {
+
+void launch_f32(hpg_ctx* c, long n, const double* x, float* y) {
+    if (n > 0) hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, y);
+}
+void launch_add(hpg_ctx* c, long n, double* ug, const float* d) {
+    if (n > 0) hipLaunchKernelGGL(k_add_delta, dim3(nblk(n)), dim3(256), 0, c->stream, n, ug, d);
+}
+
+// One outer defect-correction iteration: FP64 residual R(u0) (the judge) with the
+// linearisation data, then `ninner` multistage iterations in FP32 on
+// R(u0) + J(u0) delta = 0, then u0 += delta.
+void mixed_update(hpg_ctx* c) {
+    const long nv = (long)c->npnt * NV, nsz = (long)c->nseg * c->sm0 * NV, ni = (long)c->ntri * c->im0 * NV;
+    do_rsdl(c, c->prm.nstage, true);
+    launch_f32(c, nv, c->vprcn, c->vp32);
+    launch_f32(c, (long)c->nseg * NV, c->sprcn, c->sp32);
+    launch_f32(c, (long)c->ntri * NV, c->tprcn, c->tp32);
+    check(hipMemsetAsync(c->dlv, 0, nv * 4, c->stream), "memset");
+    if (nsz) check(hipMemsetAsync(c->dls, 0, nsz * 4, c->stream), "memset");
+    if (ni) check(hipMemsetAsync(c->dli, 0, ni * 4, c->stream), "memset");
+    for (int it = 0; it < c->ninner; ++it) {
+        check(hipMemcpyAsync(c->d0v, c->dlv, nv * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        if (nsz) check(hipMemcpyAsync(c->d0s, c->dls, nsz * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        if (ni) check(hipMemcpyAsync(c->d0i, c->dli, ni * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        for (int stage = 0; stage < c->prm.nstage; ++stage) {
+            do_lin_rsdl(c, stage, true);
+            do_minvrt32(c, (float)(c->prm.alpha[stage] * c->prm.cfl[c->log2p]));
+        }
+    }
+    launch_add(c, nv, c->ugv, c->dlv);
+    launch_add(c, nsz, c->ugs, c->dls);
+    launch_add(c, ni, c->ugi, c->dli);
+    check(hipGetLastError(), "mixed update launch");
+}
+
+}  // namespace
+
+int hpg_set_defect_inner(hpg_ctx* c, int ninner) {
+    if (!c || ninner < 1 || ninner > 1000) return -2;
+    c->ninner = ninner;
+    return 0;
+}
+
+int hpg_apply_jacobian(hpg_ctx* c, const double* dv, const double* ds, const double* di, double* ov, double* os,
+                       double* oi) {
+    if (!c || !dv || !ov) return -2;
+    if (c->log2p != c->log2pmax) { c->err = "apply_jacobian runs on the finest level"; return -2; }
+    HPG_TRY(c, {
+        const size_t nv = (size_t)c->npnt * NV, nsz = (size_t)c->nseg * c->sm0 * NV, ni = (size_t)c->ntri * c->im0 * NV;
+        do_rsdl(c, c->prm.nstage, true);
+        std::vector<float> hv(dv, dv + nv), hs(nsz), hi(ni);
+        for (size_t k = 0; k < nsz; ++k) hs[k] = (float)ds[k];
+        for (size_t k = 0; k < ni; ++k) hi[k] = (float)di[k];
+        check(hipMemcpyAsync(c->dlv, hv.data(), nv * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        if (nsz) check(hipMemcpyAsync(c->dls, hs.data(), nsz * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        if (ni) check(hipMemcpyAsync(c->dli, hi.data(), ni * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        do_lin_rsdl(c, c->prm.nstage, false);
+        check(hipMemcpyAsync(hv.data(), c->r32v, nv * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        if (nsz) check(hipMemcpyAsync(hs.data(), c->r32s, nsz * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        if (ni) check(hipMemcpyAsync(hi.data(), c->r32i, ni * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+        for (size_t k = 0; k < nv; ++k) ov[k] = hv[k];
+        for (size_t k = 0; k < nsz; ++k) os[k] = hs[k];
+        for (size_t k = 0; k < ni; ++k) oi[k] = hi[k];
+    });
+    return 0;
+}
+
 int hpg_update(hpg_ctx* c) {
     if (!c) return -2;
-    if (c->precision != HPG_FP64) { c->err = "precision mode not available"; return -3; }
+    if (c->precision == HPG_FP32_DEFECT && !c->coarse_flag) {
+        HPG_TRY(c, mixed_update(c));
+        return 0;
+    }
     HPG_TRY(c, {
         const size_t nv = (size_t)c->npnt * NV * 8, ns = (size_t)c->nseg * c->sm0 * NV * 8,
                      ni = (size_t)c->ntri * c->im0 * NV * 8;
@@ -562,9 +781,7 @@ int hpg_update(hpg_ctx* c) {
         if (ni) check(hipMemcpyAsync(c->ug0i, c->ugi, ni, hipMemcpyDeviceToDevice, c->stream), "D2D");
         for (int stage = 0; stage < c->prm.nstage; ++stage) {
             do_rsdl(c, stage);
-            do_minvrt(c);
-            const double ca = c->prm.alpha[stage] * c->prm.cfl[c->log2p];
-            modes_vsi(c, 4, c->ugv, c->ugs, c->ugi, c->ug0v, c->ug0s, c->ug0i, c->resv, c->ress, c->resi, ca);
+            do_minvrt(c, c->prm.alpha[stage] * c->prm.cfl[c->log2p]);
         }
         check(hipGetLastError(), "update launch");
     });
@@ -701,6 +918,19 @@ int hpg_basis_operator(int p, const char* name, double* out) {
     } catch (...) {
         return -1;
     }
+}
+
+// MFMA f32 16x16x4 lane-layout probe (test support; A 16x4, B 4x16 row-major, out 64x4)
+int hpg_selftest_mfma_f32(const float* A, const float* B, float* out) {
+    float *dA = nullptr, *dB = nullptr, *dO = nullptr;
+    if (hipMalloc(&dA, 64 * 4) != hipSuccess || hipMalloc(&dB, 64 * 4) != hipSuccess || hipMalloc(&dO, 256 * 4) != hipSuccess)
+        return -1;
+    hipMemcpy(dA, A, 64 * 4, hipMemcpyHostToDevice);
+    hipMemcpy(dB, B, 64 * 4, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_mfma_probe_f32, dim3(1), dim3(64), 0, 0, dA, dB, dO);
+    const hipError_t e = hipMemcpy(out, dO, 256 * 4, hipMemcpyDeviceToHost);
+    hipFree(dA); hipFree(dB); hipFree(dO);
+    return e == hipSuccess ? 0 : -1;
 }
 
 // MFMA f64 16x16x4 lane-layout probe (test support; A 16x4, B 4x16 row-major, out 64x4)

@@ -43,6 +43,46 @@ struct ElemArgs {
     const double* ops;
     int sm0, im0;
     double rho, mu, adis, beta;
+    // first colour that touches each vertex / side: that colour initialises the
+    // coefficient from the stage's carried real part instead of accumulating
+    const unsigned char* fcv;
+    const unsigned char* fcs;
+    int colour;
+    double keep;  // 1 - beta(stage) for stage < nstage, 0 for the full residual
+    // optional: per-Gauss-point linearisation data (u, v, strong residual) in FP32,
+    // stored per 16-element group in the accumulator layout; group = gbase + wave task
+    float* lin;
+    int gbase;
+};
+
+// FP32 linearised residual J(u0) * delta of one colour (mixed-precision inner solve)
+struct LinArgs {
+    const double* pnts;
+    const int* tri_pnt;
+    const int* tri_seg;
+    const int* tri_sgn;
+    const int* order;
+    int begin, end;
+    const float* dv;   // delta coefficients
+    const float* ds;
+    const float* di;
+    float* resv;       // residual (imaginary + carried real part + source)
+    float* ress;
+    float* resi;
+    float* rrv;        // carried real part
+    float* rrs;
+    float* rri;
+    const double* srcv;  // constant source added once (the FP64 residual at u0), may be null
+    const double* srcs;
+    const double* srci;
+    const float* ops;  // FP32 operator pack
+    const float* lin;  // linearisation data written by the FP64 kernel
+    int gbase;
+    int sm0, im0;
+    float rho, mu, adis, beta, keep;
+    const unsigned char* fcv;
+    const unsigned char* fcs;
+    int colour;
 };
 
 }  // namespace hpg

@@ -31,7 +31,8 @@ __device__ __forceinline__ int mode_offset(int k, int t, const int* vtx, const i
     sign = 1.0;
     if (k < 3) { kind = 0; return vtx[k] * NV; }
     if (k < D::BM) {
-        const int i = (k - 3) / D::SM, m = (k - 3) - i * D::SM;
+        constexpr int SMD = D::SM > 0 ? D::SM : 1;
+        const int i = (k - 3) / SMD, m = (k - 3) - i * SMD;
         kind = 1;
         sign = ((m & 1) && sgn[i] < 0) ? -1.0 : 1.0;
         return (seg[i] * sm0 + m) * NV;
@@ -46,6 +47,7 @@ template <int P, bool FULL>
 __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
     using D = Dims<P>;
     constexpr int GT = D::GT, KT = D::KT;
+    constexpr int SMD = D::SM > 0 ? D::SM : 1;
     __shared__ double sops[D::PACK];
     // XCD-aware block order: hardware deals consecutive blocks round-robin to the
     // 8 XCDs; give each XCD a contiguous run of elements so neighbours share L2.
@@ -200,6 +202,21 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
                 }
             R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
         }
+        if (a.lin) {
+            // linearisation data of this 16-element group, coalesced per (field, tile, reg)
+            float* L = a.lin + (size_t)(a.gbase + lb * 4 + w) * (5 * GT * 4 * 64);
+#pragma unroll
+            for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int o = (mt * 4 + i) * 64 + lane;
+                    L[0 * GT * 256 + o] = (float)U[mt][i];
+                    L[1 * GT * 256 + o] = (float)V[mt][i];
+                    L[2 * GT * 256 + o] = (float)R0[mt][i];
+                    L[3 * GT * 256 + o] = (float)R1[mt][i];
+                    L[4 * GT * 256 + o] = (float)R2[mt][i];
+                }
+        }
         // per-point tau (CALC_TAU2, rsdl.cpp:377-395) and the stabilisation (:397-421)
         const double dx1 = x0 - x2, dy1 = y0 - y2, dx2 = x1 - x0, dy2 = y1 - y0;
         const double ar2 = dx1 * dy2 - dy1 * dx2;
@@ -243,6 +260,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
     }
 
     // lftog: register i of lane l holds mode 4*i + l/16 of element l%16
+    // res = sum lf_im + res_r,  res_r = (1-beta) res_r + beta sum lf_re
+    // (tri_hp/nstage.cpp:21-82).  The first colour that touches a coefficient starts
+    // both sums from keep * res_r; interiors belong to one element.
     if (!valid) return;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -252,15 +272,29 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
         double sg;
         const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg);
         double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
-        di[off + 0] += sg * LI0[i];
-        di[off + 1] += sg * LI1[i];
-        di[off + 2] += sg * LI2[i];
+        double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+        const bool first = kind == 2 || (kind == 0 ? a.fcv[vtx[k]] : a.fcs[seg[(k - 3) / SMD]]) == a.colour;
+        const double li[3] = {sg * LI0[i], sg * LI1[i], sg * LI2[i]};
+        double lr[3] = {0.0, 0.0, 0.0};
         if constexpr (FULL) {
-            double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
             const double bt = a.beta;
-            dr[off + 0] += sg * (LR0[i] * bt);
-            dr[off + 1] += sg * (LR1[i] * bt);
-            dr[off + 2] += sg * (LR2[i] * bt);
+            lr[0] = sg * (LR0[i] * bt);
+            lr[1] = sg * (LR1[i] * bt);
+            lr[2] = sg * (LR2[i] * bt);
+        }
+        if (first) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                const double r0 = a.keep == 0.0 ? 0.0 : a.keep * dr[off + n];
+                if (FULL || a.keep != 1.0) dr[off + n] = r0 + lr[n];
+                di[off + n] = r0 + lr[n] + li[n];
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                if constexpr (FULL) dr[off + n] += lr[n];
+                di[off + n] += lr[n] + li[n];
+            }
         }
     }
 }
@@ -458,7 +492,8 @@ __global__ void k_setup_side(int nseg, const int* st_tri, const double* tprcn, d
 
 // ---------------------------------------------------------------------------------
 // minvrt, tri_hp/setup_preconditioner.cpp:102-373 (diagonal preconditioner)
-struct MinvArgs {
+template <typename T>
+struct MinvArgsT {
     const int* tri_pnt;
     const int* tri_seg;
     const int* tri_sgn;
@@ -472,29 +507,38 @@ struct MinvArgs {
     const int* st_loc;
     const unsigned char* vess;
     const unsigned char* sess;
-    double* resv;
-    double* ress;
-    double* resi;
-    const double* vprcn;
-    const double* sprcn;
-    const double* tprcn;
-    const double* lump;  // sfmv | ifmb | vfms | sdiag | sfms | bfmi | idiag
+    T* resv;
+    T* ress;
+    T* resi;
+    const T* vprcn;
+    const T* sprcn;
+    const T* tprcn;
+    const T* lump;  // sfmv | ifmb | vfms | sdiag | sfms | bfmi | idiag
     int npnt, nseg, ntri, sm, im, bm, sm0, im0;
     int o_sfmv, o_ifmb, o_vfms, o_sdiag, o_sfms, o_bfmi, o_idiag;
+    // fused stage update ug = ug0 - cfa * res (tri_hp/nstage.cpp:387-409) when ug != null
+    T* ugv;
+    T* ugs;
+    T* ugi;
+    const T* ug0v;
+    const T* ug0s;
+    const T* ug0i;
+    T cfa;
 };
 
 // vertices: subtract side and interior couplings (:110-135), scale (:151-152),
 // vertex Dirichlet (:179-183)
-__global__ void k_minv_vrtx(MinvArgs a) {
+template <typename T>
+__global__ void k_minv_vrtx(MinvArgsT<T> a) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.npnt) return;
-    double r[NV];
+    T r[NV];
     for (int n = 0; n < NV; ++n) r[n] = a.resv[v * NV + n];
     if (a.sm > 0) {
         for (int k = a.vs_ptr[v]; k < a.vs_ptr[v + 1]; ++k) {
             const int s = a.vs_side[k], i = a.vs_end[k];
             for (int m = 0; m < a.sm; ++m) {
-                const double c = a.lump[a.o_sfmv + i * a.sm + m];
+                const T c = a.lump[a.o_sfmv + i * a.sm + m];
                 for (int n = 0; n < NV; ++n) r[n] -= c * a.ress[(s * a.sm0 + m) * NV + n];
             }
         }
@@ -502,23 +546,25 @@ __global__ void k_minv_vrtx(MinvArgs a) {
             for (int k = a.vt_ptr[v]; k < a.vt_ptr[v + 1]; ++k) {
                 const int t = a.vt_tri[k], i = a.vt_loc[k];
                 for (int m = 0; m < a.im; ++m) {
-                    const double c = a.lump[a.o_ifmb + i * a.im + m];
+                    const T c = a.lump[a.o_ifmb + i * a.im + m];
                     for (int n = 0; n < NV; ++n) r[n] -= c * a.resi[(t * a.im0 + m) * NV + n];
                 }
             }
     }
     const unsigned msk = a.vess[v];
     for (int n = 0; n < NV; ++n) {
-        const double x = r[n] * a.vprcn[v * NV + n];
-        a.resv[v * NV + n] = (msk >> n) & 1 ? 0.0 : x;
+        const T x = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NV + n];
+        a.resv[v * NV + n] = x;
+        if (a.ugv) a.ugv[v * NV + n] = a.ug0v[v * NV + n] - a.cfa * x;
     }
 }
 
 // sides: subtract interior couplings (:137-148), then the vertex solution (:192-214)
-__global__ void k_minv_side(MinvArgs a) {
+template <typename T>
+__global__ void k_minv_side(MinvArgsT<T> a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    double r[5][NV];
+    T r[5][NV];
     for (int m = 0; m < a.sm; ++m)
         for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
     if (a.im > 0)
@@ -527,10 +573,10 @@ __global__ void k_minv_side(MinvArgs a) {
             if (t < 0) continue;
             const int i = a.st_loc[2 * s + k];
             const int sgn = a.tri_sgn[3 * t + i];
-            double ms = 1.0;
+            T ms = T(1);
             for (int j = 0; j < a.sm; ++j) {
                 for (int m = 0; m < a.im; ++m) {
-                    const double c = ms * a.lump[a.o_ifmb + (3 + i * a.sm + j) * a.im + m];
+                    const T c = ms * a.lump[a.o_ifmb + (3 + i * a.sm + j) * a.im + m];
                     for (int n = 0; n < NV; ++n) r[j][n] -= c * a.resi[(t * a.im0 + m) * NV + n];
                 }
                 ms *= sgn;
@@ -541,16 +587,16 @@ __global__ void k_minv_side(MinvArgs a) {
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
         const int sgn = a.tri_sgn[3 * t + i];
-        double uh[3][NV];
+        T uh[3][NV];
         for (int j = 0; j < 3; ++j) {
             const int w = a.tri_pnt[3 * t + j];
             for (int n = 0; n < NV; ++n) uh[j][n] = a.resv[w * NV + n] * a.tprcn[3 * t + n];
         }
         for (int j = 0; j < 3; ++j) {
             const int i1 = (i + j) % 3;
-            double ms = 1.0;
+            T ms = T(1);
             for (int m = 0; m < a.sm; ++m) {
-                const double c = a.lump[a.o_vfms + j * a.sm + m];
+                const T c = a.lump[a.o_vfms + j * a.sm + m];
                 for (int n = 0; n < NV; ++n) r[m][n] -= ms * c * uh[i1][n];
                 ms *= sgn;
             }
@@ -560,36 +606,38 @@ __global__ void k_minv_side(MinvArgs a) {
         for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
 }
 
-__device__ __forceinline__ double side_scaled(const MinvArgs& a, int s, int mode, int n) {
-    if ((a.sess[s] >> n) & 1) return 0.0;
+template <typename T>
+__device__ __forceinline__ T side_scaled(const MinvArgsT<T>& a, int s, int mode, int n) {
+    if ((a.sess[s] >> n) & 1) return T(0);
     return a.ress[(s * a.sm0 + mode) * NV + n] * (a.sprcn[s * NV + n] * a.lump[a.o_sdiag + mode]);
 }
 
 // remove solved side mode `mode` from the higher modes (:244-328); the solved value
 // sprcn*sdiag*res with sdirichlet is formed on the fly, stored later by k_minv_sscale
-__global__ void k_minv_sub(MinvArgs a, int mode) {
+template <typename T>
+__global__ void k_minv_sub(MinvArgsT<T> a, int mode) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    double r[5][NV];
+    T r[5][NV];
     for (int m = mode + 1; m < a.sm; ++m)
         for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
     for (int k = 0; k < 2; ++k) {
         const int t = a.st_tri[2 * s + k];
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
-        double uh[3][NV];
+        T uh[3][NV];
         int sign[3];
         for (int j = 0; j < 3; ++j) {
             const int sj = a.tri_seg[3 * t + j];
             sign[j] = a.tri_sgn[3 * t + j];
-            const double sg = (mode % 2) ? sign[j] : 1.0;
+            const T sg = (mode % 2) ? T(sign[j]) : T(1);
             for (int n = 0; n < NV; ++n) uh[j][n] = sg * side_scaled(a, sj, mode, n) * a.tprcn[3 * t + n];
         }
-        double ms = ((mode + 1) % 2) ? sign[i] : 1.0;
+        T ms = ((mode + 1) % 2) ? T(sign[i]) : T(1);
         for (int m = mode + 1; m < a.sm; ++m) {
             for (int j = 0; j < 3; ++j) {
                 const int ix = (i + j) % 3;
-                const double c = ms * a.lump[a.o_sfms + (mode * a.sm + m) * 3 + j];
+                const T c = ms * a.lump[a.o_sfms + (mode * a.sm + m) * 3 + j];
                 for (int n = 0; n < NV; ++n) r[m][n] -= c * uh[ix][n];
             }
             ms *= sign[i];
@@ -600,26 +648,32 @@ __global__ void k_minv_sub(MinvArgs a, int mode) {
 }
 
 // store the solved side modes: res.s *= sprcn*sdiag(mode), sdirichlet (:245-267, :331-343)
-__global__ void k_minv_sscale(MinvArgs a) {
+template <typename T>
+__global__ void k_minv_sscale(MinvArgsT<T> a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    double out[5][NV];
+    T out[5][NV];
     for (int m = 0; m < a.sm; ++m)
         for (int n = 0; n < NV; ++n) out[m][n] = side_scaled(a, s, m, n);
     for (int m = 0; m < a.sm; ++m)
-        for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = out[m][n];
+        for (int n = 0; n < NV; ++n) {
+            const int ix = (s * a.sm0 + m) * NV + n;
+            a.ress[ix] = out[m][n];
+            if (a.ugs) a.ugs[ix] = a.ug0s[ix] - a.cfa * out[m][n];
+        }
 }
 
 // interiors: banded solve with the literal DPBTRSNU2 indexing (utilities/myblas.c:91-122),
 // divide by tprcn, subtract the boundary solution through bfmi (:347-370)
-__global__ void k_minv_int(MinvArgs a) {
+template <typename T>
+__global__ void k_minv_int(MinvArgsT<T> a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ntri) return;
     const int im = a.im;
-    double b[3][NV];
+    T b[3][NV];
     for (int k = 0; k < im; ++k)
         for (int n = 0; n < NV; ++n) b[k][n] = a.resi[(t * a.im0 + k) * NV + n];
-    const double* ab = a.lump + a.o_idiag;  // stride 2, ofdg 1
+    const T* ab = a.lump + a.o_idiag;  // stride 2, ofdg 1
     for (int i = 0; i < im; ++i) {
         const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
         for (int j = frst; j < 1; ++j)
@@ -632,7 +686,7 @@ __global__ void k_minv_int(MinvArgs a) {
         for (int j = frst; j < 1; ++j)
             for (int n = 0; n < NV; ++n) b[ba + j][n] -= ab[i * 2 + j] * b[i][n];
     }
-    double uh[12][NV];
+    T uh[12][NV];
     for (int i = 0; i < 3; ++i) {
         const int w = a.tri_pnt[3 * t + i];
         for (int n = 0; n < NV; ++n) uh[i][n] = a.resv[w * NV + n];
@@ -640,7 +694,7 @@ __global__ void k_minv_int(MinvArgs a) {
     int cnt = 3;
     for (int i = 0; i < 3; ++i) {
         const int sd = a.tri_seg[3 * t + i], sgn = a.tri_sgn[3 * t + i];
-        double ms = 1.0;
+        T ms = T(1);
         for (int m = 0; m < a.sm; ++m) {
             for (int n = 0; n < NV; ++n) uh[cnt][n] = ms * a.ress[(sd * a.sm0 + m) * NV + n];
             ms *= sgn;
@@ -650,12 +704,16 @@ __global__ void k_minv_int(MinvArgs a) {
     for (int k = 0; k < im; ++k) {
         for (int n = 0; n < NV; ++n) b[k][n] /= a.tprcn[3 * t + n];
         for (int i = 0; i < a.bm; ++i) {
-            const double c = a.lump[a.o_bfmi + i * im + k];
+            const T c = a.lump[a.o_bfmi + i * im + k];
             for (int n = 0; n < NV; ++n) b[k][n] -= c * uh[i][n];
         }
     }
     for (int k = 0; k < im; ++k)
-        for (int n = 0; n < NV; ++n) a.resi[(t * a.im0 + k) * NV + n] = b[k][n];
+        for (int n = 0; n < NV; ++n) {
+            const int ix = (t * a.im0 + k) * NV + n;
+            a.resi[ix] = b[k][n];
+            if (a.ugi) a.ugi[ix] = a.ug0i[ix] - a.cfa * b[k][n];
+        }
 }
 
 // ---------------------------------------------------------------------------------
@@ -698,6 +756,17 @@ __global__ void k_mfma_probe(const double* A, const double* B, double* out) {
     const double bv = B[(l >> 4) * 16 + (l & 15)];
     d4 c = {0.0, 0.0, 0.0, 0.0};
     c = mfma(av, bv, c);
+    for (int i = 0; i < 4; ++i) out[l * 4 + i] = c[i];
+}
+
+// same probe for v_mfma_f32_16x16x4f32 (FP32 inputs, FP32 accumulation)
+__global__ void k_mfma_probe_f32(const float* A, const float* B, float* out) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int l = threadIdx.x;
+    const float av = A[(l & 15) * 4 + (l >> 4)];
+    const float bv = B[(l >> 4) * 16 + (l & 15)];
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, c, 0, 0, 0);
     for (int i = 0; i < 4; ++i) out[l * 4 + i] = c[i];
 }
 

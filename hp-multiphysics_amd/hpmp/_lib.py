@@ -25,7 +25,8 @@ EXPORTS = (
     "hpg_get_preconditioner", "hpg_set_tprcn", "hpg_rsdl", "hpg_setup_preconditioner", "hpg_minvrt",
     "hpg_update", "hpg_maxres", "hpg_resnorm", "hpg_mg_restrict", "hpg_mg_prolongate", "hpg_get_log2p",
     "hpg_set_precision", "hpg_set_preconditioner", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
-    "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read",
+    "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32",
+    "hpg_set_defect_inner", "hpg_apply_jacobian",
 )
 
 
@@ -84,6 +85,9 @@ def load():
     L.hpg_selftest_mfma.argtypes = [dp, dp, dp]
     L.hpg_basis_operator.argtypes = [C.c_int, C.c_char_p, dp]
     L.hpg_basis_operator.restype = C.c_int
+    L.hpg_selftest_mfma_f32.argtypes = [dp, dp, dp]
+    L.hpg_set_defect_inner.argtypes = [vp, C.c_int]
+    L.hpg_apply_jacobian.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.hpg_kernel_timing.argtypes = [vp, C.c_int]
     L.hpg_kernel_timing_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                          C.POINTER(C.c_longlong)]

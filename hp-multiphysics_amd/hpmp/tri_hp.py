@@ -161,8 +161,20 @@ class TriHpIns:
         self._check(self.L.hpg_get_log2p(self.h, C.byref(v)), "get_log2p")
         return v.value
 
-    def set_precision(self, mode: int):
+    def set_precision(self, mode: int, ninner: int | None = None):
+        """HPG_FP64 (plain) or HPG_FP32_DEFECT (FP64 defect correction around FP32
+        inner multistage iterations on the linearised residual)."""
         self._check(self.L.hpg_set_precision(self.h, mode), "set_precision")
+        if ninner is not None:
+            self._check(self.L.hpg_set_defect_inner(self.h, int(ninner)), "set_defect_inner")
+
+    def apply_jacobian(self, dv, ds, di):
+        """J(u) delta at the current state (FP32 linearised kernels); leaves R(u) in the residual."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (dv, ds, di)]
+        out = self.zeros()
+        self._check(self.L.hpg_apply_jacobian(self.h, *(_lib.ptr(x) for x in a), *(_lib.ptr(x) for x in out)),
+                    "apply_jacobian")
+        return out
 
     def synchronize(self):
         self._check(self.L.hpg_synchronize(self.h), "synchronize")

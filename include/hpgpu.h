@@ -95,10 +95,20 @@ int hpg_mg_restrict(hpg_ctx *ctx);
 int hpg_mg_prolongate(hpg_ctx *ctx);
 int hpg_get_log2p(hpg_ctx *ctx, int *log2p);
 
-/* precision and preconditioner selection (no reference counterpart: north-star
- * additions behind the same update() entry point) */
+/* precision and preconditioner selection (no reference counterpart: additions
+ * behind the same update() entry point).  In HPG_FP32_DEFECT mode hpg_update on
+ * the finest level is one defect-correction iteration: the FP64 full residual
+ * R(u0) (left in the residual arrays), then `ninner` multistage iterations of the
+ * reference's scheme in FP32 on R(u0) + J(u0) delta = 0 with the exact Jacobian of
+ * the element and edge residuals, then u0 += delta.  Coarse p-levels stay FP64. */
 int hpg_set_precision(hpg_ctx *ctx, int mode);
 int hpg_set_preconditioner(hpg_ctx *ctx, int mode);
+int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
+/* J(u) delta at the current state through the FP32 linearised kernels (finest
+ * level; overwrites the residual arrays with R(u)).  No reference counterpart;
+ * the directional derivative of tri_hp::rsdl(nstage). */
+int hpg_apply_jacobian(hpg_ctx *ctx, const double *dv, const double *ds, const double *di,
+                       double *outv, double *outs, double *outi);
 
 /* blocks until the context's stream is idle */
 int hpg_synchronize(hpg_ctx *ctx);
@@ -121,6 +131,8 @@ const char *hpg_last_error(hpg_ctx *ctx);
  *   out[lane*4 + i] is accumulator register i of lane `lane`. */
 int hpg_basis_operator(int p, const char *name, double *out);
 int hpg_selftest_mfma(const double *A, const double *B, double *out);
+/* the same for v_mfma_f32_16x16x4f32 */
+int hpg_selftest_mfma_f32(const float *A, const float *B, float *out);
 
 #ifdef __cplusplus
 }
