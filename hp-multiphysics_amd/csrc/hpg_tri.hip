@@ -17,6 +17,7 @@
 #include "tri_kernels.hip"
 #include "tri_kernels_lin.hip"
 #include "tri_ops.h"
+#include "mlp.h"
 
 using namespace hpg;
 
@@ -138,6 +139,11 @@ struct hpg_ctx {
     float *dlv = nullptr, *dls = nullptr, *dli = nullptr, *d0v = nullptr, *d0s = nullptr, *d0i = nullptr;
     float *r32v = nullptr, *r32s = nullptr, *r32i = nullptr, *rr32v = nullptr, *rr32s = nullptr, *rr32i = nullptr;
     float *vp32 = nullptr, *sp32 = nullptr, *tp32 = nullptr;
+    // learned preconditioner (the MLP is owned by the caller)
+    hpg_mlp* mlp = nullptr;
+    float* featX = nullptr;
+    float* mlpY = nullptr;
+    double* jdiag = nullptr;
     unsigned long long* d_max;
     Level lv[3];
     std::string err;
@@ -421,6 +427,14 @@ int do_setup(hpg_ctx* c) {
     s.ntri = c->ntri; s.p = d.p; s.sm = d.sm; s.im = d.im; s.tm = d.tm; s.G = d.G; s.sm0 = c->sm0; s.im0 = c->im0;
     s.rho = c->prm.rho; s.mu = c->prm.mu; s.adis = c->prm.adis;
     hipLaunchKernelGGL(k_setup_elem, dim3(nblk(c->ntri, 128)), dim3(128), 0, c->stream, s);
+    if (c->prcn_mode == HPG_PRCN_LEARNED && c->mlp) {
+        // learned scalings of the analytic tprcn (element features -> MLP -> exp)
+        hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts, c->tri_pnt,
+                           c->ugv, c->tprcn, c->prm.mu, c->prm.rho, c->featX);
+        mlp_forward_device(c->mlp, c->featX, c->ntri, c->mlpY, c->stream);
+        hipLaunchKernelGGL(k_mlp_apply, dim3(nblk((long)c->ntri * NV)), dim3(256), 0, c->stream, c->ntri, c->mlpY,
+                           c->tprcn);
+    }
     hipLaunchKernelGGL(k_setup_vrtx, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vt_ptr, c->vt_tri,
                        c->tprcn, c->vprcn, L.ops->vdiag);
     if (d.sm > 0)
@@ -587,6 +601,9 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->vp32 = c->alloc<float>(nv);
         c->sp32 = c->alloc<float>((size_t)ns * NV);
         c->tp32 = c->alloc<float>((size_t)nt * NV);
+        c->featX = c->alloc<float>((size_t)nt * MLP_IN);
+        c->mlpY = c->alloc<float>((size_t)nt * MLP_OUT);
+        c->jdiag = c->alloc<double>((size_t)nt * NV);
 
         for (int l = 0; l <= c->log2pmax; ++l) {
             Level& L = c->lv[l];
@@ -763,6 +780,59 @@ int hpg_apply_jacobian(hpg_ctx* c, const double* dv, const double* ds, const dou
         for (size_t k = 0; k < nv; ++k) ov[k] = hv[k];
         for (size_t k = 0; k < nsz; ++k) os[k] = hs[k];
         for (size_t k = 0; k < ni; ++k) oi[k] = hi[k];
+    });
+    return 0;
+}
+
+int hpg_set_mlp(hpg_ctx* c, hpg_mlp* m) {
+    if (!c) return -2;
+    c->mlp = m;
+    return 0;
+}
+
+int hpg_mlp_features(hpg_ctx* c, float* out) {
+    if (!c || !out) return -2;
+    HPG_TRY(c, {
+        hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts, c->tri_pnt,
+                           c->ugv, c->tprcn, c->prm.mu, c->prm.rho, c->featX);
+        check(hipMemcpyAsync(out, c->featX, (size_t)c->ntri * MLP_IN * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+
+int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
+    if (!c || !out) return -2;
+    if (c->log2p != c->log2pmax) { c->err = "measure_jacobian_diag runs on the finest level"; return -2; }
+    HPG_TRY(c, {
+        do_rsdl(c, c->prm.nstage, true);
+        check(hipMemsetAsync(c->jdiag, 0, (size_t)c->ntri * NV * 8, c->stream), "memset");
+        const Level& L = c->lv[c->log2p];
+        const LevelDims d(L.p);
+        LinArgs a{};
+        a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+        a.order = c->order; a.ops = L.pack32; a.lin = c->lin; a.sm0 = c->sm0; a.im0 = c->im0;
+        a.rho = (float)c->prm.rho; a.mu = (float)c->prm.mu; a.adis = (float)c->prm.adis;
+        a.beta = 1.f; a.keep = 0.f; a.fcv = c->fcv; a.fcs = c->fcs; a.diag = c->jdiag;
+        for (int mode = 0; mode < d.tm; ++mode)
+            for (int n = 0; n < NV; ++n) {
+                a.probe_m = mode; a.probe_n = n;
+                for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+                    LinArgs b = a;
+                    b.colour = (int)col; b.begin = c->coff[col]; b.end = c->coff[col + 1]; b.gbase = c->gbase[col];
+                    const int nlogical = (b.end - b.begin + 63) / 64;
+                    if (nlogical == 0) continue;
+                    const int nb = ((nlogical + 7) / 8) * 8;
+                    switch (L.p) {
+                        case 1: hipLaunchKernelGGL((k_elem_lin<1, true, true>), dim3(nb), dim3(256), 0, c->stream, b, nlogical); break;
+                        case 2: hipLaunchKernelGGL((k_elem_lin<2, true, true>), dim3(nb), dim3(256), 0, c->stream, b, nlogical); break;
+                        default: hipLaunchKernelGGL((k_elem_lin<4, true, true>), dim3(nb), dim3(256), 0, c->stream, b, nlogical); break;
+                    }
+                }
+            }
+        check(hipGetLastError(), "probe launch");
+        check(hipMemcpyAsync(out, c->jdiag, (size_t)c->ntri * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
     });
     return 0;
 }

@@ -1,0 +1,11 @@
+// Internal interface between the solver context and the preconditioner MLP.
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct hpg_mlp;
+
+namespace hpg {
+constexpr int MLP_IN = 11, MLP_OUT = 3;
+// Y[n][3] = MLP(X[n][11]) on device buffers, enqueued on `stream`
+void mlp_forward_device(const hpg_mlp* m, const float* X, int n, float* Y, hipStream_t stream);
+}  // namespace hpg

@@ -1,0 +1,409 @@
+// Preconditioner MLP: training and inference kernels for gfx950.
+//
+// Network 11 -> 64 -> 64 -> 64 -> 3, GELU (tanh form) on the hidden layers.  The
+// FP32 master weights are rounded to bfloat16 (nearest even) for every matrix
+// product and activations are rounded to bfloat16 on input to a product;
+// accumulation is FP32.  Weights are uniform across a wave, so every weight read
+// is a scalar (constant-cache) load; each lane carries one sample.
+// Training: per 64-sample block the backward pass keeps pre-activations and
+// deltas in LDS and forms the weight gradients as sample reductions; per-block
+// partial gradients are summed in a fixed order by the Adam kernel, so training
+// is bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hpgpu.h"
+#include "mlp.h"
+
+namespace hpg {
+
+constexpr int MI = 11, MH = 64, MO = 3;
+constexpr int O_W1 = 0, O_B1 = O_W1 + MI * MH, O_W2 = O_B1 + MH, O_B2 = O_W2 + MH * MH, O_W3 = O_B2 + MH,
+              O_B3 = O_W3 + MH * MH, O_W4 = O_B3 + MH, O_B4 = O_W4 + MH * MO, NPARAM = O_B4 + MO;
+constexpr float C_GELU = 0.7978845608028654f;  // sqrt(2/pi)
+
+__host__ __device__ inline float bf16r(float x) {
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+__device__ inline float gelu(float z) {
+    const float t = tanhf(C_GELU * (z + 0.044715f * z * z * z));
+    return 0.5f * z * (1.f + t);
+}
+__device__ inline float gelu_d(float z) {
+    const float u = C_GELU * (z + 0.044715f * z * z * z);
+    const float t = tanhf(u);
+    const float du = C_GELU * (1.f + 3.f * 0.044715f * z * z);
+    return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * du;
+}
+
+// one hidden layer: z[o] = b[o] + sum_i bf16(h[i]) * wq[i][o]
+template <int NIN>
+__device__ inline void layer(const float* __restrict__ wq, const float* __restrict__ h, float* z) {
+    float hb[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) hb[i] = bf16r(h[i]);
+#pragma unroll
+    for (int o = 0; o < MH; ++o) {
+        float s = wq[NIN * MH + o];  // bias follows the weight block
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) s = fmaf(hb[i], wq[i * MH + o], s);
+        z[o] = s;
+    }
+}
+
+__device__ inline void head(const float* __restrict__ wq, const float* h, float* y) {
+    float hb[MH];
+#pragma unroll
+    for (int i = 0; i < MH; ++i) hb[i] = bf16r(h[i]);
+#pragma unroll
+    for (int o = 0; o < MO; ++o) {
+        float s = wq[O_B4 + o];
+#pragma unroll
+        for (int i = 0; i < MH; ++i) s = fmaf(hb[i], wq[O_W4 + i * MO + o], s);
+        y[o] = s;
+    }
+}
+
+__global__ __launch_bounds__(128) void k_mlp_forward(int n, const float* __restrict__ X, const float* __restrict__ wq,
+                                                     float* __restrict__ Y) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float x[MI], a[MH], b[MH];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) x[i] = X[(size_t)s * MI + i];
+    layer<MI>(wq + O_W1, x, a);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) a[j] = gelu(a[j]);
+    layer<MH>(wq + O_W2, a, b);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) b[j] = gelu(b[j]);
+    layer<MH>(wq + O_W3, b, a);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) a[j] = gelu(a[j]);
+    float y[MO];
+    head(wq, a, y);
+#pragma unroll
+    for (int o = 0; o < MO; ++o) Y[(size_t)s * MO + o] = y[o];
+}
+
+// backward of one 64-sample block: partial gradient (NPARAM) and partial loss sum
+__global__ __launch_bounds__(64) void k_mlp_backward(int n, int nscale, const float* __restrict__ X,
+                                                     const float* __restrict__ T, const float* __restrict__ wq,
+                                                     float* __restrict__ partial, float* __restrict__ lossp) {
+    // [layer][unit][sample], rows padded to 65 words so that lanes reading one
+    // sample column of different units hit different banks
+    __shared__ float sZ[3][MH][65];   // pre-activations; after the deltas, bf16(gelu(z))
+    __shared__ float sD[3][MH][65];   // deltas of the hidden layers
+    __shared__ float sX[MI][65];
+    __shared__ float sY[MO][65];      // output deltas
+    const int s = threadIdx.x;
+    const int gs = blockIdx.x * 64 + s;
+    const bool ok = gs < n;
+    float x[MI], a[MH], b[MH];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) x[i] = ok ? X[(size_t)gs * MI + i] : 0.f;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) sX[i][s] = x[i];
+    layer<MI>(wq + O_W1, x, a);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) { sZ[0][j][s] = a[j]; a[j] = gelu(a[j]); }
+    layer<MH>(wq + O_W2, a, b);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) { sZ[1][j][s] = b[j]; b[j] = gelu(b[j]); }
+    layer<MH>(wq + O_W3, b, a);
+#pragma unroll
+    for (int j = 0; j < MH; ++j) { sZ[2][j][s] = a[j]; a[j] = gelu(a[j]); }
+    float y[MO];
+    head(wq, a, y);
+    float dy[MO], l = 0.f;
+#pragma unroll
+    for (int o = 0; o < MO; ++o) {
+        const float d = ok ? y[o] - T[(size_t)gs * MO + o] : 0.f;
+        l += d * d;
+        dy[o] = 2.f * d / (float)nscale;
+        sY[o][s] = dy[o];
+    }
+    // hidden deltas: dz3 = W4 dy * g'(z3); dz2 = W3 dz3 * g'(z2); dz1 = W2 dz2 * g'(z1)
+#pragma unroll
+    for (int j = 0; j < MH; ++j) {
+        float t = 0.f;
+#pragma unroll
+        for (int o = 0; o < MO; ++o) t = fmaf(wq[O_W4 + j * MO + o], dy[o], t);
+        a[j] = t * gelu_d(sZ[2][j][s]);
+        sD[2][j][s] = a[j];
+    }
+#pragma unroll
+    for (int i = 0; i < MH; ++i) {
+        float t = 0.f;
+#pragma unroll
+        for (int j = 0; j < MH; ++j) t = fmaf(wq[O_W3 + i * MH + j], a[j], t);
+        b[i] = t * gelu_d(sZ[1][i][s]);
+        sD[1][i][s] = b[i];
+    }
+#pragma unroll
+    for (int i = 0; i < MH; ++i) {
+        float t = 0.f;
+#pragma unroll
+        for (int j = 0; j < MH; ++j) t = fmaf(wq[O_W2 + i * MH + j], b[j], t);
+        sD[0][i][s] = t * gelu_d(sZ[0][i][s]);
+    }
+    // the weight gradients need the (rounded) activations, not the pre-activations;
+    // each lane converts its own sample column
+#pragma unroll
+    for (int L = 0; L < 3; ++L)
+#pragma unroll
+        for (int j = 0; j < MH; ++j) sZ[L][j][s] = bf16r(gelu(sZ[L][j][s]));
+    // block loss
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+    if (s == 0) lossp[blockIdx.x] = l;
+    __syncthreads();
+    // weight gradients: sample reductions, one parameter per lane per pass
+    float* P = partial + (size_t)blockIdx.x * NPARAM;
+    for (int p = s; p < NPARAM; p += 64) {
+        float g = 0.f;
+        if (p < O_B1) {                       // W1[i][j]
+            const int i = p / MH, j = p % MH;
+            for (int k = 0; k < 64; ++k) g = fmaf(bf16r(sX[i][k]), sD[0][j][k], g);
+        } else if (p < O_W2) {                // b1
+            const int j = p - O_B1;
+            for (int k = 0; k < 64; ++k) g += sD[0][j][k];
+        } else if (p < O_B2) {                // W2[i][j]: h1 = bf16(gelu(z1))
+            const int q = p - O_W2, i = q / MH, j = q % MH;
+            for (int k = 0; k < 64; ++k) g = fmaf(sZ[0][i][k], sD[1][j][k], g);
+        } else if (p < O_W3) {
+            const int j = p - O_B2;
+            for (int k = 0; k < 64; ++k) g += sD[1][j][k];
+        } else if (p < O_B3) {                // W3[i][j]: h2 = bf16(gelu(z2))
+            const int q = p - O_W3, i = q / MH, j = q % MH;
+            for (int k = 0; k < 64; ++k) g = fmaf(sZ[1][i][k], sD[2][j][k], g);
+        } else if (p < O_W4) {
+            const int j = p - O_B3;
+            for (int k = 0; k < 64; ++k) g += sD[2][j][k];
+        } else if (p < O_B4) {                // W4[i][o]: h3 = bf16(gelu(z3))
+            const int q = p - O_W4, i = q / MO, o = q % MO;
+            for (int k = 0; k < 64; ++k) g = fmaf(sZ[2][i][k], sY[o][k], g);
+        } else {
+            const int o = p - O_B4;
+            for (int k = 0; k < 64; ++k) g += sY[o][k];
+        }
+        P[p] = g;
+    }
+}
+
+// gradient = ordered sum of the block partials; optional Adam step on the masters
+__global__ void k_mlp_reduce_adam(int nblocks, const float* __restrict__ partial, float* __restrict__ grad,
+                                  float* master, float* wq, float* m1, float* m2, float lr, float bc1, float bc2,
+                                  int do_step) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NPARAM) return;
+    float g = 0.f;
+    for (int b = 0; b < nblocks; ++b) g += partial[(size_t)b * NPARAM + p];
+    grad[p] = g;
+    if (!do_step) return;
+    const float a = 0.9f * m1[p] + 0.1f * g;
+    const float v = 0.999f * m2[p] + 0.001f * g * g;
+    m1[p] = a;
+    m2[p] = v;
+    const float w = master[p] - lr * (a / bc1) / (sqrtf(v / bc2) + 1e-8f);
+    master[p] = w;
+    const bool is_bias = (p >= O_B1 && p < O_W2) || (p >= O_B2 && p < O_W3) || (p >= O_B3 && p < O_W4) || p >= O_B4;
+    wq[p] = is_bias ? w : bf16r(w);
+}
+
+}  // namespace hpg
+
+using namespace hpg;
+
+struct hpg_mlp {
+    hipStream_t stream = nullptr;
+    float *master = nullptr, *wq = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+    float *partial = nullptr, *lossp = nullptr, *X = nullptr, *T = nullptr, *Y = nullptr;
+    int cap_blocks = 0, cap_n = 0, step = 0;
+    std::string err;
+    ~hpg_mlp() {
+        for (float* p : {master, wq, m1, m2, grad, partial, lossp, X, T, Y})
+            if (p) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+void ck(hipError_t e, const char* w) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
+}
+void quantise(hpg_mlp* m, const std::vector<float>& w) {
+    std::vector<float> q(w);
+    for (int p = 0; p < NPARAM; ++p) {
+        const bool is_bias = (p >= O_B1 && p < O_W2) || (p >= O_B2 && p < O_W3) || (p >= O_B3 && p < O_W4) || p >= O_B4;
+        if (!is_bias) q[p] = bf16r(w[p]);
+    }
+    ck(hipMemcpyAsync(m->master, w.data(), NPARAM * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+    ck(hipMemcpyAsync(m->wq, q.data(), NPARAM * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+    ck(hipStreamSynchronize(m->stream), "sync");
+}
+void ensure_n(hpg_mlp* m, int n) {
+    if (n <= m->cap_n) return;
+    for (float** p : {&m->X, &m->T, &m->Y})
+        if (*p) { hipFree(*p); *p = nullptr; }
+    ck(hipMalloc(&m->X, (size_t)n * MI * 4), "hipMalloc");
+    ck(hipMalloc(&m->T, (size_t)n * MO * 4), "hipMalloc");
+    ck(hipMalloc(&m->Y, (size_t)n * MO * 4), "hipMalloc");
+    m->cap_n = n;
+}
+void ensure_blocks(hpg_mlp* m, int nb) {
+    if (nb <= m->cap_blocks) return;
+    if (m->partial) hipFree(m->partial);
+    if (m->lossp) hipFree(m->lossp);
+    ck(hipMalloc(&m->partial, (size_t)nb * NPARAM * 4), "hipMalloc");
+    ck(hipMalloc(&m->lossp, (size_t)nb * 4), "hipMalloc");
+    m->cap_blocks = nb;
+}
+}  // namespace
+
+namespace hpg {
+void mlp_forward_device(const hpg_mlp* m, const float* X, int n, float* Y, hipStream_t stream) {
+    if (n > 0) hipLaunchKernelGGL(k_mlp_forward, dim3((n + 127) / 128), dim3(128), 0, stream, n, X, m->wq, Y);
+}
+}  // namespace hpg
+
+extern "C" {
+
+int hpg_mlp_create(unsigned seed, hpg_mlp** out) {
+    if (!out) return -2;
+    std::unique_ptr<hpg_mlp> m(new hpg_mlp());
+    try {
+        ck(hipStreamCreate(&m->stream), "hipStreamCreate");
+        for (float** p : {&m->master, &m->wq, &m->m1, &m->m2, &m->grad}) ck(hipMalloc(p, NPARAM * 4), "hipMalloc");
+        ck(hipMemsetAsync(m->m1, 0, NPARAM * 4, m->stream), "memset");
+        ck(hipMemsetAsync(m->m2, 0, NPARAM * 4, m->stream), "memset");
+        // He-style uniform init from a fixed LCG, biases zero
+        std::vector<float> w(NPARAM, 0.f);
+        unsigned long long st = 0x9E3779B97F4A7C15ull ^ seed;
+        auto rnd = [&]() {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            return ((st >> 40) & 0xFFFFFF) / 16777216.0f * 2.f - 1.f;
+        };
+        const int fan[4] = {MI, MH, MH, MH}, offs[4] = {O_W1, O_W2, O_W3, O_W4}, nout[4] = {MH, MH, MH, MO};
+        for (int L = 0; L < 4; ++L) {
+            const float a = std::sqrt(6.f / fan[L]) * (L == 3 ? 0.1f : 1.f);
+            for (int k = 0; k < fan[L] * nout[L]; ++k) w[offs[L] + k] = a * rnd();
+        }
+        quantise(m.get(), w);
+    } catch (const std::exception&) {
+        return -1;
+    }
+    *out = m.release();
+    return 0;
+}
+
+int hpg_mlp_destroy(hpg_mlp* m) {
+    delete m;
+    return 0;
+}
+
+int hpg_mlp_num_params(void) { return NPARAM; }
+
+int hpg_mlp_get_weights(hpg_mlp* m, float* w) {
+    if (!m || !w) return -2;
+    try {
+        ck(hipMemcpyAsync(w, m->master, NPARAM * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
+        ck(hipStreamSynchronize(m->stream), "sync");
+    } catch (const std::exception& e) { m->err = e.what(); return -1; }
+    return 0;
+}
+
+int hpg_mlp_set_weights(hpg_mlp* m, const float* w) {
+    if (!m || !w) return -2;
+    try {
+        quantise(m, std::vector<float>(w, w + NPARAM));
+        m->step = 0;
+        ck(hipMemsetAsync(m->m1, 0, NPARAM * 4, m->stream), "memset");
+        ck(hipMemsetAsync(m->m2, 0, NPARAM * 4, m->stream), "memset");
+    } catch (const std::exception& e) { m->err = e.what(); return -1; }
+    return 0;
+}
+
+int hpg_mlp_forward(hpg_mlp* m, const float* X, int n, float* Y) {
+    if (!m || !X || !Y || n < 0) return -2;
+    try {
+        ensure_n(m, n);
+        ck(hipMemcpyAsync(m->X, X, (size_t)n * MI * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+        mlp_forward_device(m, m->X, n, m->Y, m->stream);
+        ck(hipMemcpyAsync(Y, m->Y, (size_t)n * MO * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
+        ck(hipStreamSynchronize(m->stream), "sync");
+    } catch (const std::exception& e) { m->err = e.what(); return -1; }
+    return 0;
+}
+
+// loss (mean square over samples and outputs) and its gradient for one batch, no update
+int hpg_mlp_loss_grad(hpg_mlp* m, const float* X, const float* T, int n, float* loss, float* grad) {
+    if (!m || !X || !T || n <= 0) return -2;
+    try {
+        ensure_n(m, n);
+        const int nb = (n + 63) / 64;
+        ensure_blocks(m, nb);
+        ck(hipMemcpyAsync(m->X, X, (size_t)n * MI * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+        ck(hipMemcpyAsync(m->T, T, (size_t)n * MO * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+        hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(64), 0, m->stream, n, n * MO, m->X, m->T, m->wq, m->partial,
+                           m->lossp);
+        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, nb, m->partial,
+                           m->grad, m->master, m->wq, m->m1, m->m2, 0.f, 1.f, 1.f, 0);
+        std::vector<float> lp(nb);
+        ck(hipMemcpyAsync(lp.data(), m->lossp, nb * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
+        if (grad) ck(hipMemcpyAsync(grad, m->grad, NPARAM * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
+        ck(hipStreamSynchronize(m->stream), "sync");
+        double s = 0.0;
+        for (float v : lp) s += v;
+        if (loss) *loss = (float)(s / ((double)n * MO));
+    } catch (const std::exception& e) { m->err = e.what(); return -1; }
+    return 0;
+}
+
+// Adam training: `epochs` passes over (X, T) in mini-batches of `batch` samples
+// (taken in order), learning rate lr; returns the mean loss of the last epoch
+int hpg_mlp_train(hpg_mlp* m, const float* X, const float* T, int n, int epochs, int batch, float lr, float* last) {
+    if (!m || !X || !T || n <= 0 || batch <= 0 || epochs <= 0) return -2;
+    try {
+        ensure_n(m, n);
+        batch = ((batch + 63) / 64) * 64;
+        ensure_blocks(m, batch / 64);
+        ck(hipMemcpyAsync(m->X, X, (size_t)n * MI * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+        ck(hipMemcpyAsync(m->T, T, (size_t)n * MO * 4, hipMemcpyHostToDevice, m->stream), "H2D");
+        double lsum = 0.0;
+        std::vector<float> lp(batch / 64);
+        for (int ep = 0; ep < epochs; ++ep) {
+            lsum = 0.0;
+            for (int b0 = 0; b0 < n; b0 += batch) {
+                const int nb_s = std::min(batch, n - b0), nb = (nb_s + 63) / 64;
+                ++m->step;
+                const float bc1 = 1.f - std::pow(0.9f, (float)m->step), bc2 = 1.f - std::pow(0.999f, (float)m->step);
+                hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(64), 0, m->stream, nb_s, nb_s * MO,
+                                   m->X + (size_t)b0 * MI, m->T + (size_t)b0 * MO, m->wq, m->partial, m->lossp);
+                hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, nb,
+                                   m->partial, m->grad, m->master, m->wq, m->m1, m->m2, lr, bc1, bc2, 1);
+                if (ep == epochs - 1) {
+                    ck(hipMemcpyAsync(lp.data(), m->lossp, nb * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
+                    ck(hipStreamSynchronize(m->stream), "sync");
+                    for (int k = 0; k < nb; ++k) lsum += lp[k];
+                }
+            }
+            ck(hipGetLastError(), "train launch");
+        }
+        ck(hipStreamSynchronize(m->stream), "sync");
+        if (last) *last = (float)(lsum / ((double)n * MO));
+    } catch (const std::exception& e) { m->err = e.what(); return -1; }
+    return 0;
+}
+
+}  // extern "C"

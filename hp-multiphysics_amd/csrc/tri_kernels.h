@@ -83,6 +83,10 @@ struct LinArgs {
     const unsigned char* fcv;
     const unsigned char* fcs;
     int colour;
+    // probing of the element Jacobian diagonal: element-local unit vector on
+    // (mode probe_m, variable probe_n); |J_e| diagonal entry accumulated into diag
+    int probe_m, probe_n;
+    double* diag;
 };
 
 }  // namespace hpg

@@ -25,7 +25,7 @@ __device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
 #endif
 __host__ __device__ constexpr int f32_row(int i, int q) { return HPG_F32_ROW_4I_PLUS_Q ? 4 * i + q : 4 * q + i; }
 
-template <int P, bool FULL>
+template <int P, bool FULL, bool PROBE = false>
 __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a, int nlogical) {
     using D = Dims<P>;
     constexpr int GT = D::GT, KT = D::KT;
@@ -67,7 +67,10 @@ __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a, int nlogical) {
         const float* src = kind == 0 ? a.dv : (kind == 1 ? a.ds : a.di);
 #pragma unroll
         for (int n = 0; n < NV; ++n) b[n][s] = 0.f;
-        if (valid && k < D::TM) {
+        if constexpr (PROBE) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) b[n][s] = (k == a.probe_m && n == a.probe_n) ? 1.f : 0.f;
+        } else if (valid && k < D::TM) {
 #pragma unroll
             for (int n = 0; n < NV; ++n) b[n][s] = (float)sg * src[off + n];
         }
@@ -239,6 +242,17 @@ __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a, int nlogical) {
     }
 
     if (!valid) return;
+    if constexpr (PROBE) {
+        // element-local diagonal entry J_e(m n, m n) of the full residual (beta = 1)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (f32_row(i, q) != a.probe_m) continue;
+            const float li[3] = {LI0[i], LI1[i], LI2[i]};
+            const float lr[3] = {LR0[i], LR1[i], LR2[i]};
+            a.diag[(size_t)t * NV + a.probe_n] += fabs((double)li[a.probe_n] + (double)lr[a.probe_n]);
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int k = f32_row(i, q);
@@ -361,5 +375,39 @@ template __global__ void k_elem_lin<2, true>(LinArgs, int);
 template __global__ void k_elem_lin<2, false>(LinArgs, int);
 template __global__ void k_elem_lin<4, true>(LinArgs, int);
 template __global__ void k_elem_lin<4, false>(LinArgs, int);
+template __global__ void k_elem_lin<1, true, true>(LinArgs, int);
+template __global__ void k_elem_lin<2, true, true>(LinArgs, int);
+template __global__ void k_elem_lin<4, true, true>(LinArgs, int);
+
+// learned-preconditioner inputs per element (11): vertex offsets from the centroid
+// scaled by sqrt(area) (6), vertex-mean u, v, p (3), log(mu / (rho sqrt(area))),
+// log(area / tprcn_u) (the analytic local pseudo-time step)
+__global__ void k_mlp_features(int ntri, const double* pnts, const int* tri_pnt, const double* ugv,
+                               const double* tprcn, double mu, double rho, float* X) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    const int v0 = tri_pnt[3 * t], v1 = tri_pnt[3 * t + 1], v2 = tri_pnt[3 * t + 2];
+    const double x[3] = {pnts[2 * v0], pnts[2 * v1], pnts[2 * v2]};
+    const double y[3] = {pnts[2 * v0 + 1], pnts[2 * v1 + 1], pnts[2 * v2 + 1]};
+    const double area = 0.5 * ((x[0] - x[2]) * (y[1] - y[0]) - (y[0] - y[2]) * (x[1] - x[0]));
+    const double L = sqrt(fabs(area));
+    const double xc = (x[0] + x[1] + x[2]) / 3.0, yc = (y[0] + y[1] + y[2]) / 3.0;
+    float* o = X + (size_t)t * 11;
+    for (int k = 0; k < 3; ++k) {
+        o[2 * k] = (float)((x[k] - xc) / L);
+        o[2 * k + 1] = (float)((y[k] - yc) / L);
+    }
+    for (int n = 0; n < NV; ++n) o[6 + n] = (float)((ugv[v0 * NV + n] + ugv[v1 * NV + n] + ugv[v2 * NV + n]) / 3.0);
+    o[9] = (float)log(1e-12 + mu / (rho * L));
+    o[10] = (float)log(fabs(area) / fmax(tprcn[3 * t], 1e-300));
+}
+
+// tprcn *= exp(clamp(y, -2, 2)) per element and variable
+__global__ void k_mlp_apply(int ntri, const float* Y, double* tprcn) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ntri * NV) return;
+    const float v = fminf(2.f, fmaxf(-2.f, Y[k]));
+    tprcn[k] *= exp((double)v);
+}
 
 }  // namespace hpg

@@ -168,6 +168,10 @@ class TriHpIns:
         if ninner is not None:
             self._check(self.L.hpg_set_defect_inner(self.h, int(ninner)), "set_defect_inner")
 
+    def set_preconditioner_mode(self, mode: int):
+        """HPG_PRCN_ANALYTIC or HPG_PRCN_LEARNED (needs an MLP attached, hpmp.learned)."""
+        self._check(self.L.hpg_set_preconditioner(self.h, mode), "set_preconditioner")
+
     def apply_jacobian(self, dv, ds, di):
         """J(u) delta at the current state (FP32 linearised kernels); leaves R(u) in the residual."""
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (dv, ds, di)]

@@ -110,6 +110,28 @@ int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
 int hpg_apply_jacobian(hpg_ctx *ctx, const double *dv, const double *ds, const double *di,
                        double *outv, double *outs, double *outi);
 
+/* Learned preconditioner (no reference counterpart).  An MLP 11 -> 64 -> 64 -> 64 -> 3
+ * (GELU, bfloat16 weights, FP32 accumulation) maps element features (vertex offsets,
+ * mean u, v, p, log viscous scale, log analytic pseudo-time step) to log scalings of
+ * the analytic tprcn of tri_hp_ins::setup_preconditioner
+ * (tri_hp/ins/setup_preconditioner.cpp:202-204); with HPG_PRCN_LEARNED and an MLP
+ * attached, hpg_setup_preconditioner applies tprcn *= exp(clamp(y, -2, 2)) before the
+ * vertex/side assembly.  Training data come from the method itself: the features
+ * and the measured element-Jacobian diagonal (sum over modes of |J_e(mn, mn)| per
+ * variable n).  The MLP must not be trained while a context that uses it runs. */
+typedef struct hpg_mlp hpg_mlp;
+int hpg_mlp_create(unsigned seed, hpg_mlp **out);
+int hpg_mlp_destroy(hpg_mlp *m);
+int hpg_mlp_num_params(void);
+int hpg_mlp_get_weights(hpg_mlp *m, float *w);
+int hpg_mlp_set_weights(hpg_mlp *m, const float *w);
+int hpg_mlp_forward(hpg_mlp *m, const float *X, int n, float *Y);
+int hpg_mlp_loss_grad(hpg_mlp *m, const float *X, const float *T, int n, float *loss, float *grad);
+int hpg_mlp_train(hpg_mlp *m, const float *X, const float *T, int n, int epochs, int batch, float lr, float *last_loss);
+int hpg_set_mlp(hpg_ctx *ctx, hpg_mlp *m);
+int hpg_mlp_features(hpg_ctx *ctx, float *out);             /* ntri x 11 */
+int hpg_measure_jacobian_diag(hpg_ctx *ctx, double *out);   /* ntri x 3, finest level */
+
 /* blocks until the context's stream is idle */
 int hpg_synchronize(hpg_ctx *ctx);
 /* Measurement support, no reference counterpart: bracket every element-residual

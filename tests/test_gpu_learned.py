@@ -1,0 +1,118 @@
+"""Learned preconditioner on the GPU: the MLP kernels against the numpy oracle
+(bfloat16 weights and activations, FP32 accumulation), training, and the
+guarded solve against the analytic preconditioner."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(tag):
+    print(f"\n[gpu-case] {tag}", flush=True)
+
+
+def _theta(seed=0, scale=0.25):
+    import mlp as OM
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(OM.n_params()) * scale).astype(np.float32)
+
+
+def test_mlp_forward_matches_oracle():
+    _case("mlp forward vs oracle n=1000")
+    import mlp as OM
+    from hpmp.learned import MLP
+    th = _theta(1)
+    X = np.random.default_rng(2).standard_normal((1000, 11)).astype(np.float32)
+    net = MLP(0)
+    net.set_weights(th)
+    Y = net.forward(X)
+    Yo, _, _ = OM.forward(th, X)
+    # identical roundings; only the FP32 summation order and tanh differ
+    assert np.abs(Y - Yo).max() <= 2e-3 * np.abs(Yo).max()
+
+
+def test_mlp_gradient_matches_oracle():
+    _case("mlp loss/gradient vs oracle n=777")
+    import mlp as OM
+    from hpmp.learned import MLP
+    th = _theta(3)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((777, 11)).astype(np.float32)
+    T = rng.standard_normal((777, 3)).astype(np.float32)
+    net = MLP(0)
+    net.set_weights(th)
+    loss, g = net.loss_grad(X, T)
+    lo, go = OM.loss_and_grad(th, X, T)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    print("grad rel err", rel_err(g, go), flush=True)
+    assert rel_err(g, go) < 2e-3
+
+
+def test_mlp_training_reduces_loss_and_is_reproducible():
+    _case("mlp training on a smooth target n=8192")
+    from hpmp.learned import MLP
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (8192, 11)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], X[:, 2] ** 2 - 0.3, np.tanh(X[:, 3] + X[:, 4])], 1).astype(np.float32)
+    losses = []
+    ws = []
+    for _ in range(2):
+        net = MLP(7)
+        l0, _ = net.loss_grad(X, T)
+        last = net.train(X, T, epochs=30, batch=1024, lr=3e-3)
+        losses.append((l0, last))
+        ws.append(net.weights())
+    print("loss before/after", losses[0], flush=True)
+    assert losses[0][1] < 0.1 * losses[0][0]
+    assert np.array_equal(ws[0], ws[1])
+
+
+def _problem(seed, n=6, log2p=1, mu=5e-2):
+    from hpmp import mesh as M, workloads as W
+    m = M.unit_square(n, 0.2, 15)
+    return m, W.noisy_freestream(m, 1 << log2p, seed=seed, scale=0.3)
+
+
+def test_learned_preconditioner_converges_to_the_same_solution_faster():
+    _case("learned preconditioner vs analytic n=6 log2p=1 mu=5e-2")
+    from hpmp.learned import MLP, GuardedSolver, calibrate, training_data
+    from hpmp.tri_hp import TriHpIns
+    m, st_test = _problem(16)
+    G = TriHpIns(m, log2p=1, mu=5e-2)
+    train_states = [_problem(s)[1] for s in (101, 102, 103, 104)]
+    X, T, off = training_data(G, train_states)
+    net = MLP(11)
+    net.train(X, T, epochs=200, batch=256, lr=2e-3)
+    c, rates = calibrate(G, net, train_states[0])
+    print("calibrated scale", c, "rates", rates, flush=True)
+
+    G.set_state(*st_test)
+    it_a, hist_a = GuardedSolver(G, None).solve(tol_rel=1e-10, max_cycles=6000)
+    sol_a = G.get_state()
+    G.set_state(*st_test)
+    gs = GuardedSolver(G, net)
+    it_l, hist_l = gs.solve(tol_rel=1e-10, max_cycles=6000)
+    print("cycles analytic", it_a, "learned", it_l, "fell back at", gs.fell_back_at, flush=True)
+    assert hist_a[-1][1] <= 1e-10 * hist_a[0][1] and hist_l[-1][1] <= 1e-10 * hist_l[0][1]
+    assert rel_err(G.get_state(), sol_a) < 1e-8
+    assert gs.fell_back_at is None and it_l < it_a
+
+
+def test_guard_falls_back_from_an_unstable_network():
+    _case("learned preconditioner fallback n=6 log2p=1")
+    from hpmp.learned import MLP, GuardedSolver, shift_output
+    from hpmp.tri_hp import TriHpIns
+    m, st = _problem(17)
+    G = TriHpIns(m, log2p=1, mu=5e-2)
+    net = MLP(3)
+    w = net.weights()
+    w[:] = 0
+    net.set_weights(w)
+    shift_output(net, [-2.0, -2.0, -2.0])     # pseudo-time steps 7.4x the analytic ones
+    G.set_state(*st)
+    gs = GuardedSolver(G, net, check_every=10)
+    it, hist = gs.solve(tol_rel=1e-6, max_cycles=6000)
+    print("fell back at", gs.fell_back_at, "cycles", it, flush=True)
+    assert gs.fell_back_at is not None and hist[-1][1] <= 1e-6 * hist[0][1]
