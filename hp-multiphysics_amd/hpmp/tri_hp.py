@@ -161,6 +161,11 @@ class TriHpIns:
         self._check(self.L.hpg_get_log2p(self.h, C.byref(v)), "get_log2p")
         return v.value
 
+    def to_finest(self):
+        """Return to the finest p-level (after a cycle was abandoned on a coarse one)."""
+        while self.log2p < self.log2pmax:
+            self.mg_prolongate()
+
     def set_precision(self, mode: int, ninner: int | None = None):
         """HPG_FP64 (plain) or HPG_FP32_DEFECT (FP64 defect correction around FP32
         inner multistage iterations on the linearised residual)."""

@@ -34,6 +34,9 @@ class ScriptedSolver:
     def rsdl(self):
         pass
 
+    def to_finest(self):
+        pass
+
     def resnorm(self):
         return self.x
 
