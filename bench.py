@@ -7,6 +7,15 @@ One step = setup_preconditioner + update() (5-stage Jacobi-preconditioned
 multistage: 5 residual evaluations, 5 minvrt sweeps, 5 stage updates), all
 device-resident.  value = unknowns x 5 residual evaluations x steps / time.
 
+Also reported (same JSON line):
+* ``mixed``: the same steps with the FP32 linearised residual inside FP64 defect
+  correction (one FP64 residual + ninner FP32 multistage iterations per update),
+  and the FP64 judge after the same number of steps for both precisions;
+* ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual with the
+  p-multigrid cycle (p = 4 -> 2 -> 1) on a smaller mesh of the same family, for
+  FP64 plain, mixed, and mixed + learned preconditioner (training time apart),
+  with each converged solution's difference from the FP64 plain one.
+
 Prints one JSON line (see the driver contract).  The CPU baseline is the oracle
 (single thread) on a bounded 128 x 128 sample of the same workload.
 """
@@ -32,6 +41,60 @@ FLOP_FULL_P4 = 19250.0
 FLOP_INV_P4 = 7250.0
 
 
+def time_to_tol(args) -> dict:
+    """Cycles and wall time to a 1e-10 relative FP64 residual: plain, mixed, mixed + learned."""
+    from hpmp import _lib, mesh as M, workloads as W
+    from hpmp.learned import MLP, GuardedSolver, calibrate, training_data
+    from hpmp.tri_hp import TriHpIns
+
+    n, mu, mg = args.tts_n, args.tts_mu, 3
+    mesh = M.unit_square(n, 0.2, 1)
+    st = W.fourier_state(mesh, 4, seed=2)
+    g = TriHpIns(mesh, log2p=2, mu=mu)
+    res = {"workload": f"tri_hp ins {n}x{n} squares ({mesh.ntri} triangles, jitter 0.2h), p=4, mu={mu}, "
+                       "Fourier state; p-multigrid cycle 4->2->1, judge every 25 cycles",
+           "tol_rel": 1e-10}
+    ref = None
+
+    def run(tag, precision, net):
+        nonlocal ref
+        g.set_precision(precision, ninner=args.ninner)
+        g.set_state(*st)
+        g.synchronize()
+        t0 = time.perf_counter()
+        gs = GuardedSolver(g, net)
+        cyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles, mglvls=mg)
+        g.synchronize()
+        t1 = time.perf_counter()
+        sol = g.get_state()
+        if ref is None:
+            ref = sol
+        num = max(np.abs(a - b).max() for a, b in zip(sol, ref) if a.size)
+        den = max(np.abs(b).max() for b in ref if b.size)
+        res[tag] = {"cycles": cyc, "seconds": t1 - t0, "final_rel": hist[-1][1] / hist[0][1],
+                    "converged": bool(hist[-1][1] <= 1e-10 * hist[0][1]),
+                    "rel_diff_vs_fp64_plain": num / den, "fell_back_at": gs.fell_back_at}
+
+    run("fp64_plain", _lib.HPG_FP64, None)
+    run("mixed", _lib.HPG_FP32_DEFECT, None)
+    t0 = time.perf_counter()
+    states = [W.fourier_state(mesh, 4, seed=s) for s in (11, 12, 13, 14)]
+    g.set_precision(_lib.HPG_FP64)
+    X, T, _ = training_data(g, states)
+    net = MLP(11)
+    loss = net.train(X, T, epochs=200, batch=256, lr=2e-3)
+    scale, rates = calibrate(g, net, states[0], mglvls=mg)
+    res["learned_training"] = {"seconds": time.perf_counter() - t0, "samples": int(X.shape[0]),
+                               "final_loss": loss, "calibrated_scale": scale}
+    run("mixed_learned", _lib.HPG_FP32_DEFECT, net)
+    run("fp64_learned", _lib.HPG_FP64, net)
+    for k in ("mixed", "mixed_learned", "fp64_learned"):
+        res[k]["speedup_vs_fp64_plain"] = res["fp64_plain"]["seconds"] / res[k]["seconds"]
+    g.close()
+    net.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,10 +104,26 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--ninner", type=int, default=2)
+    ap.add_argument("--no-tts", action="store_true")
+    ap.add_argument("--tts-n", type=int, default=16)
+    ap.add_argument("--tts-mu", type=float, default=5e-2)
+    ap.add_argument("--tts-max-cycles", type=int, default=6000)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
         return 2
+    if args.copy_probe:
+        from hpmp import _lib
+        ms = C.c_double()
+        rc = _lib.load().hpg_selftest_copy(args.copy_probe, 5, C.byref(ms))
+        print(json.dumps({"copy_bytes": args.copy_probe, "rc": rc, "ms_per_copy": ms.value,
+                          "GBps_read_plus_write": 2 * args.copy_probe / (ms.value * 1e-3) / 1e9 if ms.value else None}),
+              flush=True)
+        return 0 if rc == 0 else 1
 
     from hpmp import mesh as M, workloads as W
     from hpmp.tri_hp import TriHpIns
@@ -60,6 +139,7 @@ def main() -> int:
         if g.setup_preconditioner():
             raise SystemExit("setup_preconditioner error")
         g.update()
+    g.set_state(*st)
     g.synchronize()
     g.L.hpg_kernel_timing(g.h, 1)
     t0 = time.perf_counter()
@@ -79,6 +159,35 @@ def main() -> int:
     achieved = FLOP_FULL_P4 * full_elems / (full_ms * 1e-3) / 1e12 if full_ms > 0 else 0.0
     g.rsdl()
     judge = g.resnorm()
+
+    mixed = None
+    if not args.no_mixed:
+        from hpmp import _lib
+        g.set_state(*st)
+        g.set_precision(_lib.HPG_FP32_DEFECT, ninner=args.ninner)
+        for _ in range(args.warmup):
+            g.setup_preconditioner()
+            g.update()
+        g.set_state(*st)
+        g.synchronize()
+        m0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.setup_preconditioner()
+            g.update()
+        g.synchronize()
+        m1 = time.perf_counter()
+        g.rsdl()
+        mixed = {
+            "ninner": args.ninner,
+            "ms_per_step": 1e3 * (m1 - m0) / args.steps,
+            "fp64_residual_evals_per_step": 1,
+            "fp32_residual_evals_per_step": 5 * args.ninner,
+            "dof_evals_per_s": unknowns * (1 + 5 * args.ninner) * args.steps / (m1 - m0),
+            "judge_after_steps": g.resnorm(),
+            "fp64_plain_judge_after_steps": judge,
+        }
+        g.set_precision(_lib.HPG_FP64)
+    g.close()
 
     out = {
         "metric": METRIC,
@@ -115,6 +224,10 @@ def main() -> int:
         },
         "judge_resnorm_after": judge,
     }
+    if mixed is not None:
+        out["mixed"] = mixed
+    if not args.no_tts:
+        out["time_to_tol"] = time_to_tol(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

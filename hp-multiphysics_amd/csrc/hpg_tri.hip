@@ -1003,6 +1003,34 @@ int hpg_selftest_mfma_f32(const float* A, const float* B, float* out) {
     return e == hipSuccess ? 0 : -1;
 }
 
+// Plain device copy of a known size (test support: the reference point for HBM
+// traffic counters).  `reps` launches of k_copy16 over nbytes (multiple of 16).
+int hpg_selftest_copy(long long nbytes, int reps, double* ms_per_copy) {
+    if (nbytes <= 0 || nbytes % 16 || reps <= 0) return -2;
+    void *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, nbytes) != hipSuccess) return -1;
+    if (hipMalloc(&dst, nbytes) != hipSuccess) { hipFree(src); return -1; }
+    hipMemset(src, 1, nbytes);
+    const long long n16 = nbytes / 16;
+    const int blocks = (int)std::min<long long>((n16 + 255) / 256, 1 << 20);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const uint4*)src, (uint4*)dst, n16);
+    hipEventRecord(e1, 0);
+    const hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(src);
+    hipFree(dst);
+    return e == hipSuccess ? 0 : -1;
+}
+
 // MFMA f64 16x16x4 lane-layout probe (test support; A 16x4, B 4x16 row-major, out 64x4)
 int hpg_selftest_mfma(const double* A, const double* B, double* out) {
     double *dA = nullptr, *dB = nullptr, *dO = nullptr;

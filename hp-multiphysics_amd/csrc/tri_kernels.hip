@@ -778,4 +778,11 @@ template __global__ void k_elem_ins<2, false>(ElemArgs, int);
 template __global__ void k_elem_ins<4, true>(ElemArgs, int);
 template __global__ void k_elem_ins<4, false>(ElemArgs, int);
 
+
+// grid-stride 16-byte copy (reference kernel for traffic counters)
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n16) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (long long)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 }  // namespace hpg

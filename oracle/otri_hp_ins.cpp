@@ -9,93 +9,8 @@ namespace oracle {
 
 static const int S = 6;  // MXGP stride of the reference work arrays (tri_hp/tri_hp.h:31-33)
 
-TriHpIns::TriHpIns(const Mesh& mesh, const Params& p) : m(mesh), prm(p) {
-    for (int l = 0; l < 3; ++l) bases.emplace_back(new TriBasis(1 << l));
-    log2p = prm.log2pmax;
-    sm0 = smf = bases[log2p]->sm;
-    im0 = imf = bases[log2p]->im;
-    ugv.assign(m.npnt * NV, 0.0); ugs.assign(m.nseg * sm0 * NV, 0.0); ugi.assign(m.ntri * im0 * NV, 0.0);
-    ug0v = ugv; ug0s = ugs; ug0i = ugi;
-    resv = ugv; ress = ugs; resi = ugi;
-    res_rv = resv; res_rs = ress; res_ri = resi;
-    res0v = resv; res0s = ress; res0i = resi;
-    dresv.assign(3, resv); dress.assign(3, ress); dresi.assign(3, resi);
-    vprcn.assign(m.npnt * NV, 0.0); sprcn.assign(m.nseg * NV, 0.0); tprcn.assign(m.ntri * NV, 0.0);
+TriHpIns::TriHpIns(const Mesh& mesh, const Params& p) : TriHp<3>(mesh, p) {
     tau.assign(m.ntri * NV, 0.0);
-}
-
-// tri_mesh/tprims.cpp:66-80 (twice the signed area)
-double TriHpIns::area(int t) const {
-    const int p0 = m.tri_pnt[3 * t], p1 = m.tri_pnt[3 * t + 1], p2 = m.tri_pnt[3 * t + 2];
-    const double dx1 = m.pnts[2 * p0] - m.pnts[2 * p2], dy1 = m.pnts[2 * p0 + 1] - m.pnts[2 * p2 + 1];
-    const double dx2 = m.pnts[2 * p1] - m.pnts[2 * p0], dy2 = m.pnts[2 * p1 + 1] - m.pnts[2 * p0 + 1];
-    return dx1 * dy2 - dy1 * dx2;
-}
-
-// tri_mesh/tprims.cpp:224-242
-double TriHpIns::inscribedradius(int t) const {
-    const int p0 = m.tri_pnt[3 * t], p1 = m.tri_pnt[3 * t + 1], p2 = m.tri_pnt[3 * t + 2];
-    const double dx1 = m.pnts[2 * p0] - m.pnts[2 * p2], dy1 = m.pnts[2 * p0 + 1] - m.pnts[2 * p2 + 1];
-    const double dx2 = m.pnts[2 * p1] - m.pnts[2 * p0], dy2 = m.pnts[2 * p1 + 1] - m.pnts[2 * p0 + 1];
-    const double ar = dx1 * dy2 - dy1 * dx2;
-    const double perim = std::sqrt(dx1 * dx1 + dy1 * dy1) + std::sqrt(dx2 * dx2 + dy2 * dy2) +
-                         std::sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
-    return ar / perim;
-}
-
-// tri_hp/gtol.cpp:14-55
-void TriHpIns::ugtouht(int tind) {
-    const TriBasis& B = b();
-    for (int i = 0; i < 3; ++i) {
-        const int indx = m.tri_pnt[3 * tind + i];
-        for (int n = 0; n < NV; ++n) uht[n][i] = ugv[indx * NV + n];
-    }
-    int cnt = 3;
-    for (int i = 0; i < 3; ++i) {
-        const int sind = m.tri_seg[3 * tind + i], sign = m.tri_sgn[3 * tind + i];
-        int msgn = 1;
-        for (int mm = 0; mm < B.sm; ++mm) {
-            for (int n = 0; n < NV; ++n) uht[n][cnt] = msgn * ugs[(sind * sm0 + mm) * NV + n];
-            msgn *= sign;
-            ++cnt;
-        }
-    }
-    if (B.im > 0) {
-        int indx = 0;
-        for (int mm = 1; mm < B.sm; ++mm) {
-            for (int k = 0; k < B.sm - mm; ++k) {
-                for (int n = 0; n < NV; ++n) uht[n][cnt] = ugi[(tind * im0 + indx) * NV + n];
-                ++cnt; ++indx;
-            }
-            indx += sm0 - B.sm;
-        }
-    }
-}
-
-// tri_hp/gtol.cpp:364-402 (global residual arrays use the finest-level strides)
-void TriHpIns::lftog(int tind, double* gv, double* gs, double* gi, const double lfl[NV][15]) {
-    const TriBasis& B = b();
-    for (int k = 0; k < 3; ++k) {
-        const int g = m.tri_pnt[3 * tind + k];
-        for (int n = 0; n < NV; ++n) gv[g * NV + n] += lfl[n][k];
-    }
-    if (B.p > 1) {
-        int indx = 3;
-        for (int i = 0; i < 3; ++i) {
-            const int sind = m.tri_seg[3 * tind + i], sgn = m.tri_sgn[3 * tind + i];
-            int msgn = 1;
-            for (int mm = 0; mm < B.sm; ++mm) {
-                for (int n = 0; n < NV; ++n) gs[(sind * smf + mm) * NV + n] += msgn * lfl[n][indx];
-                msgn *= sgn;
-                ++indx;
-            }
-        }
-        indx = B.bm;
-        for (int mm = 0; mm < B.im; ++mm) {
-            for (int n = 0; n < NV; ++n) gi[(tind * imf + mm) * NV + n] += lfl[n][indx];
-            ++indx;
-        }
-    }
 }
 
 // tri_hp/ins/rsdl.cpp:17-82 and the straight branch :259-433 (steady: mvel = dugdt = 0)
@@ -245,80 +160,6 @@ void TriHpIns::edge_rsdl() {
         }
 }
 
-// tri_hp/nstage.cpp:12-110
-void TriHpIns::rsdl(int stage) {
-    const TriBasis& B = b();
-    const int nst = prm.nstage;
-    const double f = (stage < nst) ? 1.0 - prm.beta[stage] : 0.0;
-    for (int i = 0; i < m.npnt * NV; ++i) { resv[i] = 0.0; res_rv[i] = (stage < nst) ? res_rv[i] * f : 0.0; }
-    if (B.sm)
-        for (int s = 0; s < m.nseg; ++s)
-            for (int k = 0; k < B.sm; ++k)
-                for (int n = 0; n < NV; ++n) {
-                    const int ix = (s * smf + k) * NV + n;
-                    ress[ix] = 0.0;
-                    res_rs[ix] = (stage < nst) ? res_rs[ix] * f : 0.0;
-                }
-    if (B.sm && B.im)
-        for (int t = 0; t < m.ntri; ++t)
-            for (int k = 0; k < B.im; ++k)
-                for (int n = 0; n < NV; ++n) {
-                    const int ix = (t * imf + k) * NV + n;
-                    resi[ix] = 0.0;
-                    res_ri[ix] = (stage < nst) ? res_ri[ix] * f : 0.0;
-                }
-    double lf_re[NV][15], lf_im[NV][15];
-    for (int tind = 0; tind < m.ntri; ++tind) {
-        ugtouht(tind);
-        element_rsdl(tind, stage, uht, lf_re, lf_im);
-        lftog(tind, resv.data(), ress.data(), resi.data(), lf_im);
-        lftog(tind, res_rv.data(), res_rs.data(), res_ri.data(), lf_re);
-    }
-    for (int i = 0; i < m.npnt * NV; ++i) resv[i] += res_rv[i];
-    if (B.sm > 0) {
-        for (int s = 0; s < m.nseg; ++s)
-            for (int k = 0; k < B.sm; ++k)
-                for (int n = 0; n < NV; ++n) ress[(s * smf + k) * NV + n] += res_rs[(s * smf + k) * NV + n];
-        if (B.im > 0)
-            for (int t = 0; t < m.ntri; ++t)
-                for (int k = 0; k < B.im; ++k)
-                    for (int n = 0; n < NV; ++n) resi[(t * imf + k) * NV + n] += res_ri[(t * imf + k) * NV + n];
-    }
-    edge_rsdl();
-    if (coarse_flag) mg_source();
-}
-
-// tri_hp/nstage.cpp:113-129
-void TriHpIns::mg_source() {
-    const TriBasis& B = b();
-    std::vector<double>& dv = dresv[log2p];
-    std::vector<double>& ds = dress[log2p];
-    std::vector<double>& di = dresi[log2p];
-    if (isfrst) {
-        for (int i = 0; i < m.npnt * NV; ++i) dv[i] = prm.fadd * res0v[i] - resv[i];
-        for (int s = 0; s < m.nseg; ++s)
-            for (int k = 0; k < B.sm; ++k)
-                for (int n = 0; n < NV; ++n) {
-                    const int ix = (s * smf + k) * NV + n;
-                    ds[ix] = prm.fadd * res0s[ix] - ress[ix];
-                }
-        for (int t = 0; t < m.ntri; ++t)
-            for (int k = 0; k < B.im; ++k)
-                for (int n = 0; n < NV; ++n) {
-                    const int ix = (t * imf + k) * NV + n;
-                    di[ix] = prm.fadd * res0i[ix] - resi[ix];
-                }
-        isfrst = false;
-    }
-    for (int i = 0; i < m.npnt * NV; ++i) resv[i] += dv[i];
-    for (int s = 0; s < m.nseg; ++s)
-        for (int k = 0; k < B.sm; ++k)
-            for (int n = 0; n < NV; ++n) ress[(s * smf + k) * NV + n] += ds[(s * smf + k) * NV + n];
-    for (int t = 0; t < m.ntri; ++t)
-        for (int k = 0; k < B.im; ++k)
-            for (int n = 0; n < NV; ++n) resi[(t * imf + k) * NV + n] += di[(t * imf + k) * NV + n];
-}
-
 // tri_hp/ins/setup_preconditioner.cpp:9-221 (diagonal, REFINED_WAY, straight elements)
 // followed by tri_hp/setup_preconditioner.cpp:9-96.  The reference's diagonal branch
 // declares a second `err` that shadows the returned one (:15), so it breaks out of
@@ -364,289 +205,6 @@ int TriHpIns::setup_preconditioner() {
     }
     assemble_prcn(err ? 0 : m.ntri);
     return err;
-}
-
-// tri_hp/ins/setup_preconditioner.cpp:213-219 then tri_hp/setup_preconditioner.cpp:68-76
-// (the reference's loop stops at the failing element, so only the first ntri_done
-// elements contribute)
-void TriHpIns::assemble_prcn(int ntri_done) {
-    const TriBasis& B = b();
-    std::fill(vprcn.begin(), vprcn.end(), 0.0);
-    if (B.sm > 0) std::fill(sprcn.begin(), sprcn.end(), 0.0);
-    for (int tind = 0; tind < ntri_done; ++tind)
-        for (int i = 0; i < 3; ++i) {
-            const int vv = m.tri_pnt[3 * tind + i];
-            for (int n = 0; n < NV; ++n) vprcn[vv * NV + n] += tprcn[tind * NV + n];
-            if (B.sm > 0) {
-                const int side = m.tri_seg[3 * tind + i];
-                for (int n = 0; n < NV; ++n) sprcn[side * NV + n] += tprcn[tind * NV + n];
-            }
-        }
-    for (int i = 0; i < m.npnt * NV; ++i) vprcn[i] = 1.0 / (B.vdiag * vprcn[i]);
-    if (log2p)
-        for (int i = 0; i < m.nseg * NV; ++i) sprcn[i] = 1.0 / sprcn[i];
-}
-
-// tri_hp/hp_boundary.cpp:680-693
-void TriHpIns::vdirichlet() {
-    const int nebd = int(m.ebdry_start.size()) - 1;
-    for (int e = 0; e < nebd; ++e) {
-        const int mask = m.ebdry_essential[e];
-        if (!mask) continue;
-        int sind = -1;
-        for (int j = m.ebdry_start[e]; j < m.ebdry_start[e + 1]; ++j) {
-            sind = m.ebdry_seg[j];
-            const int v0 = m.seg_pnt[2 * sind];
-            for (int n = 0; n < NV; ++n) if (mask & (1 << n)) resv[v0 * NV + n] = 0.0;
-        }
-        const int v1 = m.seg_pnt[2 * sind + 1];
-        for (int n = 0; n < NV; ++n) if (mask & (1 << n)) resv[v1 * NV + n] = 0.0;
-    }
-}
-
-// tri_hp/hp_boundary.cpp:695-704
-void TriHpIns::sdirichlet(int mode) {
-    const int nebd = int(m.ebdry_start.size()) - 1;
-    for (int e = 0; e < nebd; ++e) {
-        const int mask = m.ebdry_essential[e];
-        for (int j = m.ebdry_start[e]; j < m.ebdry_start[e + 1]; ++j) {
-            const int sind = m.ebdry_seg[j];
-            for (int n = 0; n < NV; ++n) if (mask & (1 << n)) ress[(sind * smf + mode) * NV + n] = 0.0;
-        }
-    }
-}
-
-// tri_hp/setup_preconditioner.cpp:102-373 (diagonal preconditioner branch)
-void TriHpIns::minvrt() {
-    const TriBasis& B = b();
-    const int sm = B.sm, im = B.im;
-    if (sm > 0) {
-        for (int sind = 0; sind < m.nseg; ++sind)
-            for (int k = 0; k < sm; ++k)
-                for (int i = 0; i < 2; ++i) {
-                    const int v0 = m.seg_pnt[2 * sind + i];
-                    for (int n = 0; n < NV; ++n)
-                        resv[v0 * NV + n] -= B.sfmv[i * sm + k] * ress[(sind * smf + k) * NV + n];
-                }
-        if (im > 0)
-            for (int tind = 0; tind < m.ntri; ++tind) {
-                int indx2 = 3;
-                for (int i = 0; i < 3; ++i) {
-                    const int v0 = m.tri_pnt[3 * tind + i];
-                    for (int k = 0; k < im; ++k)
-                        for (int n = 0; n < NV; ++n)
-                            resv[v0 * NV + n] -= B.ifmb[i * im + k] * resi[(tind * imf + k) * NV + n];
-                    const int sind = m.tri_seg[3 * tind + i], sgn = m.tri_sgn[3 * tind + i];
-                    int msgn = 1;
-                    for (int j = 0; j < sm; ++j) {
-                        for (int k = 0; k < im; ++k)
-                            for (int n = 0; n < NV; ++n)
-                                ress[(sind * smf + j) * NV + n] -= msgn * B.ifmb[indx2 * im + k] * resi[(tind * imf + k) * NV + n];
-                        msgn *= sgn;
-                        ++indx2;
-                    }
-                }
-            }
-    }
-    for (int i = 0; i < m.npnt * NV; ++i) resv[i] *= vprcn[i];
-    vdirichlet();
-    if (sm == 0) return;
-
-    for (int tind = 0; tind < m.ntri; ++tind) {
-        double uh[NV][3];
-        for (int i = 0; i < 3; ++i) {
-            const int v0 = m.tri_pnt[3 * tind + i];
-            for (int n = 0; n < NV; ++n) uh[n][i] = resv[v0 * NV + n] * tprcn[tind * NV + n];
-        }
-        for (int i = 0; i < 3; ++i) {
-            const int sind = m.tri_seg[3 * tind + i], sgn = m.tri_sgn[3 * tind + i];
-            for (int j = 0; j < 3; ++j) {
-                const int indx1 = (i + j) % 3;
-                int msgn = 1;
-                for (int k = 0; k < sm; ++k) {
-                    for (int n = 0; n < NV; ++n)
-                        ress[(sind * smf + k) * NV + n] -= msgn * B.vfms[j * sm + k] * uh[n][indx1];
-                    msgn *= sgn;
-                }
-            }
-        }
-    }
-
-    for (int mode = 0; mode < sm - 1; ++mode) {
-        for (int s = 0; s < m.nseg; ++s)
-            for (int n = 0; n < NV; ++n) ress[(s * smf + mode) * NV + n] *= sprcn[s * NV + n] * B.sdiag[mode];
-        sdirichlet(mode);
-        for (int tind = 0; tind < m.ntri; ++tind) {
-            int side[3], sign[3];
-            double uh[NV][3];
-            for (int i = 0; i < 3; ++i) {
-                side[i] = m.tri_seg[3 * tind + i];
-                sign[i] = m.tri_sgn[3 * tind + i];
-                const int sgn = (mode % 2 ? sign[i] : 1);
-                for (int n = 0; n < NV; ++n) uh[n][i] = sgn * ress[(side[i] * smf + mode) * NV + n] * tprcn[tind * NV + n];
-            }
-            for (int i = 0; i < 3; ++i) {
-                int msgn = ((mode + 1) % 2 ? sign[i] : 1);
-                for (int mm = mode + 1; mm < sm; ++mm) {
-                    for (int j = 0; j < 3; ++j) {
-                        const int indx = (i + j) % 3;
-                        for (int n = 0; n < NV; ++n)
-                            ress[(side[i] * smf + mm) * NV + n] -= msgn * B.sfms[(mode * sm + mm) * 3 + j] * uh[n][indx];
-                    }
-                    msgn *= sign[i];
-                }
-            }
-        }
-    }
-    const int mode = sm - 1;
-    for (int s = 0; s < m.nseg; ++s)
-        for (int n = 0; n < NV; ++n) ress[(s * smf + mode) * NV + n] *= sprcn[s * NV + n] * B.sdiag[mode];
-    sdirichlet(mode);
-    if (im == 0) return;
-
-    for (int tind = 0; tind < m.ntri; ++tind) {
-        dpbtrsnu2_literal(B.idiag.data(), B.ibwth + 1, im, B.ibwth, &resi[tind * imf * NV], NV);
-        // restouht_bdry (tri_hp/setup_preconditioner.cpp:375-399)
-        double uh[NV][15];
-        for (int i = 0; i < 3; ++i) {
-            const int indx = m.tri_pnt[3 * tind + i];
-            for (int n = 0; n < NV; ++n) uh[n][i] = resv[indx * NV + n];
-        }
-        int cnt = 3;
-        for (int i = 0; i < 3; ++i) {
-            const int indx = m.tri_seg[3 * tind + i], sign = m.tri_sgn[3 * tind + i];
-            int msgn = 1;
-            for (int mm = 0; mm < sm; ++mm) {
-                for (int n = 0; n < NV; ++n) uh[n][cnt] = msgn * ress[(indx * smf + mm) * NV + n];
-                msgn *= sign;
-                ++cnt;
-            }
-        }
-        for (int k = 0; k < im; ++k) {
-            for (int n = 0; n < NV; ++n) resi[(tind * imf + k) * NV + n] /= tprcn[tind * NV + n];
-            for (int i = 0; i < B.bm; ++i)
-                for (int n = 0; n < NV; ++n) resi[(tind * imf + k) * NV + n] -= B.bfmi[i * im + k] * uh[n][i];
-        }
-    }
-}
-
-// tri_hp/nstage.cpp:256-427
-void TriHpIns::update() {
-    ug0v = ugv; ug0s = ugs; ug0i = ugi;
-    const TriBasis& B = b();
-    for (int stage = 0; stage < prm.nstage; ++stage) {
-        rsdl(stage);
-        minvrt();
-        const double cflalpha = prm.alpha[stage] * prm.cfl[log2p];
-        for (int i = 0; i < m.npnt * NV; ++i) ugv[i] = ug0v[i] - cflalpha * resv[i];
-        if (B.sm > 0) {
-            for (int s = 0; s < m.nseg; ++s)
-                for (int k = 0; k < B.sm; ++k)
-                    for (int n = 0; n < NV; ++n)
-                        ugs[(s * sm0 + k) * NV + n] = ug0s[(s * sm0 + k) * NV + n] - cflalpha * ress[(s * smf + k) * NV + n];
-            if (B.im > 0)
-                for (int t = 0; t < m.ntri; ++t) {
-                    int indx = 0, indx1 = 0;
-                    for (int mm = 1; mm < B.sm; ++mm) {
-                        for (int k = 0; k < B.sm - mm; ++k) {
-                            for (int n = 0; n < NV; ++n)
-                                ugi[(t * im0 + indx1) * NV + n] = ug0i[(t * im0 + indx1) * NV + n] - cflalpha * resi[(t * imf + indx) * NV + n];
-                            ++indx; ++indx1;
-                        }
-                        indx1 += sm0 - B.sm;
-                    }
-                }
-        }
-    }
-}
-
-// tri_hp/allocate.cpp:496-525
-double TriHpIns::maxres(double* per_var) const {
-    double mxr[NV] = {0, 0, 0};
-    for (int i = 0; i < m.npnt; ++i)
-        for (int n = 0; n < NV; ++n) mxr[n] = std::max(mxr[n], std::fabs(resv[i * NV + n]));
-    double e = 0.0;
-    for (int n = 0; n < NV; ++n) { e = std::max(e, mxr[n]); if (per_var) per_var[n] = mxr[n]; }
-    return e;
-}
-
-// tri_hp/movco.cpp:4-40 (p-level branch)
-void TriHpIns::mg_restrict() {
-    isfrst = true;
-    coarse_flag = true;
-    --log2p;
-    const TriBasis& B = b();
-    res0v = resv;
-    if (B.p > 1)
-        for (int s = 0; s < m.nseg; ++s)
-            for (int k = 0; k < B.sm; ++k)
-                for (int n = 0; n < NV; ++n) res0s[(s * smf + k) * NV + n] = ress[(s * smf + k) * NV + n];
-    if (B.p > 2)
-        for (int t = 0; t < m.ntri; ++t) {
-            int indx = 0, indx1 = 0;
-            for (int mm = 1; mm < B.sm; ++mm) {
-                for (int k = 0; k < B.sm - mm; ++k) {
-                    for (int n = 0; n < NV; ++n) res0i[(t * imf + indx) * NV + n] = resi[(t * imf + indx1) * NV + n];
-                    ++indx; ++indx1;
-                }
-                indx1 += B.p;
-            }
-        }
-}
-
-// tri_hp/movfin.cpp:7-14 (p-level branch)
-void TriHpIns::mg_prolongate() {
-    ++log2p;
-    if (log2p == prm.log2pmax) coarse_flag = false;
-}
-
-// tri_hp/nstage.cpp:131-165 (fixed mesh); K is (tm*NV) x (tm*NV), row-major
-void TriHpIns::element_jacobian(int tind, double eps_r, double eps_a, double* K) {
-    const TriBasis& B = b();
-    const int T = B.tm, N = T * NV;
-    ugtouht(tind);
-    double dw[NV] = {0, 0, 0};
-    for (int i = 0; i < 3; ++i)
-        for (int n = 0; n < NV; ++n) dw[n] = dw[n] + std::fabs(uht[n][i]);
-    for (int n = 0; n < NV; ++n) dw[n] = dw[n] * eps_r + eps_a;
-    double lre[NV][15], lim[NV][15], Rbar[NV][15];
-    element_rsdl(tind, 0, uht, lre, lim);
-    for (int i = 0; i < T; ++i)
-        for (int n = 0; n < NV; ++n) Rbar[n][i] = lre[n][i] + lim[n][i];
-    int kcol = 0;
-    for (int mode = 0; mode < T; ++mode)
-        for (int var = 0; var < NV; ++var) {
-            uht[var][mode] += dw[var];
-            element_rsdl(tind, 0, uht, lre, lim);
-            int krow = 0;
-            for (int i = 0; i < T; ++i)
-                for (int n = 0; n < NV; ++n) K[(krow++) * N + kcol] = (lre[n][i] + lim[n][i] - Rbar[n][i]) / dw[var];
-            ++kcol;
-            uht[var][mode] -= dw[var];
-        }
-}
-
-// tri_mesh/blocks.cpp:1267-1348 with ngrid = 1 (p-levels only)
-double TriHpIns::cycle(int vw, int lvl, int mglvls, int itercrsn, int iterrfne) {
-    double error = 0.0;
-    for (int vcount = 0; vcount < vw; ++vcount) {
-        if (setup_preconditioner()) throw std::runtime_error("setup_preconditioner error");
-        for (int it = 0; it < itercrsn; ++it) update();
-        if (lvl + vcount == 0) {
-            error = maxres();
-            if (!(error > 0.0)) throw std::runtime_error("maxres not positive");
-        }
-        if (lvl == mglvls - 1) return error;
-        rsdl(prm.nstage);
-        mg_restrict();
-        cycle(vw, lvl + 1, mglvls, itercrsn, iterrfne);
-        mg_prolongate();
-        if (iterrfne) {
-            if (setup_preconditioner()) throw std::runtime_error("setup_preconditioner error");
-        }
-        for (int it = 0; it < iterrfne; ++it) update();
-    }
-    return error;
 }
 
 }  // namespace oracle
