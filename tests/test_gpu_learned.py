@@ -96,7 +96,10 @@ def test_learned_preconditioner_converges_to_the_same_solution_faster():
     it_l, hist_l = gs.solve(tol_rel=1e-10, max_cycles=6000)
     print("cycles analytic", it_a, "learned", it_l, "fell back at", gs.fell_back_at, flush=True)
     assert hist_a[-1][1] <= 1e-10 * hist_a[0][1] and hist_l[-1][1] <= 1e-10 * hist_l[0][1]
-    assert rel_err(G.get_state(), sol_a) < 1e-8
+    from hpmp.tri_hp import rel_diff_mod_pressure
+    err = rel_diff_mod_pressure(G.get_state(), sol_a)
+    print("solution rel diff (mod pressure constant)", err, flush=True)
+    assert err < 1e-8
     assert gs.fell_back_at is None and it_l < it_a
 
 

@@ -94,4 +94,9 @@ def test_mixed_precision_reaches_fp64_solution(oracle_mod):
             rv = rv.copy(); rs = rs.copy(); rv[bv, :2] = 0; rs[bs, :, :2] = 0
             if max(np.abs(rv).max(), np.abs(rs).max()) < 1e-12 * r0:
                 break
-    assert rel_err(G.get_state(), O.get_state()) < 1e-10
+    # a constant pressure is a null mode (velocity prescribed on the whole boundary):
+    # different iterations converge to solutions that differ only by it
+    from hpmp.tri_hp import rel_diff_mod_pressure
+    err = rel_diff_mod_pressure(G.get_state(), O.get_state())
+    print("solution rel diff (mod pressure constant)", err, flush=True)
+    assert err < 1e-10

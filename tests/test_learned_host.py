@@ -49,7 +49,7 @@ class ScriptedSolver:
 
 def _guard(s):
     g = GuardedSolver.__new__(GuardedSolver)
-    g.s, g.mlp, g.check_every, g.grow_tol, g.learned, g.fell_back_at = s, object(), 10, 1.0, True, None
+    g.s, g.mlp, g.check_every, g.grow_tol, g.window, g.learned, g.fell_back_at = s, object(), 10, 10.0, 4, True, None
     s.set_preconditioner_mode(_lib.HPG_PRCN_LEARNED)
     return g
 
@@ -61,14 +61,39 @@ def test_contracting_learned_run_never_falls_back():
     assert g.fell_back_at is None and hist[-1][1] <= 1e-10 and it == 40
 
 
-def test_growth_restores_last_good_state_and_continues_analytically():
+def test_growth_restores_best_state_and_continues_analytically():
     s = ScriptedSolver(1.5)
     g = _guard(s)
     it, hist = g.solve(tol_rel=1e-6, max_cycles=1000)
+    # 1.5^10 = 57.7 > 10 x best (1.0) at the first check
     assert g.fell_back_at == 10
     assert s.modes[-1] == _lib.HPG_PRCN_ANALYTIC
     # restored to the initial state (1.0), then 0.9^k
     assert abs(hist[-1][1] - 0.9 ** (it - 10)) < 1e-12 and hist[-1][1] <= 1e-6
+
+
+def test_stagnation_over_the_window_falls_back():
+    s = ScriptedSolver(1.0)
+    g = _guard(s)
+    it, hist = g.solve(tol_rel=1e-6, max_cycles=1000)
+    # no contraction: the 4th check (cycle 40) is not below the value 4 checks earlier
+    assert g.fell_back_at == 40 and hist[-1][1] <= 1e-6
+
+
+class BumpySolver(ScriptedSolver):
+    """Contracts by 0.5 per cycle but jumps up x8 once at cycle 13."""
+
+    def cycle(self, **kw):
+        super().cycle(**kw)
+        if self.ncycle == 13:
+            self.x *= 2.0 ** 13
+
+
+def test_transient_rise_within_tolerance_does_not_fall_back():
+    s = BumpySolver(0.5)
+    g = _guard(s)
+    it, hist = g.solve(tol_rel=1e-10, max_cycles=1000)
+    assert g.fell_back_at is None and hist[-1][1] <= 1e-10
 
 
 def test_exception_in_learned_cycle_falls_back():
@@ -76,5 +101,5 @@ def test_exception_in_learned_cycle_falls_back():
     g = _guard(s)
     it, hist = g.solve(tol_rel=1e-8, max_cycles=2000)
     assert g.fell_back_at == 15 and hist[-1][1] <= 1e-8
-    # the last good check was at cycle 10: state 0.8^10
+    # the best check was at cycle 10: state 0.8^10
     assert abs(hist[-1][1] - 0.8 ** 10 * 0.9 ** (it - 15)) < 1e-12

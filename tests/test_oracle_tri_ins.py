@@ -92,6 +92,34 @@ def test_freestream_residual_vanishes(oracle_mod, log2p):
 
 
 @pytest.mark.parametrize("log2p", [0, 1, 2])
+def test_constant_pressure_is_a_null_mode(oracle_mod, log2p):
+    """With u, v prescribed on every boundary edge, adding a constant to the pressure
+    changes no residual row that survives the Dirichlet conditions."""
+    from hpmp import workloads as W
+    m = _mesh(6, 0.2, 5)
+    T = oracle_mod.TriHpIns(m, log2p=log2p, mu=1e-2)
+    st = W.noisy_freestream(m, 1 << log2p, seed=3)
+    T.set_state(*st)
+    T.rsdl()
+    r1 = [a.copy() for a in T.get_res()]
+    v = st[0].copy()
+    v[:, 2] += 0.37
+    T.set_state(v, st[1], st[2])
+    T.rsdl()
+    r2 = T.get_res()
+    bv, bs = m.boundary_vertex_mask(), m.boundary_segment_mask()
+    for k, (a, b) in enumerate(zip(r1, r2)):
+        if not a.size:
+            continue
+        d = np.abs(a - b)
+        if k == 0:
+            d[bv, :2] = 0
+        elif k == 1:
+            d[bs, :, :2] = 0
+        assert d.max() < 1e-13 * max(1.0, np.abs(a).max())
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
 def test_continuity_conservation(oracle_mod, log2p):
     """sum over vertices of the continuity residual = boundary integral of rho u.n,
     computed independently along every boundary edge with a 12-point rule."""
