@@ -77,3 +77,53 @@ def fourier_state(mesh: TriMesh, p: int, seed: int = 0, nmodes: int = 8, amp: fl
         i = i + noise * (int_k >= 3)[None, :, None] * rng.standard_normal(i.shape)
     _hold_boundary(mesh, p, v, s, vf, sf)
     return v, s, i
+
+
+# ---- tri_hp_cns states (primitive variables p, u, v, RT) ---------------------------
+
+def cns_freestream(mesh: TriMesh, p: int, mach: float = 0.5, gamma: float = 1.4, pr: float = 1.0, rt: float = 1.0):
+    """Uniform (p, u, v, RT) = (pr, mach * sqrt(gamma rt), 0, rt); returns (state, bc)."""
+    sm = p - 1
+    im = (p - 1) * (p - 2) // 2
+    bc = (pr, mach * np.sqrt(gamma * rt), 0.0, rt)
+    v = np.tile(np.array(bc), (mesh.npnt, 1))
+    return (v, np.zeros((mesh.nseg, sm, 4)), np.zeros((mesh.ntri, im, 4))), bc
+
+
+def _hold_cns_boundary(mesh, v, s, bc):
+    bv = mesh.boundary_vertex_mask()
+    bs = mesh.boundary_segment_mask()
+    v[bv, 1:] = np.array(bc)[1:]
+    if s.size:
+        s[bs, :, 1:] = 0.0
+
+
+def cns_gaussian_state(mesh: TriMesh, p: int, seed: int = 0, mach: float = 0.5, amp: float = 1e-2,
+                       width: float = 0.05, nbumps: int = 8, noise: float = 0.0, gamma: float = 1.4):
+    """Mach `mach` free stream plus seeded Gaussian density and temperature bumps
+    (amplitude `amp`, width `width`, centres uniform in [0.15, 0.85]^2) at the
+    vertices; p = rho RT.  Optional N(0, noise^2 2^-2k) on modes of order k >= 2.
+    u, v, RT are held at the free stream on the boundary.  Returns (state, bc)."""
+    rng = np.random.default_rng(seed)
+    (v, s, i), bc = cns_freestream(mesh, p, mach, gamma)
+    x, y = mesh.pnts[:, 0], mesh.pnts[:, 1]
+
+    def bumps():
+        f = np.zeros(mesh.npnt)
+        for _ in range(nbumps):
+            cx, cy = rng.uniform(0.15, 0.85, size=2)
+            f += np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / width ** 2)
+        return f
+
+    rho = 1.0 + amp * bumps()
+    rt = 1.0 + amp * bumps()
+    v[:, 3] = rt
+    v[:, 0] = rho * rt
+    if noise:
+        side_k, int_k = mode_orders(p)
+        if s.size:
+            s = s + noise * (2.0 ** -side_k)[None, :, None] * rng.standard_normal(s.shape)
+        if i.size:
+            i = i + noise * (2.0 ** -int_k)[None, :, None] * rng.standard_normal(i.shape)
+    _hold_cns_boundary(mesh, v, s, bc)
+    return (v, s, i), bc

@@ -68,6 +68,28 @@ def lib():
         L.otri_cycle.restype = C.c_double
         L.otri_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.otri_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
+        L.ocns_create.restype = C.c_void_p
+        L.ocns_create.argtypes = L.otri_create.argtypes
+        L.ocns_destroy.argtypes = [C.c_void_p]
+        for nm in ("ocns_set_state", "ocns_get_state", "ocns_get_res", "ocns_set_res"):
+            getattr(L, nm).argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ocns_get_prcn.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ocns_rsdl.argtypes = [C.c_void_p, C.c_int]
+        L.ocns_setup_preconditioner.argtypes = [C.c_void_p]
+        L.ocns_setup_preconditioner.restype = C.c_int
+        for nm in ("ocns_minvrt", "ocns_update", "ocns_mg_restrict", "ocns_mg_prolongate",
+                   "ocns_modify_boundary_residual", "ocns_project_new_variables"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+        L.ocns_maxres.argtypes = [C.c_void_p, _dp]
+        L.ocns_maxres.restype = C.c_double
+        L.ocns_log2p.argtypes = [C.c_void_p]
+        L.ocns_cycle.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ocns_cycle.restype = C.c_double
+        L.ocns_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.ocns_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
+        L.ocns_calculate_tau.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp, _dp]
+        L.ocns_spectral_radius.argtypes = [_dp]
+        L.ocns_spectral_radius.restype = C.c_double
         _lib = L
     return _lib
 
@@ -274,3 +296,136 @@ class TriHpIns:
         K = np.zeros((tm * 3, tm * 3))
         lib().otri_element_jacobian(self.h, tind, eps_r, eps_a, K)
         return K
+
+
+CNS_CFL_DEFAULT = (2.5, 1.5, 1.0)
+
+
+class TriHpCns:
+    """Oracle tri_hp_cns on a TriMesh; variables (p, u, v, RT); reference array layouts.
+    Boundaries whose essential mask is 0b1110 are inflow boundaries with the constant
+    values `bc` = (p, u, v, RT)."""
+
+    NV = 4
+
+    def __init__(self, mesh, log2p=2, mu=1e-3, adis=1.0, gamma=1.4, prandtl=0.713, Rgas=1.0, fadd=1.0,
+                 bc=(1.0, 0.5, 0.0, 1.0), alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA, cfl=CNS_CFL_DEFAULT):
+        self.mesh = mesh
+        self.log2pmax = log2p
+        self.p = 1 << log2p
+        self.sm0 = self.p - 1
+        self.im0 = (self.p - 1) * (self.p - 2) // 2
+        prm = np.array([mu, adis, gamma, prandtl, Rgas, fadd, *bc], dtype=np.float64)
+        al = np.array(alpha, dtype=np.float64)
+        be = np.array(beta, dtype=np.float64)
+        cf = np.array(list(cfl) + [0.5] * 4, dtype=np.float64)[:4]
+        m = mesh
+        self._keep = [np.ascontiguousarray(x) for x in (
+            m.pnts.astype(np.float64), m.seg_pnt.astype(np.int32), m.tri_pnt.astype(np.int32),
+            m.tri_seg.astype(np.int32), m.tri_sgn.astype(np.int32), m.ebdry_start.astype(np.int32),
+            m.ebdry_seg.astype(np.int32), m.ebdry_essential.astype(np.int32))]
+        k = self._keep
+        self.h = lib().ocns_create(m.npnt, k[0], m.nseg, k[1], m.ntri, k[2], k[3], k[4], m.nebd, k[5], k[6],
+                                   k[7], prm, log2p, len(al), al, be, cf)
+        if not self.h:
+            raise RuntimeError("ocns_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ocns_destroy(self.h)
+            self.h = None
+
+    def shapes(self):
+        m = self.mesh
+        return ((m.npnt, self.NV), (m.nseg, self.sm0, self.NV), (m.ntri, self.im0, self.NV))
+
+    def _alloc(self):
+        return [np.zeros(s) for s in self.shapes()]
+
+    @staticmethod
+    def _ptr(a):
+        return a if a.size else np.zeros(1)
+
+    def set_state(self, v, s, i):
+        lib().ocns_set_state(self.h, np.ascontiguousarray(v, dtype=np.float64),
+                             self._ptr(np.ascontiguousarray(s, dtype=np.float64)),
+                             self._ptr(np.ascontiguousarray(i, dtype=np.float64)))
+
+    def get_state(self):
+        v, s, i = self._alloc()
+        lib().ocns_get_state(self.h, v, self._ptr(s), self._ptr(i))
+        return v, s, i
+
+    def get_res(self):
+        v, s, i = self._alloc()
+        lib().ocns_get_res(self.h, v, self._ptr(s), self._ptr(i))
+        return v, s, i
+
+    def set_res(self, v, s, i):
+        lib().ocns_set_res(self.h, np.ascontiguousarray(v, dtype=np.float64),
+                           self._ptr(np.ascontiguousarray(s, dtype=np.float64)),
+                           self._ptr(np.ascontiguousarray(i, dtype=np.float64)))
+
+    def get_prcn(self):
+        m = self.mesh
+        out = [np.zeros((m.npnt, 4)), np.zeros((m.nseg, 4)), np.zeros((m.ntri, 4)),
+               np.zeros((m.npnt, 4, 4)), np.zeros((m.nseg, 4, 4)), np.zeros((m.ntri, 4, 4))]
+        lib().ocns_get_prcn(self.h, *out)
+        return out
+
+    def rsdl(self, stage=None):
+        lib().ocns_rsdl(self.h, len(NSTAGE_ALPHA) if stage is None else stage)
+
+    def setup_preconditioner(self) -> int:
+        return lib().ocns_setup_preconditioner(self.h)
+
+    def minvrt(self):
+        lib().ocns_minvrt(self.h)
+
+    def modify_boundary_residual(self):
+        lib().ocns_modify_boundary_residual(self.h)
+
+    def project_new_variables(self):
+        lib().ocns_project_new_variables(self.h)
+
+    def update(self):
+        lib().ocns_update(self.h)
+
+    def maxres(self):
+        pv = np.zeros(4)
+        e = lib().ocns_maxres(self.h, pv)
+        return e, pv
+
+    def mg_restrict(self):
+        lib().ocns_mg_restrict(self.h)
+
+    def mg_prolongate(self):
+        lib().ocns_mg_prolongate(self.h)
+
+    @property
+    def log2p(self):
+        return lib().ocns_log2p(self.h)
+
+    def cycle(self, vw=1, mglvls=1, itercrsn=1, iterrfne=0):
+        return lib().ocns_cycle(self.h, vw, mglvls, itercrsn, iterrfne)
+
+    def element_rsdl(self, tind, stage, uht):
+        tm = uht.shape[1]
+        lr = np.zeros((4, tm)); li = np.zeros((4, tm))
+        lib().ocns_element_rsdl(self.h, tind, stage, np.ascontiguousarray(uht, dtype=np.float64), lr, li)
+        return lr, li
+
+    def element_jacobian(self, tind, eps_r=1e-8, eps_a=1e-8):
+        tm = (self.p + 1) * (self.p + 2) // 2
+        K = np.zeros((tm * 4, tm * 4))
+        lib().ocns_element_jacobian(self.h, tind, eps_r, eps_a, K)
+        return K
+
+    def calculate_tau(self, pvu, h):
+        P = np.zeros(16); T = np.zeros(16); ts = np.zeros(1)
+        lib().ocns_calculate_tau(self.h, np.ascontiguousarray(pvu, dtype=np.float64), h, P, T, ts)
+        return P.reshape(4, 4), T.reshape(4, 4), float(ts[0])
+
+
+def spectral_radius4(A):
+    return lib().ocns_spectral_radius(np.ascontiguousarray(A, dtype=np.float64).ravel())

@@ -3,6 +3,7 @@
 // library never links this file.
 #include "otri_basis.h"
 #include "otri_hp_ins.h"
+#include "otri_hp_cns.h"
 #include "olinalg.h"
 #include <cstring>
 #include <map>
@@ -40,7 +41,7 @@ int obasis_get(int p, const char* name, double* out) {
         {"n0", &B.n0}, {"gnwtn", &B.gnwtn}, {"gnwtnn0", &B.gnwtnn0}, {"dgnwtn", &B.dgnwtn},
         {"dltn", &B.dltn}, {"dltn2", &B.dltn2}, {"norm", &B.norm}, {"sfmv", &B.sfmv},
         {"ifmb", &B.ifmb}, {"vfms", &B.vfms}, {"sdiag", &B.sdiag}, {"sfms", &B.sfms},
-        {"bfmi", &B.bfmi}, {"idiag", &B.idiag}, {"a0", &B.a0}, {"b0", &B.b0}};
+        {"bfmi", &B.bfmi}, {"idiag", &B.idiag}, {"sdiag1d", &B.sdiag1d}, {"a0", &B.a0}, {"b0", &B.b0}};
     if (std::string(name) == "vdiag") { if (out) out[0] = B.vdiag; return 1; }
     auto it = tabs.find(name);
     if (it == tabs.end()) return -1;
@@ -157,5 +158,82 @@ void otri_element_rsdl(void* h, int tind, int stage, const double* uh, double* l
     for (int n = 0; n < 3; ++n) for (int k = 0; k < tm; ++k) { lf_re[n * tm + k] = lr[n][k]; lf_im[n * tm + k] = li[n][k]; }
 }
 void otri_element_jacobian(void* h, int tind, double eps_r, double eps_a, double* K) { T(h).element_jacobian(tind, eps_r, eps_a, K); }
+
+// ---- tri_hp_cns --------------------------------------------------------------
+// prm: mu, adis, gamma, prandtl, Rgas, fadd, bc p, bc u, bc v, bc RT
+void* ocns_create(int npnt, const double* pnts, int nseg, const int* seg_pnt, int ntri, const int* tri_pnt,
+                  const int* tri_seg, const int* tri_sgn, int nebd, const int* ebdry_start, const int* ebdry_seg,
+                  const int* ebdry_essential, const double* prm10, int log2p, int nstage, const double* alpha,
+                  const double* beta, const double* cfl) {
+    try {
+        Mesh m;
+        m.npnt = npnt; m.nseg = nseg; m.ntri = ntri;
+        m.pnts.assign(pnts, pnts + 2 * npnt);
+        m.seg_pnt.assign(seg_pnt, seg_pnt + 2 * nseg);
+        m.tri_pnt.assign(tri_pnt, tri_pnt + 3 * ntri);
+        m.tri_seg.assign(tri_seg, tri_seg + 3 * ntri);
+        m.tri_sgn.assign(tri_sgn, tri_sgn + 3 * ntri);
+        m.ebdry_start.assign(ebdry_start, ebdry_start + nebd + 1);
+        m.ebdry_seg.assign(ebdry_seg, ebdry_seg + ebdry_start[nebd]);
+        m.ebdry_essential.assign(ebdry_essential, ebdry_essential + nebd);
+        Params p;
+        p.mu = prm10[0]; p.adis = prm10[1]; p.gamma = prm10[2]; p.prandtl = prm10[3]; p.Rgas = prm10[4];
+        p.fadd = prm10[5];
+        for (int n = 0; n < 4; ++n) p.bc[n] = prm10[6 + n];
+        p.log2pmax = log2p;
+        p.nstage = nstage;
+        for (int s = 0; s < nstage && s < 7; ++s) { p.alpha[s] = alpha[s]; p.beta[s] = beta[s]; }
+        p.alpha[nstage] = 1.0; p.beta[nstage] = 1.0;
+        for (int l = 0; l <= log2p; ++l) p.cfl[l] = cfl[l];
+        return new TriHpCns(m, p);
+    } catch (...) {
+        return nullptr;
+    }
+}
+void ocns_destroy(void* h) { delete static_cast<TriHpCns*>(h); }
+static TriHpCns& Cn(void* h) { return *static_cast<TriHpCns*>(h); }
+static void put(std::vector<double>& dst, const double* src) { if (!dst.empty()) std::memcpy(dst.data(), src, dst.size() * 8); }
+static void take(const std::vector<double>& src, double* dst) { if (!src.empty()) std::memcpy(dst, src.data(), src.size() * 8); }
+void ocns_set_state(void* h, const double* v, const double* s, const double* i) { auto& x = Cn(h); put(x.ugv, v); put(x.ugs, s); put(x.ugi, i); }
+void ocns_get_state(void* h, double* v, double* s, double* i) { auto& x = Cn(h); take(x.ugv, v); take(x.ugs, s); take(x.ugi, i); }
+void ocns_get_res(void* h, double* v, double* s, double* i) { auto& x = Cn(h); take(x.resv, v); take(x.ress, s); take(x.resi, i); }
+void ocns_set_res(void* h, const double* v, const double* s, const double* i) { auto& x = Cn(h); put(x.resv, v); put(x.ress, s); put(x.resi, i); }
+void ocns_get_prcn(void* h, double* vp, double* sp, double* tp, double* vpc, double* spc, double* tpc) {
+    auto& x = Cn(h);
+    take(x.vprcn, vp); take(x.sprcn, sp); take(x.tprcn, tp); take(x.vpc, vpc); take(x.spc, spc); take(x.tpc, tpc);
+}
+void ocns_rsdl(void* h, int stage) { Cn(h).rsdl(stage); }
+int ocns_setup_preconditioner(void* h) { return Cn(h).setup_preconditioner(); }
+void ocns_minvrt(void* h) { Cn(h).minvrt(); }
+void ocns_modify_boundary_residual(void* h) { Cn(h).modify_boundary_residual(); }
+void ocns_project_new_variables(void* h) { Cn(h).project_new_variables(); }
+void ocns_update(void* h) { Cn(h).update(); }
+double ocns_maxres(void* h, double* per_var) { return Cn(h).maxres(per_var); }
+void ocns_mg_restrict(void* h) { Cn(h).mg_restrict(); }
+void ocns_mg_prolongate(void* h) { Cn(h).mg_prolongate(); }
+int ocns_log2p(void* h) { return Cn(h).log2p; }
+double ocns_cycle(void* h, int vw, int mglvls, int itercrsn, int iterrfne) {
+    try { return Cn(h).cycle(vw, 0, mglvls, itercrsn, iterrfne); } catch (...) { return -1.0; }
+}
+void ocns_element_rsdl(void* h, int tind, int stage, const double* uh, double* lf_re, double* lf_im) {
+    auto& x = Cn(h);
+    double u[4][15], lr[4][15], li[4][15];
+    const int tm = x.b().tm;
+    for (int n = 0; n < 4; ++n) for (int k = 0; k < tm; ++k) u[n][k] = uh[n * tm + k];
+    x.element_rsdl(tind, stage, u, lr, li);
+    for (int n = 0; n < 4; ++n) for (int k = 0; k < tm; ++k) { lf_re[n * tm + k] = lr[n][k]; lf_im[n * tm + k] = li[n][k]; }
+}
+void ocns_element_jacobian(void* h, int tind, double eps_r, double eps_a, double* K) { Cn(h).element_jacobian(tind, eps_r, eps_a, K); }
+// calculate_tau at one state: Pinv (16), Tau (16) row-major, timestep
+void ocns_calculate_tau(void* h, const double* pvu, double hh, double* Pinv, double* Tau, double* timestep) {
+    double P[4][4], T4[4][4];
+    Cn(h).calculate_tau(pvu, hh, P, T4, *timestep);
+    for (int a = 0; a < 16; ++a) { Pinv[a] = P[a / 4][a % 4]; Tau[a] = T4[a / 4][a % 4]; }
+}
+double ocns_spectral_radius(const double* A16) {
+    double A[4][4];
+    for (int a = 0; a < 16; ++a) A[a / 4][a % 4] = A16[a];
+    return spectral_radius4(A);
+}
 
 }  // extern "C"

@@ -54,6 +54,27 @@ TriBasis::TriBasis(int p_) : p(p_) {
     pgn.assign(tm, 0.0); dpgn = pgn;
     initialize_values();
     lumpinv();
+    lumpinv1d();
+}
+
+// tri_basis/initialize.cpp:1085-1130: the 1-D mass matrix of the edge modes
+// (columns 1..p+1 of gx, Gauss-Legendre weights) and the banded Cholesky factor
+// of its side-mode block, upper band storage with bandwidth sbwth.
+void TriBasis::lumpinv1d() {
+    const int n1 = p + 1;
+    std::vector<double> mm(n1 * n1, 0.0);
+    for (int a = 1; a < p + 2; ++a)
+        for (int c = 1; c < p + 2; ++c) {
+            double s = 0.0;
+            for (int i = 0; i < gpx; ++i) s += wtx[i] * gx[i * nmodx + a] * gx[i * nmodx + c];
+            mm[(a - 1) * n1 + (c - 1)] = s;
+        }
+    sdiag1d.assign(std::max(sm, 1) * (sbwth + 1), 0.0);
+    if (p > 1) {
+        for (int j = 0; j < sm; ++j)
+            for (int i = std::max(0, j - sbwth); i <= j; ++i) sdiag1d[j * (sbwth + 1) + i - (j - sbwth)] = mm[(i + 2) * n1 + j + 2];
+        band_chol_upper(sdiag1d.data(), sm, sbwth);
+    }
 }
 
 // tri_basis/ptvalues.cpp:84-195
@@ -390,6 +411,11 @@ void TriBasis::intgrtrs(double* rslt, const double* dx, const double* dy, int st
         }
         indx += nmax;
     }
+}
+
+// tri_basis/proj1d.cpp:61-71 (linear edge field from its two end values)
+void TriBasis::proj1d(double u1, double u2, double* f) const {
+    for (int i = 0; i < gpx; ++i) f[i] = u1 * gx[i * nmodx + 1] + u2 * gx[i * nmodx + 2];
 }
 
 // tri_basis/proj1d.cpp:19-59

@@ -3,7 +3,8 @@
 // Follows tri_basis/initialize.cpp:161-455 (tables), :590-958 (lumped-mass
 // factors), tri_basis/ptvalues.cpp:84-195 (point values), tri_basis/proj.cpp:22-402
 // and tri_basis/intgrt.cpp:22-158 (transforms), tri_basis/proj1d.cpp:19-59 and
-// tri_basis/intgrt1d.cpp:17-43 (edge transforms).  The recurrence coefficients
+// tri_basis/intgrt1d.cpp:17-43 (edge transforms), tri_basis/initialize.cpp:1085-1130
+// (1-D side mass factor sdiag1d).  The recurrence coefficients
 // restate ORTHPOL's recur (utilities/myblas.c:257-459) from the closed forms of the
 // monic Jacobi recurrence; Gauss rules come from oracle::gauss_rule.
 //
@@ -16,6 +17,7 @@ namespace oracle {
 struct TriBasis {
     int p, sm, im, tm, bm, nmodx, gpx, gpn;
     static const int ibwth = 1;
+    static const int sbwth = 2;
     int arow;                              // row length of the recurrence tables
     std::vector<double> a0, b0;            // (sm+2) x (gpx+1)
     std::vector<double> gx, dgx;           // gpx x nmodx
@@ -36,6 +38,7 @@ struct TriBasis {
     std::vector<double> sfms;              // (sm-1) x sm x 3
     std::vector<double> bfmi;              // bm x im
     std::vector<double> idiag;             // im x (ibwth+1), factored
+    std::vector<double> sdiag1d;           // sm x (sbwth+1), factored 1-D side mass
     mutable std::vector<double> pgx, dpgx, pgn, dpgn;
 
     explicit TriBasis(int p);
@@ -51,11 +54,13 @@ struct TriBasis {
     void intgrtrs(double* rslt, const double* dx, const double* dy, int stride) const;
     void proj1d(const double* lin, double* f, double* dx) const;
     void proj1d(const double* lin, double* f) const;
+    void proj1d(double u1, double u2, double* f) const;
     void intgrt1d(double* rslt, const double* f) const;
 
   private:
     void initialize_values();
     void lumpinv();
+    void lumpinv1d();
 };
 
 // utilities/myblas.c:257-459 restated (ipoly 1 = Legendre, 6 = Jacobi(al,be)).
