@@ -39,6 +39,10 @@ MI355X_FP64_MFMA_TFLOPS = 78.6
 # of the reference x 2 + pointwise): full residual (beta > 0) and inviscid-only stage
 FLOP_FULL_P4 = 19250.0
 FLOP_INV_P4 = 7250.0
+# HBM bytes per launch of k_elem_ins<4, full> on this workload (7 colours, 74 898
+# elements per launch on average): FETCH_SIZE and WRITE_SIZE counter passes scaled by
+# the same counters over a plain 1 GiB copy; profiles/traffic_ins_p4_512x512_pmc.md
+TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 219.7e6
 
 
 def time_to_tol(args) -> dict:
@@ -215,7 +219,7 @@ def main() -> int:
             "peak": MI355X_FP64_MFMA_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / MI355X_FP64_MFMA_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_FULL_P4_BYTES_PER_LAUNCH if n == 512 else None,
             "avg_launch_ms": full_ms / max(1, full_launches),
             "elements_per_launch": full_elems / max(1, full_launches),
             "flop_per_element": FLOP_FULL_P4,
