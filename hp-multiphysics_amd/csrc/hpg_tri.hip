@@ -199,6 +199,8 @@ static void check(hipError_t e, const char* what) {
 }
 
 static inline unsigned nblk(long n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+// block count rounded up to a multiple of 8 for kernels that use the XCD-aware block map
+static inline unsigned nblk8(long n, int bs = 256) { return ((nblk(n, bs) + 7u) / 8u) * 8u; }
 
 namespace {
 
@@ -385,13 +387,26 @@ void do_lin_rsdl(hpg_ctx* c, int stage, bool with_source) {
 
 template <typename T>
 void launch_minv(hpg_ctx* c, const MinvArgsT<T>& a) {
-    hipLaunchKernelGGL(k_minv_vrtx<T>, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, a);
-    if (a.sm > 0) {
-        hipLaunchKernelGGL(k_minv_side<T>, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a);
-        for (int mode = 0; mode < a.sm - 1; ++mode)
-            hipLaunchKernelGGL(k_minv_sub<T>, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a, mode);
-        hipLaunchKernelGGL(k_minv_sscale<T>, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, a);
-        if (a.im > 0) hipLaunchKernelGGL(k_minv_int<T>, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, a);
+    const dim3 bv(nblk8(c->npnt)), bs(nblk8(c->nseg)), bt(nblk8(c->ntri)), th(256);
+    switch (a.sm + 1) {
+        case 1:
+            hipLaunchKernelGGL((k_minv_vrtx<T, 1>), bv, th, 0, c->stream, a);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_minv_vrtx<T, 2>), bv, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_side<T, 2>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sscale<T, 2>), bs, th, 0, c->stream, a);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_minv_vrtx<T, 4>), bv, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_side<T, 4>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sub<T, 4, 0>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sub<T, 4, 1>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sscale<T, 4>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_int<T, 4>), bt, th, 0, c->stream, a);
+            break;
+        default:
+            throw std::runtime_error("minvrt: unsupported p");
     }
     check(hipGetLastError(), "minvrt launch");
 }
@@ -426,7 +441,11 @@ int do_setup(hpg_ctx* c) {
     s.ugv = c->ugv; s.ugs = c->ugs; s.ugi = c->ugi; s.Pmat = L.Pmat; s.tprcn = c->tprcn; s.err = c->d_err;
     s.ntri = c->ntri; s.p = d.p; s.sm = d.sm; s.im = d.im; s.tm = d.tm; s.G = d.G; s.sm0 = c->sm0; s.im0 = c->im0;
     s.rho = c->prm.rho; s.mu = c->prm.mu; s.adis = c->prm.adis;
-    hipLaunchKernelGGL(k_setup_elem, dim3(nblk(c->ntri, 128)), dim3(128), 0, c->stream, s);
+    switch (d.p) {
+        case 1: hipLaunchKernelGGL(k_setup_elem<1>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
+        case 2: hipLaunchKernelGGL(k_setup_elem<2>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
+        default: hipLaunchKernelGGL(k_setup_elem<4>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
+    }
     if (c->prcn_mode == HPG_PRCN_LEARNED && c->mlp) {
         // learned scalings of the analytic tprcn (element features -> MLP -> exp)
         hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts, c->tri_pnt,

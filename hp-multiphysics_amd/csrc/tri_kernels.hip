@@ -386,6 +386,14 @@ __global__ void k_edge_ins(EdgeArgs a) {
 // ---------------------------------------------------------------------------------
 // setup_preconditioner, element part (tri_hp/ins/setup_preconditioner.cpp:33-219,
 // diagonal branch, REFINED_WAY, straight elements, steady)
+// XCD-aware block index: launches use a multiple of 8 blocks; consecutive hardware
+// blocks go round-robin to the 8 XCDs, so XCD x is given the contiguous logical
+// range [x * per, (x + 1) * per) and neighbouring entities share one L2
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
 struct SetupArgs {
     const double* pnts;
     const int* tri_pnt;
@@ -401,32 +409,37 @@ struct SetupArgs {
     double rho, mu, adis;
 };
 
-__global__ void k_setup_elem(SetupArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+template <int P>
+__global__ __launch_bounds__(128) void k_setup_elem(SetupArgs a) {
+    constexpr int TM = Dims<P>::TM, SM = Dims<P>::SM, G = Dims<P>::G;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= a.ntri) return;
     int vtx[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) vtx[k] = a.tri_pnt[3 * t + k];
-    double uh[2][15];
-    for (int k = 0; k < a.tm; ++k) {
+    double uh[2][TM];
+#pragma unroll
+    for (int k = 0; k < TM; ++k) {
         int off;
         double sg = 1.0;
         const double* src;
         if (k < 3) { off = vtx[k] * NV; src = a.ugv; }
-        else if (k < 3 * a.p) {
-            const int i = (k - 3) / a.sm, m = (k - 3) - i * a.sm;
+        else if (k < 3 * P) {
+            const int i = (k - 3) / SM, m = (k - 3) - i * SM;
             const int sgn = a.tri_sgn[3 * t + i];
             sg = ((m & 1) && sgn < 0) ? -1.0 : 1.0;
             off = (a.tri_seg[3 * t + i] * a.sm0 + m) * NV;
             src = a.ugs;
-        } else { off = (t * a.im0 + (k - 3 * a.p)) * NV; src = a.ugi; }
+        } else { off = (t * a.im0 + (k - 3 * P)) * NV; src = a.ugi; }
         uh[0][k] = sg * src[off];
         uh[1][k] = sg * src[off + 1];
     }
     double qmax = 0.0;
-    for (int g = 0; g < a.G; ++g) {
+    for (int g = 0; g < G; ++g) {
         double u = 0.0, v = 0.0;
-        for (int k = 0; k < a.tm; ++k) {
-            const double pv = a.Pmat[g * a.tm + k];
+#pragma unroll
+        for (int k = 0; k < TM; ++k) {
+            const double pv = a.Pmat[g * TM + k];
             u += pv * uh[0][k];
             v += pv * uh[1][k];
         }
@@ -526,32 +539,40 @@ struct MinvArgsT {
     T cfa;
 };
 
+
 // vertices: subtract side and interior couplings (:110-135), scale (:151-152),
 // vertex Dirichlet (:179-183)
-template <typename T>
-__global__ void k_minv_vrtx(MinvArgsT<T> a) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
+    constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM;
+    const int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= a.npnt) return;
     T r[NV];
+#pragma unroll
     for (int n = 0; n < NV; ++n) r[n] = a.resv[v * NV + n];
-    if (a.sm > 0) {
+    if constexpr (SM > 0) {
         for (int k = a.vs_ptr[v]; k < a.vs_ptr[v + 1]; ++k) {
             const int s = a.vs_side[k], i = a.vs_end[k];
-            for (int m = 0; m < a.sm; ++m) {
-                const T c = a.lump[a.o_sfmv + i * a.sm + m];
+#pragma unroll
+            for (int m = 0; m < SM; ++m) {
+                const T c = a.lump[a.o_sfmv + i * SM + m];
+#pragma unroll
                 for (int n = 0; n < NV; ++n) r[n] -= c * a.ress[(s * a.sm0 + m) * NV + n];
             }
         }
-        if (a.im > 0)
+        if constexpr (IM > 0)
             for (int k = a.vt_ptr[v]; k < a.vt_ptr[v + 1]; ++k) {
                 const int t = a.vt_tri[k], i = a.vt_loc[k];
-                for (int m = 0; m < a.im; ++m) {
-                    const T c = a.lump[a.o_ifmb + i * a.im + m];
+#pragma unroll
+                for (int m = 0; m < IM; ++m) {
+                    const T c = a.lump[a.o_ifmb + i * IM + m];
+#pragma unroll
                     for (int n = 0; n < NV; ++n) r[n] -= c * a.resi[(t * a.im0 + m) * NV + n];
                 }
             }
     }
     const unsigned msk = a.vess[v];
+#pragma unroll
     for (int n = 0; n < NV; ++n) {
         const T x = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NV + n];
         a.resv[v * NV + n] = x;
@@ -560,24 +581,35 @@ __global__ void k_minv_vrtx(MinvArgsT<T> a) {
 }
 
 // sides: subtract interior couplings (:137-148), then the vertex solution (:192-214)
-template <typename T>
-__global__ void k_minv_side(MinvArgsT<T> a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
+    constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM;
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    T r[5][NV];
-    for (int m = 0; m < a.sm; ++m)
+    T r[SM][NV];
+#pragma unroll
+    for (int m = 0; m < SM; ++m)
+#pragma unroll
         for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
-    if (a.im > 0)
+    if constexpr (IM > 0)
         for (int k = 0; k < 2; ++k) {
             const int t = a.st_tri[2 * s + k];
             if (t < 0) continue;
             const int i = a.st_loc[2 * s + k];
             const int sgn = a.tri_sgn[3 * t + i];
+            T bi[IM][NV];
+#pragma unroll
+            for (int m = 0; m < IM; ++m)
+#pragma unroll
+                for (int n = 0; n < NV; ++n) bi[m][n] = a.resi[(t * a.im0 + m) * NV + n];
             T ms = T(1);
-            for (int j = 0; j < a.sm; ++j) {
-                for (int m = 0; m < a.im; ++m) {
-                    const T c = ms * a.lump[a.o_ifmb + (3 + i * a.sm + j) * a.im + m];
-                    for (int n = 0; n < NV; ++n) r[j][n] -= c * a.resi[(t * a.im0 + m) * NV + n];
+#pragma unroll
+            for (int j = 0; j < SM; ++j) {
+#pragma unroll
+                for (int m = 0; m < IM; ++m) {
+                    const T c = ms * a.lump[a.o_ifmb + (3 + i * SM + j) * IM + m];
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) r[j][n] -= c * bi[m][n];
                 }
                 ms *= sgn;
             }
@@ -588,21 +620,27 @@ __global__ void k_minv_side(MinvArgsT<T> a) {
         const int i = a.st_loc[2 * s + k];
         const int sgn = a.tri_sgn[3 * t + i];
         T uh[3][NV];
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int w = a.tri_pnt[3 * t + j];
+            const int w = a.tri_pnt[3 * t + ((i + j) % 3)];
+#pragma unroll
             for (int n = 0; n < NV; ++n) uh[j][n] = a.resv[w * NV + n] * a.tprcn[3 * t + n];
         }
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int i1 = (i + j) % 3;
             T ms = T(1);
-            for (int m = 0; m < a.sm; ++m) {
-                const T c = a.lump[a.o_vfms + j * a.sm + m];
-                for (int n = 0; n < NV; ++n) r[m][n] -= ms * c * uh[i1][n];
+#pragma unroll
+            for (int m = 0; m < SM; ++m) {
+                const T c = a.lump[a.o_vfms + j * SM + m];
+#pragma unroll
+                for (int n = 0; n < NV; ++n) r[m][n] -= ms * c * uh[j][n];
                 ms *= sgn;
             }
         }
     }
-    for (int m = 0; m < a.sm; ++m)
+#pragma unroll
+    for (int m = 0; m < SM; ++m)
+#pragma unroll
         for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
 }
 
@@ -612,14 +650,17 @@ __device__ __forceinline__ T side_scaled(const MinvArgsT<T>& a, int s, int mode,
     return a.ress[(s * a.sm0 + mode) * NV + n] * (a.sprcn[s * NV + n] * a.lump[a.o_sdiag + mode]);
 }
 
-// remove solved side mode `mode` from the higher modes (:244-328); the solved value
+// remove solved side mode `MODE` from the higher modes (:244-328); the solved value
 // sprcn*sdiag*res with sdirichlet is formed on the fly, stored later by k_minv_sscale
-template <typename T>
-__global__ void k_minv_sub(MinvArgsT<T> a, int mode) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, int P, int MODE>
+__global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
+    constexpr int SM = Dims<P>::SM;
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    T r[5][NV];
-    for (int m = mode + 1; m < a.sm; ++m)
+    T r[SM][NV];
+#pragma unroll
+    for (int m = MODE + 1; m < SM; ++m)
+#pragma unroll
         for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
     for (int k = 0; k < 2; ++k) {
         const int t = a.st_tri[2 * s + k];
@@ -627,88 +668,115 @@ __global__ void k_minv_sub(MinvArgsT<T> a, int mode) {
         const int i = a.st_loc[2 * s + k];
         T uh[3][NV];
         int sign[3];
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int sj = a.tri_seg[3 * t + j];
-            sign[j] = a.tri_sgn[3 * t + j];
-            const T sg = (mode % 2) ? T(sign[j]) : T(1);
-            for (int n = 0; n < NV; ++n) uh[j][n] = sg * side_scaled(a, sj, mode, n) * a.tprcn[3 * t + n];
+            const int jj = (i + j) % 3;
+            const int sj = a.tri_seg[3 * t + jj];
+            sign[j] = a.tri_sgn[3 * t + jj];
+            const T sg = (MODE % 2) ? T(sign[j]) : T(1);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) uh[j][n] = sg * side_scaled(a, sj, MODE, n) * a.tprcn[3 * t + n];
         }
-        T ms = ((mode + 1) % 2) ? T(sign[i]) : T(1);
-        for (int m = mode + 1; m < a.sm; ++m) {
+        T ms = ((MODE + 1) % 2) ? T(sign[0]) : T(1);
+#pragma unroll
+        for (int m = MODE + 1; m < SM; ++m) {
+#pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int ix = (i + j) % 3;
-                const T c = ms * a.lump[a.o_sfms + (mode * a.sm + m) * 3 + j];
-                for (int n = 0; n < NV; ++n) r[m][n] -= c * uh[ix][n];
+                const T c = ms * a.lump[a.o_sfms + (MODE * SM + m) * 3 + j];
+#pragma unroll
+                for (int n = 0; n < NV; ++n) r[m][n] -= c * uh[j][n];
             }
-            ms *= sign[i];
+            ms *= sign[0];
         }
     }
-    for (int m = mode + 1; m < a.sm; ++m)
+#pragma unroll
+    for (int m = MODE + 1; m < SM; ++m)
+#pragma unroll
         for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
 }
 
 // store the solved side modes: res.s *= sprcn*sdiag(mode), sdirichlet (:245-267, :331-343)
-template <typename T>
-__global__ void k_minv_sscale(MinvArgsT<T> a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
+    constexpr int SM = Dims<P>::SM;
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    T out[5][NV];
-    for (int m = 0; m < a.sm; ++m)
-        for (int n = 0; n < NV; ++n) out[m][n] = side_scaled(a, s, m, n);
-    for (int m = 0; m < a.sm; ++m)
+#pragma unroll
+    for (int m = 0; m < SM; ++m)
+#pragma unroll
         for (int n = 0; n < NV; ++n) {
+            const T x = side_scaled(a, s, m, n);
             const int ix = (s * a.sm0 + m) * NV + n;
-            a.ress[ix] = out[m][n];
-            if (a.ugs) a.ugs[ix] = a.ug0s[ix] - a.cfa * out[m][n];
+            a.ress[ix] = x;
+            if (a.ugs) a.ugs[ix] = a.ug0s[ix] - a.cfa * x;
         }
 }
 
 // interiors: banded solve with the literal DPBTRSNU2 indexing (utilities/myblas.c:91-122),
 // divide by tprcn, subtract the boundary solution through bfmi (:347-370)
-template <typename T>
-__global__ void k_minv_int(MinvArgsT<T> a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
+    constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM, BM = Dims<P>::BM;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= a.ntri) return;
-    const int im = a.im;
-    T b[3][NV];
-    for (int k = 0; k < im; ++k)
+    T b[IM][NV];
+#pragma unroll
+    for (int k = 0; k < IM; ++k)
+#pragma unroll
         for (int n = 0; n < NV; ++n) b[k][n] = a.resi[(t * a.im0 + k) * NV + n];
     const T* ab = a.lump + a.o_idiag;  // stride 2, ofdg 1
-    for (int i = 0; i < im; ++i) {
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
         const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
+#pragma unroll
         for (int j = frst; j < 1; ++j)
+#pragma unroll
             for (int n = 0; n < NV; ++n) b[i][n] -= ab[i * 2 + j] * b[ba + j][n];
+#pragma unroll
         for (int n = 0; n < NV; ++n) b[i][n] /= ab[i * 2 + 1];
     }
-    for (int i = im - 1; i >= 0; --i) {
+#pragma unroll
+    for (int i = IM - 1; i >= 0; --i) {
+#pragma unroll
         for (int n = 0; n < NV; ++n) b[i][n] /= ab[i * 2 + 1];
         const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
+#pragma unroll
         for (int j = frst; j < 1; ++j)
+#pragma unroll
             for (int n = 0; n < NV; ++n) b[ba + j][n] -= ab[i * 2 + j] * b[i][n];
     }
-    T uh[12][NV];
+    T uh[BM][NV];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int w = a.tri_pnt[3 * t + i];
+#pragma unroll
         for (int n = 0; n < NV; ++n) uh[i][n] = a.resv[w * NV + n];
     }
-    int cnt = 3;
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int sd = a.tri_seg[3 * t + i], sgn = a.tri_sgn[3 * t + i];
         T ms = T(1);
-        for (int m = 0; m < a.sm; ++m) {
-            for (int n = 0; n < NV; ++n) uh[cnt][n] = ms * a.ress[(sd * a.sm0 + m) * NV + n];
+#pragma unroll
+        for (int m = 0; m < SM; ++m) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) uh[3 + i * SM + m][n] = ms * a.ress[(sd * a.sm0 + m) * NV + n];
             ms *= sgn;
-            ++cnt;
         }
     }
-    for (int k = 0; k < im; ++k) {
+#pragma unroll
+    for (int k = 0; k < IM; ++k) {
+#pragma unroll
         for (int n = 0; n < NV; ++n) b[k][n] /= a.tprcn[3 * t + n];
-        for (int i = 0; i < a.bm; ++i) {
-            const T c = a.lump[a.o_bfmi + i * im + k];
+#pragma unroll
+        for (int i = 0; i < BM; ++i) {
+            const T c = a.lump[a.o_bfmi + i * IM + k];
+#pragma unroll
             for (int n = 0; n < NV; ++n) b[k][n] -= c * uh[i][n];
         }
     }
-    for (int k = 0; k < im; ++k)
+#pragma unroll
+    for (int k = 0; k < IM; ++k)
+#pragma unroll
         for (int n = 0; n < NV; ++n) {
             const int ix = (t * a.im0 + k) * NV + n;
             a.resi[ix] = b[k][n];
