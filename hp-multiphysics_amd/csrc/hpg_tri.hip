@@ -16,6 +16,7 @@
 #include "../../include/hpgpu.h"
 #include "tri_kernels.hip"
 #include "tri_kernels_lin.hip"
+#include "tri_kernels_cns.hip"
 #include "tri_ops.h"
 #include "mlp.h"
 
@@ -105,7 +106,38 @@ struct Level {
     float* pack32 = nullptr;
     float* lump32 = nullptr;
     int o_sfmv, o_ifmb, o_vfms, o_sdiag, o_sfms, o_bfmi, o_idiag;
+    // cns: point-per-lane operator pack (CDims layout), edge side-mass and interior band factors
+    double* cpack = nullptr;
+    double* sdiag1d = nullptr;
+    double* idiag = nullptr;
 };
+
+// cns operator pack of one level: P, Pr, Ps as [mode][32], Dr, Ds as [point'][32],
+// Ir, Is, I as [point][16], zero padded
+std::vector<double> build_cns_pack(const TriOps& o) {
+    const int tm = o.tm, G = o.G;
+    const int OP_PR = tm * 32, OP_PS = 2 * tm * 32, OP_DR = 3 * tm * 32, OP_DS = OP_DR + G * 32;
+    const int OP_IR = OP_DS + G * 32, OP_IS = OP_IR + G * 16, OP_I = OP_IS + G * 16, SIZE = OP_I + G * 16;
+    std::vector<double> pk(SIZE, 0.0);
+    for (int k = 0; k < tm; ++k)
+        for (int g = 0; g < G; ++g) {
+            pk[k * 32 + g] = o.P[g * tm + k];
+            pk[OP_PR + k * 32 + g] = o.Pr[g * tm + k];
+            pk[OP_PS + k * 32 + g] = o.Ps[g * tm + k];
+        }
+    for (int q = 0; q < G; ++q)
+        for (int g = 0; g < G; ++g) {
+            pk[OP_DR + q * 32 + g] = o.Dr[g * G + q];
+            pk[OP_DS + q * 32 + g] = o.Ds[g * G + q];
+        }
+    for (int q = 0; q < G; ++q)
+        for (int k = 0; k < tm; ++k) {
+            pk[OP_IR + q * 16 + k] = o.Ir[k * G + q];
+            pk[OP_IS + q * 16 + k] = o.Is[k * G + q];
+            pk[OP_I + q * 16 + k] = o.I[k * G + q];
+        }
+    return pk;
+}
 
 }  // namespace
 
@@ -116,6 +148,13 @@ struct hpg_ctx {
     bool coarse_flag = false, isfrst = false;
     int precision = HPG_FP64, prcn_mode = HPG_PRCN_ANALYTIC;
     int npnt = 0, nseg = 0, ntri = 0, sm0 = 0, im0 = 0;
+    int physics = HPG_PHYSICS_INS, nv = NV;  // unknowns per vertex/mode: 3 (ins) or 4 (cns)
+    // cns: physical constants, inflow boundary state, block preconditioners, res_temp
+    CnsPhys cph{};
+    double cbc[4] = {1.0, 0.0, 0.0, 1.0};
+    double *vpc = nullptr, *spc = nullptr, *tpc = nullptr, *rtv = nullptr, *rts = nullptr;
+    int *inflow_v = nullptr, *inflow_s = nullptr;
+    int n_inflow_v = 0, n_inflow_s = 0;
     std::vector<void*> allocs;
     double* pnts = nullptr;
     int *tri_pnt = nullptr, *tri_seg = nullptr, *tri_sgn = nullptr, *seg_pnt = nullptr;
@@ -206,8 +245,11 @@ namespace {
 
 void launch_modes(hpg_ctx* c, int op, long nent, int nm, int mstride, double* x, double* y, const double* z, double k) {
     if (nent <= 0 || nm <= 0) return;
-    const long n = nent * nm * NV;
-    hipLaunchKernelGGL(k_modes, dim3(nblk(n)), dim3(256), 0, c->stream, op, nent, nm, mstride, x, y, z, k);
+    const long n = nent * nm * c->nv;
+    if (c->nv == NVC)
+        hipLaunchKernelGGL(k_modes<NVC>, dim3(nblk(n)), dim3(256), 0, c->stream, op, nent, nm, mstride, x, y, z, k);
+    else
+        hipLaunchKernelGGL(k_modes<NV>, dim3(nblk(n)), dim3(256), 0, c->stream, op, nent, nm, mstride, x, y, z, k);
 }
 
 // apply a k_modes op over the active v/s/i modes of the current level
@@ -297,7 +339,10 @@ MinvArgsT<T> minv_args(hpg_ctx* c) {
     return a;
 }
 
+void do_rsdl_cns(hpg_ctx* c, int stage);
+
 void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
+    if (c->physics == HPG_PHYSICS_CNS) { do_rsdl_cns(c, stage); return; }
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     const double f = stage < nst ? 1.0 - beta : 0.0;
@@ -385,30 +430,36 @@ void do_lin_rsdl(hpg_ctx* c, int stage, bool with_source) {
     check(hipGetLastError(), "lin rsdl launch");
 }
 
-template <typename T>
-void launch_minv(hpg_ctx* c, const MinvArgsT<T>& a) {
+template <typename T, int NVn>
+void launch_minv_nv(hpg_ctx* c, const MinvArgsT<T>& a) {
     const dim3 bv(nblk8(c->npnt)), bs(nblk8(c->nseg)), bt(nblk8(c->ntri)), th(256);
     switch (a.sm + 1) {
         case 1:
-            hipLaunchKernelGGL((k_minv_vrtx<T, 1>), bv, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_vrtx<T, 1, NVn>), bv, th, 0, c->stream, a);
             break;
         case 2:
-            hipLaunchKernelGGL((k_minv_vrtx<T, 2>), bv, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_side<T, 2>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_sscale<T, 2>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_vrtx<T, 2, NVn>), bv, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_side<T, 2, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sscale<T, 2, NVn>), bs, th, 0, c->stream, a);
             break;
         case 4:
-            hipLaunchKernelGGL((k_minv_vrtx<T, 4>), bv, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_side<T, 4>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_sub<T, 4, 0>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_sub<T, 4, 1>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_sscale<T, 4>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_int<T, 4>), bt, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_vrtx<T, 4, NVn>), bv, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_side<T, 4, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sub<T, 4, 0, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sub<T, 4, 1, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sscale<T, 4, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_int<T, 4, NVn>), bt, th, 0, c->stream, a);
             break;
         default:
             throw std::runtime_error("minvrt: unsupported p");
     }
     check(hipGetLastError(), "minvrt launch");
+}
+
+template <typename T>
+void launch_minv(hpg_ctx* c, const MinvArgsT<T>& a) {
+    if (c->nv == NVC) launch_minv_nv<T, NVC>(c, a);
+    else launch_minv_nv<T, NV>(c, a);
 }
 
 // minvrt; with update_cfa >= 0 the stage update ug = ug0 - update_cfa*res is fused
@@ -432,7 +483,160 @@ void do_minvrt32(hpg_ctx* c, float cfa) {
     launch_minv<float>(c, a);
 }
 
+// ---- tri_hp_cns drivers ------------------------------------------------------------
+template <int P>
+void launch_elem_cns(hpg_ctx* c, const CnsElemArgs& a0, bool full) {
+    for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+        CnsElemArgs a = a0;
+        a.colour = (int)col;
+        a.begin = c->coff[col];
+        a.end = c->coff[col + 1];
+        const int noct = (a.end - a.begin + 7) / 8;
+        if (noct == 0) continue;
+        const int nb = std::min(((noct + 7) / 8) * 8, 1024);
+        const size_t r = c->recs.size();
+        if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+        if (full)
+            hipLaunchKernelGGL((k_elem_cns<P, true>), dim3(nb), dim3(256), 0, c->stream, a, noct);
+        else
+            hipLaunchKernelGGL((k_elem_cns<P, false>), dim3(nb), dim3(256), 0, c->stream, a, noct);
+        if (c->timing) {
+            check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+            c->recs.push_back({full ? 1 : 0, a.end - a.begin});
+        }
+    }
+}
+
+// coarse p-level source (tri_hp/nstage.cpp:113-129)
+void mg_source(hpg_ctx* c) {
+    const int l = c->log2p;
+    if (c->isfrst) {
+        modes_vsi(c, 2, c->resv, c->ress, c->resi, c->dresv[l], c->dress[l], c->dresi[l], c->r0v, c->r0s, c->r0i,
+                  c->prm.fadd);
+        c->isfrst = false;
+    } else {
+        modes_vsi(c, 1, c->resv, c->ress, c->resi, c->dresv[l], c->dress[l], c->dresi[l], nullptr, nullptr, nullptr,
+                  0.0);
+    }
+}
+
+void do_rsdl_cns(hpg_ctx* c, int stage) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const double f = stage < nst ? 1.0 - beta : 0.0;
+    const Level& L = c->lv[c->log2p];
+    CnsElemArgs a;
+    a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+    a.order = c->order; a.begin = 0; a.end = 0;
+    a.ugv = c->ugv; a.ugs = c->ugs; a.ugi = c->ugi;
+    a.resv = c->resv; a.ress = c->ress; a.resi = c->resi;
+    a.rrv = c->rrv; a.rrs = c->rrs; a.rri = c->rri;
+    a.ops = L.cpack; a.sm0 = c->sm0; a.im0 = c->im0;
+    a.ph = c->cph; a.beta = beta; a.keep = f;
+    a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0;
+    const bool full = beta > 0.0;
+    const LevelDims d(L.p);
+    CnsEdgeArgs e;
+    e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
+    e.resv = c->resv; e.ress = c->ress;
+    e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+    e.sm0 = c->sm0; e.gamma = c->cph.gamma; e.pbc = c->cbc[0];
+    auto edges = [&](auto kern) {
+        for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
+            e.begin = c->eoff[col];
+            e.end = c->eoff[col + 1];
+            if (e.end > e.begin) hipLaunchKernelGGL(kern, dim3(nblk(e.end - e.begin, 64)), dim3(64), 0, c->stream, e);
+        }
+    };
+    switch (L.p) {
+        case 1: launch_elem_cns<1>(c, a, full); edges(k_edge_cns<1>); break;
+        case 2: launch_elem_cns<2>(c, a, full); edges(k_edge_cns<2>); break;
+        case 4: launch_elem_cns<4>(c, a, full); edges(k_edge_cns<4>); break;
+        default: throw std::runtime_error("unsupported p");
+    }
+    if (c->coarse_flag) mg_source(c);
+    check(hipGetLastError(), "cns rsdl launch");
+}
+
+int do_setup_cns(hpg_ctx* c) {
+    const Level& L = c->lv[c->log2p];
+    const LevelDims d(L.p);
+    check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
+    CnsSetupArgs s;
+    s.pnts = c->pnts; s.tri_pnt = c->tri_pnt; s.tri_seg = c->tri_seg; s.tri_sgn = c->tri_sgn;
+    s.ugv = c->ugv; s.ugs = c->ugs; s.ugi = c->ugi; s.Pmat = L.Pmat; s.tprcn = c->tprcn; s.tpc = c->tpc;
+    s.err = c->d_err; s.ntri = c->ntri; s.sm0 = c->sm0; s.im0 = c->im0; s.ph = c->cph;
+    const dim3 be(nblk8(c->ntri, 128)), te(128);
+    switch (d.p) {
+        case 1: hipLaunchKernelGGL(k_setup_elem_cns<1>, be, te, 0, c->stream, s); break;
+        case 2: hipLaunchKernelGGL(k_setup_elem_cns<2>, be, te, 0, c->stream, s); break;
+        default: hipLaunchKernelGGL(k_setup_elem_cns<4>, be, te, 0, c->stream, s); break;
+    }
+    hipLaunchKernelGGL(k_setup_vrtx_cns, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vt_ptr, c->vt_tri,
+                       c->tprcn, c->tpc, c->vprcn, c->vpc, L.ops->vdiag);
+    if (d.sm > 0)
+        hipLaunchKernelGGL(k_setup_side_cns, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->st_tri,
+                           c->tprcn, c->tpc, c->sprcn, c->spc);
+    int herr = 0;
+    check(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "memcpy err");
+    check(hipStreamSynchronize(c->stream), "sync");
+    return herr;
+}
+
+// inflow modify_boundary_residual, then project_new_variables (tri_hp/cns/update.cpp:276-357)
+void cns_boundary_and_projection(hpg_ctx* c) {
+    const Level& L = c->lv[c->log2p];
+    const LevelDims d(L.p);
+    const double* g1 = L.edge;
+    const double* w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+    CnsBcArgs b;
+    b.resv = c->resv; b.ress = c->ress; b.verts = c->inflow_v; b.sides = c->inflow_s;
+    b.nverts = c->n_inflow_v; b.nsides = c->n_inflow_s; b.seg_pnt = c->seg_pnt;
+    b.g1 = g1; b.w1 = w1; b.sdiag1d = L.sdiag1d; b.sm0 = c->sm0;
+    b.bu = c->cbc[1]; b.bv = c->cbc[2];
+    b.benergy = c->cbc[3] / (c->cph.gamma - 1.0) + 0.5 * (c->cbc[1] * c->cbc[1] + c->cbc[2] * c->cbc[2]);
+    if (b.nverts > 0) hipLaunchKernelGGL(k_mbr_vrtx, dim3(nblk(b.nverts)), dim3(256), 0, c->stream, b);
+    if (b.nsides > 0 && d.sm > 0) {
+        if (d.p == 2) hipLaunchKernelGGL(k_mbr_side<2>, dim3(nblk(b.nsides)), dim3(256), 0, c->stream, b);
+        else hipLaunchKernelGGL(k_mbr_side<4>, dim3(nblk(b.nsides)), dim3(256), 0, c->stream, b);
+    }
+    CnsProjArgs a;
+    a.ugv = c->ugv; a.ugs = c->ugs; a.ugi = c->ugi; a.resv = c->resv; a.ress = c->ress; a.resi = c->resi;
+    a.rtv = c->rtv; a.rts = c->rts; a.vpc = c->vpc; a.vess = c->vess; a.sess = c->sess;
+    a.seg_pnt = c->seg_pnt; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+    a.order = c->order; a.g1 = g1; a.w1 = w1; a.sdiag1d = L.sdiag1d; a.idiag = L.idiag; a.ops = L.cpack;
+    a.npnt = c->npnt; a.nseg = c->nseg; a.ntri = c->ntri; a.sm0 = c->sm0; a.im0 = c->im0; a.gamma = c->cph.gamma;
+    hipLaunchKernelGGL(k_pnv_vrtx, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, a);
+    if (d.sm > 0) {
+        if (d.p == 2) hipLaunchKernelGGL(k_pnv_side<2>, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, a);
+        else hipLaunchKernelGGL(k_pnv_side<4>, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, a);
+    }
+    if (d.im > 0) {
+        const int noct = (c->ntri + 7) / 8;
+        const int nb = std::min(((noct + 7) / 8) * 8, 1024);
+        hipLaunchKernelGGL(k_pnv_int<4>, dim3(nb), dim3(256), 0, c->stream, a, noct);
+    }
+    check(hipGetLastError(), "cns projection launch");
+}
+
+void update_cns(hpg_ctx* c) {
+    const size_t nv = (size_t)c->npnt * c->nv * 8, ns = (size_t)c->nseg * c->sm0 * c->nv * 8,
+                 ni = (size_t)c->ntri * c->im0 * c->nv * 8;
+    check(hipMemcpyAsync(c->ug0v, c->ugv, nv, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    if (ns) check(hipMemcpyAsync(c->ug0s, c->ugs, ns, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    if (ni) check(hipMemcpyAsync(c->ug0i, c->ugi, ni, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    for (int stage = 0; stage < c->prm.nstage; ++stage) {
+        do_rsdl_cns(c, stage);
+        do_minvrt(c);
+        cns_boundary_and_projection(c);
+        modes_vsi(c, 4, c->ugv, c->ugs, c->ugi, c->ug0v, c->ug0s, c->ug0i, c->resv, c->ress, c->resi,
+                  c->prm.alpha[stage] * c->prm.cfl[c->log2p]);
+    }
+    check(hipGetLastError(), "cns update launch");
+}
+
 int do_setup(hpg_ctx* c) {
+    if (c->physics == HPG_PHYSICS_CNS) return do_setup_cns(c);
     const Level& L = c->lv[c->log2p];
     const LevelDims d(L.p);
     check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
@@ -466,17 +670,20 @@ int do_setup(hpg_ctx* c) {
 }
 
 // max over (nent, nm of mstride, NV) of |x|, per variable
-void absmax(hpg_ctx* c, long nent, int nm, int mstride, const double* x, const unsigned char* mask, double out[NV]) {
-    unsigned long long h[NV] = {0, 0, 0};
+void absmax(hpg_ctx* c, long nent, int nm, int mstride, const double* x, const unsigned char* mask, double out[NVC]) {
+    unsigned long long h[NVC] = {0, 0, 0, 0};
     if (nent > 0 && nm > 0) {
-        check(hipMemsetAsync(c->d_max, 0, NV * sizeof(unsigned long long), c->stream), "memset");
+        check(hipMemsetAsync(c->d_max, 0, NVC * sizeof(unsigned long long), c->stream), "memset");
         const long n = nent * nm;
         const unsigned nb = std::min<unsigned>(nblk(n), 2048u);
-        hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
+        if (c->nv == NVC)
+            hipLaunchKernelGGL(k_absmax<NVC>, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
+        else
+            hipLaunchKernelGGL(k_absmax<NV>, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
         check(hipMemcpyAsync(h, c->d_max, sizeof(h), hipMemcpyDeviceToHost, c->stream), "memcpy");
         check(hipStreamSynchronize(c->stream), "sync");
     }
-    for (int k = 0; k < NV; ++k) {
+    for (int k = 0; k < NVC; ++k) {
         double v;
         std::memcpy(&v, &h[k], sizeof(double));
         out[k] = v;
@@ -490,7 +697,8 @@ extern "C" {
 int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
     if (!m || !prm || !out) return -2;
     *out = nullptr;
-    if (prm->physics != HPG_PHYSICS_INS || prm->log2p < 0 || prm->log2p > 2 || prm->nstage < 1 || prm->nstage > 7)
+    if ((prm->physics != HPG_PHYSICS_INS && prm->physics != HPG_PHYSICS_CNS) || prm->log2p < 0 || prm->log2p > 2 ||
+        prm->nstage < 1 || prm->nstage > 7)
         return -2;
     std::unique_ptr<hpg_ctx> c(new hpg_ctx());
     try {
@@ -498,6 +706,18 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->prm = *prm;
         c->prm.alpha[prm->nstage] = 1.0;
         c->prm.beta[prm->nstage] = 1.0;
+        c->physics = prm->physics;
+        c->nv = prm->physics == HPG_PHYSICS_CNS ? NVC : NV;
+        if (c->physics == HPG_PHYSICS_CNS) {
+            // tri_hp/cns/allocate.cpp:45: kcond = mu/Pr * R gamma/(gamma-1)
+            c->cph.mu = prm->mu;
+            c->cph.gamma = prm->gamma;
+            c->cph.Rgas = prm->Rgas;
+            c->cph.adis = prm->adis;
+            c->cph.kcond = prm->mu / prm->prandtl * prm->Rgas * prm->gamma / (prm->gamma - 1.0);
+            for (int n = 0; n < 4; ++n) c->cbc[n] = prm->bc[n];
+            if (!(prm->gamma > 1.0) || !(prm->prandtl > 0.0) || !(prm->Rgas > 0.0)) throw std::runtime_error("cns constants");
+        }
         c->log2pmax = c->log2p = prm->log2p;
         const int p0 = 1 << prm->log2p;
         c->sm0 = p0 - 1;
@@ -567,7 +787,7 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             }
         std::vector<unsigned char> vess(np, 0), sess(ns, 0);
         for (int b = 0; b < m->nebd; ++b) {
-            const unsigned char mk = (unsigned char)(m->ebdry_essential[b] & 7);
+            const unsigned char mk = (unsigned char)(m->ebdry_essential[b] & (c->nv == NVC ? 15 : 7));
             for (int j = m->ebdry_start[b]; j < m->ebdry_start[b + 1]; ++j) {
                 const int s = m->ebdry_seg[j];
                 sess[s] |= mk;
@@ -586,7 +806,8 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->vess = c->upload(vess.data(), vess.size());
         c->sess = c->upload(sess.data(), sess.size());
 
-        const size_t nv = (size_t)np * NV, nsz = (size_t)ns * c->sm0 * NV, ni = (size_t)nt * c->im0 * NV;
+        const int NVX = c->nv;
+        const size_t nv = (size_t)np * NVX, nsz = (size_t)ns * c->sm0 * NVX, ni = (size_t)nt * c->im0 * NVX;
         double** vsi[] = {&c->ugv, &c->ug0v, &c->resv, &c->rrv, &c->r0v};
         double** ssi[] = {&c->ugs, &c->ug0s, &c->ress, &c->rrs, &c->r0s};
         double** isi[] = {&c->ugi, &c->ug0i, &c->resi, &c->rri, &c->r0i};
@@ -599,10 +820,35 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             c->dresi[l] = c->alloc<double>(ni);
         }
         c->vprcn = c->alloc<double>(nv);
-        c->sprcn = c->alloc<double>((size_t)ns * NV);
-        c->tprcn = c->alloc<double>((size_t)nt * NV);
+        c->sprcn = c->alloc<double>((size_t)ns * NVX);
+        c->tprcn = c->alloc<double>((size_t)nt * NVX);
         c->d_err = c->alloc<int>(1);
-        c->d_max = c->alloc<unsigned long long>(NV);
+        c->d_max = c->alloc<unsigned long long>(NVC);
+        if (c->physics == HPG_PHYSICS_CNS) {
+            c->vpc = c->alloc<double>((size_t)np * 16);
+            c->spc = c->alloc<double>((size_t)ns * 16);
+            c->tpc = c->alloc<double>((size_t)nt * 16);
+            c->rtv = c->alloc<double>(nv);
+            c->rts = c->alloc<double>(nsz);
+            // inflow boundaries (essential u, v, RT): their vertices (once each) and sides
+            std::vector<int> iv, is;
+            std::vector<unsigned char> seen(np, 0);
+            for (int b = 0; b < m->nebd; ++b) {
+                if (m->ebdry_essential[b] != 0b1110) continue;
+                for (int j = m->ebdry_start[b]; j < m->ebdry_start[b + 1]; ++j) {
+                    const int sg = m->ebdry_seg[j];
+                    is.push_back(sg);
+                    for (int e = 0; e < 2; ++e) {
+                        const int v = m->seg_pnt[2 * sg + e];
+                        if (!seen[v]) { seen[v] = 1; iv.push_back(v); }
+                    }
+                }
+            }
+            c->n_inflow_v = (int)iv.size();
+            c->n_inflow_s = (int)is.size();
+            c->inflow_v = c->upload(iv.data(), iv.size());
+            c->inflow_s = c->upload(is.data(), is.size());
+        }
 
         // mixed precision: 16-element groups per colour, FP32 inner arrays
         c->gbase.assign(c->coff.size(), 0);
@@ -647,6 +893,13 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             ed.insert(ed.end(), o.dg1.begin(), o.dg1.end());
             ed.insert(ed.end(), o.w1.begin(), o.w1.end());
             L.edge = c->upload(ed.data(), ed.size());
+            if (c->physics == HPG_PHYSICS_CNS) {
+                const std::vector<double> cp = build_cns_pack(o);
+                L.cpack = c->upload(cp.data(), cp.size());
+                L.sdiag1d = c->upload(o.sdiag1d.data(), o.sdiag1d.size());
+                const std::vector<double> idg = o.idiag.empty() ? std::vector<double>(2, 1.0) : o.idiag;
+                L.idiag = c->upload(idg.data(), idg.size());
+            }
         }
         check(hipStreamSynchronize(c->stream), "create sync");
     } catch (const std::exception& ex) {
@@ -664,8 +917,8 @@ int hpg_destroy(hpg_ctx* c) {
 static int copy3(hpg_ctx* c, bool h2d, double* dv, double* ds, double* di, const double* hv, const double* hs,
                  const double* hi, double* ov, double* os, double* oi) {
     HPG_TRY(c, {
-        const size_t nv = (size_t)c->npnt * NV * 8, ns = (size_t)c->nseg * c->sm0 * NV * 8,
-                     ni = (size_t)c->ntri * c->im0 * NV * 8;
+        const size_t nv = (size_t)c->npnt * c->nv * 8, ns = (size_t)c->nseg * c->sm0 * c->nv * 8,
+                     ni = (size_t)c->ntri * c->im0 * c->nv * 8;
         if (h2d) {
             check(hipMemcpyAsync(dv, hv, nv, hipMemcpyHostToDevice, c->stream), "H2D");
             if (ns) check(hipMemcpyAsync(ds, hs, ns, hipMemcpyHostToDevice, c->stream), "H2D");
@@ -699,17 +952,28 @@ int hpg_set_residual(hpg_ctx* c, const double* v, const double* s, const double*
 int hpg_get_preconditioner(hpg_ctx* c, double* vp, double* sp, double* tp) {
     if (!c) return -2;
     HPG_TRY(c, {
-        check(hipMemcpyAsync(vp, c->vprcn, (size_t)c->npnt * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
-        check(hipMemcpyAsync(sp, c->sprcn, (size_t)c->nseg * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
-        check(hipMemcpyAsync(tp, c->tprcn, (size_t)c->ntri * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(vp, c->vprcn, (size_t)c->npnt * c->nv * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(sp, c->sprcn, (size_t)c->nseg * c->nv * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(tp, c->tprcn, (size_t)c->ntri * c->nv * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
         check(hipStreamSynchronize(c->stream), "sync");
     });
     return 0;
 }
+int hpg_get_block_preconditioner(hpg_ctx* c, double* vpc, double* spc, double* tpc) {
+    if (!c || c->physics != HPG_PHYSICS_CNS) return -2;
+    HPG_TRY(c, {
+        if (vpc) check(hipMemcpyAsync(vpc, c->vpc, (size_t)c->npnt * 16 * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        if (spc) check(hipMemcpyAsync(spc, c->spc, (size_t)c->nseg * 16 * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        if (tpc) check(hipMemcpyAsync(tpc, c->tpc, (size_t)c->ntri * 16 * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+
 int hpg_set_tprcn(hpg_ctx* c, const double* tp) {
     if (!c) return -2;
     HPG_TRY(c, {
-        check(hipMemcpyAsync(c->tprcn, tp, (size_t)c->ntri * NV * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        check(hipMemcpyAsync(c->tprcn, tp, (size_t)c->ntri * c->nv * 8, hipMemcpyHostToDevice, c->stream), "H2D");
         check(hipStreamSynchronize(c->stream), "sync");
     });
     return 0;
@@ -781,6 +1045,7 @@ int hpg_set_defect_inner(hpg_ctx* c, int ninner) {
 int hpg_apply_jacobian(hpg_ctx* c, const double* dv, const double* ds, const double* di, double* ov, double* os,
                        double* oi) {
     if (!c || !dv || !ov) return -2;
+    if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     if (c->log2p != c->log2pmax) { c->err = "apply_jacobian runs on the finest level"; return -2; }
     HPG_TRY(c, {
         const size_t nv = (size_t)c->npnt * NV, nsz = (size_t)c->nseg * c->sm0 * NV, ni = (size_t)c->ntri * c->im0 * NV;
@@ -811,6 +1076,7 @@ int hpg_set_mlp(hpg_ctx* c, hpg_mlp* m) {
 
 int hpg_mlp_features(hpg_ctx* c, float* out) {
     if (!c || !out) return -2;
+    if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     HPG_TRY(c, {
         hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts, c->tri_pnt,
                            c->ugv, c->tprcn, c->prm.mu, c->prm.rho, c->featX);
@@ -822,6 +1088,7 @@ int hpg_mlp_features(hpg_ctx* c, float* out) {
 
 int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
     if (!c || !out) return -2;
+    if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     if (c->log2p != c->log2pmax) { c->err = "measure_jacobian_diag runs on the finest level"; return -2; }
     HPG_TRY(c, {
         do_rsdl(c, c->prm.nstage, true);
@@ -858,6 +1125,10 @@ int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
 
 int hpg_update(hpg_ctx* c) {
     if (!c) return -2;
+    if (c->physics == HPG_PHYSICS_CNS) {
+        HPG_TRY(c, update_cns(c));
+        return 0;
+    }
     if (c->precision == HPG_FP32_DEFECT && !c->coarse_flag) {
         HPG_TRY(c, mixed_update(c));
         return 0;
@@ -880,10 +1151,10 @@ int hpg_update(hpg_ctx* c) {
 int hpg_maxres(hpg_ctx* c, double* per_var, double* out) {
     if (!c) return -2;
     HPG_TRY(c, {
-        double mx[NV];
+        double mx[NVC];
         absmax(c, c->npnt, 1, 1, c->resv, nullptr, mx);
         double e = 0.0;
-        for (int n = 0; n < NV; ++n) {
+        for (int n = 0; n < c->nv; ++n) {
             e = std::max(e, mx[n]);
             if (per_var) per_var[n] = mx[n];
         }
@@ -896,12 +1167,12 @@ int hpg_resnorm(hpg_ctx* c, double* out) {
     if (!c || !out) return -2;
     HPG_TRY(c, {
         const LevelDims d(1 << c->log2p);
-        double a[NV], b[NV] = {0, 0, 0}, e[NV] = {0, 0, 0};
+        double a[NVC], b[NVC] = {0, 0, 0, 0}, e[NVC] = {0, 0, 0, 0};
         absmax(c, c->npnt, 1, 1, c->resv, c->vess, a);
         if (d.sm) absmax(c, c->nseg, d.sm, c->sm0, c->ress, c->sess, b);
         if (d.im) absmax(c, c->ntri, d.im, c->im0, c->resi, nullptr, e);
         double r = 0.0;
-        for (int n = 0; n < NV; ++n) r = std::max(r, std::max(a[n], std::max(b[n], e[n])));
+        for (int n = 0; n < c->nv; ++n) r = std::max(r, std::max(a[n], std::max(b[n], e[n])));
         *out = r;
     });
     return 0;
@@ -934,12 +1205,14 @@ int hpg_get_log2p(hpg_ctx* c, int* l) {
 
 int hpg_set_precision(hpg_ctx* c, int mode) {
     if (!c || (mode != HPG_FP64 && mode != HPG_FP32_DEFECT)) return -2;
+    if (c->physics != HPG_PHYSICS_INS && mode != HPG_FP64) { c->err = "the FP32 defect-correction path is built for ins"; return -2; }
     c->precision = mode;
     return 0;
 }
 
 int hpg_set_preconditioner(hpg_ctx* c, int mode) {
     if (!c || (mode != HPG_PRCN_ANALYTIC && mode != HPG_PRCN_LEARNED)) return -2;
+    if (c->physics != HPG_PHYSICS_INS && mode != HPG_PRCN_ANALYTIC) { c->err = "the learned preconditioner is built for ins"; return -2; }
     c->prcn_mode = mode;
     return 0;
 }

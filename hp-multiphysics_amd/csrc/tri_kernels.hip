@@ -24,21 +24,21 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
 // ---------------------------------------------------------------------------------
 // tri_hp::ugtouht / lftog index map for mode k of element t (tri_hp/gtol.cpp:14-55,
 // 364-402): returns the offset of the NV-vector and the side sign sgn^m.
-template <int P>
+template <int P, int NVn = NV>
 __device__ __forceinline__ int mode_offset(int k, int t, const int* vtx, const int* seg, const int* sgn,
                                            int sm0, int im0, int& kind, double& sign) {
     using D = Dims<P>;
     sign = 1.0;
-    if (k < 3) { kind = 0; return vtx[k] * NV; }
+    if (k < 3) { kind = 0; return vtx[k] * NVn; }
     if (k < D::BM) {
         constexpr int SMD = D::SM > 0 ? D::SM : 1;
         const int i = (k - 3) / SMD, m = (k - 3) - i * SMD;
         kind = 1;
         sign = ((m & 1) && sgn[i] < 0) ? -1.0 : 1.0;
-        return (seg[i] * sm0 + m) * NV;
+        return (seg[i] * sm0 + m) * NVn;
     }
     kind = 2;
-    return (t * im0 + (k - D::BM)) * NV;
+    return (t * im0 + (k - D::BM)) * NVn;
 }
 
 // tri_hp_ins::element_rsdl, straight branch (tri_hp/ins/rsdl.cpp:259-433), steady,
@@ -306,13 +306,14 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
 // op 2: y = c*z - x, x += y     (mg_source first visit, nstage.cpp:113-128)
 // op 3: y = x                   (mg_restrict, movco.cpp:18-37)
 // op 4: x = y - c*z             (update, nstage.cpp:387-409)
+template <int NVn>
 __global__ void k_modes(int op, long nent, int nm, int mstride, double* __restrict__ x, double* __restrict__ y,
                         const double* __restrict__ zz, double c) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long span = (long)nm * NV;
+    const long span = (long)nm * NVn;
     if (k >= nent * span) return;
     const long ent = k / span, r = k - ent * span;
-    const long ix = ent * (long)mstride * NV + r;
+    const long ix = ent * (long)mstride * NVn + r;
     switch (op) {
         case 0: x[ix] = 0.0; y[ix] *= c; break;
         case 1: x[ix] += y[ix]; break;
@@ -384,8 +385,6 @@ __global__ void k_edge_ins(EdgeArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
-// setup_preconditioner, element part (tri_hp/ins/setup_preconditioner.cpp:33-219,
-// diagonal branch, REFINED_WAY, straight elements, steady)
 // XCD-aware block index: launches use a multiple of 8 blocks; consecutive hardware
 // blocks go round-robin to the 8 XCDs, so XCD x is given the contiguous logical
 // range [x * per, (x + 1) * per) and neighbouring entities share one L2
@@ -394,6 +393,8 @@ __device__ __forceinline__ int xcd_block() {
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// setup_preconditioner, element part (tri_hp/ins/setup_preconditioner.cpp:33-219,
+// diagonal branch, REFINED_WAY, straight elements, steady)
 struct SetupArgs {
     const double* pnts;
     const int* tri_pnt;
@@ -542,14 +543,14 @@ struct MinvArgsT {
 
 // vertices: subtract side and interior couplings (:110-135), scale (:151-152),
 // vertex Dirichlet (:179-183)
-template <typename T, int P>
+template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM;
     const int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= a.npnt) return;
-    T r[NV];
+    T r[NVn];
 #pragma unroll
-    for (int n = 0; n < NV; ++n) r[n] = a.resv[v * NV + n];
+    for (int n = 0; n < NVn; ++n) r[n] = a.resv[v * NVn + n];
     if constexpr (SM > 0) {
         for (int k = a.vs_ptr[v]; k < a.vs_ptr[v + 1]; ++k) {
             const int s = a.vs_side[k], i = a.vs_end[k];
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
             for (int m = 0; m < SM; ++m) {
                 const T c = a.lump[a.o_sfmv + i * SM + m];
 #pragma unroll
-                for (int n = 0; n < NV; ++n) r[n] -= c * a.ress[(s * a.sm0 + m) * NV + n];
+                for (int n = 0; n < NVn; ++n) r[n] -= c * a.ress[(s * a.sm0 + m) * NVn + n];
             }
         }
         if constexpr (IM > 0)
@@ -567,41 +568,41 @@ __global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
                 for (int m = 0; m < IM; ++m) {
                     const T c = a.lump[a.o_ifmb + i * IM + m];
 #pragma unroll
-                    for (int n = 0; n < NV; ++n) r[n] -= c * a.resi[(t * a.im0 + m) * NV + n];
+                    for (int n = 0; n < NVn; ++n) r[n] -= c * a.resi[(t * a.im0 + m) * NVn + n];
                 }
             }
     }
     const unsigned msk = a.vess[v];
 #pragma unroll
-    for (int n = 0; n < NV; ++n) {
-        const T x = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NV + n];
-        a.resv[v * NV + n] = x;
-        if (a.ugv) a.ugv[v * NV + n] = a.ug0v[v * NV + n] - a.cfa * x;
+    for (int n = 0; n < NVn; ++n) {
+        const T x = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NVn + n];
+        a.resv[v * NVn + n] = x;
+        if (a.ugv) a.ugv[v * NVn + n] = a.ug0v[v * NVn + n] - a.cfa * x;
     }
 }
 
 // sides: subtract interior couplings (:137-148), then the vertex solution (:192-214)
-template <typename T, int P>
+template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    T r[SM][NV];
+    T r[SM][NVn];
 #pragma unroll
     for (int m = 0; m < SM; ++m)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
+        for (int n = 0; n < NVn; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NVn + n];
     if constexpr (IM > 0)
         for (int k = 0; k < 2; ++k) {
             const int t = a.st_tri[2 * s + k];
             if (t < 0) continue;
             const int i = a.st_loc[2 * s + k];
             const int sgn = a.tri_sgn[3 * t + i];
-            T bi[IM][NV];
+            T bi[IM][NVn];
 #pragma unroll
             for (int m = 0; m < IM; ++m)
 #pragma unroll
-                for (int n = 0; n < NV; ++n) bi[m][n] = a.resi[(t * a.im0 + m) * NV + n];
+                for (int n = 0; n < NVn; ++n) bi[m][n] = a.resi[(t * a.im0 + m) * NVn + n];
             T ms = T(1);
 #pragma unroll
             for (int j = 0; j < SM; ++j) {
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
                 for (int m = 0; m < IM; ++m) {
                     const T c = ms * a.lump[a.o_ifmb + (3 + i * SM + j) * IM + m];
 #pragma unroll
-                    for (int n = 0; n < NV; ++n) r[j][n] -= c * bi[m][n];
+                    for (int n = 0; n < NVn; ++n) r[j][n] -= c * bi[m][n];
                 }
                 ms *= sgn;
             }
@@ -619,12 +620,12 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
         const int sgn = a.tri_sgn[3 * t + i];
-        T uh[3][NV];
+        T uh[3][NVn];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const int w = a.tri_pnt[3 * t + ((i + j) % 3)];
 #pragma unroll
-            for (int n = 0; n < NV; ++n) uh[j][n] = a.resv[w * NV + n] * a.tprcn[3 * t + n];
+            for (int n = 0; n < NVn; ++n) uh[j][n] = a.resv[w * NVn + n] * a.tprcn[NVn * t + n];
         }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -633,7 +634,7 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
             for (int m = 0; m < SM; ++m) {
                 const T c = a.lump[a.o_vfms + j * SM + m];
 #pragma unroll
-                for (int n = 0; n < NV; ++n) r[m][n] -= ms * c * uh[j][n];
+                for (int n = 0; n < NVn; ++n) r[m][n] -= ms * c * uh[j][n];
                 ms *= sgn;
             }
         }
@@ -641,32 +642,32 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
 #pragma unroll
     for (int m = 0; m < SM; ++m)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
+        for (int n = 0; n < NVn; ++n) a.ress[(s * a.sm0 + m) * NVn + n] = r[m][n];
 }
 
-template <typename T>
+template <typename T, int NVn>
 __device__ __forceinline__ T side_scaled(const MinvArgsT<T>& a, int s, int mode, int n) {
     if ((a.sess[s] >> n) & 1) return T(0);
-    return a.ress[(s * a.sm0 + mode) * NV + n] * (a.sprcn[s * NV + n] * a.lump[a.o_sdiag + mode]);
+    return a.ress[(s * a.sm0 + mode) * NVn + n] * (a.sprcn[s * NVn + n] * a.lump[a.o_sdiag + mode]);
 }
 
 // remove solved side mode `MODE` from the higher modes (:244-328); the solved value
 // sprcn*sdiag*res with sdirichlet is formed on the fly, stored later by k_minv_sscale
-template <typename T, int P, int MODE>
+template <typename T, int P, int MODE, int NVn>
 __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
-    T r[SM][NV];
+    T r[SM][NVn];
 #pragma unroll
     for (int m = MODE + 1; m < SM; ++m)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NV + n];
+        for (int n = 0; n < NVn; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NVn + n];
     for (int k = 0; k < 2; ++k) {
         const int t = a.st_tri[2 * s + k];
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
-        T uh[3][NV];
+        T uh[3][NVn];
         int sign[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -675,7 +676,7 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
             sign[j] = a.tri_sgn[3 * t + jj];
             const T sg = (MODE % 2) ? T(sign[j]) : T(1);
 #pragma unroll
-            for (int n = 0; n < NV; ++n) uh[j][n] = sg * side_scaled(a, sj, MODE, n) * a.tprcn[3 * t + n];
+            for (int n = 0; n < NVn; ++n) uh[j][n] = sg * side_scaled<T, NVn>(a, sj, MODE, n) * a.tprcn[NVn * t + n];
         }
         T ms = ((MODE + 1) % 2) ? T(sign[0]) : T(1);
 #pragma unroll
@@ -684,7 +685,7 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
             for (int j = 0; j < 3; ++j) {
                 const T c = ms * a.lump[a.o_sfms + (MODE * SM + m) * 3 + j];
 #pragma unroll
-                for (int n = 0; n < NV; ++n) r[m][n] -= c * uh[j][n];
+                for (int n = 0; n < NVn; ++n) r[m][n] -= c * uh[j][n];
             }
             ms *= sign[0];
         }
@@ -692,11 +693,11 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
 #pragma unroll
     for (int m = MODE + 1; m < SM; ++m)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) a.ress[(s * a.sm0 + m) * NV + n] = r[m][n];
+        for (int n = 0; n < NVn; ++n) a.ress[(s * a.sm0 + m) * NVn + n] = r[m][n];
 }
 
 // store the solved side modes: res.s *= sprcn*sdiag(mode), sdirichlet (:245-267, :331-343)
-template <typename T, int P>
+template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
@@ -704,9 +705,9 @@ __global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
 #pragma unroll
     for (int m = 0; m < SM; ++m)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) {
-            const T x = side_scaled(a, s, m, n);
-            const int ix = (s * a.sm0 + m) * NV + n;
+        for (int n = 0; n < NVn; ++n) {
+            const T x = side_scaled<T, NVn>(a, s, m, n);
+            const int ix = (s * a.sm0 + m) * NVn + n;
             a.ress[ix] = x;
             if (a.ugs) a.ugs[ix] = a.ug0s[ix] - a.cfa * x;
         }
@@ -714,16 +715,16 @@ __global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
 
 // interiors: banded solve with the literal DPBTRSNU2 indexing (utilities/myblas.c:91-122),
 // divide by tprcn, subtract the boundary solution through bfmi (:347-370)
-template <typename T, int P>
+template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM, BM = Dims<P>::BM;
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= a.ntri) return;
-    T b[IM][NV];
+    T b[IM][NVn];
 #pragma unroll
     for (int k = 0; k < IM; ++k)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) b[k][n] = a.resi[(t * a.im0 + k) * NV + n];
+        for (int n = 0; n < NVn; ++n) b[k][n] = a.resi[(t * a.im0 + k) * NVn + n];
     const T* ab = a.lump + a.o_idiag;  // stride 2, ofdg 1
 #pragma unroll
     for (int i = 0; i < IM; ++i) {
@@ -731,26 +732,26 @@ __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
 #pragma unroll
         for (int j = frst; j < 1; ++j)
 #pragma unroll
-            for (int n = 0; n < NV; ++n) b[i][n] -= ab[i * 2 + j] * b[ba + j][n];
+            for (int n = 0; n < NVn; ++n) b[i][n] -= ab[i * 2 + j] * b[ba + j][n];
 #pragma unroll
-        for (int n = 0; n < NV; ++n) b[i][n] /= ab[i * 2 + 1];
+        for (int n = 0; n < NVn; ++n) b[i][n] /= ab[i * 2 + 1];
     }
 #pragma unroll
     for (int i = IM - 1; i >= 0; --i) {
 #pragma unroll
-        for (int n = 0; n < NV; ++n) b[i][n] /= ab[i * 2 + 1];
+        for (int n = 0; n < NVn; ++n) b[i][n] /= ab[i * 2 + 1];
         const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
 #pragma unroll
         for (int j = frst; j < 1; ++j)
 #pragma unroll
-            for (int n = 0; n < NV; ++n) b[ba + j][n] -= ab[i * 2 + j] * b[i][n];
+            for (int n = 0; n < NVn; ++n) b[ba + j][n] -= ab[i * 2 + j] * b[i][n];
     }
-    T uh[BM][NV];
+    T uh[BM][NVn];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int w = a.tri_pnt[3 * t + i];
 #pragma unroll
-        for (int n = 0; n < NV; ++n) uh[i][n] = a.resv[w * NV + n];
+        for (int n = 0; n < NVn; ++n) uh[i][n] = a.resv[w * NVn + n];
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -759,26 +760,26 @@ __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
 #pragma unroll
         for (int m = 0; m < SM; ++m) {
 #pragma unroll
-            for (int n = 0; n < NV; ++n) uh[3 + i * SM + m][n] = ms * a.ress[(sd * a.sm0 + m) * NV + n];
+            for (int n = 0; n < NVn; ++n) uh[3 + i * SM + m][n] = ms * a.ress[(sd * a.sm0 + m) * NVn + n];
             ms *= sgn;
         }
     }
 #pragma unroll
     for (int k = 0; k < IM; ++k) {
 #pragma unroll
-        for (int n = 0; n < NV; ++n) b[k][n] /= a.tprcn[3 * t + n];
+        for (int n = 0; n < NVn; ++n) b[k][n] /= a.tprcn[NVn * t + n];
 #pragma unroll
         for (int i = 0; i < BM; ++i) {
             const T c = a.lump[a.o_bfmi + i * IM + k];
 #pragma unroll
-            for (int n = 0; n < NV; ++n) b[k][n] -= c * uh[i][n];
+            for (int n = 0; n < NVn; ++n) b[k][n] -= c * uh[i][n];
         }
     }
 #pragma unroll
     for (int k = 0; k < IM; ++k)
 #pragma unroll
-        for (int n = 0; n < NV; ++n) {
-            const int ix = (t * a.im0 + k) * NV + n;
+        for (int n = 0; n < NVn; ++n) {
+            const int ix = (t * a.im0 + k) * NVn + n;
             a.resi[ix] = b[k][n];
             if (a.ugi) a.ugi[ix] = a.ug0i[ix] - a.cfa * b[k][n];
         }
@@ -787,33 +788,34 @@ __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
 // ---------------------------------------------------------------------------------
 // max |x| over (nent, nm active modes of mstride, NV), per variable, into out[NV] as
 // ordered bit patterns of non-negative doubles; mask (optional) skips essential entries
+template <int NVn>
 __global__ void k_absmax(long nent, int nm, int mstride, const double* __restrict__ x,
                          const unsigned char* __restrict__ mask, unsigned long long* out) {
-    __shared__ unsigned long long red[NV][256];
-    unsigned long long mx[NV] = {0ull, 0ull, 0ull};
+    __shared__ unsigned long long red[NVn][256];
+    unsigned long long mx[NVn] = {};
     const long span = (long)nm;
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < nent * span; k += (long)gridDim.x * blockDim.x) {
         const long ent = k / span, m = k - ent * span;
         const unsigned msk = mask ? mask[ent] : 0u;
-        for (int n = 0; n < NV; ++n) {
+        for (int n = 0; n < NVn; ++n) {
             if ((msk >> n) & 1) continue;
-            const double v = fabs(x[(ent * mstride + m) * NV + n]);
+            const double v = fabs(x[(ent * mstride + m) * NVn + n]);
             const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
             mx[n] = bits > mx[n] ? bits : mx[n];
         }
     }
-    for (int n = 0; n < NV; ++n) red[n][threadIdx.x] = mx[n];
+    for (int n = 0; n < NVn; ++n) red[n][threadIdx.x] = mx[n];
     __syncthreads();
     for (int st = blockDim.x / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st)
-            for (int n = 0; n < NV; ++n) {
+            for (int n = 0; n < NVn; ++n) {
                 const unsigned long long o = red[n][threadIdx.x + st];
                 red[n][threadIdx.x] = o > red[n][threadIdx.x] ? o : red[n][threadIdx.x];
             }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int n = 0; n < NV; ++n) atomicMax(&out[n], red[n][0]);
+        for (int n = 0; n < NVn; ++n) atomicMax(&out[n], red[n][0]);
 }
 
 // MFMA lane-layout self test: D = A(16x4) * B(4x16) with the layout the element

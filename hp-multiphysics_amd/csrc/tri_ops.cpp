@@ -236,10 +236,38 @@ TriOps::TriOps(int p_) : p(p_) {
     g1.assign(gpx * n1, 0.0); dg1 = g1; w1 = wx;
     for (int i = 0; i < gpx; ++i)
         for (int k = 0; k < n1; ++k) {
-            const int xm = (k < 2) ? k + 1 : k + 1;
-            g1[i * n1 + k] = gxv[i * nmodx + xm];
-            dg1[i * n1 + k] = dgxv[i * nmodx + xm];
+            g1[i * n1 + k] = gxv[i * nmodx + k + 1];
+            dg1[i * n1 + k] = dgxv[i * nmodx + k + 1];
         }
+
+    // 1-D edge mass matrix and the banded Cholesky factor of its side block
+    {
+        std::vector<double> mm(n1 * n1, 0.0);
+        for (int a = 0; a < n1; ++a)
+            for (int c = 0; c < n1; ++c) {
+                double acc = 0.0;
+                for (int i = 0; i < gpx; ++i) acc += wx[i] * g1[i * n1 + a] * g1[i * n1 + c];
+                mm[a * n1 + c] = acc;
+            }
+        const int kd = 2, ld = 3;
+        sdiag1d.assign(std::max(sm, 1) * ld, 0.0);
+        for (int j = 0; j < sm; ++j)
+            for (int i = std::max(0, j - kd); i <= j; ++i) sdiag1d[j * ld + kd + i - j] = mm[(i + 2) * n1 + j + 2];
+        for (int j = 0; j < sm; ++j) {
+            const int i0 = std::max(0, j - kd);
+            for (int i = i0; i <= j; ++i) {
+                double acc = sdiag1d[j * ld + kd + i - j];
+                for (int k = std::max(i0, std::max(0, i - kd)); k < i; ++k)
+                    acc -= sdiag1d[i * ld + kd + k - i] * sdiag1d[j * ld + kd + k - j];
+                if (i == j) {
+                    if (!(acc > 0.0)) throw std::runtime_error("edge mass matrix not positive definite");
+                    sdiag1d[j * ld + kd] = std::sqrt(acc);
+                } else {
+                    sdiag1d[j * ld + kd + i - j] = acc / sdiag1d[i * ld + kd];
+                }
+            }
+        }
+    }
 
     // ---- lumped mass inversion (tri_basis/initialize.cpp:590-958) ------------------------
     mass.assign(tm * tm, 0.0);

@@ -28,6 +28,9 @@ struct TriOps {
     std::vector<double> I, Ir, Is;   // tm x G
     std::vector<double> Dr, Ds;      // G x G
     std::vector<double> g1, dg1, w1; // edge basis: gpx x (sm+2) values / d/dxi, weights
+    // upper band (bandwidth 2, row j = column j of U: U(j-2,j), U(j-1,j), U(j,j)) Cholesky
+    // factor of the side-mode block of the edge mass matrix (tri_basis/initialize.cpp:1097-1130)
+    std::vector<double> sdiag1d;
     // lumped-mass inversion factors, layouts as in tri_basis.h:124-141
     double vdiag;
     std::vector<double> sfmv, ifmb, vfms, sdiag, sfms, bfmi, idiag;

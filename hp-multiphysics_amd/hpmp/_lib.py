@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libhpgpu.so")
 
 HPG_PHYSICS_INS = 0
+HPG_PHYSICS_CNS = 1
 HPG_FP64 = 0
 HPG_FP32_DEFECT = 1
 HPG_PRCN_ANALYTIC = 0
@@ -29,6 +30,7 @@ EXPORTS = (
     "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
+    "hpg_get_block_preconditioner",
 )
 
 
@@ -48,6 +50,7 @@ class ParamsC(C.Structure):
         ("rho", C.c_double), ("mu", C.c_double), ("adis", C.c_double),
         ("bc_pressure", C.c_double), ("fadd", C.c_double),
         ("nstage", C.c_int), ("alpha", C.c_double * 8), ("beta", C.c_double * 8), ("cfl", C.c_double * 4),
+        ("gamma", C.c_double), ("prandtl", C.c_double), ("Rgas", C.c_double), ("bc", C.c_double * 4),
     ]
 
 
@@ -73,6 +76,7 @@ def load():
     for nm in ("hpg_set_state", "hpg_get_state", "hpg_get_residual", "hpg_set_residual", "hpg_get_preconditioner"):
         getattr(L, nm).argtypes = [vp, dp, dp, dp]
     L.hpg_set_tprcn.argtypes = [vp, dp]
+    L.hpg_get_block_preconditioner.argtypes = [vp, dp, dp, dp]
     L.hpg_rsdl.argtypes = [vp, C.c_int]
     for nm in ("hpg_setup_preconditioner", "hpg_minvrt", "hpg_update", "hpg_mg_restrict", "hpg_mg_prolongate",
                "hpg_synchronize"):

@@ -39,10 +39,11 @@ def rel_diff_mod_pressure(a, b) -> float:
 
 class TriHpIns:
     NV = 3
+    PHYSICS = _lib.HPG_PHYSICS_INS
 
     def __init__(self, mesh: TriMesh, log2p: int = 2, rho: float = 1.0, mu: float = 0.0, adis: float = 1.0,
                  bc_pressure: float = 0.0, fadd: float = 1.0, alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA,
-                 cfl=CFL_DEFAULT):
+                 cfl=CFL_DEFAULT, **extra):
         self.L = _lib.load()
         self.mesh = mesh
         self.log2pmax = int(log2p)
@@ -59,7 +60,7 @@ class TriHpIns:
         cm = _lib.TriMeshC(m.npnt, m.nseg, m.ntri, *(C.c_void_p(a.ctypes.data) for a in k[:5]), m.nebd,
                            *(C.c_void_p(a.ctypes.data) for a in k[5:]))
         prm = _lib.ParamsC()
-        prm.physics = _lib.HPG_PHYSICS_INS
+        prm.physics = self.PHYSICS
         prm.log2p = self.log2pmax
         prm.rho, prm.mu, prm.adis, prm.bc_pressure, prm.fadd = rho, mu, adis, bc_pressure, fadd
         prm.nstage = self.nstage
@@ -69,12 +70,17 @@ class TriHpIns:
         for lvl in range(4):
             prm.cfl[lvl] = cfl[lvl] if lvl < len(cfl) else 0.5
         self.params = dict(rho=rho, mu=mu, adis=adis, bc_pressure=bc_pressure, fadd=fadd,
-                           alpha=tuple(alpha), beta=tuple(beta), cfl=tuple(cfl))
+                           alpha=tuple(alpha), beta=tuple(beta), cfl=tuple(cfl), **extra)
+        self._fill_extra(prm, **extra)
         h = C.c_void_p()
         rc = self.L.hpg_create(C.byref(cm), C.byref(prm), C.byref(h))
         if rc != 0 or not h.value:
             raise _lib.HpgError(f"hpg_create failed ({rc})")
         self.h = h
+
+    def _fill_extra(self, prm, **extra):
+        if extra:
+            raise TypeError(f"unexpected arguments {sorted(extra)}")
 
     @classmethod
     def from_input_map(cls, mesh: TriMesh, inmap: InputMap, prefix: str = "b0") -> "TriHpIns":
@@ -132,7 +138,7 @@ class TriHpIns:
 
     def get_preconditioner(self):
         m = self.mesh
-        vp, sp, tp = np.zeros((m.npnt, 3)), np.zeros((m.nseg, 3)), np.zeros((m.ntri, 3))
+        vp, sp, tp = np.zeros((m.npnt, self.NV)), np.zeros((m.nseg, self.NV)), np.zeros((m.ntri, self.NV))
         self._check(self.L.hpg_get_preconditioner(self.h, _lib.ptr(vp), _lib.ptr(sp), _lib.ptr(tp)), "get_prcn")
         return vp, sp, tp
 
@@ -154,7 +160,7 @@ class TriHpIns:
         self._check(self.L.hpg_update(self.h), "update")
 
     def maxres(self):
-        pv = np.zeros(3)
+        pv = np.zeros(self.NV)
         e = np.zeros(1)
         self._check(self.L.hpg_maxres(self.h, _lib.ptr(pv), _lib.ptr(e)), "maxres")
         return float(e[0]), pv
@@ -227,3 +233,33 @@ class TriHpIns:
             for _ in range(iterrfne):
                 self.update()
         return error
+
+
+CNS_CFL_DEFAULT = (2.5, 1.5, 1.0)
+
+
+class TriHpCns(TriHpIns):
+    """tri_hp_cns on the GPU (tri_hp/cns/tri_hp_cns.h:19-72): primitive unknowns
+    (p, u, v, RT), NV = 4.  Boundaries with essential mask 0b1110 are inflow
+    boundaries at the constant state `bc` = (p, u, v, RT), which also gives the
+    boundary pressure of the edge flux (tri_hp/cns/bdry.cpp:192-245)."""
+
+    NV = 4
+    PHYSICS = _lib.HPG_PHYSICS_CNS
+
+    def __init__(self, mesh: TriMesh, log2p: int = 2, mu: float = 1e-3, adis: float = 1.0, gamma: float = 1.4,
+                 prandtl: float = 0.713, Rgas: float = 1.0, bc=(1.0, 0.5, 0.0, 1.0), fadd: float = 1.0,
+                 alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA, cfl=CNS_CFL_DEFAULT):
+        super().__init__(mesh, log2p=log2p, rho=1.0, mu=mu, adis=adis, bc_pressure=bc[0], fadd=fadd, alpha=alpha,
+                         beta=beta, cfl=cfl, gamma=gamma, prandtl=prandtl, Rgas=Rgas, bc=tuple(bc))
+
+    def _fill_extra(self, prm, gamma, prandtl, Rgas, bc):
+        prm.gamma, prm.prandtl, prm.Rgas = gamma, prandtl, Rgas
+        for n in range(4):
+            prm.bc[n] = bc[n]
+
+    def get_block_preconditioner(self):
+        m = self.mesh
+        out = [np.zeros((m.npnt, 4, 4)), np.zeros((m.nseg, 4, 4)), np.zeros((m.ntri, 4, 4))]
+        self._check(self.L.hpg_get_block_preconditioner(self.h, *(_lib.ptr(x) for x in out)), "get_block_preconditioner")
+        return out

@@ -40,21 +40,31 @@ typedef struct {
     const int *ebdry_essential;
 } hpg_tri_mesh;
 
-enum { HPG_PHYSICS_INS = 0 };
+/* HPG_PHYSICS_INS: tri_hp_ins, unknowns (u, v, p), NV = 3.
+ * HPG_PHYSICS_CNS: tri_hp_cns, primitive unknowns (p, u, v, RT), NV = 4
+ * (tri_hp/cns/tri_hp_cns.h:19-72); every array below then carries 4 values per
+ * vertex, side mode and interior mode.  Boundaries whose essential mask is 0b1110
+ * (u, v, RT fixed) are the reference's cns "inflow" boundaries
+ * (tri_hp/cns/bdry_cns.h:56-66) with the constant state bc[]. */
+enum { HPG_PHYSICS_INS = 0, HPG_PHYSICS_CNS = 1 };
 enum { HPG_FP64 = 0, HPG_FP32_DEFECT = 1 };
 enum { HPG_PRCN_ANALYTIC = 0, HPG_PRCN_LEARNED = 1 };
 
 /* Input-map values the hot path reads (tri_hp/ins/allocate.cpp:13-44,
  * tri_mesh/blocks.cpp:1126-1141, tri_hp/allocate.cpp:275-276, :198). */
 typedef struct {
-    int physics;             /* HPG_PHYSICS_INS */
+    int physics;             /* HPG_PHYSICS_INS or HPG_PHYSICS_CNS */
     int log2p;               /* 0,1,2 -> p = 1,2,4 */
-    double rho, mu, adis;    /* b0_rho, b0_mu, b0_dissipation */
-    double bc_pressure;      /* pressure of the freestream ibc on edge fluxes */
+    double rho, mu, adis;    /* b0_rho (ins), b0_mu, b0_dissipation */
+    double bc_pressure;      /* ins: pressure of the freestream ibc on edge fluxes */
     double fadd;             /* hp_fadd */
     int nstage;              /* <= 7 */
     double alpha[8], beta[8];/* stage coefficients; entry nstage is set to 1 */
     double cfl[4];           /* per log2p */
+    /* cns only (tri_hp/cns/allocate.cpp:32-45): gamma, prandtl, Rgas, and the
+     * inflow / boundary-flux state (p, u, v, RT) of the ibc */
+    double gamma, prandtl, Rgas;
+    double bc[4];
 } hpg_params;
 
 /* replaces tri_hp::init / tri_hp_ins::init (tri_hp/allocate.cpp, tri_hp/ins/allocate.cpp:13-44) */
@@ -70,18 +80,26 @@ int hpg_set_residual(hpg_ctx *ctx, const double *resv, const double *ress, const
 /* diagonal preconditioner (vprcn, sprcn, tprcn of tri_hp::hp_global, tri_hp.h:138-139) */
 int hpg_get_preconditioner(hpg_ctx *ctx, double *vprcn, double *sprcn, double *tprcn);
 int hpg_set_tprcn(hpg_ctx *ctx, const double *tprcn);
+/* cns block preconditioners, 16 doubles (row-major 4x4) per entity: the vertex and
+ * side sums and the per-element tpreconditioner (tri_hp_cns::hp_cns_global::
+ * vpreconditioner/spreconditioner/tpreconditioner, tri_hp/cns/tri_hp_cns.h:33) */
+int hpg_get_block_preconditioner(hpg_ctx *ctx, double *vpc, double *spc, double *tpc);
 
 /* replaces tri_hp::rsdl(int stage) (tri_hp/nstage.cpp:12-110) with
- * tri_hp_ins::element_rsdl (tri_hp/ins/rsdl.cpp:17-436) and the edge-boundary
- * fluxes (tri_hp/hp_boundary.cpp:973-1049); stage == nstage is the full residual */
+ * tri_hp_ins::element_rsdl (tri_hp/ins/rsdl.cpp:17-436) or tri_hp_cns::element_rsdl
+ * (tri_hp/cns/rsdl.cpp:16-535) and the edge-boundary fluxes
+ * (tri_hp/hp_boundary.cpp:973-1049); stage == nstage is the full residual */
 int hpg_rsdl(hpg_ctx *ctx, int stage);
 /* replaces tri_hp_ins::setup_preconditioner (tri_hp/ins/setup_preconditioner.cpp:9-310)
+ * or tri_hp_cns::setup_preconditioner (tri_hp/cns/setup_preconditioner.cpp:7-239)
  * + tri_hp::setup_preconditioner (tri_hp/setup_preconditioner.cpp:9-96); returns the
  * error count (non-positive element size or NaN in the solution) */
 int hpg_setup_preconditioner(hpg_ctx *ctx);
 /* replaces tri_hp::minvrt (tri_hp/setup_preconditioner.cpp:102-373) */
 int hpg_minvrt(hpg_ctx *ctx);
-/* replaces tri_hp::update (tri_hp/nstage.cpp:256-427) */
+/* replaces tri_hp::update (tri_hp/nstage.cpp:256-427) or, for cns, tri_hp_cns::update
+ * (tri_hp/cns/update.cpp:276-357: minvrt, inflow modify_boundary_residual,
+ * project_new_variables, stage update) */
 int hpg_update(hpg_ctx *ctx);
 /* replaces tri_hp::maxres (tri_hp/allocate.cpp:496-525); per_var may be NULL */
 int hpg_maxres(hpg_ctx *ctx, double *per_var, double *out);

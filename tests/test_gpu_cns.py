@@ -1,0 +1,123 @@
+"""GPU parity of the tri_hp_cns path: libhpgpu.so (through the C-ABI) against the
+CPU oracle.  Residuals to 1e-12 relative in max norm; every test prints its case
+before touching the GPU."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+RSDL_TOL = 1e-12
+INFLOW = 0b1110
+
+
+def _case(tag):
+    print(f"\n[gpu-case] {tag}", flush=True)
+
+
+def _pair(oracle_mod, n, log2p, jitter=0.2, seed=1, mu=1e-2, state_seed=11, amp=0.05, width=0.2, noise=2e-2):
+    from hpmp import mesh as M, workloads as W
+    from hpmp.tri_hp import TriHpCns
+    m = M.unit_square(n, jitter, seed, essential_mask=INFLOW)
+    st, bc = W.cns_gaussian_state(m, 1 << log2p, seed=state_seed, amp=amp, width=width, noise=noise)
+    G = TriHpCns(m, log2p=log2p, mu=mu, bc=bc)
+    O = oracle_mod.TriHpCns(m, log2p=log2p, mu=mu, bc=bc)
+    G.set_state(*st)
+    O.set_state(*st)
+    return m, st, G, O
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
+@pytest.mark.parametrize("stage", [5, 0, 1, 2])
+def test_cns_rsdl_parity(oracle_mod, log2p, stage):
+    _case(f"cns rsdl log2p={log2p} stage={stage}")
+    m, st, G, O = _pair(oracle_mod, 8, log2p, seed=3)
+    for s in range(stage + 1) if stage < 5 else [5]:
+        G.rsdl(s); O.rsdl(s)
+    err = rel_err(G.get_residual(), O.get_res())
+    print("rel err", err, flush=True)
+    assert err < RSDL_TOL
+
+
+def test_cns_freestream_residual_vanishes_full_size():
+    _case("cns free stream 724x724 p=4 (1 048 352 triangles)")
+    from hpmp import mesh as M, workloads as W
+    from hpmp.tri_hp import TriHpCns
+    m = M.unit_square(724, 0.2, 1, essential_mask=INFLOW)
+    st, bc = W.cns_freestream(m, 4, mach=0.5)
+    G = TriHpCns(m, log2p=2, mu=1e-3, bc=bc)
+    G.set_state(*st)
+    G.rsdl()
+    mx = max(np.abs(a).max() for a in G.get_residual() if a.size)
+    print("max |res|", mx, flush=True)
+    assert mx < 1e-13
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
+def test_cns_setup_preconditioner_parity(oracle_mod, log2p):
+    _case(f"cns setup_preconditioner log2p={log2p}")
+    m, st, G, O = _pair(oracle_mod, 7, log2p, seed=4)
+    assert G.setup_preconditioner() == 0 and O.setup_preconditioner() == 0
+    gv, gs, gt = G.get_preconditioner()
+    gvp, gsp, gtp = G.get_block_preconditioner()
+    ov, os_, ot, ovp, osp, otp = O.get_prcn()
+    assert rel_err([gt], [ot]) < 1e-13
+    assert rel_err([gv], [ov]) < 1e-13
+    assert rel_err([gtp], [otp]) < 1e-12
+    assert rel_err([gvp], [ovp]) < 1e-12
+    if log2p:
+        assert rel_err([gs], [os_]) < 1e-13
+        assert rel_err([gsp], [osp]) < 1e-12
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
+def test_cns_minvrt_parity(oracle_mod, log2p):
+    _case(f"cns minvrt log2p={log2p}")
+    m, st, G, O = _pair(oracle_mod, 7, log2p, seed=6)
+    G.setup_preconditioner(); O.setup_preconditioner()
+    O.rsdl()
+    G.set_residual(*O.get_res())
+    O.minvrt(); G.minvrt()
+    assert rel_err(G.get_residual(), O.get_res()) < RSDL_TOL
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
+def test_cns_update_parity(oracle_mod, log2p):
+    _case(f"cns update x3 log2p={log2p}")
+    m, st, G, O = _pair(oracle_mod, 7, log2p, seed=8)
+    for _ in range(3):
+        assert G.setup_preconditioner() == 0 and O.setup_preconditioner() == 0
+        G.update(); O.update()
+    err_s = rel_err(G.get_state(), O.get_state())
+    err_r = rel_err(G.get_residual(), O.get_res())
+    print("state rel err", err_s, "residual rel err", err_r, flush=True)
+    assert err_s < 1e-12
+    assert err_r < 1e-10
+
+
+def test_cns_pmg_cycle_parity(oracle_mod):
+    _case("cns p-multigrid cycle x3 log2p=2 mglvls=3")
+    m, st, G, O = _pair(oracle_mod, 7, 2, seed=10)
+    for _ in range(3):
+        eg = G.cycle(vw=1, mglvls=3, itercrsn=1, iterrfne=1)
+        eo = O.cycle(vw=1, mglvls=3, itercrsn=1, iterrfne=1)
+        assert abs(eg - eo) <= 1e-10 * abs(eo)
+    assert G.log2p == 2
+    assert rel_err(G.get_state(), O.get_state()) < 1e-11
+
+
+def test_cns_rsdl_is_deterministic():
+    _case("cns rsdl run-to-run bitwise log2p=2")
+    from hpmp import mesh as M, workloads as W
+    from hpmp.tri_hp import TriHpCns
+    m = M.unit_square(24, 0.2, 2, essential_mask=INFLOW)
+    st, bc = W.cns_gaussian_state(m, 4, seed=1, noise=1e-2)
+    G = TriHpCns(m, log2p=2, mu=1e-3, bc=bc)
+    G.set_state(*st)
+    G.rsdl()
+    a = G.get_residual()
+    for _ in range(2):
+        G.rsdl()
+        b = G.get_residual()
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
