@@ -15,13 +15,14 @@ def _case(tag):
     print(f"\n[gpu-case] {tag}", flush=True)
 
 
-def _pair(oracle_mod, n, log2p, jitter=0.2, seed=1, mu=1e-2, state_seed=11, amp=0.05, width=0.2, noise=2e-2):
+def _pair(oracle_mod, n, log2p, jitter=0.2, seed=1, mu=1e-2, state_seed=11, amp=0.05, width=0.2, noise=2e-2,
+          cfl=(2.5, 1.5, 1.0)):
     from hpmp import mesh as M, workloads as W
     from hpmp.tri_hp import TriHpCns
     m = M.unit_square(n, jitter, seed, essential_mask=INFLOW)
     st, bc = W.cns_gaussian_state(m, 1 << log2p, seed=state_seed, amp=amp, width=width, noise=noise)
-    G = TriHpCns(m, log2p=log2p, mu=mu, bc=bc)
-    O = oracle_mod.TriHpCns(m, log2p=log2p, mu=mu, bc=bc)
+    G = TriHpCns(m, log2p=log2p, mu=mu, bc=bc, cfl=cfl)
+    O = oracle_mod.TriHpCns(m, log2p=log2p, mu=mu, bc=bc, cfl=cfl)
     G.set_state(*st)
     O.set_state(*st)
     return m, st, G, O
@@ -83,8 +84,11 @@ def test_cns_minvrt_parity(oracle_mod, log2p):
 
 @pytest.mark.parametrize("log2p", [0, 1, 2])
 def test_cns_update_parity(oracle_mod, log2p):
-    _case(f"cns update x3 log2p={log2p}")
-    m, st, G, O = _pair(oracle_mod, 7, log2p, seed=8)
+    # the explicit cns relaxation of the reference (the reference's cns decks drive
+    # PETSc instead) leaves the admissible states at the default CFL on this mesh:
+    # the trajectory is compared at CFL 0.01, where three updates stay admissible
+    _case(f"cns update x3 log2p={log2p} cfl=0.01")
+    m, st, G, O = _pair(oracle_mod, 7, log2p, seed=8, cfl=(0.01, 0.01, 0.01))
     for _ in range(3):
         assert G.setup_preconditioner() == 0 and O.setup_preconditioner() == 0
         G.update(); O.update()
@@ -96,8 +100,8 @@ def test_cns_update_parity(oracle_mod, log2p):
 
 
 def test_cns_pmg_cycle_parity(oracle_mod):
-    _case("cns p-multigrid cycle x3 log2p=2 mglvls=3")
-    m, st, G, O = _pair(oracle_mod, 7, 2, seed=10)
+    _case("cns p-multigrid cycle x3 log2p=2 mglvls=3 cfl=0.01")
+    m, st, G, O = _pair(oracle_mod, 7, 2, seed=10, cfl=(0.01, 0.01, 0.01))
     for _ in range(3):
         eg = G.cycle(vw=1, mglvls=3, itercrsn=1, iterrfne=1)
         eo = O.cycle(vw=1, mglvls=3, itercrsn=1, iterrfne=1)
