@@ -11,6 +11,8 @@ Also reported (same JSON line):
 * ``mixed``: the same steps with the FP32 linearised residual inside FP64 defect
   correction (one FP64 residual + ninner FP32 multistage iterations per update),
   and the FP64 judge after the same number of steps for both precisions;
+* ``cns``: the compressible path on the 724 x 724 p = 4 workload (FP64 step
+  throughput and the element kernel's rate);
 * ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual with the
   p-multigrid cycle (p = 4 -> 2 -> 1) on a smaller mesh of the same family, for
   FP64 plain, mixed, and mixed + learned preconditioner (training time apart),
@@ -43,6 +45,58 @@ FLOP_INV_P4 = 7250.0
 # elements per launch on average): FETCH_SIZE and WRITE_SIZE counter passes scaled by
 # the same counters over a plain 1 GiB copy; profiles/traffic_ins_p4_512x512_pmc.md
 TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 219.7e6
+
+
+# tri_hp cns p = 4 full residual, algorithmic flops per element: transforms 30 G T +
+# 12 g^3 MAC = 12 750 MAC (25 500 flop) plus about 700 flop of pointwise tau, flux
+# Jacobians and stabilisation at each of the 25 Gauss points
+FLOP_CNS_FULL_P4 = 43000.0
+
+
+def cns_throughput(args) -> dict:
+    """tri_hp cns on the 724 x 724 p = 4 workload (Mach 0.5 free stream + Gaussian density
+    and temperature bumps): FP64 step throughput and the element kernel's rate."""
+    from hpmp import mesh as M, workloads as W
+    from hpmp.tri_hp import TriHpCns
+
+    n = args.cns_n
+    mesh = M.unit_square(n, 0.2, 1, essential_mask=0b1110)
+    st, bc = W.cns_gaussian_state(mesh, 4, seed=3)
+    # CFL 0: every step does the full work (5 residual evaluations, minvrt, inflow
+    # residual, primitive projection) on the same admissible state; the reference's
+    # explicit cns relaxation leaves admissible states at any fixed CFL on this fine
+    # unit-scale mesh, and the arithmetic per step does not depend on the step size
+    g = TriHpCns(mesh, log2p=2, mu=1e-3, bc=bc, cfl=(0.0, 0.0, 0.0))
+    g.set_state(*st)
+    unknowns = 4 * (mesh.npnt + mesh.nseg * 3 + mesh.ntri * 3)
+    g.setup_preconditioner()
+    g.update()
+    g.set_state(*st)
+    g.synchronize()
+    g.L.hpg_kernel_timing(g.h, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.cns_steps):
+        if g.setup_preconditioner():
+            raise SystemExit("cns setup_preconditioner error")
+        g.update()
+    g.synchronize()
+    t1 = time.perf_counter()
+    tot = C.c_double(); nl = C.c_longlong(); ne = C.c_longlong()
+    g.L.hpg_kernel_timing_read(g.h, 1, C.byref(tot), C.byref(nl), C.byref(ne))
+    g.L.hpg_kernel_timing(g.h, 0)
+    ach = FLOP_CNS_FULL_P4 * ne.value / (tot.value * 1e-3) / 1e12 if tot.value > 0 else 0.0
+    out = {"workload": f"tri_hp cns {n}x{n} squares -> {mesh.ntri} triangles (jitter 0.2h), p=4, mu=1e-3, Mach 0.5 + "
+                       "Gaussian rho/T bumps (amplitude 1e-2, width 0.05), inflow on all sides; step = setup_preconditioner + "
+                       "5-stage update at CFL 0 (full work, state held)",
+           "unknowns": unknowns, "steps": args.cns_steps,
+           "ms_per_step": 1e3 * (t1 - t0) / args.cns_steps,
+           "dof_evals_per_s": unknowns * 5 * args.cns_steps / (t1 - t0),
+           "roofline": {"bound": "fp64 valu", "kernel": "k_elem_cns<4,full>", "achieved": ach,
+                        "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
+                        "avg_launch_ms": tot.value / max(1, nl.value),
+                        "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_CNS_FULL_P4}}
+    g.close()
+    return out
 
 
 def time_to_tol(args) -> dict:
@@ -113,6 +167,9 @@ def main() -> int:
     ap.add_argument("--tts-n", type=int, default=16)
     ap.add_argument("--tts-mu", type=float, default=5e-2)
     ap.add_argument("--tts-max-cycles", type=int, default=6000)
+    ap.add_argument("--no-cns", action="store_true")
+    ap.add_argument("--cns-n", type=int, default=724)
+    ap.add_argument("--cns-steps", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
@@ -231,6 +288,8 @@ def main() -> int:
         out["mixed"] = mixed
     if not args.no_tts:
         out["time_to_tol"] = time_to_tol(args)
+    if not args.no_cns:
+        out["cns"] = cns_throughput(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
