@@ -155,6 +155,12 @@ struct hpg_ctx {
     double *vpc = nullptr, *spc = nullptr, *tpc = nullptr, *rtv = nullptr, *rts = nullptr;
     int *inflow_v = nullptr, *inflow_s = nullptr;
     int n_inflow_v = 0, n_inflow_s = 0;
+    // h-multigrid: this context is a coarse mesh level linked to `fine`
+    hpg_ctx* fine = nullptr;
+    bool coarse_level = false;
+    hipEvent_t xev = nullptr;  // orders this context's stream against a neighbour level's
+    int *hr_ptr = nullptr, *hr_fidx = nullptr, *hi_fv = nullptr, *hp_cv = nullptr;
+    double *hr_wt = nullptr, *hi_wt = nullptr, *hp_wt = nullptr, *vug_frst = nullptr;
     std::vector<void*> allocs;
     double* pnts = nullptr;
     int *tri_pnt = nullptr, *tri_seg = nullptr, *tri_sgn = nullptr, *seg_pnt = nullptr;
@@ -221,6 +227,7 @@ struct hpg_ctx {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
         for (hipEvent_t e : events) hipEventDestroy(e);
+        if (xev) hipEventDestroy(xev);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -1178,8 +1185,88 @@ int hpg_resnorm(hpg_ctx* c, double* out) {
     return 0;
 }
 
+int hpg_link_fine(hpg_ctx* c, hpg_ctx* f, const int* cc_tri, const double* cc_wt, const int* fc_tri,
+                  const double* fc_wt) {
+    if (!c || !f || c == f || !cc_tri || !cc_wt || !fc_tri || !fc_wt) return -2;
+    if (c->nv != f->nv || c->physics != f->physics || c->log2pmax != 0) {
+        c->err = "a coarse level has the fine level's physics and runs at p = 1";
+        return -2;
+    }
+    HPG_TRY(c, {
+        const int nf = f->npnt, nc = c->npnt;
+        std::vector<int> th(3 * (size_t)c->ntri);
+        check(hipMemcpy(th.data(), c->tri_pnt, th.size() * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        std::vector<int> fh(3 * (size_t)f->ntri);
+        check(hipMemcpy(fh.data(), f->tri_pnt, fh.size() * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        // restriction gather: coarse vertex <- (fine vertex, weight), ascending fine vertex
+        std::vector<int> ptr(nc + 1, 0), cv(3 * (size_t)nf);
+        for (int i = 0; i < nf; ++i) {
+            const int t = cc_tri[i];
+            if (t < 0 || t >= c->ntri) throw std::runtime_error("ccnnct triangle out of range");
+            for (int j = 0; j < 3; ++j) { cv[3 * i + j] = th[3 * t + j]; ptr[th[3 * t + j] + 1]++; }
+        }
+        for (int v = 0; v < nc; ++v) ptr[v + 1] += ptr[v];
+        std::vector<int> fidx(ptr[nc]);
+        std::vector<double> wt(ptr[nc]);
+        {
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (int i = 0; i < nf; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const int v = cv[3 * i + j];
+                    fidx[fill[v]] = i;
+                    wt[fill[v]] = cc_wt[3 * i + j];
+                    fill[v]++;
+                }
+        }
+        std::vector<int> fv(3 * (size_t)nc);
+        for (int v = 0; v < nc; ++v) {
+            const int t = fc_tri[v];
+            if (t < 0 || t >= f->ntri) throw std::runtime_error("fcnnct triangle out of range");
+            for (int j = 0; j < 3; ++j) fv[3 * v + j] = fh[3 * t + j];
+        }
+        c->hr_ptr = c->upload(ptr.data(), ptr.size());
+        c->hr_fidx = c->upload(fidx.data(), fidx.size());
+        c->hr_wt = c->upload(wt.data(), wt.size());
+        c->hi_fv = c->upload(fv.data(), fv.size());
+        c->hi_wt = c->upload(fc_wt, 3 * (size_t)nc);
+        c->hp_cv = c->upload(cv.data(), cv.size());
+        c->hp_wt = c->upload(cc_wt, 3 * (size_t)nf);
+        c->vug_frst = c->alloc<double>((size_t)nc * c->nv);
+        if (!c->xev) check(hipEventCreateWithFlags(&c->xev, hipEventDisableTiming), "hipEventCreate");
+        if (!f->xev) check(hipEventCreateWithFlags(&f->xev, hipEventDisableTiming), "hipEventCreate");
+        check(hipStreamSynchronize(c->stream), "sync");
+        c->fine = f;
+        c->coarse_level = true;
+    });
+    return 0;
+}
+
 int hpg_mg_restrict(hpg_ctx* c) {
-    if (!c || c->log2p == 0) return -2;
+    if (!c) return -2;
+    if (c->coarse_level) {
+        // tri_hp/movco.cpp:41-102: fine residual and state onto this coarse mesh
+        HPG_TRY(c, {
+            hpg_ctx* f = c->fine;
+            check(hipEventRecord(f->xev, f->stream), "hipEventRecord");
+            check(hipStreamWaitEvent(c->stream, f->xev, 0), "hipStreamWaitEvent");
+            c->isfrst = true;
+            c->coarse_flag = true;
+            if (c->nv == NVC) {
+                hipLaunchKernelGGL(k_hrestrict<NVC>, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->hr_ptr,
+                                   c->hr_fidx, c->hr_wt, f->resv, c->r0v);
+                hipLaunchKernelGGL(k_hinterp<NVC>, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->hi_fv,
+                                   c->hi_wt, f->ugv, c->ugv, c->vug_frst);
+            } else {
+                hipLaunchKernelGGL(k_hrestrict<NV>, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->hr_ptr,
+                                   c->hr_fidx, c->hr_wt, f->resv, c->r0v);
+                hipLaunchKernelGGL(k_hinterp<NV>, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->hi_fv,
+                                   c->hi_wt, f->ugv, c->ugv, c->vug_frst);
+            }
+            check(hipGetLastError(), "h restrict");
+        });
+        return 0;
+    }
+    if (c->log2p == 0) return -2;
     HPG_TRY(c, {
         c->isfrst = true;
         c->coarse_flag = true;
@@ -1191,7 +1278,26 @@ int hpg_mg_restrict(hpg_ctx* c) {
 }
 
 int hpg_mg_prolongate(hpg_ctx* c) {
-    if (!c || c->log2p >= c->log2pmax) return -2;
+    if (!c) return -2;
+    if (c->coarse_level) {
+        // tri_hp/movfin.cpp:16-82: coarse correction interpolated onto the fine vertices
+        HPG_TRY(c, {
+            hpg_ctx* f = c->fine;
+            const long n = (long)c->npnt * c->nv;
+            hipLaunchKernelGGL(k_hcorr, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->vug_frst, c->ugv);
+            if (c->nv == NVC)
+                hipLaunchKernelGGL(k_hprolong<NVC>, dim3(nblk(f->npnt)), dim3(256), 0, c->stream, f->npnt, c->hp_cv,
+                                   c->hp_wt, c->vug_frst, f->resv, f->ugv);
+            else
+                hipLaunchKernelGGL(k_hprolong<NV>, dim3(nblk(f->npnt)), dim3(256), 0, c->stream, f->npnt, c->hp_cv,
+                                   c->hp_wt, c->vug_frst, f->resv, f->ugv);
+            check(hipGetLastError(), "h prolongate");
+            check(hipEventRecord(c->xev, c->stream), "hipEventRecord");
+            check(hipStreamWaitEvent(f->stream, c->xev, 0), "hipStreamWaitEvent");
+        });
+        return 0;
+    }
+    if (c->log2p >= c->log2pmax) return -2;
     ++c->log2p;
     if (c->log2p == c->log2pmax) c->coarse_flag = false;
     return 0;

@@ -818,6 +818,79 @@ __global__ void k_absmax(long nent, int nm, int mstride, const double* __restric
         for (int n = 0; n < NVn; ++n) atomicMax(&out[n], red[n][0]);
 }
 
+// ---------------------------------------------------------------------------------
+// h-multigrid transfers between a mesh and its coarse neighbour (tri_hp/movco.cpp:41-102,
+// tri_hp/movfin.cpp:16-82), vertex values only (coarse levels run at p = 1)
+
+// res0(coarse v) = sum over the fine vertices held by triangles of v, in ascending fine
+// order, of weight x fine residual (ccnnct transposed into a gather)
+template <int NVn>
+__global__ void k_hrestrict(int nc, const int* __restrict__ ptr, const int* __restrict__ fidx,
+                            const double* __restrict__ wt, const double* __restrict__ fres, double* __restrict__ res0) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nc) return;
+    double s[NVn];
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) s[n] = 0.0;
+    for (int k = ptr[v]; k < ptr[v + 1]; ++k) {
+        const int i = fidx[k];
+        const double w = wt[k];
+#pragma unroll
+        for (int n = 0; n < NVn; ++n) s[n] += w * fres[i * NVn + n];
+    }
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) res0[v * NVn + n] = s[n];
+}
+
+// coarse ug = fine ug interpolated at the coarse vertices (fcnnct); vug_frst = ug
+template <int NVn>
+__global__ void k_hinterp(int nc, const int* __restrict__ fv, const double* __restrict__ w,
+                          const double* __restrict__ fug, double* __restrict__ ug, double* __restrict__ vug) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nc) return;
+    double s[NVn];
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) s[n] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int f = fv[3 * v + j];
+        const double ww = w[3 * v + j];
+#pragma unroll
+        for (int n = 0; n < NVn; ++n) s[n] += ww * fug[f * NVn + n];
+    }
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) { ug[v * NVn + n] = s[n]; vug[v * NVn + n] = s[n]; }
+}
+
+// vug_frst -= ug: minus the coarse correction
+__global__ void k_hcorr(long n, double* __restrict__ vug, const double* __restrict__ ug) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) vug[k] -= ug[k];
+}
+
+// fine res = interpolated coarse correction (ccnnct), fine ug += res
+template <int NVn>
+__global__ void k_hprolong(int nf, const int* __restrict__ cv, const double* __restrict__ w,
+                           const double* __restrict__ vug, double* __restrict__ fres, double* __restrict__ fug) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    double r[NVn];
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) r[n] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int c = cv[3 * i + j];
+        const double ww = w[3 * i + j];
+#pragma unroll
+        for (int n = 0; n < NVn; ++n) r[n] -= ww * vug[c * NVn + n];
+    }
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) {
+        fres[i * NVn + n] = r[n];
+        fug[i * NVn + n] += r[n];
+    }
+}
+
 // MFMA lane-layout self test: D = A(16x4) * B(4x16) with the layout the element
 // kernel assumes; out[lane*4+i] = D element held by (lane, register i)
 __global__ void k_mfma_probe(const double* A, const double* B, double* out) {

@@ -30,7 +30,7 @@ EXPORTS = (
     "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
-    "hpg_get_block_preconditioner",
+    "hpg_get_block_preconditioner", "hpg_link_fine",
 )
 
 
@@ -77,6 +77,7 @@ def load():
         getattr(L, nm).argtypes = [vp, dp, dp, dp]
     L.hpg_set_tprcn.argtypes = [vp, dp]
     L.hpg_get_block_preconditioner.argtypes = [vp, dp, dp, dp]
+    L.hpg_link_fine.argtypes = [vp, vp, dp, dp, dp, dp]
     L.hpg_rsdl.argtypes = [vp, C.c_int]
     for nm in ("hpg_setup_preconditioner", "hpg_minvrt", "hpg_update", "hpg_mg_restrict", "hpg_mg_prolongate",
                "hpg_synchronize"):

@@ -148,6 +148,69 @@ def unit_square(n: int, jitter: float = 0.0, seed: int = 0, essential_mask: int 
     return m
 
 
+def coarsen_square(fine: TriMesh, n_fine: int) -> TriMesh:
+    """The next coarser multigrid mesh of ``unit_square(n_fine, ...)``: the n/2 x n/2
+    structured mesh whose vertex (i, j) is the fine vertex (2i, 2j), so the jittered
+    positions are inherited and the coarse vertices are a subset of the fine ones
+    (the reference coarsens its meshes by vertex removal, tri_mesh/blocks.cpp:1245-1260)."""
+    if n_fine % 2:
+        raise ValueError("coarsening needs an even number of squares")
+    nc = n_fine // 2
+    c = unit_square(nc, 0.0, 0, essential_mask=int(fine.ebdry_essential[0]))
+    N1f, N1c = n_fine + 1, nc + 1
+    jj, ii = np.divmod(np.arange(N1c * N1c), N1c)
+    c.pnts = fine.pnts[(2 * jj) * N1f + 2 * ii].copy()
+    c.ebdry_essential = fine.ebdry_essential.copy()
+    check_mesh(c)
+    return c
+
+
+def _locate(points: np.ndarray, mesh: TriMesh, n: int):
+    """Triangle of the structured n x n ``mesh`` holding each point and its barycentric
+    weights (the triangle with the largest smallest weight among the two triangles of
+    the nominal cell and of its 8 neighbours; ties go to the lowest triangle index)."""
+    x = np.clip((points[:, 0] * n).astype(np.int64), 0, n - 1)
+    y = np.clip((points[:, 1] * n).astype(np.int64), 0, n - 1)
+    best_t = np.full(points.shape[0], -1, dtype=np.int64)
+    best_m = np.full(points.shape[0], -np.inf)
+    best_w = np.zeros((points.shape[0], 3))
+    cand = []
+    for dj in (-1, 0, 1):
+        for di in (-1, 0, 1):
+            cx, cy = x + di, y + dj
+            ok = (cx >= 0) & (cx < n) & (cy >= 0) & (cy < n)
+            sq = np.where(ok, cy * n + cx, 0)
+            for k in (0, 1):
+                cand.append((np.where(ok, 2 * sq + k, -1)))
+    cand = np.stack(cand, 1)
+    cand_sorted = np.sort(np.where(cand < 0, np.iinfo(np.int64).max, cand), axis=1)
+    for col in range(cand_sorted.shape[1]):
+        t = cand_sorted[:, col]
+        ok = t != np.iinfo(np.int64).max
+        tt = np.where(ok, t, 0)
+        P = mesh.pnts[mesh.tri_pnt[tt]]
+        d = (P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1]) - (P[:, 2, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1])
+        w1 = ((points[:, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1]) - (P[:, 2, 0] - P[:, 0, 0]) * (points[:, 1] - P[:, 0, 1])) / d
+        w2 = ((P[:, 1, 0] - P[:, 0, 0]) * (points[:, 1] - P[:, 0, 1]) - (points[:, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1])) / d
+        w0 = 1.0 - w1 - w2
+        mn = np.minimum(w0, np.minimum(w1, w2))
+        better = ok & (mn > best_m + 1e-14)
+        best_t = np.where(better, t, best_t)
+        best_m = np.where(better, mn, best_m)
+        best_w = np.where(better[:, None], np.stack([w0, w1, w2], 1), best_w)
+    assert np.all(best_t >= 0)
+    return best_t.astype(np.int32), best_w
+
+
+def connect(fine: TriMesh, n_fine: int, coarse: TriMesh):
+    """Vertex connections between a mesh and its coarse neighbour (tri_mesh.h:197-201):
+    ccnnct = (coarse triangle, weights) of every fine vertex, fcnnct = (fine triangle,
+    weights) of every coarse vertex."""
+    cc_tri, cc_wt = _locate(fine.pnts, coarse, n_fine // 2)
+    fc_tri, fc_wt = _locate(coarse.pnts, fine, n_fine)
+    return cc_tri, cc_wt, fc_tri, fc_wt
+
+
 def check_mesh(m: TriMesh) -> None:
     """Structural invariants every mesh handed to the solver must satisfy."""
     assert np.all(m.area2() > 0.0), "triangles must be counter-clockwise"

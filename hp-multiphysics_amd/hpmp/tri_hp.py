@@ -209,6 +209,26 @@ class TriHpIns:
     def synchronize(self):
         self._check(self.L.hpg_synchronize(self.h), "synchronize")
 
+    def link_fine(self, fine: "TriHpIns", cc_tri, cc_wt, fc_tri, fc_wt):
+        """Make this level (a coarser mesh at log2p = 0) the h-multigrid neighbour of
+        ``fine`` (tri_mesh::connect, tri_mesh/blocks.cpp:1245-1260); afterwards
+        mg_restrict/mg_prolongate on this level move between the two meshes
+        (tri_hp/movco.cpp:41-102, tri_hp/movfin.cpp:16-82)."""
+        m, f = self.mesh, fine.mesh
+        cc_tri = np.ascontiguousarray(cc_tri, dtype=np.int32)
+        fc_tri = np.ascontiguousarray(fc_tri, dtype=np.int32)
+        cc_wt = np.ascontiguousarray(cc_wt, dtype=np.float64)
+        fc_wt = np.ascontiguousarray(fc_wt, dtype=np.float64)
+        if cc_tri.shape != (f.npnt,) or cc_wt.shape != (f.npnt, 3):
+            raise ValueError("cc_tri/cc_wt must hold one coarse triangle and 3 weights per fine vertex")
+        if fc_tri.shape != (m.npnt,) or fc_wt.shape != (m.npnt, 3):
+            raise ValueError("fc_tri/fc_wt must hold one fine triangle and 3 weights per coarse vertex")
+        if cc_tri.min() < 0 or cc_tri.max() >= m.ntri or fc_tri.min() < 0 or fc_tri.max() >= f.ntri:
+            raise ValueError("connection triangle index out of range")
+        self._check(self.L.hpg_link_fine(self.h, fine.h, _lib.ptr(cc_tri), _lib.ptr(cc_wt), _lib.ptr(fc_tri),
+                                         _lib.ptr(fc_wt)), "link_fine")
+        self._fine = fine
+
     def cycle(self, vw: int = 1, mglvls: int = 1, itercrsn: int = 1, iterrfne: int = 0, lvl: int = 0) -> float:
         """block::cycle (tri_mesh/blocks.cpp:1267-1348) on one grid; p-levels below."""
         error = 0.0

@@ -101,6 +101,16 @@ int hpg_minvrt(hpg_ctx *ctx);
  * (tri_hp/cns/update.cpp:276-357: minvrt, inflow modify_boundary_residual,
  * project_new_variables, stage update) */
 int hpg_update(hpg_ctx *ctx);
+/* h-multigrid: make `coarse` (a context created on the coarse mesh with log2p = 0)
+ * the coarse neighbour of `fine` (replaces tri_mesh::connect and the ccnnct/fcnnct
+ * tables, tri_mesh/tri_mesh.h:197-201, tri_mesh/blocks.cpp:1245-1260).  cc_tri/cc_wt:
+ * for every fine vertex the coarse triangle holding it and its 3 barycentric weights;
+ * fc_tri/fc_wt: for every coarse vertex the fine triangle and weights.  Afterwards
+ * hpg_mg_restrict(coarse) restricts the fine residual and state onto the coarse mesh
+ * (tri_hp/movco.cpp:41-102) and hpg_mg_prolongate(coarse) adds the coarse correction
+ * to the fine state (tri_hp/movfin.cpp:16-82).  The fine context must outlive it. */
+int hpg_link_fine(hpg_ctx *coarse, hpg_ctx *fine, const int *cc_tri, const double *cc_wt, const int *fc_tri,
+                  const double *fc_wt);
 /* replaces tri_hp::maxres (tri_hp/allocate.cpp:496-525); per_var may be NULL */
 int hpg_maxres(hpg_ctx *ctx, double *per_var, double *out);
 /* raw-residual judge: max |res| over every vertex, side and interior coefficient

@@ -68,6 +68,10 @@ def lib():
         L.otri_cycle.restype = C.c_double
         L.otri_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.otri_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
+        L.otri_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
+        L.ocns_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
+        L.otri_get_mg.argtypes = [C.c_void_p, _dp, _dp]
+        L.ocns_get_mg.argtypes = [C.c_void_p, _dp, _dp]
         L.ocns_create.restype = C.c_void_p
         L.ocns_create.argtypes = L.otri_create.argtypes
         L.ocns_destroy.argtypes = [C.c_void_p]
@@ -284,6 +288,19 @@ class TriHpIns:
     def cycle(self, vw=1, mglvls=1, itercrsn=1, iterrfne=0):
         return lib().otri_cycle(self.h, vw, mglvls, itercrsn, iterrfne)
 
+    def link_fine(self, fine, cc_tri, cc_wt, fc_tri, fc_wt):
+        """Make this (coarse mesh) level the h-multigrid neighbour of `fine`."""
+        self._fine = fine
+        self._conn = [np.ascontiguousarray(cc_tri, dtype=np.int32), np.ascontiguousarray(cc_wt, dtype=np.float64),
+                      np.ascontiguousarray(fc_tri, dtype=np.int32), np.ascontiguousarray(fc_wt, dtype=np.float64)]
+        lib().otri_link_fine(self.h, fine.h, *self._conn)
+
+    def get_mg(self):
+        """(res0 at the vertices, vug_frst): the multigrid entry residual and state."""
+        r0 = np.zeros((self.mesh.npnt, 3)); vug = np.zeros((self.mesh.npnt, 3))
+        lib().otri_get_mg(self.h, r0, vug)
+        return r0, vug
+
     def element_rsdl(self, tind, stage, uht):
         tm = (self.p + 1) * (self.p + 2) // 2 if self.log2p == self.log2pmax else None
         tm = uht.shape[1]
@@ -420,6 +437,19 @@ class TriHpCns:
         K = np.zeros((tm * 4, tm * 4))
         lib().ocns_element_jacobian(self.h, tind, eps_r, eps_a, K)
         return K
+
+    def link_fine(self, fine, cc_tri, cc_wt, fc_tri, fc_wt):
+        """Make this (coarse mesh) level the h-multigrid neighbour of `fine`."""
+        self._fine = fine
+        self._conn = [np.ascontiguousarray(cc_tri, dtype=np.int32), np.ascontiguousarray(cc_wt, dtype=np.float64),
+                      np.ascontiguousarray(fc_tri, dtype=np.int32), np.ascontiguousarray(fc_wt, dtype=np.float64)]
+        lib().ocns_link_fine(self.h, fine.h, *self._conn)
+
+    def get_mg(self):
+        """(res0 at the vertices, vug_frst): the multigrid entry residual and state."""
+        r0 = np.zeros((self.mesh.npnt, 4)); vug = np.zeros((self.mesh.npnt, 4))
+        lib().ocns_get_mg(self.h, r0, vug)
+        return r0, vug
 
     def calculate_tau(self, pvu, h):
         P = np.zeros(16); T = np.zeros(16); ts = np.zeros(1)

@@ -159,6 +159,12 @@ void otri_element_rsdl(void* h, int tind, int stage, const double* uh, double* l
 }
 void otri_element_jacobian(void* h, int tind, double eps_r, double eps_a, double* K) { T(h).element_jacobian(tind, eps_r, eps_a, K); }
 
+// h-multigrid link of a coarse level to its fine neighbour (ins and cns)
+void otri_get_mg(void* h, double* res0v, double* vug) { auto& x = T(h); std::memcpy(res0v, x.res0v.data(), x.m.npnt * 3 * 8); if (!x.vug_frst.empty()) std::memcpy(vug, x.vug_frst.data(), x.vug_frst.size() * 8); }
+void otri_link_fine(void* coarse, void* fine, const int* cc_tri, const double* cc_wt, const int* fc_tri, const double* fc_wt) {
+    static_cast<TriHpIns*>(coarse)->link_fine(static_cast<TriHpIns*>(fine), cc_tri, cc_wt, fc_tri, fc_wt);
+}
+
 // ---- tri_hp_cns --------------------------------------------------------------
 // prm: mu, adis, gamma, prandtl, Rgas, fadd, bc p, bc u, bc v, bc RT
 void* ocns_create(int npnt, const double* pnts, int nseg, const int* seg_pnt, int ntri, const int* tri_pnt,
@@ -208,6 +214,7 @@ void ocns_minvrt(void* h) { Cn(h).minvrt(); }
 void ocns_modify_boundary_residual(void* h) { Cn(h).modify_boundary_residual(); }
 void ocns_project_new_variables(void* h) { Cn(h).project_new_variables(); }
 void ocns_update(void* h) { Cn(h).update(); }
+void ocns_get_mg(void* h, double* res0v, double* vug) { auto& x = Cn(h); std::memcpy(res0v, x.res0v.data(), x.m.npnt * 4 * 8); if (!x.vug_frst.empty()) std::memcpy(vug, x.vug_frst.data(), x.vug_frst.size() * 8); }
 double ocns_maxres(void* h, double* per_var) { return Cn(h).maxres(per_var); }
 void ocns_mg_restrict(void* h) { Cn(h).mg_restrict(); }
 void ocns_mg_prolongate(void* h) { Cn(h).mg_prolongate(); }
@@ -229,6 +236,9 @@ void ocns_calculate_tau(void* h, const double* pvu, double hh, double* Pinv, dou
     double P[4][4], T4[4][4];
     Cn(h).calculate_tau(pvu, hh, P, T4, *timestep);
     for (int a = 0; a < 16; ++a) { Pinv[a] = P[a / 4][a % 4]; Tau[a] = T4[a / 4][a % 4]; }
+}
+void ocns_link_fine(void* coarse, void* fine, const int* cc_tri, const double* cc_wt, const int* fc_tri, const double* fc_wt) {
+    static_cast<TriHpCns*>(coarse)->link_fine(static_cast<TriHpCns*>(fine), cc_tri, cc_wt, fc_tri, fc_wt);
 }
 double ocns_spectral_radius(const double* A16) {
     double A[4][4];

@@ -390,9 +390,45 @@ double TriHp<NV>::maxres(double* per_var) const {
 
 // tri_hp/movco.cpp:4-40 (p-level branch)
 template <int NV>
+void TriHp<NV>::link_fine(TriHp* f, const int* cctri, const double* ccwt, const int* fctri, const double* fcwt) {
+    fine = f;
+    coarse_level = true;
+    cc_tri.assign(cctri, cctri + f->m.npnt);
+    cc_wt.assign(ccwt, ccwt + 3 * f->m.npnt);
+    fc_tri.assign(fctri, fctri + m.npnt);
+    fc_wt.assign(fcwt, fcwt + 3 * m.npnt);
+    vug_frst.assign(m.npnt * NV, 0.0);
+}
+
+// tri_hp/movco.cpp:4-106.  p-levels (!coarse_level): truncate the residual into res0.
+// h-levels: res0 = sum over fine vertices of weight x fine residual on the coarse
+// triangle that holds the fine vertex; ug = fine ug interpolated at the coarse
+// vertices, kept in vug_frst.  (The reference shares the residual arrays between the
+// levels of a block; here the fine residual is read from the fine level.)
+template <int NV>
 void TriHp<NV>::mg_restrict() {
     isfrst = true;
     coarse_flag = true;
+    if (coarse_level) {
+        std::fill(res0v.begin(), res0v.end(), 0.0);
+        for (int i = 0; i < fine->m.npnt; ++i) {
+            const int tind = cc_tri[i];
+            for (int j = 0; j < 3; ++j) {
+                const int v0 = m.tri_pnt[3 * tind + j];
+                for (int n = 0; n < NV; ++n) res0v[v0 * NV + n] += cc_wt[3 * i + j] * fine->resv[i * NV + n];
+            }
+        }
+        for (int i = 0; i < m.npnt; ++i) {
+            const int tind = fc_tri[i];
+            for (int n = 0; n < NV; ++n) ugv[i * NV + n] = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                const int fv = fine->m.tri_pnt[3 * tind + j];
+                for (int n = 0; n < NV; ++n) ugv[i * NV + n] += fc_wt[3 * i + j] * fine->ugv[fv * NV + n];
+            }
+            for (int n = 0; n < NV; ++n) vug_frst[i * NV + n] = ugv[i * NV + n];
+        }
+        return;
+    }
     --log2p;
     const TriBasis& B = b();
     res0v = resv;
@@ -413,14 +449,29 @@ void TriHp<NV>::mg_restrict() {
         }
 }
 
-// tri_hp/movfin.cpp:7-14 (p-level branch)
+// tri_hp/movfin.cpp:7-82.  p-levels: back to the finer p.  h-levels: the coarse
+// correction (vug_frst - ug, negated) is interpolated to the fine vertices through
+// the fine vertices' coarse triangles, written into the fine residual and added to
+// the fine ug.
 template <int NV>
 void TriHp<NV>::mg_prolongate() {
+    if (coarse_level) {
+        for (int i = 0; i < m.npnt * NV; ++i) vug_frst[i] -= ugv[i];
+        for (int i = 0; i < fine->m.npnt; ++i) {
+            const int tind = cc_tri[i];
+            for (int n = 0; n < NV; ++n) fine->resv[i * NV + n] = 0.0;
+            for (int j = 0; j < 3; ++j) {
+                const int ind = m.tri_pnt[3 * tind + j];
+                for (int n = 0; n < NV; ++n) fine->resv[i * NV + n] -= cc_wt[3 * i + j] * vug_frst[ind * NV + n];
+            }
+        }
+        for (int i = 0; i < fine->m.npnt * NV; ++i) fine->ugv[i] += fine->resv[i];
+        return;
+    }
     ++log2p;
     if (log2p == prm.log2pmax) coarse_flag = false;
 }
 
-// tri_hp/nstage.cpp:131-165 (fixed mesh); K is (tm*NV) x (tm*NV), row-major
 template <int NV>
 void TriHp<NV>::element_jacobian(int tind, double eps_r, double eps_a, double* K) {
     const TriBasis& B = b();

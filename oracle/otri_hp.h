@@ -10,7 +10,8 @@
 //   preconditioner assembly  tri_hp/setup_preconditioner.cpp:9-96
 //   minvrt                   tri_hp/setup_preconditioner.cpp:102-399 (diagonal branch)
 //   maxres                   tri_hp/allocate.cpp:496-525
-//   mg_restrict/prolongate   tri_hp/movco.cpp:4-40, tri_hp/movfin.cpp:7-14 (p-levels)
+//   mg_restrict/prolongate   tri_hp/movco.cpp:4-106, tri_hp/movfin.cpp:7-82 (p-levels, and
+//                            h-levels through the fine/coarse vertex connections)
 //   element_jacobian         tri_hp/nstage.cpp:131-165 (fixed mesh)
 //   cycle                    tri_mesh/blocks.cpp:1267-1348 (one grid, extra_finest_levels)
 //   area, inscribedradius    tri_mesh/tprims.cpp:66-80, 224-242
@@ -63,6 +64,15 @@ class TriHp {
     std::vector<std::unique_ptr<TriBasis>> bases;  // log2p = 0,1,2 -> p = 1,2,4
     int log2p, sm0, im0, smf, imf;
     bool coarse_flag = false, isfrst = false;
+    // h-multigrid: a coarse mesh level linked to its fine neighbour.  ccnnct of the
+    // fine mesh (tri_mesh.h:197-201): for every fine vertex the coarse triangle that
+    // holds it and its barycentric weights; fcnnct of this mesh: for every coarse
+    // vertex the fine triangle and weights.  vug_frst keeps the restricted state.
+    bool coarse_level = false;
+    TriHp* fine = nullptr;
+    std::vector<int> cc_tri, fc_tri;
+    std::vector<double> cc_wt, fc_wt, vug_frst;
+    void link_fine(TriHp* f, const int* cctri, const double* ccwt, const int* fctri, const double* fcwt);
 
     // solution and work arrays, reference layouts v(npnt,NV), s(nseg,sm,NV), i(ntri,im,NV)
     std::vector<double> ugv, ugs, ugi;
