@@ -50,6 +50,7 @@ class ScriptedSolver:
 def _guard(s):
     g = GuardedSolver.__new__(GuardedSolver)
     g.s, g.mlp, g.check_every, g.grow_tol, g.window, g.learned, g.fell_back_at = s, object(), 10, 10.0, 4, True, None
+    g.grids = None
     s.set_preconditioner_mode(_lib.HPG_PRCN_LEARNED)
     return g
 
