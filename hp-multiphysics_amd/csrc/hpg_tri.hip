@@ -268,6 +268,19 @@ void modes_vsi(hpg_ctx* c, int op, double* xv, double* xs, double* xi, double* y
     if (d.im > 0) launch_modes(c, op, c->ntri, d.im, c->im0, xi, yi, zi, k);
 }
 
+// blocks of 256 threads of `kernel` that the whole device holds at once (a multiple
+// of 8: one share per XCD); persistent kernels launch at most this many
+static int resident_blocks(const void* kernel) {
+    int dev = 0, cus = 0, per = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0), "hipOccupancy");
+    return std::max(8, (cus * std::max(per, 1)) / 8 * 8);
+}
+
+// grid of a persistent launch over `want` blocks' worth of work
+static int persistent_grid(int want, int cap) { return std::min((want + 7) / 8 * 8, cap); }
+
 template <int P>
 void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full) {
     for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
@@ -278,13 +291,16 @@ void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full) {
         a.gbase = c->gbase[col];
         const int nlogical = (a.end - a.begin + 63) / 64;
         if (nlogical == 0) continue;
-        const int nb = ((nlogical + 7) / 8) * 8;
         const size_t r = c->recs.size();
         if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
-        if (full)
-            hipLaunchKernelGGL((k_elem_ins<P, true>), dim3(nb), dim3(256), 0, c->stream, a, nlogical);
-        else
-            hipLaunchKernelGGL((k_elem_ins<P, false>), dim3(nb), dim3(256), 0, c->stream, a, nlogical);
+        if (full) {
+            static const int cap = resident_blocks((const void*)k_elem_ins<P, true>);
+            hipLaunchKernelGGL((k_elem_ins<P, true>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream, a);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_ins<P, false>);
+            hipLaunchKernelGGL((k_elem_ins<P, false>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream,
+                               a);
+        }
         if (c->timing) {
             check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
             c->recs.push_back({full ? 1 : 0, a.end - a.begin});
@@ -620,8 +636,8 @@ void cns_boundary_and_projection(hpg_ctx* c) {
     }
     if (d.im > 0) {
         const int noct = (c->ntri + 7) / 8;
-        const int nb = std::min(((noct + 7) / 8) * 8, 1024);
-        hipLaunchKernelGGL(k_pnv_int<4>, dim3(nb), dim3(256), 0, c->stream, a, noct);
+        static const int cap = resident_blocks((const void*)k_pnv_int<4>);
+        hipLaunchKernelGGL(k_pnv_int<4>, dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
     }
     check(hipGetLastError(), "cns projection launch");
 }

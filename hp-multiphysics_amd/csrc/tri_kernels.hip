@@ -24,278 +24,373 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
 // ---------------------------------------------------------------------------------
 // tri_hp::ugtouht / lftog index map for mode k of element t (tri_hp/gtol.cpp:14-55,
 // 364-402): returns the offset of the NV-vector and the side sign sgn^m.
+// a[i] for a lane-varying i < 3 without dynamic register indexing (which would spill
+// the array to scratch)
+__device__ __forceinline__ int sel3(const int* a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
 template <int P, int NVn = NV>
 __device__ __forceinline__ int mode_offset(int k, int t, const int* vtx, const int* seg, const int* sgn,
                                            int sm0, int im0, int& kind, double& sign) {
     using D = Dims<P>;
     sign = 1.0;
-    if (k < 3) { kind = 0; return vtx[k] * NVn; }
+    if (k < 3) { kind = 0; return sel3(vtx, k) * NVn; }
     if (k < D::BM) {
         constexpr int SMD = D::SM > 0 ? D::SM : 1;
         const int i = (k - 3) / SMD, m = (k - 3) - i * SMD;
         kind = 1;
-        sign = ((m & 1) && sgn[i] < 0) ? -1.0 : 1.0;
-        return (seg[i] * sm0 + m) * NVn;
+        sign = ((m & 1) && sel3(sgn, i) < 0) ? -1.0 : 1.0;
+        return (sel3(seg, i) * sm0 + m) * NVn;
     }
     kind = 2;
     return (t * im0 + (k - D::BM)) * NVn;
 }
 
+// Per-lane gather state of one element, loaded one group ahead of its use.
+// Slot s of lane l is mode 4s + l/16 of element l%16: the K-step of the projection
+// that reads it and the accumulator register that returns its residual.
+struct ElemConn {
+    int t;  // element, -1 for a padding lane
+    int vtx[3], seg[3], sgn[3];
+};
+
+template <int P>
+struct ElemSlots {
+    static constexpr int KT = Dims<P>::KT;
+    int t;
+    int off[KT];   // offset of the slot's NV-vector, bitwise-negated for a side sign of -1
+    int first;     // bit s: the slot's first visit in this stage
+    double x[3], y[3];
+};
+
+// kind of mode k: 0 vertex, 1 side, 2 interior (mode_offset's classification)
+template <int P>
+__device__ __forceinline__ int mode_kind(int k) { return k < 3 ? 0 : (k < Dims<P>::BM ? 1 : 2); }
+
+__device__ __forceinline__ int elem_of_group(const ElemArgs& a, int g, int gend, int e) {
+    const int idx = a.begin + g * 16 + e;
+    return (g < gend && idx < a.end) ? a.order[idx] : -1;
+}
+
+__device__ __forceinline__ ElemConn elem_conn(const ElemArgs& a, int t) {
+    ElemConn c;
+    c.t = t;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c.vtx[k] = t >= 0 ? a.tri_pnt[3 * t + k] : 0;
+        c.seg[k] = t >= 0 ? a.tri_seg[3 * t + k] : 0;
+        c.sgn[k] = t >= 0 ? a.tri_sgn[3 * t + k] : 1;
+    }
+    return c;
+}
+
+template <int P>
+__device__ __forceinline__ ElemSlots<P> elem_slots(const ElemArgs& a, const ElemConn& c, int q) {
+    using D = Dims<P>;
+    constexpr int SMD = D::SM > 0 ? D::SM : 1;
+    ElemSlots<P> S;
+    S.t = c.t;
+    const bool valid = c.t >= 0;
+    // padding lanes compute on the reference triangle with a zero state
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        S.x[k] = valid ? a.pnts[2 * c.vtx[k]] : (k == 1 ? 1.0 : 0.0);
+        S.y[k] = valid ? a.pnts[2 * c.vtx[k] + 1] : (k == 2 ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int s = 0; s < D::KT; ++s) {
+        const int k = 4 * s + q;
+        int kind;
+        double sg;
+        const int off = mode_offset<P>(k, valid ? c.t : 0, c.vtx, c.seg, c.sgn, a.sm0, a.im0, kind, sg);
+        S.off[s] = sg < 0.0 ? ~off : off;
+        const bool live = valid && k < D::TM;
+        const bool first = live && (kind == 2 || (kind == 0 ? a.fcv[sel3(c.vtx, k)]
+                                                             : a.fcs[sel3(c.seg, (k - 3) / SMD)]) == a.colour);
+        if (s == 0) S.first = 0;
+        S.first |= first ? (1 << s) : 0;
+    }
+    return S;
+}
+
 // tri_hp_ins::element_rsdl, straight branch (tri_hp/ins/rsdl.cpp:259-433), steady,
 // for the elements order[begin:end) (one colour: no two share a vertex or side).
+// Persistent: each wave walks 16-element groups of its XCD's contiguous share of the
+// colour and keeps a three-deep load pipeline (element ids three groups ahead,
+// connectivity two ahead, modes/geometry/first-visit flags one ahead; the old
+// residual values of the current group are requested before its contractions), so
+// the dependent gathers overlap the MFMA work of earlier groups.  The grid is a
+// multiple of 8 blocks of 4 waves.
 template <int P, bool FULL>
-__global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a, int nlogical) {
+__global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
     using D = Dims<P>;
     constexpr int GT = D::GT, KT = D::KT;
-    constexpr int SMD = D::SM > 0 ? D::SM : 1;
     __shared__ double sops[D::PACK];
-    // XCD-aware block order: hardware deals consecutive blocks round-robin to the
-    // 8 XCDs; give each XCD a contiguous run of elements so neighbours share L2.
-    const int per = gridDim.x >> 3;
-    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (lb >= nlogical) return;
     for (int k = threadIdx.x; k < D::PACK; k += blockDim.x) sops[k] = a.ops[k];
     __syncthreads();
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int e = lane & 15, q = lane >> 4;
-    const int idx = a.begin + (lb * 4 + w) * 16 + e;
-    const bool valid = idx < a.end;
-    if (__builtin_amdgcn_ballot_w64(valid) == 0) return;  // whole wave past the colour end
-    const int t = valid ? a.order[idx] : a.order[a.begin];
+    // XCD-aware split: hardware deals consecutive blocks round-robin to the 8 XCDs;
+    // XCD x owns groups [x*gpx, (x+1)*gpx) so neighbouring elements share its L2
+    const int WX = (gridDim.x >> 3) * 4;
+    const int rank = (blockIdx.x >> 3) * 4 + w;
+    const int xcd = blockIdx.x & 7;
+    const int ng = (a.end - a.begin + 15) >> 4;
+    const int gpx = (ng + 7) >> 3;
+    const int g0 = xcd * gpx + rank;
+    const int gend = min(ng, (xcd + 1) * gpx);
+    if (g0 >= gend) return;
 
-    int vtx[3], seg[3], sgn[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        vtx[k] = a.tri_pnt[3 * t + k];
-        seg[k] = a.tri_seg[3 * t + k];
-        sgn[k] = a.tri_sgn[3 * t + k];
-    }
-    const double x0 = a.pnts[2 * vtx[0]], y0 = a.pnts[2 * vtx[0] + 1];
-    const double x1 = a.pnts[2 * vtx[1]], y1 = a.pnts[2 * vtx[1] + 1];
-    const double x2 = a.pnts[2 * vtx[2]], y2 = a.pnts[2 * vtx[2] + 1];
-    // ldcrd(n,0) = dx_n/dr, ldcrd(n,1) = dx_n/ds (rsdl.cpp:44-47)
-    const double l00 = 0.5 * (x2 - x1), l01 = 0.5 * (x0 - x1);
-    const double l10 = 0.5 * (y2 - y1), l11 = 0.5 * (y0 - y1);
+    ElemSlots<P> S = elem_slots<P>(a, elem_conn(a, elem_of_group(a, g0, gend, e)), q);
+    ElemConn cnext = elem_conn(a, elem_of_group(a, g0 + WX, gend, e));
+    int tnext2 = elem_of_group(a, g0 + 2 * WX, gend, e);
 
-    // gather the B operands: K-step s of lane l holds mode 4s + l/16 of element l%16
-    double b[NV][KT];
-#pragma unroll
-    for (int s = 0; s < KT; ++s) {
-        const int k = 4 * s + q;
-        int kind;
-        double sg;
-        const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg);
-        const double* src = kind == 0 ? a.ugv : (kind == 1 ? a.ugs : a.ugi);
-#pragma unroll
-        for (int n = 0; n < NV; ++n) b[n][s] = 0.0;
-        if (valid && k < D::TM) {
-#pragma unroll
-            for (int n = 0; n < NV; ++n) b[n][s] = sg * src[off + n];
-        }
-    }
-
-    const d4 z = {0.0, 0.0, 0.0, 0.0};
-    d4 U[GT], V[GT], Pp[GT];
-#pragma unroll
-    for (int mt = 0; mt < GT; ++mt) {
-        d4 cu = z, cv = z, cp = z;
-#pragma unroll
-        for (int s = 0; s < KT; ++s) {
-            const double av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
-            cu = mfma(av, b[0][s], cu);
-            cv = mfma(av, b[1][s], cv);
-            cp = mfma(av, b[2][s], cp);
-        }
-        U[mt] = cu; V[mt] = cv; Pp[mt] = cp;
-    }
-
-    // inviscid fluxes (rsdl.cpp:262-291): continuity C, momentum A
-    d4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
-    const double rho = a.rho;
-#pragma unroll
-    for (int mt = 0; mt < GT; ++mt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const double u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
-            const double fx = rho * u, fy = rho * v;
-            const double c0 = +l11 * fx - l01 * fy;
-            const double c1 = -l10 * fx + l00 * fy;
-            C0[mt][i] = c0; C1[mt][i] = c1;
-            A00[mt][i] = u * c0 + l11 * pr;
-            A01[mt][i] = u * c1 - l10 * pr;
-            A10[mt][i] = v * c0 - l01 * pr;
-            A11[mt][i] = v * c1 + l00 * pr;
-        }
-
-    // lf_im = intgrtrs of the inviscid fluxes (rsdl.cpp:292-294)
-    d4 LI0 = z, LI1 = z, LI2 = z;
-#pragma unroll
-    for (int kt = 0; kt < GT; ++kt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int kk = kt * 4 + i;
-            const double ar = sops[D::OFF_IR + kk * 64 + lane];
-            const double as = sops[D::OFF_IS + kk * 64 + lane];
-            LI0 = mfma(ar, A00[kt][i], LI0); LI0 = mfma(as, A01[kt][i], LI0);
-            LI1 = mfma(ar, A10[kt][i], LI1); LI1 = mfma(as, A11[kt][i], LI1);
-            LI2 = mfma(ar, C0[kt][i], LI2);  LI2 = mfma(as, C1[kt][i], LI2);
-        }
-
-    d4 LR0 = z, LR1 = z, LR2 = z;
-    if constexpr (FULL) {
-        // derivatives of u, v at the Gauss points (proj with r/s derivatives)
-        d4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
-#pragma unroll
-        for (int mt = 0; mt < GT; ++mt) {
-            d4 c1 = z, c2 = z, c3 = z, c4 = z;
+    for (int g = g0; g < gend; g += WX) {
+        // old residual values of this group, consumed by the scatter at the end and
+        // requested late (register pressure): before the last contraction of the full
+        // residual, after the contractions of the inviscid one
+        double ro[NV][KT], io[NV][KT];
+        auto load_old = [&]() {
 #pragma unroll
             for (int s = 0; s < KT; ++s) {
-                const double ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
-                const double as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
-                c1 = mfma(ar, b[0][s], c1); c2 = mfma(as, b[0][s], c2);
-                c3 = mfma(ar, b[1][s], c3); c4 = mfma(as, b[1][s], c4);
+                const int k = 4 * s + q, kind = mode_kind<P>(k);
+                const bool live = S.t >= 0 && k < D::TM;
+                const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
+                const double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
+                const double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) {
+                    ro[n][s] = (live && (FULL || a.keep != 0.0)) ? dr[off + n] : 0.0;
+                    io[n][s] = (live && !((S.first >> s) & 1)) ? di[off + n] : 0.0;
+                }
             }
-            Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
+        };
+        // prefetch: next group's slots, connectivity two ahead, element ids three ahead
+        const ElemSlots<P> Sn = elem_slots<P>(a, cnext, q);
+        const ElemConn cn2 = elem_conn(a, tnext2);
+        const int tn3 = elem_of_group(a, g + 3 * WX, gend, e);
+
+        // modal coefficients of this group: K-step s of lane l is mode 4s + l/16
+        double b[NV][KT];
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+            const int k = 4 * s + q, kind = mode_kind<P>(k);
+            const bool live = S.t >= 0 && k < D::TM;
+            const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
+            const double sg = S.off[s] < 0 ? -1.0 : 1.0;
+            const double* src = kind == 0 ? a.ugv : (kind == 1 ? a.ugs : a.ugi);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) b[n][s] = live ? sg * src[off + n] : 0.0;
         }
-        // viscous stress tensor (rsdl.cpp:303-326); steady so rhobd0 = 0
-        const double cjcb = l00 * l11 - l10 * l01;
-        const double cjcbi = a.mu / cjcb;
-        const double v0000 = -cjcbi * (2. * l11 * l11 + l01 * l01);
-        const double v0011 = -cjcbi * (2. * l10 * l10 + l00 * l00);
-        const double v0001 = cjcbi * (2. * l11 * l10 + l01 * l00);
-        const double v1100 = -cjcbi * (l11 * l11 + 2. * l01 * l01);
-        const double v1111 = -cjcbi * (l10 * l10 + 2. * l00 * l00);
-        const double v1101 = cjcbi * (l11 * l10 + 2. * l01 * l00);
-        const double v0100 = cjcbi * l01 * l11;
-        const double v0111 = cjcbi * l00 * l10;
-        const double v0101 = -cjcbi * l01 * l10;
-        const double v0110 = -cjcbi * l00 * l11;
-        d4 F00[GT], F01[GT], F10[GT], F11[GT];
-#pragma unroll
-        for (int mt = 0; mt < GT; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
-                const double f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
-                const double f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
-                const double f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
-                const double f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
-                F00[mt][i] = f00; F01[mt][i] = f01; F10[mt][i] = f10; F11[mt][i] = f11;
-                A00[mt][i] += f00; A01[mt][i] += f01; A10[mt][i] += f10; A11[mt][i] += f11;
-            }
-        // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:369-375)
-        d4 R0[GT], R1[GT], R2[GT];
-#pragma unroll
+        const double x0 = S.x[0], y0 = S.y[0], x1 = S.x[1], y1 = S.y[1], x2 = S.x[2], y2 = S.y[2];
+        // ldcrd(n,0) = dx_n/dr, ldcrd(n,1) = dx_n/ds (rsdl.cpp:44-47)
+        const double l00 = 0.5 * (x2 - x1), l01 = 0.5 * (x0 - x1);
+        const double l10 = 0.5 * (y2 - y1), l11 = 0.5 * (y0 - y1);
+
+        const d4 z = {0.0, 0.0, 0.0, 0.0};
+        d4 U[GT], V[GT], Pp[GT];
+    #pragma unroll
         for (int mt = 0; mt < GT; ++mt) {
-            d4 r0 = z, r1 = z, r2 = z;
-#pragma unroll
-            for (int kt = 0; kt < GT; ++kt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int kk = (mt * GT + kt) * 4 + i;
-                    const double dr = sops[D::OFF_DR + kk * 64 + lane];
-                    const double ds = sops[D::OFF_DS + kk * 64 + lane];
-                    r0 = mfma(dr, A00[kt][i], r0); r0 = mfma(ds, A01[kt][i], r0);
-                    r1 = mfma(dr, A10[kt][i], r1); r1 = mfma(ds, A11[kt][i], r1);
-                    r2 = mfma(dr, C0[kt][i], r2);  r2 = mfma(ds, C1[kt][i], r2);
-                }
-            R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
-        }
-        if (a.lin) {
-            // linearisation data of this 16-element group, coalesced per (field, tile, reg)
-            float* L = a.lin + (size_t)(a.gbase + lb * 4 + w) * (5 * GT * 4 * 64);
-#pragma unroll
-            for (int mt = 0; mt < GT; ++mt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int o = (mt * 4 + i) * 64 + lane;
-                    L[0 * GT * 256 + o] = (float)U[mt][i];
-                    L[1 * GT * 256 + o] = (float)V[mt][i];
-                    L[2 * GT * 256 + o] = (float)R0[mt][i];
-                    L[3 * GT * 256 + o] = (float)R1[mt][i];
-                    L[4 * GT * 256 + o] = (float)R2[mt][i];
-                }
-        }
-        // per-point tau (CALC_TAU2, rsdl.cpp:377-395) and the stabilisation (:397-421)
-        const double dx1 = x0 - x2, dy1 = y0 - y2, dx2 = x1 - x0, dy2 = y1 - y0;
-        const double ar2 = dx1 * dy2 - dy1 * dx2;
-        const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
-                             sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
-        const double h = (ar2 / perim) / (0.25 * (P + 1) * (P + 1));
-        const double nu = a.mu / rho;
-        const double gv = 2. * nu / h;
-        const double gvis = gv * gv;
-        const double tfac = a.adis * h / cjcb;
-#pragma unroll
-        for (int mt = 0; mt < GT; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double u = U[mt][i], v = V[mt][i];
-                const double qq = u * u + v * v;
-                double gam = 3.0 * qq + gvis;
-                if (a.mu == 0.0) gam = fmax(gam, 0.1);
-                const double tau0 = tfac / sqrt(gam);
-                const double tau2 = sqrt(qq) * tau0;
-                const double t0 = tau0 * R0[mt][i], t1 = tau0 * R1[mt][i], t2 = tau2 * R2[mt][i];
-                F00[mt][i] -= (l11 * (2 * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
-                F01[mt][i] -= (-l10 * (2 * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
-                F10[mt][i] -= +l11 * v * t0 + (l11 * u - l01 * (2. * v)) * t1 - l01 * t2;
-                F11[mt][i] -= -l10 * v * t0 + (-l10 * u + l00 * (2. * v)) * t1 + l00 * t2;
-                C0[mt][i] = -(l11 * t0 - l01 * t1);
-                C1[mt][i] = -(-l10 * t0 + l00 * t1);
+            d4 cu = z, cv = z, cp = z;
+    #pragma unroll
+            for (int s = 0; s < KT; ++s) {
+                const double av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
+                cu = mfma(av, b[0][s], cu);
+                cv = mfma(av, b[1][s], cv);
+                cp = mfma(av, b[2][s], cp);
             }
-        // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
-#pragma unroll
+            U[mt] = cu; V[mt] = cv; Pp[mt] = cp;
+        }
+
+        // inviscid fluxes (rsdl.cpp:262-291): continuity C, momentum A
+        d4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
+        const double rho = a.rho;
+    #pragma unroll
+        for (int mt = 0; mt < GT; ++mt)
+    #pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
+                const double fx = rho * u, fy = rho * v;
+                const double c0 = +l11 * fx - l01 * fy;
+                const double c1 = -l10 * fx + l00 * fy;
+                C0[mt][i] = c0; C1[mt][i] = c1;
+                A00[mt][i] = u * c0 + l11 * pr;
+                A01[mt][i] = u * c1 - l10 * pr;
+                A10[mt][i] = v * c0 - l01 * pr;
+                A11[mt][i] = v * c1 + l00 * pr;
+            }
+
+        // lf_im = intgrtrs of the inviscid fluxes (rsdl.cpp:292-294)
+        d4 LI0 = z, LI1 = z, LI2 = z;
+    #pragma unroll
         for (int kt = 0; kt < GT; ++kt)
-#pragma unroll
+    #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int kk = kt * 4 + i;
                 const double ar = sops[D::OFF_IR + kk * 64 + lane];
                 const double as = sops[D::OFF_IS + kk * 64 + lane];
-                LR0 = mfma(ar, F00[kt][i], LR0); LR0 = mfma(as, F01[kt][i], LR0);
-                LR1 = mfma(ar, F10[kt][i], LR1); LR1 = mfma(as, F11[kt][i], LR1);
-                LR2 = mfma(ar, C0[kt][i], LR2);  LR2 = mfma(as, C1[kt][i], LR2);
+                LI0 = mfma(ar, A00[kt][i], LI0); LI0 = mfma(as, A01[kt][i], LI0);
+                LI1 = mfma(ar, A10[kt][i], LI1); LI1 = mfma(as, A11[kt][i], LI1);
+                LI2 = mfma(ar, C0[kt][i], LI2);  LI2 = mfma(as, C1[kt][i], LI2);
             }
-    }
 
-    // lftog: register i of lane l holds mode 4*i + l/16 of element l%16
-    // res = sum lf_im + res_r,  res_r = (1-beta) res_r + beta sum lf_re
-    // (tri_hp/nstage.cpp:21-82).  The first colour that touches a coefficient starts
-    // both sums from keep * res_r; interiors belong to one element.
-    if (!valid) return;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int k = 4 * i + q;
-        if (k >= D::TM) continue;
-        int kind;
-        double sg;
-        const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg);
-        double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
-        double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
-        const bool first = kind == 2 || (kind == 0 ? a.fcv[vtx[k]] : a.fcs[seg[(k - 3) / SMD]]) == a.colour;
-        const double li[3] = {sg * LI0[i], sg * LI1[i], sg * LI2[i]};
-        double lr[3] = {0.0, 0.0, 0.0};
+        d4 LR0 = z, LR1 = z, LR2 = z;
         if constexpr (FULL) {
-            const double bt = a.beta;
-            lr[0] = sg * (LR0[i] * bt);
-            lr[1] = sg * (LR1[i] * bt);
-            lr[2] = sg * (LR2[i] * bt);
-        }
-        if (first) {
-#pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                const double r0 = a.keep == 0.0 ? 0.0 : a.keep * dr[off + n];
-                if (FULL || a.keep != 1.0) dr[off + n] = r0 + lr[n];
-                di[off + n] = r0 + lr[n] + li[n];
+            // derivatives of u, v at the Gauss points (proj with r/s derivatives)
+            d4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
+    #pragma unroll
+            for (int mt = 0; mt < GT; ++mt) {
+                d4 c1 = z, c2 = z, c3 = z, c4 = z;
+    #pragma unroll
+                for (int s = 0; s < KT; ++s) {
+                    const double ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
+                    const double as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
+                    c1 = mfma(ar, b[0][s], c1); c2 = mfma(as, b[0][s], c2);
+                    c3 = mfma(ar, b[1][s], c3); c4 = mfma(as, b[1][s], c4);
+                }
+                Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
             }
-        } else {
+            // viscous stress tensor (rsdl.cpp:303-326); steady so rhobd0 = 0
+            const double cjcb = l00 * l11 - l10 * l01;
+            const double cjcbi = a.mu / cjcb;
+            const double v0000 = -cjcbi * (2. * l11 * l11 + l01 * l01);
+            const double v0011 = -cjcbi * (2. * l10 * l10 + l00 * l00);
+            const double v0001 = cjcbi * (2. * l11 * l10 + l01 * l00);
+            const double v1100 = -cjcbi * (l11 * l11 + 2. * l01 * l01);
+            const double v1111 = -cjcbi * (l10 * l10 + 2. * l00 * l00);
+            const double v1101 = cjcbi * (l11 * l10 + 2. * l01 * l00);
+            const double v0100 = cjcbi * l01 * l11;
+            const double v0111 = cjcbi * l00 * l10;
+            const double v0101 = -cjcbi * l01 * l10;
+            const double v0110 = -cjcbi * l00 * l11;
+            d4 F00[GT], F01[GT], F10[GT], F11[GT];
+    #pragma unroll
+            for (int mt = 0; mt < GT; ++mt)
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
+                    const double f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
+                    const double f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
+                    const double f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
+                    const double f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
+                    F00[mt][i] = f00; F01[mt][i] = f01; F10[mt][i] = f10; F11[mt][i] = f11;
+                    A00[mt][i] += f00; A01[mt][i] += f01; A10[mt][i] += f10; A11[mt][i] += f11;
+                }
+            // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:369-375)
+            d4 R0[GT], R1[GT], R2[GT];
+    #pragma unroll
+            for (int mt = 0; mt < GT; ++mt) {
+                d4 r0 = z, r1 = z, r2 = z;
+    #pragma unroll
+                for (int kt = 0; kt < GT; ++kt)
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int kk = (mt * GT + kt) * 4 + i;
+                        const double dr = sops[D::OFF_DR + kk * 64 + lane];
+                        const double ds = sops[D::OFF_DS + kk * 64 + lane];
+                        r0 = mfma(dr, A00[kt][i], r0); r0 = mfma(ds, A01[kt][i], r0);
+                        r1 = mfma(dr, A10[kt][i], r1); r1 = mfma(ds, A11[kt][i], r1);
+                        r2 = mfma(dr, C0[kt][i], r2);  r2 = mfma(ds, C1[kt][i], r2);
+                    }
+                R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
+            }
+            if (a.lin) {
+                // linearisation data of this 16-element group, coalesced per (field, tile, reg)
+                float* L = a.lin + (size_t)(a.gbase + g) * (5 * GT * 4 * 64);
+    #pragma unroll
+                for (int mt = 0; mt < GT; ++mt)
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int o = (mt * 4 + i) * 64 + lane;
+                        L[0 * GT * 256 + o] = (float)U[mt][i];
+                        L[1 * GT * 256 + o] = (float)V[mt][i];
+                        L[2 * GT * 256 + o] = (float)R0[mt][i];
+                        L[3 * GT * 256 + o] = (float)R1[mt][i];
+                        L[4 * GT * 256 + o] = (float)R2[mt][i];
+                    }
+            }
+            // per-point tau (CALC_TAU2, rsdl.cpp:377-395) and the stabilisation (:397-421)
+            const double dx1 = x0 - x2, dy1 = y0 - y2, dx2 = x1 - x0, dy2 = y1 - y0;
+            const double ar2 = dx1 * dy2 - dy1 * dx2;
+            const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
+                                 sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
+            const double h = (ar2 / perim) / (0.25 * (P + 1) * (P + 1));
+            const double nu = a.mu / rho;
+            const double gv = 2. * nu / h;
+            const double gvis = gv * gv;
+            const double tfac = a.adis * h / cjcb;
+    #pragma unroll
+            for (int mt = 0; mt < GT; ++mt)
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double u = U[mt][i], v = V[mt][i];
+                    const double qq = u * u + v * v;
+                    double gam = 3.0 * qq + gvis;
+                    if (a.mu == 0.0) gam = fmax(gam, 0.1);
+                    const double tau0 = tfac / sqrt(gam);
+                    const double tau2 = sqrt(qq) * tau0;
+                    const double t0 = tau0 * R0[mt][i], t1 = tau0 * R1[mt][i], t2 = tau2 * R2[mt][i];
+                    F00[mt][i] -= (l11 * (2 * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
+                    F01[mt][i] -= (-l10 * (2 * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
+                    F10[mt][i] -= +l11 * v * t0 + (l11 * u - l01 * (2. * v)) * t1 - l01 * t2;
+                    F11[mt][i] -= -l10 * v * t0 + (-l10 * u + l00 * (2. * v)) * t1 + l00 * t2;
+                    C0[mt][i] = -(l11 * t0 - l01 * t1);
+                    C1[mt][i] = -(-l10 * t0 + l00 * t1);
+                }
+            load_old();
+            // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
+    #pragma unroll
+            for (int kt = 0; kt < GT; ++kt)
+    #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = kt * 4 + i;
+                    const double ar = sops[D::OFF_IR + kk * 64 + lane];
+                    const double as = sops[D::OFF_IS + kk * 64 + lane];
+                    LR0 = mfma(ar, F00[kt][i], LR0); LR0 = mfma(as, F01[kt][i], LR0);
+                    LR1 = mfma(ar, F10[kt][i], LR1); LR1 = mfma(as, F11[kt][i], LR1);
+                    LR2 = mfma(ar, C0[kt][i], LR2);  LR2 = mfma(as, C1[kt][i], LR2);
+                }
+        }
+
+
+        if constexpr (!FULL) load_old();
+        // lftog: res = sum lf_im + res_r,  res_r = (1-beta) res_r + beta sum lf_re
+        // (tri_hp/nstage.cpp:21-82).  The first colour that touches a coefficient
+        // starts both sums from keep * res_r; interiors belong to one element.
 #pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                if constexpr (FULL) dr[off + n] += lr[n];
-                di[off + n] += lr[n] + li[n];
+        for (int s = 0; s < KT; ++s) {
+            const int k = 4 * s + q, kind = mode_kind<P>(k);
+            if (S.t < 0 || k >= D::TM) continue;
+            const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
+            const double sg = S.off[s] < 0 ? -1.0 : 1.0;
+            double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
+            double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+            const double li[3] = {sg * LI0[s], sg * LI1[s], sg * LI2[s]};
+            double lr[3] = {0.0, 0.0, 0.0};
+            if constexpr (FULL) {
+                const double bt = a.beta;
+                lr[0] = sg * (LR0[s] * bt);
+                lr[1] = sg * (LR1[s] * bt);
+                lr[2] = sg * (LR2[s] * bt);
+            }
+            if ((S.first >> s) & 1) {
+#pragma unroll
+                for (int n = 0; n < NV; ++n) {
+                    const double r0 = a.keep == 0.0 ? 0.0 : a.keep * ro[n][s];
+                    if (FULL || a.keep != 1.0) dr[off + n] = r0 + lr[n];
+                    di[off + n] = r0 + lr[n] + li[n];
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < NV; ++n) {
+                    if constexpr (FULL) dr[off + n] = ro[n][s] + lr[n];
+                    di[off + n] = io[n][s] + lr[n] + li[n];
+                }
             }
         }
+        S = Sn;
+        cnext = cn2;
+        tnext2 = tn3;
     }
 }
 
@@ -572,12 +667,18 @@ __global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
                 }
             }
     }
+    // every load before the first store: the stores could alias later loads
     const unsigned msk = a.vess[v];
+    T x[NVn], u0[NVn];
 #pragma unroll
     for (int n = 0; n < NVn; ++n) {
-        const T x = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NVn + n];
-        a.resv[v * NVn + n] = x;
-        if (a.ugv) a.ugv[v * NVn + n] = a.ug0v[v * NVn + n] - a.cfa * x;
+        x[n] = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NVn + n];
+        u0[n] = a.ugv ? a.ug0v[v * NVn + n] : T(0);
+    }
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) {
+        a.resv[v * NVn + n] = x[n];
+        if (a.ugv) a.ugv[v * NVn + n] = u0[n] - a.cfa * x[n];
     }
 }
 
@@ -702,14 +803,22 @@ __global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nseg) return;
+    // every load before the first store: the stores could alias later loads
+    T x[SM][NVn], u0[SM][NVn];
 #pragma unroll
     for (int m = 0; m < SM; ++m)
 #pragma unroll
         for (int n = 0; n < NVn; ++n) {
-            const T x = side_scaled<T, NVn>(a, s, m, n);
+            x[m][n] = side_scaled<T, NVn>(a, s, m, n);
+            u0[m][n] = a.ugs ? a.ug0s[(s * a.sm0 + m) * NVn + n] : T(0);
+        }
+#pragma unroll
+    for (int m = 0; m < SM; ++m)
+#pragma unroll
+        for (int n = 0; n < NVn; ++n) {
             const int ix = (s * a.sm0 + m) * NVn + n;
-            a.ress[ix] = x;
-            if (a.ugs) a.ugs[ix] = a.ug0s[ix] - a.cfa * x;
+            a.ress[ix] = x[m][n];
+            if (a.ugs) a.ugs[ix] = u0[m][n] - a.cfa * x[m][n];
         }
 }
 
@@ -775,13 +884,18 @@ __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
             for (int n = 0; n < NVn; ++n) b[k][n] -= c * uh[i][n];
         }
     }
+    T u0[IM][NVn];
+#pragma unroll
+    for (int k = 0; k < IM; ++k)
+#pragma unroll
+        for (int n = 0; n < NVn; ++n) u0[k][n] = a.ugi ? a.ug0i[(t * a.im0 + k) * NVn + n] : T(0);
 #pragma unroll
     for (int k = 0; k < IM; ++k)
 #pragma unroll
         for (int n = 0; n < NVn; ++n) {
             const int ix = (t * a.im0 + k) * NVn + n;
             a.resi[ix] = b[k][n];
-            if (a.ugi) a.ugi[ix] = a.ug0i[ix] - a.cfa * b[k][n];
+            if (a.ugi) a.ugi[ix] = u0[k][n] - a.cfa * b[k][n];
         }
 }
 
@@ -914,12 +1028,12 @@ __global__ void k_mfma_probe_f32(const float* A, const float* B, float* out) {
 }
 
 // explicit instantiations used by the host context
-template __global__ void k_elem_ins<1, true>(ElemArgs, int);
-template __global__ void k_elem_ins<1, false>(ElemArgs, int);
-template __global__ void k_elem_ins<2, true>(ElemArgs, int);
-template __global__ void k_elem_ins<2, false>(ElemArgs, int);
-template __global__ void k_elem_ins<4, true>(ElemArgs, int);
-template __global__ void k_elem_ins<4, false>(ElemArgs, int);
+template __global__ void k_elem_ins<1, true>(ElemArgs);
+template __global__ void k_elem_ins<1, false>(ElemArgs);
+template __global__ void k_elem_ins<2, true>(ElemArgs);
+template __global__ void k_elem_ins<2, false>(ElemArgs);
+template __global__ void k_elem_ins<4, true>(ElemArgs);
+template __global__ void k_elem_ins<4, false>(ElemArgs);
 
 
 // grid-stride 16-byte copy (reference kernel for traffic counters)

@@ -279,6 +279,16 @@ struct CnsElemArgs {
 
 // octet loop of the persistent element kernels: XCD x of the 8 walks the contiguous
 // octet range [x*per, (x+1)*per) with the gridDim/8 blocks it is given
+// LDS exchange among the lanes of one wave.  Every LDS buffer written inside the
+// octet loops is private to a half-wave, so ordering the wave's own LDS accesses is
+// enough: a wave's LDS instructions complete in issue order, and the fences and
+// wave barrier keep the compiler from moving memory operations across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 #define HPG_OCTET_LOOP(noct, oc)                                                          \
     const int nbx_ = gridDim.x >> 3, xcd_ = blockIdx.x & 7, jb_ = blockIdx.x >> 3;       \
     const int per_ = ((noct) + 7) / 8;                                                     \
@@ -296,9 +306,10 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     const double gm1 = a.ph.gamma - 1.0, gogm1 = a.ph.gamma / gm1;
+    __syncthreads();  // operators staged
 
     HPG_OCTET_LOOP(noct, oc) {
-        __syncthreads();  // operators staged / previous octet done with the LDS buffers
+        wave_lds_sync();  // previous octet done with this half-wave's LDS buffers
         const int idx = a.begin + oc * 8 + hw;
         const bool valid = idx < a.end;
         const int t = valid ? a.order[idx] : a.order[a.begin];
@@ -325,7 +336,7 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
             for (int n = 0; n < NVC; ++n)
                 suh[hw][n][g] = valid ? sg * src[off + n] : ((g < 3 && (n == 0 || n == 3)) ? 1.0 : 0.0);
         }
-        __syncthreads();
+        wave_lds_sync();
 
         // values (and derivatives of u, v, RT) at this lane's Gauss point
         double u[NVC], ur[NVC], us[NVC];
@@ -362,7 +373,7 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
         }
 #pragma unroll
         for (int n = 0; n < NVC; ++n) { sf[hw][2 * n][g] = cv[n][0]; sf[hw][2 * n + 1][g] = cv[n][1]; }
-        __syncthreads();
+        wave_lds_sync();
         // lf_im = intgrtrs of the convective fluxes; lane g owns outputs g and g + 32
         double li[2] = {0.0, 0.0};
 #pragma unroll
@@ -406,13 +417,13 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
             df[3][1] = k01 * ur[3] + k11 * us[3];
             df[3][0] += df[1][0] * u[1] + df[2][0] * u[2];
             df[3][1] += df[1][1] * u[1] + df[2][1] * u[2];
-            __syncthreads();  // everyone done reading the convective fluxes
+            wave_lds_sync();  // the half-wave is done reading the convective fluxes
 #pragma unroll
             for (int n = 0; n < NVC; ++n) {
                 sf[hw][2 * n][g] = cv[n][0] + df[n][0];
                 sf[hw][2 * n + 1][g] = cv[n][1] + df[n][1];
             }
-            __syncthreads();
+            wave_lds_sync();
             // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:425-429)
             double res[NVC] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 5
@@ -461,10 +472,10 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
                         df[m][1] -= (-l10 * A[m][n] + l00 * B[m][n]) * tres[n];
                     }
             }
-            __syncthreads();  // everyone done reading the total fluxes
+            wave_lds_sync();  // the half-wave is done reading the total fluxes
 #pragma unroll
             for (int n = 0; n < NVC; ++n) { sf[hw][2 * n][g] = df[n][0]; sf[hw][2 * n + 1][g] = df[n][1]; }
-            __syncthreads();
+            wave_lds_sync();
             // lf_re = beta * intgrtrs(viscous - stabilisation fluxes) (rsdl.cpp:490-497)
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
@@ -492,7 +503,7 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
                 const int off = mode_offset<P, NVC>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg) + n;
                 double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
                 double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
-                const bool first = kind == 2 || (kind == 0 ? a.fcv[vtx[k]] : a.fcs[seg[(k - 3) / SMD]]) == a.colour;
+                const bool first = kind == 2 || (kind == 0 ? a.fcv[sel3(vtx, k)] : a.fcs[sel3(seg, (k - 3) / SMD)]) == a.colour;
                 const double vli = sg * li[r], vlr = sg * lr[r];
                 if (first) {
                     const double r0 = a.keep == 0.0 ? 0.0 : a.keep * dr[off];
@@ -932,8 +943,9 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
     for (int k = threadIdx.x; k < G * 16; k += blockDim.x) sI[k] = a.ops[D::OP_I + k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     const double gm1 = a.gamma - 1.0;
+    __syncthreads();  // operators staged
     HPG_OCTET_LOOP(noct, oc) {
-        __syncthreads();
+        wave_lds_sync();  // previous octet done with this half-wave's LDS buffers
         const int idx = oc * 8 + hw;
         const bool valid = idx < a.ntri;
         const int t = valid ? idx : 0;
@@ -950,11 +962,12 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
             for (int n = 0; n < NVC; ++n) {
                 double u, rr, tt;
                 if (k < 3) {
-                    u = a.ugv[vtx[k] * NVC + n]; rr = a.rtv[vtx[k] * NVC + n]; tt = a.resv[vtx[k] * NVC + n];
+                    const int ix = sel3(vtx, k) * NVC + n;
+                    u = a.ugv[ix]; rr = a.rtv[ix]; tt = a.resv[ix];
                 } else if (k < BM) {
                     const int i = (k - 3) / SM, m = (k - 3) - i * SM;
-                    const double sg = ((m & 1) && sgn[i] < 0) ? -1.0 : 1.0;
-                    const int ix = (seg[i] * a.sm0 + m) * NVC + n;
+                    const double sg = ((m & 1) && sel3(sgn, i) < 0) ? -1.0 : 1.0;
+                    const int ix = (sel3(seg, i) * a.sm0 + m) * NVC + n;
                     u = sg * a.ugs[ix]; rr = sg * a.rts[ix]; tt = sg * a.ress[ix];
                 } else {
                     const int ix = (t * a.im0 + (k - BM)) * NVC + n;
@@ -965,7 +978,7 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
                 sc[hw][2][n][k] = tt;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         double w[NVC], r[NVC], tp[NVC];
 #pragma unroll
         for (int n = 0; n < NVC; ++n) { w[n] = 0.0; r[n] = 0.0; tp[n] = 0.0; }
@@ -992,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
         lower4(L, r);
 #pragma unroll
         for (int n = 0; n < NVC; ++n) sf[hw][n][g] = tp[n] - r[n];
-        __syncthreads();
+        wave_lds_sync();
         // intgrt onto the interior modes, then the interior banded solve (dpbtrs)
         if (g < NVC * IM) {
             const int n = g / IM, m = g - n * IM, k = BM + m;
@@ -1001,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
             for (int q = 0; q < G; ++q) s += sI[q * 16 + k] * sf[hw][n][q];
             so[hw][n][m] = s;
         }
-        __syncthreads();
+        wave_lds_sync();
         if (valid && g < NVC) {
             const int n = g;
             double x[IM];

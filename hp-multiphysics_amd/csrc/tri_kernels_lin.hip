@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a, int nlogical) {
         float* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
         float* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
         const double* sr = kind == 0 ? a.srcv : (kind == 1 ? a.srcs : a.srci);
-        const bool first = kind == 2 || (kind == 0 ? a.fcv[vtx[k]] : a.fcs[seg[(k - 3) / SMD]]) == a.colour;
+        const bool first = kind == 2 || (kind == 0 ? a.fcv[sel3(vtx, k)] : a.fcs[sel3(seg, (k - 3) / SMD)]) == a.colour;
         const float li[3] = {sg * LI0[i], sg * LI1[i], sg * LI2[i]};
         float lr[3] = {0.f, 0.f, 0.f};
         if constexpr (FULL) {
