@@ -13,10 +13,14 @@ Also reported (same JSON line):
   and the FP64 judge after the same number of steps for both precisions;
 * ``cns``: the compressible path on the 724 x 724 p = 4 workload (FP64 step
   throughput and the element kernel's rate);
+* ``tet``: the tetrahedral ins path on 64^3 cubes -> 1 572 864 tetrahedra, p = 2
+  (step throughput and the element kernel's rate);
 * ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual with the
-  p-multigrid cycle (p = 4 -> 2 -> 1) on a smaller mesh of the same family, for
-  FP64 plain, mixed, and mixed + learned preconditioner (training time apart),
-  with each converged solution's difference from the FP64 plain one.
+  h+p multigrid cycle on the same 512 x 512 p = 4 mesh family at mu = 1e-2, for FP64
+  plain, mixed, mixed + learned preconditioner and FP64 + learned (training time
+  apart), with each converged solution's difference from the FP64 plain one.  (At
+  mu = 1e-3 the reference's explicit relaxation does not reach 1e-10 on meshes of
+  128 x 128 and finer with this cycle.)
 
 Prints one JSON line (see the driver contract).  The CPU baseline is the oracle
 (single thread) on one step of the same 512 x 512 workload.
@@ -99,18 +103,73 @@ def cns_throughput(args) -> dict:
     return out
 
 
+# tet_hp ins p = 2: the reference's sum-factorised transforms, 13 000 MAC per element
+# (4 fields x proj with r, s, t derivatives, intgrtrst x 8, intgrt x 4, deriv r/s/t),
+# plus about 150 pointwise flop at each of the 27 Gauss points
+FLOP_TET_FULL_P2 = 30000.0
+
+
+def tet_throughput(args) -> dict:
+    """tet_hp ins on the 64^3 (1 572 864 tetrahedra) p = 2 workload: solenoidal Fourier
+    velocity, inflow on every face; FP64 step throughput and the element kernel's rate."""
+    from hpmp import tetmesh as TM, workloads as W
+    from hpmp.tet_hp import TetHpIns
+
+    n = args.tet_n
+    mesh = TM.unit_cube(n)
+    st = W.tet_solenoidal_state(mesh, 2, seed=4)
+    # CFL 0 holds the state: every step does the full work of setup_preconditioner
+    # + 5-stage update on the same admissible state
+    g = TetHpIns(mesh, log2p=1, mu=1e-2, cfl=(0.0, 0.0))
+    g.set_state(*st)
+    unknowns = 4 * (mesh.npnt + mesh.nseg)
+    g.setup_preconditioner()
+    g.update()
+    g.synchronize()
+    g.L.hpg_tet_kernel_timing(g.h, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.tet_steps):
+        if g.setup_preconditioner():
+            raise SystemExit("tet setup_preconditioner error")
+        g.update()
+    g.synchronize()
+    t1 = time.perf_counter()
+    tot = C.c_double(); nl = C.c_longlong(); ne = C.c_longlong()
+    g.L.hpg_tet_kernel_timing_read(g.h, 1, C.byref(tot), C.byref(nl), C.byref(ne))
+    g.L.hpg_tet_kernel_timing(g.h, 0)
+    ach = FLOP_TET_FULL_P2 * ne.value / (tot.value * 1e-3) / 1e12 if tot.value > 0 else 0.0
+    out = {"workload": f"tet_hp ins {n}^3 cubes -> {mesh.ntet} tetrahedra (6 per cube), p=2, mu=1e-2, "
+                       "u = curl psi (16 Fourier modes), inflow on every face; step = setup_preconditioner + "
+                       "5-stage update at CFL 0 (full work, state held)",
+           "unknowns": unknowns, "steps": args.tet_steps,
+           "ms_per_step": 1e3 * (t1 - t0) / args.tet_steps,
+           "dof_evals_per_s": unknowns * 5 * args.tet_steps / (t1 - t0),
+           "roofline": {"bound": "fp64 valu", "kernel": "k_elem_tet<2,full>", "achieved": ach,
+                        "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
+                        "avg_launch_ms": tot.value / max(1, nl.value),
+                        "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_TET_FULL_P2}}
+    g.close()
+    return out
+
+
 def time_to_tol(args) -> dict:
-    """Cycles and wall time to a 1e-10 relative FP64 residual: plain, mixed, mixed + learned."""
-    from hpmp import _lib, mesh as M, workloads as W
+    """Cycles and wall time to a 1e-10 relative FP64 residual with the h+p multigrid
+    cycle (block::cycle over p = 4 -> 2 -> 1 on the fine mesh, then every coarser mesh
+    down to 4 x 4 squares): FP64 plain, mixed, mixed + learned, FP64 + learned."""
+    from hpmp import _lib, mesh as M, multigrid as MG, workloads as W
     from hpmp.learned import MLP, GuardedSolver, calibrate, training_data
     from hpmp.tri_hp import TriHpIns, rel_diff_mod_pressure
 
-    n, mu, mg = args.tts_n, args.tts_mu, 3
+    n, mu = args.tts_n, args.tts_mu
+    ngrid = int(np.log2(n // 4)) + 1
     mesh = M.unit_square(n, 0.2, 1)
     st = W.fourier_state(mesh, 4, seed=2)
+    meshes, links = MG.square_hierarchy(mesh, n, ngrid)
     g = TriHpIns(mesh, log2p=2, mu=mu)
+    grids = MG.build_stack(TriHpIns, g, meshes, links, mu=mu)
     res = {"workload": f"tri_hp ins {n}x{n} squares ({mesh.ntri} triangles, jitter 0.2h), p=4, mu={mu}, "
-                       "Fourier state; p-multigrid cycle 4->2->1, judge every 25 cycles",
+                       f"Fourier state; h+p multigrid cycle (p 4->2->1, then {ngrid - 1} coarser meshes to 4x4), "
+                       "one update before and one after each coarse correction, judge every 25 cycles",
            "tol_rel": 1e-10}
     ref = None
 
@@ -120,8 +179,8 @@ def time_to_tol(args) -> dict:
         g.set_state(*st)
         g.synchronize()
         t0 = time.perf_counter()
-        gs = GuardedSolver(g, net)
-        cyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles, mglvls=mg)
+        gs = GuardedSolver(g, net, grids=grids)
+        cyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles)
         g.synchronize()
         t1 = time.perf_counter()
         sol = g.get_state()
@@ -131,6 +190,7 @@ def time_to_tol(args) -> dict:
                     "converged": bool(hist[-1][1] <= 1e-10 * hist[0][1]),
                     "rel_diff_vs_fp64_plain_mod_pressure_constant": rel_diff_mod_pressure(sol, ref),
                     "fell_back_at": gs.fell_back_at}
+        print(f"# time_to_tol {tag}: {cyc} cycles, {t1 - t0:.1f} s", file=sys.stderr, flush=True)
 
     run("fp64_plain", _lib.HPG_FP64, None)
     run("mixed", _lib.HPG_FP32_DEFECT, None)
@@ -138,15 +198,19 @@ def time_to_tol(args) -> dict:
     states = [W.fourier_state(mesh, 4, seed=s) for s in (11, 12, 13, 14)]
     g.set_precision(_lib.HPG_FP64)
     X, T, _ = training_data(g, states)
+    pick = np.random.default_rng(0).permutation(X.shape[0])[:args.tts_samples]
+    X, T = X[pick], T[pick]
     net = MLP(11)
     loss = net.train(X, T, epochs=200, batch=256, lr=2e-3)
-    scale, rates = calibrate(g, net, states[0], mglvls=mg)
+    scale, rates = calibrate(g, net, states[0], cycles=50, grids=grids)
     res["learned_training"] = {"seconds": time.perf_counter() - t0, "samples": int(X.shape[0]),
                                "final_loss": loss, "calibrated_scale": scale}
     run("mixed_learned", _lib.HPG_FP32_DEFECT, net)
     run("fp64_learned", _lib.HPG_FP64, net)
     for k in ("mixed", "mixed_learned", "fp64_learned"):
         res[k]["speedup_vs_fp64_plain"] = res["fp64_plain"]["seconds"] / res[k]["seconds"]
+    for x in grids[:0:-1]:
+        x.close()
     g.close()
     net.close()
     return res
@@ -164,12 +228,16 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--ninner", type=int, default=2)
     ap.add_argument("--no-tts", action="store_true")
-    ap.add_argument("--tts-n", type=int, default=16)
-    ap.add_argument("--tts-mu", type=float, default=5e-2)
-    ap.add_argument("--tts-max-cycles", type=int, default=6000)
+    ap.add_argument("--tts-n", type=int, default=512)
+    ap.add_argument("--tts-mu", type=float, default=1e-2)
+    ap.add_argument("--tts-max-cycles", type=int, default=3000)
+    ap.add_argument("--tts-samples", type=int, default=65536)
     ap.add_argument("--no-cns", action="store_true")
     ap.add_argument("--cns-n", type=int, default=724)
     ap.add_argument("--cns-steps", type=int, default=3)
+    ap.add_argument("--no-tet", action="store_true")
+    ap.add_argument("--tet-n", type=int, default=64)
+    ap.add_argument("--tet-steps", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
@@ -290,6 +358,8 @@ def main() -> int:
         out["time_to_tol"] = time_to_tol(args)
     if not args.no_cns:
         out["cns"] = cns_throughput(args)
+    if not args.no_tet:
+        out["tet"] = tet_throughput(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

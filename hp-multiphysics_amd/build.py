@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhpgpu.so")
-SOURCES = ["hpg_tri.hip", "mlp.hip", "tri_ops.cpp"]
+SOURCES = ["hpg_tri.hip", "mlp.hip", "tri_ops.cpp", "tet_ops.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -1,0 +1,494 @@
+// Context and C-ABI of the tet_hp_ins residual/relaxation path (include/hpgpu.h,
+// hpg_tet_*).  Compiled in the same translation unit as hpg_tri.hip (included at its
+// end), whose host helpers (check, nblk8, HPG_TRY, resident_blocks, persistent_grid)
+// and shared kernels (k_modes, k_absmax) it uses.
+#include "tet_kernels.hip"
+#include "tet_ops.h"
+
+namespace {
+
+struct TetLevel {
+    int p = 0;
+    double* ops = nullptr;   // TDims<p>::SIZE
+    double* P2 = nullptr;    // face values GF x tm2d
+    double* W2 = nullptr;    // face weights GF
+    double* vfms = nullptr;  // 4 x 10
+    double vdiag = 0.0, ediag = 0.0, sfmv0 = 0.0, sfmv1 = 0.0;
+};
+
+template <int P>
+std::vector<double> build_tet_pack(const TetOps& o) {
+    using D = TDims<P>;
+    std::vector<double> pk(D::SIZE, 0.0);
+    const std::vector<double>* proj[4] = {&o.P, &o.Pr, &o.Ps, &o.Pt};
+    for (int op = 0; op < 4; ++op)
+        for (int m = 0; m < D::TM; ++m)
+            for (int g = 0; g < D::G; ++g) pk[op * D::TM * 32 + m * 32 + g] = (*proj[op])[g * D::TM + m];
+    const std::vector<double>* integ[3] = {&o.Ir, &o.Is, &o.It};
+    for (int op = 0; op < 3; ++op)
+        for (int g = 0; g < D::G; ++g)
+            for (int m = 0; m < D::TM; ++m) pk[D::OP_IR + op * D::G * 16 + g * 16 + m] = (*integ[op])[m * D::G + g];
+    const std::vector<double>* der[3] = {&o.Dr, &o.Ds, &o.Dt};
+    for (int op = 0; op < 3; ++op)
+        for (int q = 0; q < D::G; ++q)
+            for (int g = 0; g < D::G; ++g) pk[D::OP_DR + op * D::G * 32 + q * 32 + g] = (*der[op])[g * D::G + q];
+    return pk;
+}
+
+// CSR of entity -> codes, built from (entity, code) pairs in ascending code order
+void csr(int nent, const std::vector<std::pair<int, int>>& pairs, std::vector<int>& ptr, std::vector<int>& code) {
+    ptr.assign(nent + 1, 0);
+    for (auto& pr : pairs) ptr[pr.first + 1]++;
+    for (int i = 0; i < nent; ++i) ptr[i + 1] += ptr[i];
+    code.assign(pairs.size(), 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (auto& pr : pairs) code[fill[pr.first]++] = pr.second;
+}
+
+}  // namespace
+
+struct hpg_tet_ctx {
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    std::string err;
+    int npnt = 0, nseg = 0, ntet = 0, nbf = 0, log2pmax = 0, log2p = 0, em0 = 0;
+    hpg_params prm{};
+    bool coarse_flag = false, isfrst = false;
+    double *pnts = nullptr;
+    int *tet_pnt = nullptr, *tet_seg = nullptr, *bf_pnt = nullptr, *bf_seg = nullptr;
+    unsigned char *vess = nullptr, *eess = nullptr;
+    int *vt_ptr = nullptr, *vt_code = nullptr, *et_ptr = nullptr, *et_code = nullptr;
+    int *ve_ptr = nullptr, *ve_code = nullptr, *vf_ptr = nullptr, *vf_code = nullptr, *ef_ptr = nullptr,
+        *ef_code = nullptr;
+    double *ugv = nullptr, *uge = nullptr, *ug0v = nullptr, *ug0e = nullptr, *resv = nullptr, *rese = nullptr;
+    double *rrv = nullptr, *rre = nullptr, *r0v = nullptr, *r0e = nullptr;
+    double *dresv[2] = {nullptr, nullptr}, *drese[2] = {nullptr, nullptr};
+    double *vprcn = nullptr, *eprcn = nullptr, *iprcn = nullptr, *tau = nullptr, *lf = nullptr, *flf = nullptr;
+    int* d_err = nullptr;
+    unsigned long long* d_max = nullptr;
+    TetLevel lv[2];
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    struct Rec { int full, nelem; };
+    std::vector<Rec> recs;
+
+    hipEvent_t event(size_t k) {
+        while (events.size() <= k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+            events.push_back(e);
+        }
+        return events[k];
+    }
+    template <class T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+        allocs.push_back(p);
+        check(hipMemsetAsync(p, 0, n * sizeof(T), stream), "memset");
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* upload(const T* h, size_t n) {
+        T* d = alloc<T>(n);
+        if (n) check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream), "H2D");
+        return d;
+    }
+    ~hpg_tet_ctx() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : allocs) hipFree(p);
+        for (hipEvent_t e : events) hipEventDestroy(e);
+        if (stream) hipStreamDestroy(stream);
+    }
+    int em() const { return (1 << log2p) - 1; }
+};
+
+namespace {
+
+template <int P>
+void launch_elem_tet(hpg_tet_ctx* c, int stage) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    TetElemArgs a;
+    a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.tet_sgn = nullptr;
+    a.ntet = c->ntet; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
+    a.ops = c->lv[c->log2p].ops; a.rho = c->prm.rho; a.mu = c->prm.mu; a.beta = beta;
+    const int noct = (c->ntet + 7) / 8;
+    const size_t r = c->recs.size();
+    if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+    if (beta > 0.0) {
+        static const int cap = resident_blocks((const void*)k_elem_tet<P, true>);
+        hipLaunchKernelGGL((k_elem_tet<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+    } else {
+        static const int cap = resident_blocks((const void*)k_elem_tet<P, false>);
+        hipLaunchKernelGGL((k_elem_tet<P, false>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+    }
+    if (c->timing) {
+        check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+        c->recs.push_back({beta > 0.0 ? 1 : 0, c->ntet});
+    }
+}
+
+void tet_modes(hpg_tet_ctx* c, int op, double* xv, double* xe, double* yv, double* ye, const double* zv,
+               const double* ze, double k) {
+    const long nv = (long)c->npnt * TNV;
+    hipLaunchKernelGGL(k_modes<TNV>, dim3(nblk(nv)), dim3(256), 0, c->stream, op, (long)c->npnt, 1, 1, xv, yv, zv, k);
+    const int em = c->em();
+    if (em > 0) {
+        const long ne = (long)c->nseg * em * TNV;
+        hipLaunchKernelGGL(k_modes<TNV>, dim3(nblk(ne)), dim3(256), 0, c->stream, op, (long)c->nseg, em, c->em0, xe, ye,
+                           ze, k);
+    }
+}
+
+// tet_hp::rsdl (tet_hp/update.cpp:9-131)
+void do_rsdl_tet(hpg_tet_ctx* c, int stage) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const TetLevel& L = c->lv[c->log2p];
+    if (L.p == 1) launch_elem_tet<1>(c, stage); else launch_elem_tet<2>(c, stage);
+    if (c->nbf > 0) {
+        if (L.p == 1)
+            hipLaunchKernelGGL(k_face_tet<1>, dim3(nblk8(c->nbf)), dim3(256), 0, c->stream, c->nbf, c->bf_pnt, c->bf_seg,
+                               c->pnts, c->ugv, c->uge, c->em0, L.P2, L.W2, c->prm.rho, c->flf);
+        else
+            hipLaunchKernelGGL(k_face_tet<2>, dim3(nblk8(c->nbf)), dim3(256), 0, c->stream, c->nbf, c->bf_pnt, c->bf_seg,
+                               c->pnts, c->ugv, c->uge, c->em0, L.P2, L.W2, c->prm.rho, c->flf);
+    }
+    const int tm = L.p == 1 ? 4 : 10, tm2 = L.p == 1 ? 3 : 6;
+    TetGatherArgs g;
+    g.tm = tm; g.tm2 = tm2; g.lf = c->lf; g.flf = c->flf; g.omb = 1.0 - beta;
+    g.nent = c->npnt; g.mbase = 0; g.fbase = 0; g.stride = TNV; g.ptr = c->vt_ptr; g.code = c->vt_code;
+    g.fptr = c->vf_ptr; g.fcode = c->vf_code; g.res = c->resv; g.rr = c->rrv;
+    hipLaunchKernelGGL(k_gather_tet, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, g);
+    if (c->em() > 0) {
+        g.nent = c->nseg; g.mbase = 4; g.fbase = 3; g.stride = c->em0 * TNV; g.ptr = c->et_ptr; g.code = c->et_code;
+        g.fptr = c->ef_ptr; g.fcode = c->ef_code; g.res = c->rese; g.rr = c->rre;
+        hipLaunchKernelGGL(k_gather_tet, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, g);
+    }
+    if (c->coarse_flag) {
+        const int l = c->log2p;
+        if (c->isfrst) {
+            tet_modes(c, 2, c->resv, c->rese, c->dresv[l], c->drese[l], c->r0v, c->r0e, c->prm.fadd);
+            c->isfrst = false;
+        } else {
+            tet_modes(c, 1, c->resv, c->rese, c->dresv[l], c->drese[l], nullptr, nullptr, 0.0);
+        }
+    }
+    check(hipGetLastError(), "tet rsdl launch");
+}
+
+TetMinvArgs tet_minv_args(hpg_tet_ctx* c, bool pre_dirichlet) {
+    const TetLevel& L = c->lv[c->log2p];
+    TetMinvArgs a{};
+    a.npnt = c->npnt; a.nseg = c->nseg; a.em0 = c->em0; a.em = c->em(); a.pre_dirichlet = pre_dirichlet ? 1 : 0;
+    a.ve_ptr = c->ve_ptr; a.ve_code = c->ve_code; a.et_ptr = c->et_ptr; a.et_code = c->et_code;
+    a.tet_pnt = c->tet_pnt; a.vess = c->vess; a.eess = c->eess; a.resv = c->resv; a.rese = c->rese;
+    a.vprcn = c->vprcn; a.eprcn = c->eprcn; a.iprcn = c->iprcn;
+    a.vdiag = L.vdiag; a.ediag = L.ediag; a.sfmv0 = L.sfmv0; a.sfmv1 = L.sfmv1; a.vfms = L.vfms;
+    return a;
+}
+
+void do_minvrt_tet(hpg_tet_ctx* c, bool pre_dirichlet, double cfa) {
+    TetMinvArgs a = tet_minv_args(c, pre_dirichlet);
+    if (cfa >= 0.0) {
+        a.ugv = c->ugv; a.ug0v = c->ug0v; a.cfa = cfa;
+        if (a.em > 0) { a.uge = c->uge; a.ug0e = c->ug0e; }
+    }
+    hipLaunchKernelGGL(k_minv_vrtx_tet, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, a);
+    if (a.em > 0) hipLaunchKernelGGL(k_minv_edge_tet, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, a);
+    check(hipGetLastError(), "tet minvrt launch");
+}
+
+int do_setup_tet(hpg_tet_ctx* c) {
+    TetSetupArgs a;
+    a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.ugv = c->ugv; a.ntet = c->ntet; a.p = c->lv[c->log2p].p;
+    a.rho = c->prm.rho; a.mu = c->prm.mu; a.adis = c->prm.adis; a.iprcn = c->iprcn; a.tau = c->tau; a.err = c->d_err;
+    check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
+    hipLaunchKernelGGL(k_setup_tet, dim3(nblk8(c->ntet)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_setup_gather_tet, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vt_ptr,
+                       c->vt_code, c->iprcn, c->vprcn);
+    if (c->em() > 0)
+        hipLaunchKernelGGL(k_setup_gather_tet, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->et_ptr,
+                           c->et_code, c->iprcn, c->eprcn);
+    check(hipGetLastError(), "tet setup launch");
+    int h = 0;
+    check(hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "D2H");
+    check(hipStreamSynchronize(c->stream), "sync");
+    return h;
+}
+
+double tet_absmax(hpg_tet_ctx* c, long nent, int nm, int mstride, const double* x, const unsigned char* mask,
+                  double out[TNV]) {
+    unsigned long long h[TNV] = {0, 0, 0, 0};
+    if (nent > 0 && nm > 0) {
+        check(hipMemsetAsync(c->d_max, 0, TNV * sizeof(unsigned long long), c->stream), "memset");
+        const long n = nent * nm;
+        const unsigned nb = std::min<unsigned>(nblk(n), 2048u);
+        hipLaunchKernelGGL(k_absmax<TNV>, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
+        check(hipMemcpyAsync(h, c->d_max, sizeof(h), hipMemcpyDeviceToHost, c->stream), "memcpy");
+        check(hipStreamSynchronize(c->stream), "sync");
+    }
+    double r = 0.0;
+    for (int n = 0; n < TNV; ++n) {
+        double v;
+        std::memcpy(&v, &h[n], sizeof(double));
+        out[n] = v;
+        r = std::max(r, v);
+    }
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** out) {
+    if (!m || !prm || !out) return -2;
+    *out = nullptr;
+    if (prm->physics != HPG_PHYSICS_INS || prm->log2p < 0 || prm->log2p > 1 || prm->nstage < 1 || prm->nstage > 7)
+        return -2;
+    if (m->npnt <= 0 || m->nseg <= 0 || m->ntet <= 0 || m->nbf < 0) return -2;
+    std::unique_ptr<hpg_tet_ctx> c(new hpg_tet_ctx);
+    try {
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        c->prm = *prm;
+        c->prm.alpha[prm->nstage] = 1.0;
+        c->prm.beta[prm->nstage] = 1.0;
+        c->npnt = m->npnt; c->nseg = m->nseg; c->ntet = m->ntet; c->nbf = m->nbf;
+        c->log2pmax = c->log2p = prm->log2p;
+        c->em0 = (1 << prm->log2p) - 1;
+        const int np = m->npnt, ns = m->nseg, nt = m->ntet, nf = m->nbf;
+        // host checks of the index arrays the kernels gather through
+        for (long k = 0; k < 4L * nt; ++k)
+            if (m->tet_pnt[k] < 0 || m->tet_pnt[k] >= np) throw std::runtime_error("tet_pnt out of range");
+        for (long k = 0; k < 6L * nt; ++k)
+            if (m->tet_seg[k] < 0 || m->tet_seg[k] >= ns) throw std::runtime_error("tet_seg out of range");
+        for (long k = 0; k < 2L * ns; ++k)
+            if (m->seg_pnt[k] < 0 || m->seg_pnt[k] >= np) throw std::runtime_error("seg_pnt out of range");
+        for (long k = 0; k < 3L * nf; ++k)
+            if (m->bf_pnt[k] < 0 || m->bf_pnt[k] >= np || m->bf_seg[k] < 0 || m->bf_seg[k] >= ns)
+                throw std::runtime_error("boundary face index out of range");
+        if ((long)nt * 16 > 2147483647L || (long)nf * 8 > 2147483647L) throw std::runtime_error("mesh too large");
+        c->pnts = c->upload(m->pnts, 3 * (size_t)np);
+        c->tet_pnt = c->upload(m->tet_pnt, 4 * (size_t)nt);
+        c->tet_seg = c->upload(m->tet_seg, 6 * (size_t)nt);
+        c->bf_pnt = c->upload(m->bf_pnt, 3 * (size_t)nf);
+        c->bf_seg = c->upload(m->bf_seg, 3 * (size_t)nf);
+        // essential velocity (bits of u, v, w) on the vertices / edges of the inflow faces
+        std::vector<unsigned char> vess(np, 0), eess(ns, 0);
+        for (long k = 0; k < 3L * nf; ++k) { vess[m->bf_pnt[k]] = 7; eess[m->bf_seg[k]] = 7; }
+        c->vess = c->upload(vess.data(), vess.size());
+        c->eess = c->upload(eess.data(), eess.size());
+        // incidence in ascending element / edge / face order
+        std::vector<std::pair<int, int>> vt, et, ve, vf, ef;
+        vt.reserve(4L * nt); et.reserve(6L * nt);
+        for (int t = 0; t < nt; ++t) {
+            for (int i = 0; i < 4; ++i) vt.push_back({m->tet_pnt[4 * t + i], 16 * t + i});
+            for (int i = 0; i < 6; ++i) et.push_back({m->tet_seg[6 * t + i], 16 * t + i});
+        }
+        for (int e = 0; e < ns; ++e)
+            for (int i = 0; i < 2; ++i) ve.push_back({m->seg_pnt[2 * e + i], 2 * e + i});
+        for (int f = 0; f < nf; ++f)
+            for (int j = 0; j < 3; ++j) {
+                vf.push_back({m->bf_pnt[3 * f + j], 8 * f + j});
+                ef.push_back({m->bf_seg[3 * f + j], 8 * f + j});
+            }
+        std::vector<int> ptr, code;
+        csr(np, vt, ptr, code); c->vt_ptr = c->upload(ptr.data(), ptr.size()); c->vt_code = c->upload(code.data(), code.size());
+        csr(ns, et, ptr, code); c->et_ptr = c->upload(ptr.data(), ptr.size()); c->et_code = c->upload(code.data(), code.size());
+        csr(np, ve, ptr, code); c->ve_ptr = c->upload(ptr.data(), ptr.size()); c->ve_code = c->upload(code.data(), code.size());
+        csr(np, vf, ptr, code); c->vf_ptr = c->upload(ptr.data(), ptr.size()); c->vf_code = c->upload(code.data(), code.size());
+        csr(ns, ef, ptr, code); c->ef_ptr = c->upload(ptr.data(), ptr.size()); c->ef_code = c->upload(code.data(), code.size());
+        const size_t nv = (size_t)np * TNV, ne = (size_t)ns * c->em0 * TNV;
+        double** vv[] = {&c->ugv, &c->ug0v, &c->resv, &c->rrv, &c->r0v};
+        double** ee[] = {&c->uge, &c->ug0e, &c->rese, &c->rre, &c->r0e};
+        for (auto p : vv) *p = c->alloc<double>(nv);
+        for (auto p : ee) *p = c->alloc<double>(ne);
+        for (int l = 0; l <= c->log2pmax; ++l) {
+            c->dresv[l] = c->alloc<double>(nv);
+            c->drese[l] = c->alloc<double>(ne);
+        }
+        c->vprcn = c->alloc<double>(nv);
+        c->eprcn = c->alloc<double>((size_t)ns * TNV);
+        c->iprcn = c->alloc<double>((size_t)nt * TNV);
+        c->tau = c->alloc<double>((size_t)nt * TNV);
+        c->lf = c->alloc<double>((size_t)nt * 2 * TNV * (c->log2pmax ? 10 : 4));
+        c->flf = c->alloc<double>((size_t)nf * 6);
+        c->d_err = c->alloc<int>(1);
+        c->d_max = c->alloc<unsigned long long>(TNV);
+        for (int l = 0; l <= c->log2pmax; ++l) {
+            const TetOps o(1 << l);
+            TetLevel& L = c->lv[l];
+            L.p = o.p;
+            const std::vector<double> pk = o.p == 1 ? build_tet_pack<1>(o) : build_tet_pack<2>(o);
+            L.ops = c->upload(pk.data(), pk.size());
+            L.P2 = c->upload(o.P2.data(), o.P2.size());
+            L.W2 = c->upload(o.W2.data(), o.W2.size());
+            L.vdiag = o.vdiag;
+            if (o.em > 0) {
+                L.ediag = o.ediag[0]; L.sfmv0 = o.sfmv[0]; L.sfmv1 = o.sfmv[1];
+                L.vfms = c->upload(o.vfms.data(), o.vfms.size());
+            }
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+    } catch (const std::exception&) {
+        return -1;
+    }
+    *out = c.release();
+    return 0;
+}
+
+int hpg_tet_destroy(hpg_tet_ctx* c) {
+    delete c;
+    return 0;
+}
+
+static int tet_copy(hpg_tet_ctx* c, bool h2d, double* dv, double* de, const double* hv, const double* he, double* ov,
+                    double* oe) {
+    HPG_TRY(c, {
+        const size_t nv = (size_t)c->npnt * TNV * 8, ne = (size_t)c->nseg * c->em0 * TNV * 8;
+        if (h2d) {
+            check(hipMemcpyAsync(dv, hv, nv, hipMemcpyHostToDevice, c->stream), "H2D");
+            if (ne) check(hipMemcpyAsync(de, he, ne, hipMemcpyHostToDevice, c->stream), "H2D");
+        } else {
+            check(hipMemcpyAsync(ov, dv, nv, hipMemcpyDeviceToHost, c->stream), "D2H");
+            if (ne) check(hipMemcpyAsync(oe, de, ne, hipMemcpyDeviceToHost, c->stream), "D2H");
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+
+int hpg_tet_set_state(hpg_tet_ctx* c, const double* v, const double* e) {
+    if (!c || !v || (c->em0 && !e)) return -2;
+    return tet_copy(c, true, c->ugv, c->uge, v, e, nullptr, nullptr);
+}
+int hpg_tet_get_state(hpg_tet_ctx* c, double* v, double* e) {
+    if (!c || !v || (c->em0 && !e)) return -2;
+    return tet_copy(c, false, c->ugv, c->uge, nullptr, nullptr, v, e);
+}
+int hpg_tet_get_residual(hpg_tet_ctx* c, double* v, double* e) {
+    if (!c || !v || (c->em0 && !e)) return -2;
+    return tet_copy(c, false, c->resv, c->rese, nullptr, nullptr, v, e);
+}
+int hpg_tet_set_residual(hpg_tet_ctx* c, const double* v, const double* e) {
+    if (!c || !v || (c->em0 && !e)) return -2;
+    return tet_copy(c, true, c->resv, c->rese, v, e, nullptr, nullptr);
+}
+int hpg_tet_get_preconditioner(hpg_tet_ctx* c, double* vprcn, double* eprcn, double* iprcn, double* tau) {
+    if (!c || !vprcn || !eprcn || !iprcn || !tau) return -2;
+    HPG_TRY(c, {
+        check(hipMemcpyAsync(vprcn, c->vprcn, (size_t)c->npnt * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(eprcn, c->eprcn, (size_t)c->nseg * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(iprcn, c->iprcn, (size_t)c->ntet * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(tau, c->tau, (size_t)c->ntet * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+int hpg_tet_rsdl(hpg_tet_ctx* c, int stage) {
+    if (!c || stage < 0 || stage > c->prm.nstage) return -2;
+    HPG_TRY(c, { do_rsdl_tet(c, stage); });
+    return 0;
+}
+int hpg_tet_setup_preconditioner(hpg_tet_ctx* c) {
+    if (!c) return -2;
+    int e = 0;
+    HPG_TRY(c, { e = do_setup_tet(c); });
+    return e;
+}
+int hpg_tet_minvrt(hpg_tet_ctx* c) {
+    if (!c) return -2;
+    HPG_TRY(c, { do_minvrt_tet(c, false, -1.0); });
+    return 0;
+}
+int hpg_tet_update(hpg_tet_ctx* c) {
+    if (!c) return -2;
+    HPG_TRY(c, {
+        check(hipMemcpyAsync(c->ug0v, c->ugv, (size_t)c->npnt * TNV * 8, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        if (c->em0)
+            check(hipMemcpyAsync(c->ug0e, c->uge, (size_t)c->nseg * c->em0 * TNV * 8, hipMemcpyDeviceToDevice,
+                                 c->stream), "D2D");
+        for (int s = 0; s < c->prm.nstage; ++s) {
+            do_rsdl_tet(c, s);
+            do_minvrt_tet(c, true, c->prm.alpha[s] * c->prm.cfl[c->log2p]);
+        }
+    });
+    return 0;
+}
+int hpg_tet_maxres(hpg_tet_ctx* c, double* per_var, double* out) {
+    if (!c || !out) return -2;
+    HPG_TRY(c, {
+        double a[TNV];
+        *out = tet_absmax(c, c->npnt, 1, 1, c->resv, nullptr, a);
+        if (per_var)
+            for (int n = 0; n < TNV; ++n) per_var[n] = a[n];
+    });
+    return 0;
+}
+int hpg_tet_resnorm(hpg_tet_ctx* c, double* out) {
+    if (!c || !out) return -2;
+    HPG_TRY(c, {
+        double a[TNV], b[TNV] = {0, 0, 0, 0};
+        double r = tet_absmax(c, c->npnt, 1, 1, c->resv, c->vess, a);
+        if (c->em() > 0) r = std::max(r, tet_absmax(c, c->nseg, c->em(), c->em0, c->rese, c->eess, b));
+        *out = r;
+    });
+    return 0;
+}
+int hpg_tet_mg_restrict(hpg_tet_ctx* c) {
+    if (!c || c->log2p == 0) return -2;
+    HPG_TRY(c, {
+        c->isfrst = true;
+        c->coarse_flag = true;
+        --c->log2p;
+        tet_modes(c, 3, c->resv, c->rese, c->r0v, c->r0e, nullptr, nullptr, 0.0);
+        check(hipGetLastError(), "tet restrict");
+    });
+    return 0;
+}
+int hpg_tet_mg_prolongate(hpg_tet_ctx* c) {
+    if (!c || c->log2p >= c->log2pmax) return -2;
+    ++c->log2p;
+    if (c->log2p == c->log2pmax) c->coarse_flag = false;
+    return 0;
+}
+int hpg_tet_get_log2p(hpg_tet_ctx* c, int* l) {
+    if (!c || !l) return -2;
+    *l = c->log2p;
+    return 0;
+}
+int hpg_tet_synchronize(hpg_tet_ctx* c) {
+    if (!c) return -2;
+    HPG_TRY(c, { check(hipStreamSynchronize(c->stream), "sync"); });
+    return 0;
+}
+const char* hpg_tet_last_error(hpg_tet_ctx* c) { return c ? c->err.c_str() : "null context"; }
+int hpg_tet_kernel_timing(hpg_tet_ctx* c, int on) {
+    if (!c) return -2;
+    c->timing = on != 0;
+    c->recs.clear();
+    return 0;
+}
+int hpg_tet_kernel_timing_read(hpg_tet_ctx* c, int full, double* total_ms, long long* launches, long long* elements) {
+    if (!c || !total_ms || !launches || !elements) return -2;
+    HPG_TRY(c, {
+        check(hipStreamSynchronize(c->stream), "sync");
+        double t = 0.0;
+        long long nl = 0, ne = 0;
+        for (size_t r = 0; r < c->recs.size(); ++r) {
+            if (c->recs[r].full != full) continue;
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, c->events[2 * r], c->events[2 * r + 1]), "hipEventElapsedTime");
+            t += ms;
+            ++nl;
+            ne += c->recs[r].nelem;
+        }
+        *total_ms = t; *launches = nl; *elements = ne;
+    });
+    return 0;
+}
+
+}  // extern "C"

@@ -1459,3 +1459,6 @@ int hpg_selftest_mfma(const double* A, const double* B, double* out) {
 }
 
 }  // extern "C"
+
+// tet_hp_ins path (same translation unit: shares the host helpers above)
+#include "hpg_tet.hip"

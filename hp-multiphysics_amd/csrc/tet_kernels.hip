@@ -1,0 +1,516 @@
+// HIP kernels of the tet_hp / tet_hp_ins residual and relaxation path (gfx950),
+// p in {1, 2}: vertex and edge modes, NV = 4 (u, v, w, p).
+//
+// Element residual: one element per half-wave, one Gauss point per lane (27 points
+// at p = 2, 8 at p = 1).  The transform operators sit in LDS, lane-contiguous; the
+// point fields a reduction needs are exchanged through a half-wave's private LDS
+// rows.  Element residuals go to an element-major buffer lf[t][re/im][n][m] and the
+// gather kernels sum them per vertex / edge over the incident elements in
+// ascending element order, the order of the reference's element loop, so the
+// assembly is deterministic without colouring.
+#include <hip/hip_runtime.h>
+
+namespace hpg {
+
+constexpr int TNV = 4;
+
+template <int P>
+struct TDims {
+    static constexpr int TM = (P + 1) * (P + 2) * (P + 3) / 6;
+    static constexpr int GP = P + 1;
+    static constexpr int G = GP * GP * GP;
+    static constexpr int EM = P - 1;
+    static constexpr int TM2 = 3 + 3 * EM;
+    static constexpr int GF = GP * GP;
+    // operator pack (doubles): P, Pr, Ps, Pt as [m][32] (lane = Gauss point);
+    // Ir, Is, It as [g][16] (lane = mode); Dr, Ds, Dt as [g'][32] (lane g reads D[g][g'])
+    static constexpr int OP_P = 0, OP_PR = TM * 32, OP_PS = 2 * TM * 32, OP_PT = 3 * TM * 32;
+    static constexpr int OP_IR = 4 * TM * 32, OP_IS = OP_IR + G * 16, OP_IT = OP_IS + G * 16;
+    static constexpr int OP_DR = OP_IT + G * 16, OP_DS = OP_DR + G * 32, OP_DT = OP_DS + G * 32;
+    static constexpr int SIZE = OP_DT + G * 32;
+};
+
+struct TetElemArgs {
+    const double* pnts;
+    const int* tet_pnt;
+    const int* tet_seg;
+    const int* tet_sgn;
+    int ntet, em0;
+    const double* ugv;
+    const double* uge;
+    const double* tau;  // ntet x 4 (0: momentum, 3: continuity)
+    double* lf;         // ntet x 2 x 4 x TM: [t][0 = imaginary, 1 = real][n][m]
+    const double* ops;
+    double rho, mu, beta;
+};
+
+// tet_hp_ins::element_rsdl, straight branch (tet_hp/ins/rsdl.cpp:15-87, 442-785),
+// steady, tau frozen by setup_preconditioner
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
+    using D = TDims<P>;
+    constexpr int TM = D::TM, G = D::G;
+    __shared__ double sops[D::SIZE];
+    __shared__ double suh[8][TNV][16];
+    __shared__ double sf[8][12][32];
+    for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
+    const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
+    __syncthreads();  // operators staged
+    HPG_OCTET_LOOP(noct, oc) {
+        wave_lds_sync();  // previous octet done with this half-wave's LDS rows
+        const int t = oc * 8 + hw;
+        const bool valid = t < a.ntet;
+        const int tt = valid ? t : 0;
+        int vtx[4], seg[6];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vtx[k] = a.tet_pnt[4 * tt + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) seg[k] = a.tet_seg[6 * tt + k];
+        // ugtouht (tet_hp/gtol.cpp:17-44); edge mode 0 carries sgn^0 = 1
+        if (g < TM) {
+#pragma unroll
+            for (int n = 0; n < TNV; ++n) {
+                double v;
+                if (g < 4) {
+                    const int ix = (g == 0 ? vtx[0] : g == 1 ? vtx[1] : g == 2 ? vtx[2] : vtx[3]) * TNV + n;
+                    v = a.ugv[ix];
+                } else {
+                    const int e = g - 4;
+                    const int s = e == 0 ? seg[0] : e == 1 ? seg[1] : e == 2 ? seg[2] : e == 3 ? seg[3] : e == 4 ? seg[4] : seg[5];
+                    v = a.uge[(s * a.em0) * TNV + n];
+                }
+                suh[hw][n][g] = valid ? v : (n == 0 && g < 4 ? 1.0 : 0.0);
+            }
+        }
+        // coordinate derivatives (rsdl.cpp:58-62) and cofactors (:449-458); padding
+        // half-waves use the reference tet
+        double X[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) X[k][n] = valid ? a.pnts[3 * vtx[k] + n] : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? 1.0 : -1.0);
+        double ld[3][3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            ld[n][0] = 0.5 * (X[3][n] - X[2][n]);
+            ld[n][1] = 0.5 * (X[1][n] - X[2][n]);
+            ld[n][2] = 0.5 * (X[0][n] - X[2][n]);
+        }
+        double d[3][3];
+        d[0][0] = ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1];
+        d[0][1] = -ld[0][1] * ld[2][2] + ld[0][2] * ld[2][1];
+        d[0][2] = ld[0][1] * ld[1][2] - ld[0][2] * ld[1][1];
+        d[1][0] = -ld[1][0] * ld[2][2] + ld[1][2] * ld[2][0];
+        d[1][1] = ld[0][0] * ld[2][2] - ld[0][2] * ld[2][0];
+        d[1][2] = -ld[0][0] * ld[1][2] + ld[0][2] * ld[1][0];
+        d[2][0] = ld[1][0] * ld[2][1] - ld[1][1] * ld[2][0];
+        d[2][1] = -ld[0][0] * ld[2][1] + ld[0][1] * ld[2][0];
+        d[2][2] = ld[0][0] * ld[1][1] - ld[0][1] * ld[1][0];
+        const double cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
+                            ld[0][1] * (ld[1][0] * ld[2][2] - ld[2][0] * ld[1][2]) +
+                            ld[0][2] * (ld[1][0] * ld[2][1] - ld[2][0] * ld[1][1]);
+        wave_lds_sync();
+        // values (and velocity r, s, t derivatives) at this lane's Gauss point
+        double u[TNV], du[3][3];
+#pragma unroll
+        for (int n = 0; n < TNV; ++n) u[n] = 0.0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) du[n][c] = 0.0;
+        if (g < G) {
+#pragma unroll
+            for (int k = 0; k < TM; ++k) {
+                const double pk = sops[D::OP_P + k * 32 + g];
+#pragma unroll
+                for (int n = 0; n < TNV; ++n) u[n] += pk * suh[hw][n][k];
+                if constexpr (FULL) {
+                    const double pr = sops[D::OP_PR + k * 32 + g], ps = sops[D::OP_PS + k * 32 + g],
+                                 pt = sops[D::OP_PT + k * 32 + g];
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) {
+                        du[n][0] += pr * suh[hw][n][k];
+                        du[n][1] += ps * suh[hw][n][k];
+                        du[n][2] += pt * suh[hw][n][k];
+                    }
+                }
+            }
+        }
+        // convective and pressure fluxes (rsdl.cpp:461-508)
+        double cv[3][3], c3[3];
+        {
+            const double fx = a.rho * u[0], fy = a.rho * u[1], fz = a.rho * u[2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) c3[c] = d[c][0] * fx + d[c][1] * fy + d[c][2] * fz;
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cv[n][c] = u[n] * c3[c] + d[c][n] * u[3];
+        }
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = cv[n][c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sf[hw][9 + c][g] = c3[c];
+        wave_lds_sync();
+        // lf_im = intgrtrst of the fluxes (rsdl.cpp:509-511): lane = mode
+        double* lfo = a.lf + (size_t)tt * (2 * TNV * TM);
+        if (valid && g < TM) {
+#pragma unroll
+            for (int n = 0; n < TNV; ++n) {
+                double s = 0.0;
+#pragma unroll 3
+                for (int q = 0; q < G; ++q)
+                    s += sops[D::OP_IR + q * 16 + g] * sf[hw][3 * n + 0][q] + sops[D::OP_IS + q * 16 + g] * sf[hw][3 * n + 1][q] +
+                         sops[D::OP_IT + q * 16 + g] * sf[hw][3 * n + 2][q];
+                lfo[n * TM + g] = s;
+            }
+        }
+        if constexpr (FULL) {
+            // viscous fluxes (rsdl.cpp:518-674): visc(n,m)(a,b) = -mu/cjcb (delta_nm
+            // sum_c d(a)(c) d(b)(c) + d(a)(m) d(b)(n)) applied to du(m,b)
+            const double cjcbi = a.mu / cjcb;
+            double df[3][3];
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+#pragma unroll
+                for (int aa = 0; aa < 3; ++aa) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int bb = 0; bb < 3; ++bb) {
+                        const double dd = d[aa][0] * d[bb][0] + d[aa][1] * d[bb][1] + d[aa][2] * d[bb][2];
+#pragma unroll
+                        for (int m = 0; m < 3; ++m) s += (d[aa][m] * d[bb][n] + (m == n ? dd : 0.0)) * du[m][bb];
+                    }
+                    df[n][aa] = -cjcbi * s;
+                }
+            wave_lds_sync();  // the half-wave is done reading the convective fluxes
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = cv[n][c] + df[n][c];
+            wave_lds_sync();
+            // strong residual: derivr/derivs/derivt of the total fluxes (rsdl.cpp:683-692)
+            double res[TNV] = {0.0, 0.0, 0.0, 0.0};
+            if (g < G) {
+#pragma unroll 3
+                for (int q = 0; q < G; ++q) {
+                    const double dr = sops[D::OP_DR + q * 32 + g], ds = sops[D::OP_DS + q * 32 + g],
+                                 dt = sops[D::OP_DT + q * 32 + g];
+#pragma unroll
+                    for (int n = 0; n < TNV; ++n)
+                        res[n] += dr * sf[hw][3 * n + 0][q] + ds * sf[hw][3 * n + 1][q] + dt * sf[hw][3 * n + 2][q];
+                }
+            }
+            // stabilisation (rsdl.cpp:695-767)
+            const double t0 = a.tau[4 * tt] * res[0], t1 = a.tau[4 * tt] * res[1], t2 = a.tau[4 * tt] * res[2],
+                         t3 = a.tau[4 * tt + 3] * res[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                df[0][c] -= (d[c][0] * (2. * u[0]) + d[c][1] * u[1] + d[c][2] * u[2]) * t0 + d[c][1] * u[0] * t1 +
+                            d[c][2] * u[0] * t2 + d[c][0] * t3;
+                df[1][c] -= d[c][0] * u[1] * t0 + (d[c][0] * u[0] + d[c][1] * (2. * u[1]) + d[c][2] * u[2]) * t1 +
+                            d[c][2] * u[1] * t2 + d[c][1] * t3;
+                df[2][c] -= d[c][0] * u[2] * t0 + d[c][1] * u[2] * t1 +
+                            (d[c][0] * u[0] + d[c][1] * u[1] + d[c][2] * (2. * u[2])) * t2 + d[c][2] * t3;
+                c3[c] = -d[c][0] * t0 - d[c][1] * t1 - d[c][2] * t2;
+            }
+            wave_lds_sync();  // the half-wave is done reading the total fluxes
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? df[n][c] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sf[hw][9 + c][g] = g < G ? c3[c] : 0.0;
+            wave_lds_sync();
+            // lf_re = beta * intgrtrst (rsdl.cpp:768-776)
+            if (valid && g < TM) {
+#pragma unroll
+                for (int n = 0; n < TNV; ++n) {
+                    double s = 0.0;
+#pragma unroll 3
+                for (int q = 0; q < G; ++q)
+                        s += sops[D::OP_IR + q * 16 + g] * sf[hw][3 * n + 0][q] +
+                             sops[D::OP_IS + q * 16 + g] * sf[hw][3 * n + 1][q] +
+                             sops[D::OP_IT + q * 16 + g] * sf[hw][3 * n + 2][q];
+                    lfo[(TNV + n) * TM + g] = s * a.beta;
+                }
+            }
+        } else {
+            if (valid && g < TM)
+#pragma unroll
+                for (int n = 0; n < TNV; ++n) lfo[(TNV + n) * TM + g] = 0.0;
+        }
+    }
+}
+
+// inflow face flux, continuity only (tet_hp/ins/bdry.cpp:59-102, bdry_ins.h:122-134):
+// flf[f][m] = integral over the face of phi_m rho u.n, n = dx/dr x dx/ds, which for a
+// straight face is (p2 - p1) x (p0 - p1) / 4 at every Gauss point
+template <int P>
+__global__ __launch_bounds__(256) void k_face_tet(int nbf, const int* __restrict__ bf_pnt, const int* __restrict__ bf_seg,
+                                                   const double* __restrict__ pnts, const double* __restrict__ ugv,
+                                                   const double* __restrict__ uge, int em0, const double* __restrict__ P2,
+                                                   const double* __restrict__ W2, double rho, double* __restrict__ flf) {
+    using D = TDims<P>;
+    const int f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= nbf) return;
+    int pv[3], se[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { pv[j] = bf_pnt[3 * f + j]; se[j] = bf_seg[3 * f + j]; }
+    double x[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) x[j][n] = pnts[3 * pv[j] + n];
+    double v1[3], v2[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        v1[n] = 0.5 * (x[2][n] - x[1][n]);
+        v2[n] = 0.5 * (x[0][n] - x[1][n]);
+    }
+    const double nrm[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2], v1[0] * v2[1] - v1[1] * v2[0]};
+    double uh[3][6];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) uh[n][j] = ugv[pv[j] * TNV + n];
+#pragma unroll
+        for (int j = 0; j < 3 * D::EM; ++j) uh[n][3 + j] = uge[(se[j] * em0) * TNV + n];
+    }
+    double lf[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < D::GF; ++q) {
+        double un[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < D::TM2; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) un[n] += P2[q * D::TM2 + m] * uh[n][m];
+        const double flux = rho * (un[0] * nrm[0] + un[1] * nrm[1] + un[2] * nrm[2]) * W2[q];
+#pragma unroll
+        for (int m = 0; m < D::TM2; ++m) lf[m] += P2[q * D::TM2 + m] * flux;
+    }
+#pragma unroll
+    for (int m = 0; m < D::TM2; ++m) flf[f * D::TM2 + m] = lf[m];
+}
+
+// assembly of one entity kind (vertices: mode = local vertex; edges: mode = 4 + local
+// edge): res_r = omb res_r + sum lf_re; res = face fluxes + sum lf_im + res_r
+// (tet_hp/update.cpp:13-31, 45-70, bdry.cpp:16-57); elements and faces ascending
+struct TetGatherArgs {
+    int nent, tm, tm2, mbase, fbase, stride;
+    const int* ptr;   // CSR entity -> (element, local index) codes 16*t + i
+    const int* code;
+    const int* fptr;  // CSR entity -> (face, local index) codes 8*f + j
+    const int* fcode;
+    const double* lf;
+    const double* flf;
+    double* res;
+    double* rr;
+    double omb;
+};
+
+__global__ __launch_bounds__(256) void k_gather_tet(TetGatherArgs a) {
+    const int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= a.nent) return;
+    double si[TNV] = {0.0, 0.0, 0.0, 0.0}, sr[TNV];
+    for (int k = a.fptr[v]; k < a.fptr[v + 1]; ++k) {
+        const int c = a.fcode[k], f = c >> 3, j = c & 7;
+        si[TNV - 1] += a.flf[f * a.tm2 + a.fbase + j];
+    }
+    double* R = a.res + (size_t)v * a.stride;
+    double* RR = a.rr + (size_t)v * a.stride;
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) sr[n] = a.omb * RR[n];
+    for (int k = a.ptr[v]; k < a.ptr[v + 1]; ++k) {
+        const int c = a.code[k], t = c >> 4, i = c & 15;
+        const double* L = a.lf + (size_t)t * (2 * TNV * a.tm) + a.mbase + i;
+#pragma unroll
+        for (int n = 0; n < TNV; ++n) {
+            si[n] += L[n * a.tm];
+            sr[n] += L[(TNV + n) * a.tm];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) {
+        RR[n] = sr[n];
+        R[n] = si[n] + sr[n];
+    }
+}
+
+// setup_preconditioner element part (tet_hp/ins/setup_preconditioner.cpp:37-118)
+struct TetSetupArgs {
+    const double* pnts;
+    const int* tet_pnt;
+    const double* ugv;
+    int ntet, p;
+    double rho, mu, adis;
+    double* iprcn;
+    double* tau;
+    int* err;
+};
+
+__global__ __launch_bounds__(256) void k_setup_tet(TetSetupArgs a) {
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= a.ntet) return;
+    int v[4];
+    double X[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = a.tet_pnt[4 * t + k];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) X[k][n] = a.pnts[3 * v[k] + n];
+    }
+    // vol = |(v3-v2) x (v1-v2) . (v0-v2)| (tet_mesh/tet_prims.cpp:180-191), jcb = vol/8
+    double d1[3], d2[3], d3[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) { d1[n] = X[3][n] - X[2][n]; d2[n] = X[1][n] - X[2][n]; d3[n] = X[0][n] - X[2][n]; }
+    const double cx = d1[1] * d2[2] - d1[2] * d2[1], cy = d1[2] * d2[0] - d1[0] * d2[2], cz = d1[0] * d2[1] - d1[1] * d2[0];
+    double jcb = 0.125 * fabs(cx * d3[0] + cy * d3[1] + cz * d3[2]);
+    // largest / smallest face area, faces ordered as tet_mesh/setup.cpp:305-308
+    const int vf[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
+    double amin = 1.0e99, amax = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        double e1[3], e2[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            e1[n] = X[vf[j][0]][n] - X[vf[j][1]][n];
+            e2[n] = X[vf[j][0]][n] - X[vf[j][2]][n];
+        }
+        const double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2], c2 = e1[0] * e2[1] - e1[1] * e2[0];
+        const double ar = .5 * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+        amax = ar > amax ? ar : amax;
+        amin = ar < amin ? ar : amin;
+    }
+    const double pp = (a.p + 1) * (a.p + 1);
+    const double h = 4.0 * jcb / (0.25 * pp * amax);
+    const double hmax = 4.0 * jcb / (0.25 * pp * amin);
+    double qmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double u = a.ugv[v[k] * TNV], w = a.ugv[v[k] * TNV + 1], z = a.ugv[v[k] * TNV + 2];
+        const double q = u * u + w * w + z * z;
+        qmax = q > qmax ? q : qmax;
+    }
+    if (!(jcb > 0.0) || !(qmax >= 0.0)) atomicOr(a.err, 1);
+    const double nu = a.mu / a.rho;
+    double gam = 3.0 * qmax + (2. * nu / hmax) * (2. * nu / hmax);
+    if (a.mu == 0.0) gam = fmax(gam, 0.1);
+    const double q = sqrt(qmax);
+    const double lam1 = q + sqrt(qmax + gam);
+    a.tau[4 * t] = a.adis * h / (jcb * sqrt(gam));
+    a.tau[4 * t + 3] = qmax * a.tau[4 * t];
+    jcb *= 2. * nu * (1. / (h * h) + 1. / (h * h) + 1. / (h * h)) + 3 * lam1 / h;
+    a.iprcn[4 * t + 0] = a.rho * jcb;
+    a.iprcn[4 * t + 1] = a.rho * jcb;
+    a.iprcn[4 * t + 2] = a.rho * jcb;
+    a.iprcn[4 * t + 3] = jcb / gam;
+}
+
+// prcn(entity) = 1 / sum of the incident elements' iprcn, ascending elements
+// (tet_hp/ins/setup_preconditioner.cpp:119-126, tet_hp/setup_preconditioner.cpp:455-476)
+__global__ __launch_bounds__(256) void k_setup_gather_tet(int nent, const int* __restrict__ ptr,
+                                                          const int* __restrict__ code, const double* __restrict__ iprcn,
+                                                          double* __restrict__ prcn) {
+    const int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= nent) return;
+    double s[TNV] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = ptr[v]; k < ptr[v + 1]; ++k) {
+        const int t = code[k] >> 4;
+#pragma unroll
+        for (int n = 0; n < TNV; ++n) s[n] += iprcn[4 * t + n];
+    }
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) prcn[v * TNV + n] = 1.0 / s[n];
+}
+
+// minvrt for p <= 2 (tet_hp/setup_preconditioner.cpp:11-210), diagonal preconditioner.
+// pre_dirichlet: the update's vdirichlet/edirichlet before minvrt (update.cpp:218-223)
+struct TetMinvArgs {
+    int npnt, nseg, em0, em, pre_dirichlet;
+    const int* ve_ptr;   // vertex -> (edge, end) codes 2*e + end, ascending edges
+    const int* ve_code;
+    const int* et_ptr;   // edge -> (element, local edge) codes 16*t + i, ascending elements
+    const int* et_code;
+    const int* tet_pnt;
+    const unsigned char* vess;
+    const unsigned char* eess;
+    double* resv;
+    double* rese;
+    const double* vprcn;
+    const double* eprcn;
+    const double* iprcn;
+    double vdiag, ediag, sfmv0, sfmv1;
+    const double* vfms;  // 4 x 10 vertex rows of the mass matrix
+    // fused stage update ug = ug0 - cfa res when ugv != null
+    double* ugv;
+    double* uge;
+    const double* ug0v;
+    const double* ug0e;
+    double cfa;
+};
+
+__global__ __launch_bounds__(256) void k_minv_vrtx_tet(TetMinvArgs a) {
+    const int v = xcd_block() * blockDim.x + threadIdx.x;
+    if (v >= a.npnt) return;
+    double r[TNV];
+    const bool vd = a.vess[v] != 0;
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) r[n] = (a.pre_dirichlet && vd && n < 3) ? 0.0 : a.resv[v * TNV + n];
+    if (a.em > 0)
+        for (int k = a.ve_ptr[v]; k < a.ve_ptr[v + 1]; ++k) {
+            const int c = a.ve_code[k], e = c >> 1;
+            const double sf = (c & 1) ? a.sfmv1 : a.sfmv0;
+            const bool ed = a.pre_dirichlet && a.eess[e];
+#pragma unroll
+            for (int n = 0; n < TNV; ++n) r[n] -= sf * ((ed && n < 3) ? 0.0 : a.rese[(e * a.em0) * TNV + n]);
+        }
+    double x[TNV], u0[TNV];
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) {
+        x[n] = (vd && n < 3) ? 0.0 : r[n] * (a.vprcn[v * TNV + n] * a.vdiag);
+        u0[n] = a.ugv ? a.ug0v[v * TNV + n] : 0.0;
+    }
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) {
+        a.resv[v * TNV + n] = x[n];
+        if (a.ugv) a.ugv[v * TNV + n] = u0[n] - a.cfa * x[n];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_minv_edge_tet(TetMinvArgs a) {
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= a.nseg) return;
+    const bool ed = a.eess[e] != 0;
+    double r[TNV];
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) r[n] = (a.pre_dirichlet && ed && n < 3) ? 0.0 : a.rese[(e * a.em0) * TNV + n];
+    for (int k = a.et_ptr[e]; k < a.et_ptr[e + 1]; ++k) {
+        const int c = a.et_code[k], t = c >> 4, i = c & 15;
+        double ip[TNV];
+#pragma unroll
+        for (int n = 0; n < TNV; ++n) ip[n] = a.iprcn[4 * t + n];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int w = a.tet_pnt[4 * t + j];
+            const double cf = a.vfms[j * 10 + 4 + i];
+#pragma unroll
+            for (int n = 0; n < TNV; ++n) r[n] -= cf * (a.resv[w * TNV + n] * ip[n]);
+        }
+    }
+    double x[TNV], u0[TNV];
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) {
+        x[n] = (ed && n < 3) ? 0.0 : r[n] * (a.eprcn[e * TNV + n] * a.ediag);
+        u0[n] = a.uge ? a.ug0e[(e * a.em0) * TNV + n] : 0.0;
+    }
+#pragma unroll
+    for (int n = 0; n < TNV; ++n) {
+        a.rese[(e * a.em0) * TNV + n] = x[n];
+        if (a.uge) a.uge[(e * a.em0) * TNV + n] = u0[n] - a.cfa * x[n];
+    }
+}
+
+}  // namespace hpg

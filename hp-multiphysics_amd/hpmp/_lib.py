@@ -31,6 +31,11 @@ EXPORTS = (
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
     "hpg_get_block_preconditioner", "hpg_link_fine",
+    "hpg_tet_create", "hpg_tet_destroy", "hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual",
+    "hpg_tet_set_residual", "hpg_tet_get_preconditioner", "hpg_tet_rsdl", "hpg_tet_setup_preconditioner",
+    "hpg_tet_minvrt", "hpg_tet_update", "hpg_tet_maxres", "hpg_tet_resnorm", "hpg_tet_mg_restrict",
+    "hpg_tet_mg_prolongate", "hpg_tet_get_log2p", "hpg_tet_synchronize", "hpg_tet_last_error",
+    "hpg_tet_kernel_timing", "hpg_tet_kernel_timing_read",
 )
 
 
@@ -41,6 +46,14 @@ class TriMeshC(C.Structure):
         ("tri_seg", C.c_void_p), ("tri_sgn", C.c_void_p),
         ("nebd", C.c_int), ("ebdry_start", C.c_void_p), ("ebdry_seg", C.c_void_p),
         ("ebdry_essential", C.c_void_p),
+    ]
+
+
+class TetMeshC(C.Structure):
+    _fields_ = [
+        ("npnt", C.c_int), ("nseg", C.c_int), ("ntet", C.c_int), ("nbf", C.c_int),
+        ("pnts", C.c_void_p), ("seg_pnt", C.c_void_p), ("tet_pnt", C.c_void_p), ("tet_seg", C.c_void_p),
+        ("tet_sgn", C.c_void_p), ("bf_pnt", C.c_void_p), ("bf_seg", C.c_void_p), ("bf_sgn", C.c_void_p),
     ]
 
 
@@ -78,6 +91,23 @@ def load():
     L.hpg_set_tprcn.argtypes = [vp, dp]
     L.hpg_get_block_preconditioner.argtypes = [vp, dp, dp, dp]
     L.hpg_link_fine.argtypes = [vp, vp, dp, dp, dp, dp]
+    L.hpg_tet_create.argtypes = [vp, vp, vp]
+    L.hpg_tet_destroy.argtypes = [vp]
+    for nm in ("hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual", "hpg_tet_set_residual"):
+        getattr(L, nm).argtypes = [vp, dp, dp]
+    L.hpg_tet_get_preconditioner.argtypes = [vp, dp, dp, dp, dp]
+    L.hpg_tet_rsdl.argtypes = [vp, C.c_int]
+    for nm in ("hpg_tet_setup_preconditioner", "hpg_tet_minvrt", "hpg_tet_update", "hpg_tet_mg_restrict",
+               "hpg_tet_mg_prolongate", "hpg_tet_synchronize"):
+        getattr(L, nm).argtypes = [vp]
+    L.hpg_tet_maxres.argtypes = [vp, dp, dp]
+    L.hpg_tet_resnorm.argtypes = [vp, dp]
+    L.hpg_tet_get_log2p.argtypes = [vp, C.POINTER(C.c_int)]
+    L.hpg_tet_last_error.argtypes = [vp]
+    L.hpg_tet_last_error.restype = C.c_char_p
+    L.hpg_tet_kernel_timing.argtypes = [vp, C.c_int]
+    L.hpg_tet_kernel_timing_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                             C.POINTER(C.c_longlong)]
     L.hpg_rsdl.argtypes = [vp, C.c_int]
     for nm in ("hpg_setup_preconditioner", "hpg_minvrt", "hpg_update", "hpg_mg_restrict", "hpg_mg_prolongate",
                "hpg_synchronize"):

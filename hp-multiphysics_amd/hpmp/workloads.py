@@ -127,3 +127,34 @@ def cns_gaussian_state(mesh: TriMesh, p: int, seed: int = 0, mach: float = 0.5, 
             i = i + noise * (2.0 ** -int_k)[None, :, None] * rng.standard_normal(i.shape)
     _hold_cns_boundary(mesh, v, s, bc)
     return (v, s, i), bc
+
+
+def tet_solenoidal_state(mesh, p: int, seed: int = 0, nmodes: int = 16, kmax: int = 4):
+    """Velocity u = curl psi, psi = sum of `nmodes` seeded Fourier modes A sin(k.x + phi)
+    (so div u = 0 exactly), scaled to max |u| = 1 at the vertices, pressure 0.  Vertex
+    values are point values; the p = 2 edge coefficient interpolates the edge midpoint:
+    u(mid) = (u_a + u_b)/2 + c/4 for the edge mode lambda_a lambda_b.  Returns (v, e)
+    in the layouts v(npnt, 4), e(nseg, p-1, 4)."""
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(-kmax, kmax + 1, size=(nmodes, 3)).astype(np.float64) * np.pi
+    ks[np.all(ks == 0, axis=1)] = np.pi
+    amps = rng.standard_normal((nmodes, 3))
+    phis = rng.uniform(0, 2 * np.pi, nmodes)
+
+    def vel(x):
+        u = np.zeros_like(x)
+        for k, a, ph in zip(ks, amps, phis):
+            u += np.cross(k, a)[None, :] * np.cos(x @ k + ph)[:, None]
+        return u
+
+    uv = vel(mesh.pnts)
+    scale = 1.0 / np.abs(uv).max()
+    v = np.zeros((mesh.npnt, 4))
+    v[:, :3] = uv * scale
+    em = p - 1
+    e = np.zeros((mesh.nseg, em, 4))
+    if em:
+        a, b = mesh.seg_pnt[:, 0], mesh.seg_pnt[:, 1]
+        mid = 0.5 * (mesh.pnts[a] + mesh.pnts[b])
+        e[:, 0, :3] = 4.0 * (vel(mid) * scale - 0.5 * (v[a, :3] + v[b, :3]))
+    return v, e

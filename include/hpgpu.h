@@ -188,6 +188,65 @@ int hpg_selftest_mfma_f32(const float *A, const float *B, float *out);
  * traffic counters of the other kernels are scaled against. */
 int hpg_selftest_copy(long long nbytes, int reps, double *ms_per_copy);
 
+/* ---- tet_hp_ins (straight tetrahedra, p = 1, 2, NV = 4: u, v, w, p) ------------------
+ * One context per tetrahedral block.  Mesh arrays of tet_mesh (tet_mesh/tet_mesh.h:
+ * 73-137): edge i of a tet joins its local vertices (2,3), (3,1), (2,1), (1,0), (2,0),
+ * (3,0) with sign +1 when the edge's first vertex is the first of that pair
+ * (tet_mesh/setup.cpp:230-275); tets positively oriented; boundary faces ordered so
+ * that (p2 - p1) x (p0 - p1) points out, edge j of a face opposite its vertex j.
+ * Every boundary face is an inflow face (tet_hp/ins/bdry_ins.h:118-183): u, v, w
+ * essential on its vertices and edges, continuity flux rho u.n through it. */
+typedef struct hpg_tet_ctx hpg_tet_ctx;
+typedef struct {
+    int npnt, nseg, ntet, nbf;
+    const double *pnts;      /* npnt x 3 */
+    const int *seg_pnt;      /* nseg x 2 */
+    const int *tet_pnt;      /* ntet x 4 */
+    const int *tet_seg;      /* ntet x 6 */
+    const int *tet_sgn;      /* ntet x 6 */
+    const int *bf_pnt;       /* nbf x 3 */
+    const int *bf_seg;       /* nbf x 3 */
+    const int *bf_sgn;       /* nbf x 3 */
+} hpg_tet_mesh;
+/* replaces tet_hp::init / tet_hp_ins::init (tet_hp/allocate.cpp, tet_hp/ins/allocate.cpp);
+ * prm: physics HPG_PHYSICS_INS, log2p 0 or 1, rho, mu, adis, fadd, nstage, alpha, beta, cfl */
+int hpg_tet_create(const hpg_tet_mesh *mesh, const hpg_params *prm, hpg_tet_ctx **out);
+int hpg_tet_destroy(hpg_tet_ctx *ctx);
+/* tet_hp::ug (tet_hp/tet_hp.h:89-94): v(npnt,4), e(nseg,em0,4), em0 = p - 1 at log2p */
+int hpg_tet_set_state(hpg_tet_ctx *ctx, const double *ugv, const double *uge);
+int hpg_tet_get_state(hpg_tet_ctx *ctx, double *ugv, double *uge);
+/* tet_hp::hp_global::res (tet_hp/tet_hp.h:120-130) */
+int hpg_tet_get_residual(hpg_tet_ctx *ctx, double *resv, double *rese);
+int hpg_tet_set_residual(hpg_tet_ctx *ctx, const double *resv, const double *rese);
+/* vprcn, eprcn (inverted), iprcn (per element) and tau (tet_hp_ins::tau) */
+int hpg_tet_get_preconditioner(hpg_tet_ctx *ctx, double *vprcn, double *eprcn, double *iprcn, double *tau);
+/* replaces tet_hp::rsdl(stage) (tet_hp/update.cpp:9-131, tet_hp.h:229) with
+ * tet_hp_ins::element_rsdl (tet_hp/ins/rsdl.cpp:15-785, straight, steady) */
+int hpg_tet_rsdl(hpg_tet_ctx *ctx, int stage);
+/* replaces tet_hp_ins::setup_preconditioner (tet_hp/ins/setup_preconditioner.cpp:7-225)
+ * and tet_hp::setup_preconditioner (tet_hp/setup_preconditioner.cpp:428-525);
+ * returns the error count */
+int hpg_tet_setup_preconditioner(hpg_tet_ctx *ctx);
+/* replaces tet_hp::minvrt (tet_hp/setup_preconditioner.cpp:11-351), p <= 2 */
+int hpg_tet_minvrt(hpg_tet_ctx *ctx);
+/* replaces tet_hp::update (tet_hp/update.cpp:181-417) */
+int hpg_tet_update(hpg_tet_ctx *ctx);
+/* replaces tet_hp::maxres (tet_hp/allocate.cpp:530-559) */
+int hpg_tet_maxres(hpg_tet_ctx *ctx, double *per_var, double *out);
+/* max |res| over vertex and edge modes, essential components excluded */
+int hpg_tet_resnorm(hpg_tet_ctx *ctx, double *out);
+/* replace tet_hp::mg_restrict / mg_prolongate, p-levels (tet_hp/mg_restrict.cpp:4-25,
+ * tet_hp/mg_prolongate.cpp:7-14) */
+int hpg_tet_mg_restrict(hpg_tet_ctx *ctx);
+int hpg_tet_mg_prolongate(hpg_tet_ctx *ctx);
+int hpg_tet_get_log2p(hpg_tet_ctx *ctx, int *log2p);
+int hpg_tet_synchronize(hpg_tet_ctx *ctx);
+const char *hpg_tet_last_error(hpg_tet_ctx *ctx);
+/* per-launch HIP-event timing of the tet element kernel (full = beta > 0 stages) */
+int hpg_tet_kernel_timing(hpg_tet_ctx *ctx, int on);
+int hpg_tet_kernel_timing_read(hpg_tet_ctx *ctx, int full, double *total_ms, long long *launches,
+                               long long *elements);
+
 #ifdef __cplusplus
 }
 #endif
