@@ -48,7 +48,7 @@ FLOP_INV_P4 = 7250.0
 # HBM bytes per launch of k_elem_ins<4, full> on this workload (7 colours, 74 898
 # elements per launch on average): FETCH_SIZE and WRITE_SIZE counter passes scaled by
 # the same counters over a plain 1 GiB copy; profiles/traffic_ins_p4_512x512_pmc.md
-TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 219.7e6
+TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 218.0e6
 
 
 # tri_hp cns p = 4 full residual, algorithmic flops per element: transforms 30 G T +

@@ -189,10 +189,10 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
 
         const d4 z = {0.0, 0.0, 0.0, 0.0};
         d4 U[GT], V[GT], Pp[GT];
-    #pragma unroll
+#pragma unroll
         for (int mt = 0; mt < GT; ++mt) {
             d4 cu = z, cv = z, cp = z;
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < KT; ++s) {
                 const double av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
                 cu = mfma(av, b[0][s], cu);
@@ -205,9 +205,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         // inviscid fluxes (rsdl.cpp:262-291): continuity C, momentum A
         d4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
         const double rho = a.rho;
-    #pragma unroll
+#pragma unroll
         for (int mt = 0; mt < GT; ++mt)
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const double u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
                 const double fx = rho * u, fy = rho * v;
@@ -222,9 +222,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
 
         // lf_im = intgrtrs of the inviscid fluxes (rsdl.cpp:292-294)
         d4 LI0 = z, LI1 = z, LI2 = z;
-    #pragma unroll
+#pragma unroll
         for (int kt = 0; kt < GT; ++kt)
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int kk = kt * 4 + i;
                 const double ar = sops[D::OFF_IR + kk * 64 + lane];
@@ -238,10 +238,10 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         if constexpr (FULL) {
             // derivatives of u, v at the Gauss points (proj with r/s derivatives)
             d4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
-    #pragma unroll
+#pragma unroll
             for (int mt = 0; mt < GT; ++mt) {
                 d4 c1 = z, c2 = z, c3 = z, c4 = z;
-    #pragma unroll
+#pragma unroll
                 for (int s = 0; s < KT; ++s) {
                     const double ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
                     const double as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
@@ -264,9 +264,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
             const double v0101 = -cjcbi * l01 * l10;
             const double v0110 = -cjcbi * l00 * l11;
             d4 F00[GT], F01[GT], F10[GT], F11[GT];
-    #pragma unroll
+#pragma unroll
             for (int mt = 0; mt < GT; ++mt)
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const double ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
                     const double f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
@@ -278,12 +278,12 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
                 }
             // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:369-375)
             d4 R0[GT], R1[GT], R2[GT];
-    #pragma unroll
+#pragma unroll
             for (int mt = 0; mt < GT; ++mt) {
                 d4 r0 = z, r1 = z, r2 = z;
-    #pragma unroll
+#pragma unroll
                 for (int kt = 0; kt < GT; ++kt)
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int kk = (mt * GT + kt) * 4 + i;
                         const double dr = sops[D::OFF_DR + kk * 64 + lane];
@@ -297,9 +297,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
             if (a.lin) {
                 // linearisation data of this 16-element group, coalesced per (field, tile, reg)
                 float* L = a.lin + (size_t)(a.gbase + g) * (5 * GT * 4 * 64);
-    #pragma unroll
+#pragma unroll
                 for (int mt = 0; mt < GT; ++mt)
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int o = (mt * 4 + i) * 64 + lane;
                         L[0 * GT * 256 + o] = (float)U[mt][i];
@@ -319,9 +319,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
             const double gv = 2. * nu / h;
             const double gvis = gv * gv;
             const double tfac = a.adis * h / cjcb;
-    #pragma unroll
+#pragma unroll
             for (int mt = 0; mt < GT; ++mt)
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const double u = U[mt][i], v = V[mt][i];
                     const double qq = u * u + v * v;
@@ -339,9 +339,9 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
                 }
             load_old();
             // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
-    #pragma unroll
+#pragma unroll
             for (int kt = 0; kt < GT; ++kt)
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = kt * 4 + i;
                     const double ar = sops[D::OFF_IR + kk * 64 + lane];
