@@ -455,7 +455,9 @@ void do_lin_rsdl(hpg_ctx* c, int stage, bool with_source) {
 
 template <typename T, int NVn>
 void launch_minv_nv(hpg_ctx* c, const MinvArgsT<T>& a) {
-    const dim3 bv(nblk8(c->npnt)), bs(nblk8(c->nseg)), bt(nblk8(c->ntri)), th(256);
+    // one lane per (entity, variable); k_minv_sscale one lane per (side, mode, variable)
+    const dim3 bv(nblk8((long)c->npnt * NVn)), bs(nblk8((long)c->nseg * NVn)), bt(nblk8((long)c->ntri * NVn));
+    const dim3 bq(nblk8((long)c->nseg * a.sm * NVn)), th(256);
     switch (a.sm + 1) {
         case 1:
             hipLaunchKernelGGL((k_minv_vrtx<T, 1, NVn>), bv, th, 0, c->stream, a);
@@ -463,14 +465,14 @@ void launch_minv_nv(hpg_ctx* c, const MinvArgsT<T>& a) {
         case 2:
             hipLaunchKernelGGL((k_minv_vrtx<T, 2, NVn>), bv, th, 0, c->stream, a);
             hipLaunchKernelGGL((k_minv_side<T, 2, NVn>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_sscale<T, 2, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sscale<T, 2, NVn>), bq, th, 0, c->stream, a);
             break;
         case 4:
             hipLaunchKernelGGL((k_minv_vrtx<T, 4, NVn>), bv, th, 0, c->stream, a);
             hipLaunchKernelGGL((k_minv_side<T, 4, NVn>), bs, th, 0, c->stream, a);
             hipLaunchKernelGGL((k_minv_sub<T, 4, 0, NVn>), bs, th, 0, c->stream, a);
             hipLaunchKernelGGL((k_minv_sub<T, 4, 1, NVn>), bs, th, 0, c->stream, a);
-            hipLaunchKernelGGL((k_minv_sscale<T, 4, NVn>), bs, th, 0, c->stream, a);
+            hipLaunchKernelGGL((k_minv_sscale<T, 4, NVn>), bq, th, 0, c->stream, a);
             hipLaunchKernelGGL((k_minv_int<T, 4, NVn>), bt, th, 0, c->stream, a);
             break;
         default:
@@ -747,6 +749,23 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->im0 = (p0 - 1) * (p0 - 2) / 2;
         c->npnt = m->npnt; c->nseg = m->nseg; c->ntri = m->ntri;
         const int np = m->npnt, ns = m->nseg, nt = m->ntri;
+        // host checks of every index array the kernels gather through
+        if (np <= 0 || ns <= 0 || nt <= 0 || m->nebd < 0) throw std::runtime_error("empty mesh");
+        if (!m->pnts || !m->seg_pnt || !m->tri_pnt || !m->tri_seg || !m->tri_sgn) throw std::runtime_error("null mesh array");
+        for (long k = 0; k < 3L * nt; ++k) {
+            if (m->tri_pnt[k] < 0 || m->tri_pnt[k] >= np) throw std::runtime_error("tri_pnt out of range");
+            if (m->tri_seg[k] < 0 || m->tri_seg[k] >= ns) throw std::runtime_error("tri_seg out of range");
+            if (m->tri_sgn[k] != 1 && m->tri_sgn[k] != -1) throw std::runtime_error("tri_sgn must be +-1");
+        }
+        for (long k = 0; k < 2L * ns; ++k)
+            if (m->seg_pnt[k] < 0 || m->seg_pnt[k] >= np) throw std::runtime_error("seg_pnt out of range");
+        if (m->nebd > 0) {
+            if (!m->ebdry_start || !m->ebdry_essential || m->ebdry_start[0] != 0) throw std::runtime_error("ebdry_start");
+            for (int b = 0; b < m->nebd; ++b)
+                if (m->ebdry_start[b + 1] < m->ebdry_start[b]) throw std::runtime_error("ebdry_start not ascending");
+            for (int k = 0; k < m->ebdry_start[m->nebd]; ++k)
+                if (m->ebdry_seg[k] < 0 || m->ebdry_seg[k] >= ns) throw std::runtime_error("ebdry_seg out of range");
+        }
 
         c->pnts = c->upload(m->pnts, 2 * (size_t)np);
         c->tri_pnt = c->upload(m->tri_pnt, 3 * (size_t)nt);

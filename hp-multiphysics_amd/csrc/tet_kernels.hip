@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_setup_tet(TetSetupArgs a) {
     for (int k = 0; k < 4; ++k) {
         const double u = a.ugv[v[k] * TNV], w = a.ugv[v[k] * TNV + 1], z = a.ugv[v[k] * TNV + 2];
         const double q = u * u + w * w + z * z;
-        qmax = q > qmax ? q : qmax;
+        qmax = (qmax > q) ? qmax : q;  // MAX of sys/param.h: a NaN q propagates
     }
     if (!(jcb > 0.0) || !(qmax >= 0.0)) atomicOr(a.err, 1);
     const double nu = a.mu / a.rho;

@@ -636,82 +636,72 @@ struct MinvArgsT {
 };
 
 
+// The minvrt sweeps run one lane per (entity, variable): the diagonal preconditioner
+// never couples the variables, so lane (e, n) carries variable n of entity e through
+// every subtraction in the same order as the per-entity loop.  Consecutive lanes then
+// read consecutive doubles of the (entity, mode, variable) records, and the gathers of
+// neighbouring records touch a third of the cache lines a lane-per-entity sweep does.
+template <typename T, int NVn>
+__device__ __forceinline__ bool minv_lane(int count, int& e, int& n) {
+    const long g = (long)xcd_block() * blockDim.x + threadIdx.x;
+    e = (int)(g / NVn);
+    n = (int)(g - (long)e * NVn);
+    return e < count;
+}
+
 // vertices: subtract side and interior couplings (:110-135), scale (:151-152),
 // vertex Dirichlet (:179-183)
 template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM;
-    const int v = xcd_block() * blockDim.x + threadIdx.x;
-    if (v >= a.npnt) return;
-    T r[NVn];
-#pragma unroll
-    for (int n = 0; n < NVn; ++n) r[n] = a.resv[v * NVn + n];
+    int v, n;
+    if (!minv_lane<T, NVn>(a.npnt, v, n)) return;
+    T r = a.resv[v * NVn + n];
     if constexpr (SM > 0) {
         for (int k = a.vs_ptr[v]; k < a.vs_ptr[v + 1]; ++k) {
             const int s = a.vs_side[k], i = a.vs_end[k];
 #pragma unroll
-            for (int m = 0; m < SM; ++m) {
-                const T c = a.lump[a.o_sfmv + i * SM + m];
-#pragma unroll
-                for (int n = 0; n < NVn; ++n) r[n] -= c * a.ress[(s * a.sm0 + m) * NVn + n];
-            }
+            for (int m = 0; m < SM; ++m) r -= a.lump[a.o_sfmv + i * SM + m] * a.ress[(s * a.sm0 + m) * NVn + n];
         }
         if constexpr (IM > 0)
             for (int k = a.vt_ptr[v]; k < a.vt_ptr[v + 1]; ++k) {
                 const int t = a.vt_tri[k], i = a.vt_loc[k];
 #pragma unroll
-                for (int m = 0; m < IM; ++m) {
-                    const T c = a.lump[a.o_ifmb + i * IM + m];
-#pragma unroll
-                    for (int n = 0; n < NVn; ++n) r[n] -= c * a.resi[(t * a.im0 + m) * NVn + n];
-                }
+                for (int m = 0; m < IM; ++m) r -= a.lump[a.o_ifmb + i * IM + m] * a.resi[(t * a.im0 + m) * NVn + n];
             }
     }
     // every load before the first store: the stores could alias later loads
-    const unsigned msk = a.vess[v];
-    T x[NVn], u0[NVn];
-#pragma unroll
-    for (int n = 0; n < NVn; ++n) {
-        x[n] = (msk >> n) & 1 ? T(0) : r[n] * a.vprcn[v * NVn + n];
-        u0[n] = a.ugv ? a.ug0v[v * NVn + n] : T(0);
-    }
-#pragma unroll
-    for (int n = 0; n < NVn; ++n) {
-        a.resv[v * NVn + n] = x[n];
-        if (a.ugv) a.ugv[v * NVn + n] = u0[n] - a.cfa * x[n];
-    }
+    const T x = (a.vess[v] >> n) & 1 ? T(0) : r * a.vprcn[v * NVn + n];
+    const T u0 = a.ugv ? a.ug0v[v * NVn + n] : T(0);
+    a.resv[v * NVn + n] = x;
+    if (a.ugv) a.ugv[v * NVn + n] = u0 - a.cfa * x;
 }
 
 // sides: subtract interior couplings (:137-148), then the vertex solution (:192-214)
 template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= a.nseg) return;
-    T r[SM][NVn];
+    int s, n;
+    if (!minv_lane<T, NVn>(a.nseg, s, n)) return;
+    T r[SM];
 #pragma unroll
-    for (int m = 0; m < SM; ++m)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NVn + n];
+    for (int m = 0; m < SM; ++m) r[m] = a.ress[(s * a.sm0 + m) * NVn + n];
     if constexpr (IM > 0)
         for (int k = 0; k < 2; ++k) {
             const int t = a.st_tri[2 * s + k];
             if (t < 0) continue;
             const int i = a.st_loc[2 * s + k];
             const int sgn = a.tri_sgn[3 * t + i];
-            T bi[IM][NVn];
+            T bi[IM];
 #pragma unroll
-            for (int m = 0; m < IM; ++m)
-#pragma unroll
-                for (int n = 0; n < NVn; ++n) bi[m][n] = a.resi[(t * a.im0 + m) * NVn + n];
+            for (int m = 0; m < IM; ++m) bi[m] = a.resi[(t * a.im0 + m) * NVn + n];
             T ms = T(1);
 #pragma unroll
             for (int j = 0; j < SM; ++j) {
 #pragma unroll
                 for (int m = 0; m < IM; ++m) {
                     const T c = ms * a.lump[a.o_ifmb + (3 + i * SM + j) * IM + m];
-#pragma unroll
-                    for (int n = 0; n < NVn; ++n) r[j][n] -= c * bi[m][n];
+                    r[j] -= c * bi[m];
                 }
                 ms *= sgn;
             }
@@ -721,29 +711,22 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
         const int sgn = a.tri_sgn[3 * t + i];
-        T uh[3][NVn];
+        const T tp = a.tprcn[NVn * t + n];
+        T uh[3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int w = a.tri_pnt[3 * t + ((i + j) % 3)];
-#pragma unroll
-            for (int n = 0; n < NVn; ++n) uh[j][n] = a.resv[w * NVn + n] * a.tprcn[NVn * t + n];
-        }
+        for (int j = 0; j < 3; ++j) uh[j] = a.resv[a.tri_pnt[3 * t + (i + j) % 3] * NVn + n] * tp;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             T ms = T(1);
 #pragma unroll
             for (int m = 0; m < SM; ++m) {
-                const T c = a.lump[a.o_vfms + j * SM + m];
-#pragma unroll
-                for (int n = 0; n < NVn; ++n) r[m][n] -= ms * c * uh[j][n];
+                r[m] -= ms * a.lump[a.o_vfms + j * SM + m] * uh[j];
                 ms *= sgn;
             }
         }
     }
 #pragma unroll
-    for (int m = 0; m < SM; ++m)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) a.ress[(s * a.sm0 + m) * NVn + n] = r[m][n];
+    for (int m = 0; m < SM; ++m) a.ress[(s * a.sm0 + m) * NVn + n] = r[m];
 }
 
 template <typename T, int NVn>
@@ -757,18 +740,17 @@ __device__ __forceinline__ T side_scaled(const MinvArgsT<T>& a, int s, int mode,
 template <typename T, int P, int MODE, int NVn>
 __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= a.nseg) return;
-    T r[SM][NVn];
+    int s, n;
+    if (!minv_lane<T, NVn>(a.nseg, s, n)) return;
+    T r[SM];
 #pragma unroll
-    for (int m = MODE + 1; m < SM; ++m)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) r[m][n] = a.ress[(s * a.sm0 + m) * NVn + n];
+    for (int m = MODE + 1; m < SM; ++m) r[m] = a.ress[(s * a.sm0 + m) * NVn + n];
     for (int k = 0; k < 2; ++k) {
         const int t = a.st_tri[2 * s + k];
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
-        T uh[3][NVn];
+        const T tp = a.tprcn[NVn * t + n];
+        T uh[3];
         int sign[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -776,8 +758,7 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
             const int sj = a.tri_seg[3 * t + jj];
             sign[j] = a.tri_sgn[3 * t + jj];
             const T sg = (MODE % 2) ? T(sign[j]) : T(1);
-#pragma unroll
-            for (int n = 0; n < NVn; ++n) uh[j][n] = sg * side_scaled<T, NVn>(a, sj, MODE, n) * a.tprcn[NVn * t + n];
+            uh[j] = sg * side_scaled<T, NVn>(a, sj, MODE, n) * tp;
         }
         T ms = ((MODE + 1) % 2) ? T(sign[0]) : T(1);
 #pragma unroll
@@ -785,41 +766,31 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const T c = ms * a.lump[a.o_sfms + (MODE * SM + m) * 3 + j];
-#pragma unroll
-                for (int n = 0; n < NVn; ++n) r[m][n] -= c * uh[j][n];
+                r[m] -= c * uh[j];
             }
             ms *= sign[0];
         }
     }
 #pragma unroll
-    for (int m = MODE + 1; m < SM; ++m)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) a.ress[(s * a.sm0 + m) * NVn + n] = r[m][n];
+    for (int m = MODE + 1; m < SM; ++m) a.ress[(s * a.sm0 + m) * NVn + n] = r[m];
 }
 
-// store the solved side modes: res.s *= sprcn*sdiag(mode), sdirichlet (:245-267, :331-343)
+// store the solved side modes: res.s *= sprcn*sdiag(mode), sdirichlet (:245-267, :331-343);
+// one lane per (side, mode, variable), so on the finest level the sweep is a plain
+// contiguous stream
 template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    const long g = (long)xcd_block() * blockDim.x + threadIdx.x;
+    const int s = (int)(g / (SM * NVn));
     if (s >= a.nseg) return;
+    const int mn = (int)(g - (long)s * (SM * NVn)), m = mn / NVn, n = mn - m * NVn;
+    const int ix = (s * a.sm0 + m) * NVn + n;
     // every load before the first store: the stores could alias later loads
-    T x[SM][NVn], u0[SM][NVn];
-#pragma unroll
-    for (int m = 0; m < SM; ++m)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) {
-            x[m][n] = side_scaled<T, NVn>(a, s, m, n);
-            u0[m][n] = a.ugs ? a.ug0s[(s * a.sm0 + m) * NVn + n] : T(0);
-        }
-#pragma unroll
-    for (int m = 0; m < SM; ++m)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) {
-            const int ix = (s * a.sm0 + m) * NVn + n;
-            a.ress[ix] = x[m][n];
-            if (a.ugs) a.ugs[ix] = u0[m][n] - a.cfa * x[m][n];
-        }
+    const T x = side_scaled<T, NVn>(a, s, m, n);
+    const T u0 = a.ugs ? a.ug0s[ix] : T(0);
+    a.ress[ix] = x;
+    if (a.ugs) a.ugs[ix] = u0 - a.cfa * x;
 }
 
 // interiors: banded solve with the literal DPBTRSNU2 indexing (utilities/myblas.c:91-122),
@@ -827,76 +798,55 @@ __global__ __launch_bounds__(256) void k_minv_sscale(MinvArgsT<T> a) {
 template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
     constexpr int SM = Dims<P>::SM, IM = Dims<P>::IM, BM = Dims<P>::BM;
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= a.ntri) return;
-    T b[IM][NVn];
+    int t, n;
+    if (!minv_lane<T, NVn>(a.ntri, t, n)) return;
+    T b[IM];
 #pragma unroll
-    for (int k = 0; k < IM; ++k)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) b[k][n] = a.resi[(t * a.im0 + k) * NVn + n];
+    for (int k = 0; k < IM; ++k) b[k] = a.resi[(t * a.im0 + k) * NVn + n];
     const T* ab = a.lump + a.o_idiag;  // stride 2, ofdg 1
 #pragma unroll
     for (int i = 0; i < IM; ++i) {
         const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
 #pragma unroll
-        for (int j = frst; j < 1; ++j)
-#pragma unroll
-            for (int n = 0; n < NVn; ++n) b[i][n] -= ab[i * 2 + j] * b[ba + j][n];
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) b[i][n] /= ab[i * 2 + 1];
+        for (int j = frst; j < 1; ++j) b[i] -= ab[i * 2 + j] * b[ba + j];
+        b[i] /= ab[i * 2 + 1];
     }
 #pragma unroll
     for (int i = IM - 1; i >= 0; --i) {
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) b[i][n] /= ab[i * 2 + 1];
+        b[i] /= ab[i * 2 + 1];
         const int frst = (1 - i > 0 ? 1 - i : 0), ba = -frst;
 #pragma unroll
-        for (int j = frst; j < 1; ++j)
-#pragma unroll
-            for (int n = 0; n < NVn; ++n) b[ba + j][n] -= ab[i * 2 + j] * b[i][n];
+        for (int j = frst; j < 1; ++j) b[ba + j] -= ab[i * 2 + j] * b[i];
     }
-    T uh[BM][NVn];
+    T uh[BM];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const int w = a.tri_pnt[3 * t + i];
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) uh[i][n] = a.resv[w * NVn + n];
-    }
+    for (int i = 0; i < 3; ++i) uh[i] = a.resv[a.tri_pnt[3 * t + i] * NVn + n];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int sd = a.tri_seg[3 * t + i], sgn = a.tri_sgn[3 * t + i];
         T ms = T(1);
 #pragma unroll
         for (int m = 0; m < SM; ++m) {
-#pragma unroll
-            for (int n = 0; n < NVn; ++n) uh[3 + i * SM + m][n] = ms * a.ress[(sd * a.sm0 + m) * NVn + n];
+            uh[3 + i * SM + m] = ms * a.ress[(sd * a.sm0 + m) * NVn + n];
             ms *= sgn;
         }
     }
+    const T tp = a.tprcn[NVn * t + n];
 #pragma unroll
     for (int k = 0; k < IM; ++k) {
+        b[k] /= tp;
 #pragma unroll
-        for (int n = 0; n < NVn; ++n) b[k][n] /= a.tprcn[NVn * t + n];
-#pragma unroll
-        for (int i = 0; i < BM; ++i) {
-            const T c = a.lump[a.o_bfmi + i * IM + k];
-#pragma unroll
-            for (int n = 0; n < NVn; ++n) b[k][n] -= c * uh[i][n];
-        }
+        for (int i = 0; i < BM; ++i) b[k] -= a.lump[a.o_bfmi + i * IM + k] * uh[i];
     }
-    T u0[IM][NVn];
+    T u0[IM];
 #pragma unroll
-    for (int k = 0; k < IM; ++k)
+    for (int k = 0; k < IM; ++k) u0[k] = a.ugi ? a.ug0i[(t * a.im0 + k) * NVn + n] : T(0);
 #pragma unroll
-        for (int n = 0; n < NVn; ++n) u0[k][n] = a.ugi ? a.ug0i[(t * a.im0 + k) * NVn + n] : T(0);
-#pragma unroll
-    for (int k = 0; k < IM; ++k)
-#pragma unroll
-        for (int n = 0; n < NVn; ++n) {
-            const int ix = (t * a.im0 + k) * NVn + n;
-            a.resi[ix] = b[k][n];
-            if (a.ugi) a.ugi[ix] = u0[k][n] - a.cfa * b[k][n];
-        }
+    for (int k = 0; k < IM; ++k) {
+        const int ix = (t * a.im0 + k) * NVn + n;
+        a.resi[ix] = b[k];
+        if (a.ugi) a.ugi[ix] = u0[k] - a.cfa * b[k];
+    }
 }
 
 // ---------------------------------------------------------------------------------

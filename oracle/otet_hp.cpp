@@ -324,7 +324,7 @@ int TetHpIns::setup_preconditioner() {
             const int v = m.tet_pnt[4 * t + j];
             const double q = std::pow(ugv[v * NV + 0], 2.0) + std::pow(ugv[v * NV + 1], 2.0) +
                              std::pow(ugv[v * NV + 2], 2.0);
-            qmax = std::max(qmax, q);
+            qmax = (qmax > q) ? qmax : q;  // MAX of sys/param.h (a NaN q propagates)
         }
         if (!(jcb > 0.0)) err = 1;
         if (!(qmax >= 0.0)) err = 1;

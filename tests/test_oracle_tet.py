@@ -230,3 +230,39 @@ def test_relaxation_and_p_multigrid_converge_to_one_solution(OT):
     dv[:, 3] -= dv[:, 3].mean()
     de = a[1][1] - b[1][1]
     assert max(np.abs(dv).max(), np.abs(de).max()) < 1e-10
+
+
+def test_nan_state_reports_setup_error_in_max_order(OT, oracle_mod):
+    """setup_preconditioner returns 1 when qmax is NaN (the reference's
+    `!(qmax >= 0.0)` check, tet_hp/ins/setup_preconditioner.cpp:59-76).  qmax is
+    accumulated with MAX of sys/param.h, ((a) > (b) ? (a) : (b)), so a NaN q survives
+    only as the last vertex of an element: a NaN velocity at a vertex is caught exactly
+    when that vertex is the last one of some element.  std::max, or the operands the
+    other way round, would change which vertices are caught."""
+    from hpmp import mesh as M, workloads as W
+    m = TM.unit_cube(2)
+    v = np.tile([1.0, 0.0, 0.0, 0.0], (m.npnt, 1))
+    e = np.zeros((m.nseg, 1, 4))
+    O = OT.TetHpIns(m, log2p=1, mu=1e-2)
+    O.set_state(v, e)
+    assert O.setup_preconditioner() == 0
+    last = set(m.tet_pnt[:, 3].tolist())
+    assert 0 < len(last) < m.npnt          # both outcomes occur on this mesh
+    for pos in range(m.npnt):
+        w = v.copy()
+        w[pos, pos % 3] = np.nan
+        O.set_state(w, e)
+        assert O.setup_preconditioner() == (1 if pos in last else 0), pos
+    t = M.unit_square(3, 0.1, 1)
+    st = list(W.noisy_freestream(t, 4, seed=1))
+    Ot = oracle_mod.TriHpIns(t, log2p=2, mu=1e-2)
+    Ot.set_state(*st)
+    assert Ot.setup_preconditioner() == 0
+    # tri_hp_ins takes qmax over the Gauss points of the projected velocity
+    # (tri_hp/ins/setup_preconditioner.cpp:98-125): every point sees every vertex
+    # mode, so a NaN at any vertex reaches the last point and is caught
+    for pos in range(t.npnt):
+        w = [x.copy() for x in st]
+        w[0][pos, 1] = np.nan
+        Ot.set_state(*w)
+        assert Ot.setup_preconditioner() == 1, pos
