@@ -3,14 +3,14 @@
 // Network 11 -> 64 -> 64 -> 64 -> 3, GELU (tanh form) on the hidden layers.  The
 // FP32 master weights are rounded to bfloat16 (nearest even) for every matrix
 // product and activations are rounded to bfloat16 on input to a product;
-// accumulation is FP32.  Weights are uniform across a wave, so every weight read
-// is a scalar (constant-cache) load; each lane carries one sample.
-// Training: per 64-sample block the backward pass keeps pre-activations and
+// accumulation is FP32.  Inference runs on the matrix cores (bf16 MFMA, 16 samples
+// per tile).  Training: per 64-sample block the backward pass keeps pre-activations and
 // deltas in LDS and forms the weight gradients as sample reductions; per-block
 // partial gradients are summed in a fixed order by the Adam kernel, so training
 // is bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -75,26 +75,125 @@ __device__ inline void head(const float* __restrict__ wq, const float* h, float*
     }
 }
 
-__global__ __launch_bounds__(128) void k_mlp_forward(int n, const float* __restrict__ X, const float* __restrict__ wq,
-                                                     float* __restrict__ Y) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float x[MI], a[MH], b[MH];
+// ---- inference on the matrix cores -------------------------------------------------
+// v_mfma_f32_16x16x16_bf16 with the hidden units as rows and 16 samples as columns.
+// Operand layout: lane l holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16] (j = 0..3);
+// accumulator register i of lane l holds D[4(l/16)+i][l%16] (checked on the device
+// by hpg_selftest_mfma_bf16).  The accumulators of output tile mt of one layer are
+// therefore the B operand of K-step kt = mt of the next layer, lane for lane: the
+// activations never leave registers.  Each wave holds every weight as A fragments
+// (80 VGPRs) and sweeps 16-sample tiles; products of bfloat16 operands are exact in
+// FP32 and accumulate in FP32, so only the summation order differs from the
+// sample-per-lane forward pass of the training kernel below.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline short bf16_bits(float x) {
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ inline f32x4_t mfma_bf16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// GELU of the accumulators, rounded to bfloat16: the B operands of the next layer
+__device__ inline void gelu_fragments(const f32x4_t z[4], bf16x4_t h[4]) {
 #pragma unroll
-    for (int i = 0; i < MI; ++i) x[i] = X[(size_t)s * MI + i];
-    layer<MI>(wq + O_W1, x, a);
+    for (int kt = 0; kt < 4; ++kt)
 #pragma unroll
-    for (int j = 0; j < MH; ++j) a[j] = gelu(a[j]);
-    layer<MH>(wq + O_W2, a, b);
+        for (int j = 0; j < 4; ++j) h[kt][j] = bf16_bits(gelu(z[kt][j]));
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward_mfma(int n, const float* __restrict__ X,
+                                                          const float* __restrict__ wq, float* __restrict__ Y) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, c = lane & 15;
+    bf16x4_t A1[4], A2[4][4], A3[4][4], A4[4];
+    f32x4_t b1[4], b2[4], b3[4], b4;
 #pragma unroll
-    for (int j = 0; j < MH; ++j) b[j] = gelu(b[j]);
-    layer<MH>(wq + O_W3, b, a);
+    for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
-    for (int j = 0; j < MH; ++j) a[j] = gelu(a[j]);
-    float y[MO];
-    head(wq, a, y);
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * q + j;
+            A1[mt][j] = k < MI ? bf16_bits(wq[O_W1 + k * MH + 16 * mt + c]) : (short)0;
+        }
 #pragma unroll
-    for (int o = 0; o < MO; ++o) Y[(size_t)s * MO + o] = y[o];
+        for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = 16 * kt + 4 * q + j;
+                A2[mt][kt][j] = bf16_bits(wq[O_W2 + k * MH + 16 * mt + c]);
+                A3[mt][kt][j] = bf16_bits(wq[O_W3 + k * MH + 16 * mt + c]);
+            }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            b1[mt][i] = wq[O_B1 + 16 * mt + 4 * q + i];
+            b2[mt][i] = wq[O_B2 + 16 * mt + 4 * q + i];
+            b3[mt][i] = wq[O_B3 + 16 * mt + 4 * q + i];
+        }
+    }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 16 * kt + 4 * q + j;
+            A4[kt][j] = c < MO ? bf16_bits(wq[O_W4 + k * MO + c]) : (short)0;
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b4[i] = 4 * q + i < MO ? wq[O_B4 + 4 * q + i] : 0.f;
+
+    const long ntile = ((long)n + 15) / 16;
+    for (long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntile; tile += (long)gridDim.x * 4) {
+        const long s = tile * 16 + c;
+        bf16x4_t h[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * q + j;
+            h[0][j] = (s < n && k < MI) ? bf16_bits(X[s * MI + k]) : (short)0;
+        }
+        f32x4_t z[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) z[mt] = mfma_bf16(A1[mt], h[0], b1[mt]);
+        gelu_fragments(z, h);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t acc = b2[mt];
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) acc = mfma_bf16(A2[mt][kt], h[kt], acc);
+            z[mt] = acc;
+        }
+        gelu_fragments(z, h);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t acc = b3[mt];
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) acc = mfma_bf16(A3[mt][kt], h[kt], acc);
+            z[mt] = acc;
+        }
+        gelu_fragments(z, h);
+        f32x4_t y = b4;
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) y = mfma_bf16(A4[kt], h[kt], y);
+        if (q == 0 && s < n)
+#pragma unroll
+            for (int i = 0; i < MO; ++i) Y[s * MO + i] = y[i];
+    }
+}
+
+// one v_mfma_f32_16x16x16_bf16 on A, B (16 x 16 row-major, rounded to bfloat16)
+__global__ void k_mfma_probe_bf16(const float* A, const float* B, float* out) {
+    const int l = threadIdx.x, q = l >> 4, c = l & 15;
+    bf16x4_t a, b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        a[j] = bf16_bits(A[c * 16 + 4 * q + j]);
+        b[j] = bf16_bits(B[(4 * q + j) * 16 + c]);
+    }
+    f32x4_t d = {0.f, 0.f, 0.f, 0.f};
+    d = mfma_bf16(a, b, d);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[l * 4 + i] = d[i];
 }
 
 // backward of one 64-sample block: partial gradient (NPARAM) and partial loss sum
@@ -273,7 +372,11 @@ void ensure_blocks(hpg_mlp* m, int nb) {
 
 namespace hpg {
 void mlp_forward_device(const hpg_mlp* m, const float* X, int n, float* Y, hipStream_t stream) {
-    if (n > 0) hipLaunchKernelGGL(k_mlp_forward, dim3((n + 127) / 128), dim3(128), 0, stream, n, X, m->wq, Y);
+    if (n <= 0) return;
+    // four 16-sample tiles per block and pass; at most 2048 blocks, each wave loops
+    const long tiles = ((long)n + 15) / 16;
+    const long blocks = std::min<long>((tiles + 3) / 4, 2048);
+    hipLaunchKernelGGL(k_mlp_forward_mfma, dim3((unsigned)blocks), dim3(256), 0, stream, n, X, m->wq, Y);
 }
 }  // namespace hpg
 
@@ -332,6 +435,23 @@ int hpg_mlp_set_weights(hpg_mlp* m, const float* w) {
         ck(hipMemsetAsync(m->m2, 0, NPARAM * 4, m->stream), "memset");
     } catch (const std::exception& e) { m->err = e.what(); return -1; }
     return 0;
+}
+
+// MFMA bf16 16x16x16 lane-layout probe (test support; A, B 16x16 row-major, out 64x4)
+int hpg_selftest_mfma_bf16(const float* A, const float* B, float* out) {
+    if (!A || !B || !out) return -2;
+    float *dA = nullptr, *dB = nullptr, *dO = nullptr;
+    if (hipMalloc(&dA, 256 * 4) != hipSuccess || hipMalloc(&dB, 256 * 4) != hipSuccess ||
+        hipMalloc(&dO, 256 * 4) != hipSuccess) {
+        hipFree(dA); hipFree(dB);
+        return -1;
+    }
+    hipMemcpy(dA, A, 256 * 4, hipMemcpyHostToDevice);
+    hipMemcpy(dB, B, 256 * 4, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_mfma_probe_bf16, dim3(1), dim3(64), 0, 0, dA, dB, dO);
+    const hipError_t e = hipMemcpy(out, dO, 256 * 4, hipMemcpyDeviceToHost);
+    hipFree(dA); hipFree(dB); hipFree(dO);
+    return e == hipSuccess ? 0 : -1;
 }
 
 int hpg_mlp_forward(hpg_mlp* m, const float* X, int n, float* Y) {

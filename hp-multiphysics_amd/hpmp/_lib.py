@@ -26,7 +26,7 @@ EXPORTS = (
     "hpg_get_preconditioner", "hpg_set_tprcn", "hpg_rsdl", "hpg_setup_preconditioner", "hpg_minvrt",
     "hpg_update", "hpg_maxres", "hpg_resnorm", "hpg_mg_restrict", "hpg_mg_prolongate", "hpg_get_log2p",
     "hpg_set_precision", "hpg_set_preconditioner", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
-    "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32",
+    "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32", "hpg_selftest_mfma_bf16",
     "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
@@ -123,6 +123,7 @@ def load():
     L.hpg_basis_operator.argtypes = [C.c_int, C.c_char_p, dp]
     L.hpg_basis_operator.restype = C.c_int
     L.hpg_selftest_mfma_f32.argtypes = [dp, dp, dp]
+    L.hpg_selftest_mfma_bf16.argtypes = [dp, dp, dp]
     L.hpg_selftest_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.hpg_set_defect_inner.argtypes = [vp, C.c_int]
     L.hpg_apply_jacobian.argtypes = [vp, dp, dp, dp, dp, dp, dp]

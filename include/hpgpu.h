@@ -183,6 +183,10 @@ int hpg_basis_operator(int p, const char *name, double *out);
 int hpg_selftest_mfma(const double *A, const double *B, double *out);
 /* the same for v_mfma_f32_16x16x4f32 */
 int hpg_selftest_mfma_f32(const float *A, const float *B, float *out);
+/* test support, no reference counterpart: one v_mfma_f32_16x16x16_bf16 on A and B
+ * (16x16 row-major, rounded to bfloat16 on the device); out[lane*4 + i] is
+ * accumulator register i of lane `lane` (the layout the MLP inference kernel uses). */
+int hpg_selftest_mfma_bf16(const float *A, const float *B, float *out);
 /* test support, no reference counterpart: `reps` plain device copies of nbytes
  * (multiple of 16) between two fresh buffers; the known-size kernel that HBM
  * traffic counters of the other kernels are scaled against. */

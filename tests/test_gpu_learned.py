@@ -33,6 +33,37 @@ def test_mlp_forward_matches_oracle():
     assert np.abs(Y - Yo).max() <= 2e-3 * np.abs(Yo).max()
 
 
+def test_mfma_bf16_lane_layout():
+    _case("v_mfma_f32_16x16x16_bf16 lane layout")
+    from hpmp import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(9)
+    # small integers: exact in bfloat16, and every sum of 16 products is exact in FP32
+    A = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    B = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    out = np.zeros(256, dtype=np.float32)
+    assert L.hpg_selftest_mfma_bf16(_lib.ptr(A), _lib.ptr(B), _lib.ptr(out)) == 0
+    D = A.astype(np.float64) @ B.astype(np.float64)
+    # register i of lane l holds row 4*(l//16) + i, column l % 16
+    exp = np.array([[D[4 * (l // 16) + i, l % 16] for i in range(4)] for l in range(64)]).ravel()
+    assert np.array_equal(out, exp.astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 100003])
+def test_mlp_forward_ragged_sizes(n):
+    _case(f"mlp forward (MFMA tiles of 16) n={n}")
+    import mlp as OM
+    from hpmp.learned import MLP
+    th = _theta(7)
+    X = np.random.default_rng(n).standard_normal((n, 11)).astype(np.float32)
+    net = MLP(0)
+    net.set_weights(th)
+    Y = net.forward(X)
+    Yo, _, _ = OM.forward(th, X)
+    assert np.all(np.isfinite(Y))
+    assert np.abs(Y - Yo).max() <= 2e-3 * np.abs(Yo).max()
+
+
 def test_mlp_gradient_matches_oracle():
     _case("mlp loss/gradient vs oracle n=777")
     import mlp as OM
