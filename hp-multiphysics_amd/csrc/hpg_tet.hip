@@ -1,8 +1,9 @@
-// Context and C-ABI of the tet_hp_ins residual/relaxation path (include/hpgpu.h,
-// hpg_tet_*).  Compiled in the same translation unit as hpg_tri.hip (included at its
+// Context and C-ABI of the tet_hp_ins and tet_hp_cns residual/relaxation paths
+// (include/hpgpu.h, hpg_tet_*).  Compiled in the same translation unit as hpg_tri.hip (included at its
 // end), whose host helpers (check, nblk8, HPG_TRY, resident_blocks, persistent_grid)
 // and shared kernels (k_modes, k_absmax) it uses.
 #include "tet_kernels.hip"
+#include "tet_kernels_cns.hip"
 #include "tet_ops.h"
 
 namespace {
@@ -13,6 +14,7 @@ struct TetLevel {
     double* P2 = nullptr;    // face values GF x tm2d
     double* W2 = nullptr;    // face weights GF
     double* vfms = nullptr;  // 4 x 10
+    double* Pg = nullptr;    // G x tm values at the Gauss points (cns setup)
     double vdiag = 0.0, ediag = 0.0, sfmv0 = 0.0, sfmv1 = 0.0;
 };
 
@@ -52,10 +54,12 @@ struct hpg_tet_ctx {
     std::vector<void*> allocs;
     std::string err;
     int npnt = 0, nseg = 0, ntet = 0, nbf = 0, log2pmax = 0, log2p = 0, em0 = 0;
+    int nv = TNV;            // 4 (ins: u, v, w, p) or 5 (cns: p, u, v, w, RT)
     hpg_params prm{};
+    double kcond = 0.0;      // cns: R mu / prandtl * gamma / (gamma - 1)
     bool coarse_flag = false, isfrst = false;
     double *pnts = nullptr;
-    int *tet_pnt = nullptr, *tet_seg = nullptr, *bf_pnt = nullptr, *bf_seg = nullptr;
+    int *tet_pnt = nullptr, *tet_seg = nullptr, *seg_pnt = nullptr, *bf_pnt = nullptr, *bf_seg = nullptr;
     unsigned char *vess = nullptr, *eess = nullptr;
     int *vt_ptr = nullptr, *vt_code = nullptr, *et_ptr = nullptr, *et_code = nullptr;
     int *ve_ptr = nullptr, *ve_code = nullptr, *vf_ptr = nullptr, *vf_code = nullptr, *ef_ptr = nullptr,
@@ -64,6 +68,7 @@ struct hpg_tet_ctx {
     double *rrv = nullptr, *rre = nullptr, *r0v = nullptr, *r0e = nullptr;
     double *dresv[2] = {nullptr, nullptr}, *drese[2] = {nullptr, nullptr};
     double *vprcn = nullptr, *eprcn = nullptr, *iprcn = nullptr, *tau = nullptr, *lf = nullptr, *flf = nullptr;
+    double *tpc = nullptr, *vpc = nullptr, *epc = nullptr;  // cns: 5 x 5 blocks per element / vertex / edge
     int* d_err = nullptr;
     unsigned long long* d_max = nullptr;
     TetLevel lv[2];
@@ -102,14 +107,42 @@ struct hpg_tet_ctx {
         if (stream) hipStreamDestroy(stream);
     }
     int em() const { return (1 << log2p) - 1; }
+    bool cns() const { return nv == CNV; }
 };
 
 namespace {
 
 template <int P>
+void launch_elem_tetc(hpg_tet_ctx* c, double beta) {
+    TetcElemArgs a;
+    a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg;
+    a.ntet = c->ntet; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
+    a.ops = c->lv[c->log2p].ops; a.mu = c->prm.mu; a.kcond = c->kcond; a.R = c->prm.Rgas; a.gamma = c->prm.gamma;
+    a.atm = 0.0; a.beta = beta;
+    const int noct = (c->ntet + 7) / 8;
+    if (beta > 0.0) {
+        static const int cap = resident_blocks((const void*)k_elem_tetc<P, true>);
+        hipLaunchKernelGGL((k_elem_tetc<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+    } else {
+        static const int cap = resident_blocks((const void*)k_elem_tetc<P, false>);
+        hipLaunchKernelGGL((k_elem_tetc<P, false>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+    }
+}
+
+template <int P>
 void launch_elem_tet(hpg_tet_ctx* c, int stage) {
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    if (c->cns()) {
+        const size_t r = c->recs.size();
+        if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+        launch_elem_tetc<P>(c, beta);
+        if (c->timing) {
+            check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+            c->recs.push_back({beta > 0.0 ? 1 : 0, c->ntet});
+        }
+        return;
+    }
     TetElemArgs a;
     a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.tet_sgn = nullptr;
     a.ntet = c->ntet; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
@@ -130,16 +163,23 @@ void launch_elem_tet(hpg_tet_ctx* c, int stage) {
     }
 }
 
-void tet_modes(hpg_tet_ctx* c, int op, double* xv, double* xe, double* yv, double* ye, const double* zv,
-               const double* ze, double k) {
-    const long nv = (long)c->npnt * TNV;
-    hipLaunchKernelGGL(k_modes<TNV>, dim3(nblk(nv)), dim3(256), 0, c->stream, op, (long)c->npnt, 1, 1, xv, yv, zv, k);
+template <int NVn>
+void tet_modes_nv(hpg_tet_ctx* c, int op, double* xv, double* xe, double* yv, double* ye, const double* zv,
+                  const double* ze, double k) {
+    const long nv = (long)c->npnt * NVn;
+    hipLaunchKernelGGL(k_modes<NVn>, dim3(nblk(nv)), dim3(256), 0, c->stream, op, (long)c->npnt, 1, 1, xv, yv, zv, k);
     const int em = c->em();
     if (em > 0) {
-        const long ne = (long)c->nseg * em * TNV;
-        hipLaunchKernelGGL(k_modes<TNV>, dim3(nblk(ne)), dim3(256), 0, c->stream, op, (long)c->nseg, em, c->em0, xe, ye,
+        const long ne = (long)c->nseg * em * NVn;
+        hipLaunchKernelGGL(k_modes<NVn>, dim3(nblk(ne)), dim3(256), 0, c->stream, op, (long)c->nseg, em, c->em0, xe, ye,
                            ze, k);
     }
+}
+
+void tet_modes(hpg_tet_ctx* c, int op, double* xv, double* xe, double* yv, double* ye, const double* zv,
+               const double* ze, double k) {
+    if (c->cns()) tet_modes_nv<CNV>(c, op, xv, xe, yv, ye, zv, ze, k);
+    else tet_modes_nv<TNV>(c, op, xv, xe, yv, ye, zv, ze, k);
 }
 
 // tet_hp::rsdl (tet_hp/update.cpp:9-131)
@@ -148,7 +188,17 @@ void do_rsdl_tet(hpg_tet_ctx* c, int stage) {
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     const TetLevel& L = c->lv[c->log2p];
     if (L.p == 1) launch_elem_tet<1>(c, stage); else launch_elem_tet<2>(c, stage);
-    if (c->nbf > 0) {
+    if (c->nbf > 0 && c->cns()) {
+        const double* b = c->prm.bc;
+        if (L.p == 1)
+            hipLaunchKernelGGL(k_face_tetc<1>, dim3(nblk8(c->nbf)), dim3(256), 0, c->stream, c->nbf, c->bf_pnt, c->bf_seg,
+                               c->pnts, c->ugv, c->uge, c->em0, L.P2, L.W2, c->prm.gamma, 0.0, b[1], b[2], b[3], b[4],
+                               c->flf);
+        else
+            hipLaunchKernelGGL(k_face_tetc<2>, dim3(nblk8(c->nbf)), dim3(256), 0, c->stream, c->nbf, c->bf_pnt, c->bf_seg,
+                               c->pnts, c->ugv, c->uge, c->em0, L.P2, L.W2, c->prm.gamma, 0.0, b[1], b[2], b[3], b[4],
+                               c->flf);
+    } else if (c->nbf > 0) {
         if (L.p == 1)
             hipLaunchKernelGGL(k_face_tet<1>, dim3(nblk8(c->nbf)), dim3(256), 0, c->stream, c->nbf, c->bf_pnt, c->bf_seg,
                                c->pnts, c->ugv, c->uge, c->em0, L.P2, L.W2, c->prm.rho, c->flf);
@@ -158,14 +208,17 @@ void do_rsdl_tet(hpg_tet_ctx* c, int stage) {
     }
     const int tm = L.p == 1 ? 4 : 10, tm2 = L.p == 1 ? 3 : 6;
     TetGatherArgs g;
-    g.tm = tm; g.tm2 = tm2; g.lf = c->lf; g.flf = c->flf; g.omb = 1.0 - beta;
-    g.nent = c->npnt; g.mbase = 0; g.fbase = 0; g.stride = TNV; g.ptr = c->vt_ptr; g.code = c->vt_code;
+    const int nvv = c->nv;
+    g.tm = tm; g.tm2 = tm2; g.lf = c->lf; g.flf = c->flf; g.omb = 1.0 - beta; g.fnv = c->cns() ? CNV : 1;
+    g.nent = c->npnt; g.mbase = 0; g.fbase = 0; g.stride = nvv; g.ptr = c->vt_ptr; g.code = c->vt_code;
     g.fptr = c->vf_ptr; g.fcode = c->vf_code; g.res = c->resv; g.rr = c->rrv;
-    hipLaunchKernelGGL(k_gather_tet, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, g);
+    if (c->cns()) hipLaunchKernelGGL(k_gather_tet<CNV>, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, g);
+    else hipLaunchKernelGGL(k_gather_tet<TNV>, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, g);
     if (c->em() > 0) {
-        g.nent = c->nseg; g.mbase = 4; g.fbase = 3; g.stride = c->em0 * TNV; g.ptr = c->et_ptr; g.code = c->et_code;
+        g.nent = c->nseg; g.mbase = 4; g.fbase = 3; g.stride = c->em0 * nvv; g.ptr = c->et_ptr; g.code = c->et_code;
         g.fptr = c->ef_ptr; g.fcode = c->ef_code; g.res = c->rese; g.rr = c->rre;
-        hipLaunchKernelGGL(k_gather_tet, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, g);
+        if (c->cns()) hipLaunchKernelGGL(k_gather_tet<CNV>, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, g);
+        else hipLaunchKernelGGL(k_gather_tet<TNV>, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, g);
     }
     if (c->coarse_flag) {
         const int l = c->log2p;
@@ -190,7 +243,39 @@ TetMinvArgs tet_minv_args(hpg_tet_ctx* c, bool pre_dirichlet) {
     return a;
 }
 
+// tet_hp_cns::minvrt with the stage update fused where the data allow it: edges in the
+// edge sweep; vertices in the vertex sweep only without edge modes, else afterwards
+// (the edge sweep reads the stage state of the vertices)
+void do_minvrt_tetc(hpg_tet_ctx* c, double cfa) {
+    const TetLevel& L = c->lv[c->log2p];
+    TetcMinvArgs a{};
+    a.npnt = c->npnt; a.nseg = c->nseg; a.em0 = c->em0; a.em = c->em();
+    a.ve_ptr = c->ve_ptr; a.ve_code = c->ve_code; a.et_ptr = c->et_ptr; a.et_code = c->et_code;
+    a.tet_pnt = c->tet_pnt; a.seg_pnt = c->seg_pnt; a.vess = c->vess; a.eess = c->eess;
+    a.resv = c->resv; a.rese = c->rese; a.vprcn = c->vprcn; a.eprcn = c->eprcn; a.iprcn = c->iprcn;
+    a.vpc = c->vpc; a.epc = c->epc; a.ugv = c->ugv;
+    a.vdiag = L.vdiag; a.sfmv0 = L.sfmv0; a.sfmv1 = L.sfmv1; a.gm1 = c->prm.gamma - 1.0; a.atm = 0.0; a.vfms = L.vfms;
+    const bool upd = cfa >= 0.0;
+    a.cfa = cfa;
+    if (upd && a.em == 0) { a.ugv_out = c->ugv; a.ug0v = c->ug0v; }
+    hipLaunchKernelGGL(k_minv_vrtx_tetc, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, a);
+    if (a.em > 0) {
+        if (upd) { a.uge_out = c->uge; a.ug0e = c->ug0e; }
+        hipLaunchKernelGGL(k_minv_edge_tetc, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, a);
+        if (upd) {
+            const long n = (long)c->npnt * CNV;
+            hipLaunchKernelGGL(k_modes<CNV>, dim3(nblk(n)), dim3(256), 0, c->stream, 4, (long)c->npnt, 1, 1, c->ugv,
+                               c->ug0v, c->resv, cfa);
+        }
+    }
+    check(hipGetLastError(), "tet cns minvrt launch");
+}
+
 void do_minvrt_tet(hpg_tet_ctx* c, bool pre_dirichlet, double cfa) {
+    if (c->cns()) {
+        do_minvrt_tetc(c, cfa);
+        return;
+    }
     TetMinvArgs a = tet_minv_args(c, pre_dirichlet);
     if (cfa >= 0.0) {
         a.ugv = c->ugv; a.ug0v = c->ug0v; a.cfa = cfa;
@@ -201,7 +286,33 @@ void do_minvrt_tet(hpg_tet_ctx* c, bool pre_dirichlet, double cfa) {
     check(hipGetLastError(), "tet minvrt launch");
 }
 
+template <int P>
+void launch_setup_tetc(hpg_tet_ctx* c) {
+    TetcSetupArgs a;
+    a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.ugv = c->ugv; a.uge = c->uge;
+    a.Pg = c->lv[c->log2p].Pg; a.ntet = c->ntet; a.em0 = c->em0; a.p = P; a.prec = 1;
+    a.mu = c->prm.mu; a.adis = c->prm.adis; a.gamma = c->prm.gamma; a.kcond = c->kcond; a.R = c->prm.Rgas; a.atm = 0.0;
+    a.tau = c->tau; a.tpc = c->tpc; a.iprcn = c->iprcn; a.err = c->d_err;
+    hipLaunchKernelGGL(k_setup_tetc<P>, dim3(nblk8(c->ntet)), dim3(256), 0, c->stream, a);
+}
+
+int do_setup_tetc(hpg_tet_ctx* c) {
+    check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
+    if (c->lv[c->log2p].p == 1) launch_setup_tetc<1>(c); else launch_setup_tetc<2>(c);
+    hipLaunchKernelGGL(k_setup_gather_tetc, dim3(nblk8(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vt_ptr,
+                       c->vt_code, c->iprcn, c->tpc, c->vprcn, c->vpc);
+    if (c->em() > 0)
+        hipLaunchKernelGGL(k_setup_gather_tetc, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->et_ptr,
+                           c->et_code, c->iprcn, c->tpc, c->eprcn, c->epc);
+    check(hipGetLastError(), "tet cns setup launch");
+    int h = 0;
+    check(hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "D2H");
+    check(hipStreamSynchronize(c->stream), "sync");
+    return h;
+}
+
 int do_setup_tet(hpg_tet_ctx* c) {
+    if (c->cns()) return do_setup_tetc(c);
     TetSetupArgs a;
     a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.ugv = c->ugv; a.ntet = c->ntet; a.p = c->lv[c->log2p].p;
     a.rho = c->prm.rho; a.mu = c->prm.mu; a.adis = c->prm.adis; a.iprcn = c->iprcn; a.tau = c->tau; a.err = c->d_err;
@@ -220,18 +331,21 @@ int do_setup_tet(hpg_tet_ctx* c) {
 }
 
 double tet_absmax(hpg_tet_ctx* c, long nent, int nm, int mstride, const double* x, const unsigned char* mask,
-                  double out[TNV]) {
-    unsigned long long h[TNV] = {0, 0, 0, 0};
+                  double out[CNV]) {
+    unsigned long long h[CNV] = {0, 0, 0, 0, 0};
     if (nent > 0 && nm > 0) {
-        check(hipMemsetAsync(c->d_max, 0, TNV * sizeof(unsigned long long), c->stream), "memset");
+        check(hipMemsetAsync(c->d_max, 0, CNV * sizeof(unsigned long long), c->stream), "memset");
         const long n = nent * nm;
         const unsigned nb = std::min<unsigned>(nblk(n), 2048u);
-        hipLaunchKernelGGL(k_absmax<TNV>, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
-        check(hipMemcpyAsync(h, c->d_max, sizeof(h), hipMemcpyDeviceToHost, c->stream), "memcpy");
+        if (c->cns())
+            hipLaunchKernelGGL(k_absmax<CNV>, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
+        else
+            hipLaunchKernelGGL(k_absmax<TNV>, dim3(nb), dim3(256), 0, c->stream, nent, nm, mstride, x, mask, c->d_max);
+        check(hipMemcpyAsync(h, c->d_max, c->nv * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream), "memcpy");
         check(hipStreamSynchronize(c->stream), "sync");
     }
     double r = 0.0;
-    for (int n = 0; n < TNV; ++n) {
+    for (int n = 0; n < c->nv; ++n) {
         double v;
         std::memcpy(&v, &h[n], sizeof(double));
         out[n] = v;
@@ -247,13 +361,19 @@ extern "C" {
 int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** out) {
     if (!m || !prm || !out) return -2;
     *out = nullptr;
-    if (prm->physics != HPG_PHYSICS_INS || prm->log2p < 0 || prm->log2p > 1 || prm->nstage < 1 || prm->nstage > 7)
+    if ((prm->physics != HPG_PHYSICS_INS && prm->physics != HPG_PHYSICS_CNS) || prm->log2p < 0 || prm->log2p > 1 ||
+        prm->nstage < 1 || prm->nstage > 7)
+        return -2;
+    if (prm->physics == HPG_PHYSICS_CNS && !(prm->gamma > 1.0 && prm->prandtl > 0.0 && prm->Rgas > 0.0 && prm->bc[4] > 0.0))
         return -2;
     if (m->npnt <= 0 || m->nseg <= 0 || m->ntet <= 0 || m->nbf < 0) return -2;
     std::unique_ptr<hpg_tet_ctx> c(new hpg_tet_ctx);
     try {
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         c->prm = *prm;
+        c->nv = prm->physics == HPG_PHYSICS_CNS ? CNV : TNV;
+        // tet_hp/cns/allocate.cpp:50
+        c->kcond = prm->Rgas * prm->mu / prm->prandtl * prm->gamma / (prm->gamma - 1.0);
         c->prm.alpha[prm->nstage] = 1.0;
         c->prm.beta[prm->nstage] = 1.0;
         c->npnt = m->npnt; c->nseg = m->nseg; c->ntet = m->ntet; c->nbf = m->nbf;
@@ -274,11 +394,14 @@ int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** o
         c->pnts = c->upload(m->pnts, 3 * (size_t)np);
         c->tet_pnt = c->upload(m->tet_pnt, 4 * (size_t)nt);
         c->tet_seg = c->upload(m->tet_seg, 6 * (size_t)nt);
+        c->seg_pnt = c->upload(m->seg_pnt, 2 * (size_t)ns);
         c->bf_pnt = c->upload(m->bf_pnt, 3 * (size_t)nf);
         c->bf_seg = c->upload(m->bf_seg, 3 * (size_t)nf);
-        // essential velocity (bits of u, v, w) on the vertices / edges of the inflow faces
+        // essential variables on the vertices / edges of the inflow faces: u, v, w (ins,
+        // bits 0-2) or u, v, w, RT (cns, bits 1-4)
+        const unsigned char ebits = c->nv == CNV ? 30 : 7;
         std::vector<unsigned char> vess(np, 0), eess(ns, 0);
-        for (long k = 0; k < 3L * nf; ++k) { vess[m->bf_pnt[k]] = 7; eess[m->bf_seg[k]] = 7; }
+        for (long k = 0; k < 3L * nf; ++k) { vess[m->bf_pnt[k]] = ebits; eess[m->bf_seg[k]] = ebits; }
         c->vess = c->upload(vess.data(), vess.size());
         c->eess = c->upload(eess.data(), eess.size());
         // incidence in ascending element / edge / face order
@@ -301,7 +424,8 @@ int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** o
         csr(np, ve, ptr, code); c->ve_ptr = c->upload(ptr.data(), ptr.size()); c->ve_code = c->upload(code.data(), code.size());
         csr(np, vf, ptr, code); c->vf_ptr = c->upload(ptr.data(), ptr.size()); c->vf_code = c->upload(code.data(), code.size());
         csr(ns, ef, ptr, code); c->ef_ptr = c->upload(ptr.data(), ptr.size()); c->ef_code = c->upload(code.data(), code.size());
-        const size_t nv = (size_t)np * TNV, ne = (size_t)ns * c->em0 * TNV;
+        const int NVc = c->nv;
+        const size_t nv = (size_t)np * NVc, ne = (size_t)ns * c->em0 * NVc;
         double** vv[] = {&c->ugv, &c->ug0v, &c->resv, &c->rrv, &c->r0v};
         double** ee[] = {&c->uge, &c->ug0e, &c->rese, &c->rre, &c->r0e};
         for (auto p : vv) *p = c->alloc<double>(nv);
@@ -311,13 +435,18 @@ int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** o
             c->drese[l] = c->alloc<double>(ne);
         }
         c->vprcn = c->alloc<double>(nv);
-        c->eprcn = c->alloc<double>((size_t)ns * TNV);
-        c->iprcn = c->alloc<double>((size_t)nt * TNV);
-        c->tau = c->alloc<double>((size_t)nt * TNV);
-        c->lf = c->alloc<double>((size_t)nt * 2 * TNV * (c->log2pmax ? 10 : 4));
-        c->flf = c->alloc<double>((size_t)nf * 6);
+        c->eprcn = c->alloc<double>((size_t)ns * NVc);
+        c->iprcn = c->alloc<double>((size_t)nt * NVc);
+        c->tau = c->alloc<double>((size_t)nt * (c->nv == CNV ? 25 : TNV));
+        c->lf = c->alloc<double>((size_t)nt * 2 * NVc * (c->log2pmax ? 10 : 4));
+        c->flf = c->alloc<double>((size_t)nf * 6 * (c->nv == CNV ? CNV : 1));
+        if (c->nv == CNV) {
+            c->tpc = c->alloc<double>((size_t)nt * 25);
+            c->vpc = c->alloc<double>((size_t)np * 25);
+            c->epc = c->alloc<double>((size_t)ns * 25);
+        }
         c->d_err = c->alloc<int>(1);
-        c->d_max = c->alloc<unsigned long long>(TNV);
+        c->d_max = c->alloc<unsigned long long>(CNV);
         for (int l = 0; l <= c->log2pmax; ++l) {
             const TetOps o(1 << l);
             TetLevel& L = c->lv[l];
@@ -326,6 +455,7 @@ int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** o
             L.ops = c->upload(pk.data(), pk.size());
             L.P2 = c->upload(o.P2.data(), o.P2.size());
             L.W2 = c->upload(o.W2.data(), o.W2.size());
+            L.Pg = c->upload(o.P.data(), o.P.size());
             L.vdiag = o.vdiag;
             if (o.em > 0) {
                 L.ediag = o.ediag[0]; L.sfmv0 = o.sfmv[0]; L.sfmv1 = o.sfmv[1];
@@ -348,7 +478,7 @@ int hpg_tet_destroy(hpg_tet_ctx* c) {
 static int tet_copy(hpg_tet_ctx* c, bool h2d, double* dv, double* de, const double* hv, const double* he, double* ov,
                     double* oe) {
     HPG_TRY(c, {
-        const size_t nv = (size_t)c->npnt * TNV * 8, ne = (size_t)c->nseg * c->em0 * TNV * 8;
+        const size_t nv = (size_t)c->npnt * c->nv * 8, ne = (size_t)c->nseg * c->em0 * c->nv * 8;
         if (h2d) {
             check(hipMemcpyAsync(dv, hv, nv, hipMemcpyHostToDevice, c->stream), "H2D");
             if (ne) check(hipMemcpyAsync(de, he, ne, hipMemcpyHostToDevice, c->stream), "H2D");
@@ -380,10 +510,21 @@ int hpg_tet_set_residual(hpg_tet_ctx* c, const double* v, const double* e) {
 int hpg_tet_get_preconditioner(hpg_tet_ctx* c, double* vprcn, double* eprcn, double* iprcn, double* tau) {
     if (!c || !vprcn || !eprcn || !iprcn || !tau) return -2;
     HPG_TRY(c, {
-        check(hipMemcpyAsync(vprcn, c->vprcn, (size_t)c->npnt * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
-        check(hipMemcpyAsync(eprcn, c->eprcn, (size_t)c->nseg * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
-        check(hipMemcpyAsync(iprcn, c->iprcn, (size_t)c->ntet * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
-        check(hipMemcpyAsync(tau, c->tau, (size_t)c->ntet * TNV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        const size_t k = c->nv, kt = c->cns() ? 25 : TNV;
+        check(hipMemcpyAsync(vprcn, c->vprcn, (size_t)c->npnt * k * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(eprcn, c->eprcn, (size_t)c->nseg * k * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(iprcn, c->iprcn, (size_t)c->ntet * k * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(tau, c->tau, (size_t)c->ntet * kt * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+int hpg_tet_get_cns_blocks(hpg_tet_ctx* c, double* vpc, double* epc) {
+    if (!c || !vpc || !epc) return -2;
+    if (!c->cns()) { c->err = "5 x 5 preconditioner blocks exist for cns only"; return -2; }
+    HPG_TRY(c, {
+        check(hipMemcpyAsync(vpc, c->vpc, (size_t)c->npnt * 25 * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(epc, c->epc, (size_t)c->nseg * 25 * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
         check(hipStreamSynchronize(c->stream), "sync");
     });
     return 0;
@@ -407,13 +548,15 @@ int hpg_tet_minvrt(hpg_tet_ctx* c) {
 int hpg_tet_update(hpg_tet_ctx* c) {
     if (!c) return -2;
     HPG_TRY(c, {
-        check(hipMemcpyAsync(c->ug0v, c->ugv, (size_t)c->npnt * TNV * 8, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        // tet_hp::update (update.cpp:181-417) and tet_hp_cns::update (cns/update.cpp:18-143,
+        // no Dirichlet pass before minvrt)
+        check(hipMemcpyAsync(c->ug0v, c->ugv, (size_t)c->npnt * c->nv * 8, hipMemcpyDeviceToDevice, c->stream), "D2D");
         if (c->em0)
-            check(hipMemcpyAsync(c->ug0e, c->uge, (size_t)c->nseg * c->em0 * TNV * 8, hipMemcpyDeviceToDevice,
+            check(hipMemcpyAsync(c->ug0e, c->uge, (size_t)c->nseg * c->em0 * c->nv * 8, hipMemcpyDeviceToDevice,
                                  c->stream), "D2D");
         for (int s = 0; s < c->prm.nstage; ++s) {
             do_rsdl_tet(c, s);
-            do_minvrt_tet(c, true, c->prm.alpha[s] * c->prm.cfl[c->log2p]);
+            do_minvrt_tet(c, !c->cns(), c->prm.alpha[s] * c->prm.cfl[c->log2p]);
         }
     });
     return 0;
@@ -421,17 +564,17 @@ int hpg_tet_update(hpg_tet_ctx* c) {
 int hpg_tet_maxres(hpg_tet_ctx* c, double* per_var, double* out) {
     if (!c || !out) return -2;
     HPG_TRY(c, {
-        double a[TNV];
+        double a[CNV];
         *out = tet_absmax(c, c->npnt, 1, 1, c->resv, nullptr, a);
         if (per_var)
-            for (int n = 0; n < TNV; ++n) per_var[n] = a[n];
+            for (int n = 0; n < c->nv; ++n) per_var[n] = a[n];
     });
     return 0;
 }
 int hpg_tet_resnorm(hpg_tet_ctx* c, double* out) {
     if (!c || !out) return -2;
     HPG_TRY(c, {
-        double a[TNV], b[TNV] = {0, 0, 0, 0};
+        double a[CNV], b[CNV] = {0, 0, 0, 0, 0};
         double r = tet_absmax(c, c->npnt, 1, 1, c->resv, c->vess, a);
         if (c->em() > 0) r = std::max(r, tet_absmax(c, c->nseg, c->em(), c->em0, c->rese, c->eess, b));
         *out = r;

@@ -300,6 +300,7 @@ __global__ __launch_bounds__(256) void k_face_tet(int nbf, const int* __restrict
 // (tet_hp/update.cpp:13-31, 45-70, bdry.cpp:16-57); elements and faces ascending
 struct TetGatherArgs {
     int nent, tm, tm2, mbase, fbase, stride;
+    int fnv;          // face variables: 1 (continuity into the last variable) or NV, flf[f][fnv][tm2]
     const int* ptr;   // CSR entity -> (element, local index) codes 16*t + i
     const int* code;
     const int* fptr;  // CSR entity -> (face, local index) codes 8*f + j
@@ -311,29 +312,37 @@ struct TetGatherArgs {
     double omb;
 };
 
+template <int NV>
 __global__ __launch_bounds__(256) void k_gather_tet(TetGatherArgs a) {
     const int v = xcd_block() * blockDim.x + threadIdx.x;
     if (v >= a.nent) return;
-    double si[TNV] = {0.0, 0.0, 0.0, 0.0}, sr[TNV];
+    double si[NV], sr[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) si[n] = 0.0;
     for (int k = a.fptr[v]; k < a.fptr[v + 1]; ++k) {
         const int c = a.fcode[k], f = c >> 3, j = c & 7;
-        si[TNV - 1] += a.flf[f * a.tm2 + a.fbase + j];
+        if (a.fnv == 1) {
+            si[NV - 1] += a.flf[f * a.tm2 + a.fbase + j];
+        } else {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) si[n] += a.flf[(f * NV + n) * a.tm2 + a.fbase + j];
+        }
     }
     double* R = a.res + (size_t)v * a.stride;
     double* RR = a.rr + (size_t)v * a.stride;
 #pragma unroll
-    for (int n = 0; n < TNV; ++n) sr[n] = a.omb * RR[n];
+    for (int n = 0; n < NV; ++n) sr[n] = a.omb * RR[n];
     for (int k = a.ptr[v]; k < a.ptr[v + 1]; ++k) {
         const int c = a.code[k], t = c >> 4, i = c & 15;
-        const double* L = a.lf + (size_t)t * (2 * TNV * a.tm) + a.mbase + i;
+        const double* L = a.lf + (size_t)t * (2 * NV * a.tm) + a.mbase + i;
 #pragma unroll
-        for (int n = 0; n < TNV; ++n) {
+        for (int n = 0; n < NV; ++n) {
             si[n] += L[n * a.tm];
-            sr[n] += L[(TNV + n) * a.tm];
+            sr[n] += L[(NV + n) * a.tm];
         }
     }
 #pragma unroll
-    for (int n = 0; n < TNV; ++n) {
+    for (int n = 0; n < NV; ++n) {
         RR[n] = sr[n];
         R[n] = si[n] + sr[n];
     }

@@ -32,7 +32,7 @@ EXPORTS = (
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
     "hpg_get_block_preconditioner", "hpg_link_fine",
     "hpg_tet_create", "hpg_tet_destroy", "hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual",
-    "hpg_tet_set_residual", "hpg_tet_get_preconditioner", "hpg_tet_rsdl", "hpg_tet_setup_preconditioner",
+    "hpg_tet_set_residual", "hpg_tet_get_preconditioner", "hpg_tet_get_cns_blocks", "hpg_tet_rsdl", "hpg_tet_setup_preconditioner",
     "hpg_tet_minvrt", "hpg_tet_update", "hpg_tet_maxres", "hpg_tet_resnorm", "hpg_tet_mg_restrict",
     "hpg_tet_mg_prolongate", "hpg_tet_get_log2p", "hpg_tet_synchronize", "hpg_tet_last_error",
     "hpg_tet_kernel_timing", "hpg_tet_kernel_timing_read",
@@ -63,7 +63,7 @@ class ParamsC(C.Structure):
         ("rho", C.c_double), ("mu", C.c_double), ("adis", C.c_double),
         ("bc_pressure", C.c_double), ("fadd", C.c_double),
         ("nstage", C.c_int), ("alpha", C.c_double * 8), ("beta", C.c_double * 8), ("cfl", C.c_double * 4),
-        ("gamma", C.c_double), ("prandtl", C.c_double), ("Rgas", C.c_double), ("bc", C.c_double * 4),
+        ("gamma", C.c_double), ("prandtl", C.c_double), ("Rgas", C.c_double), ("bc", C.c_double * 5),
     ]
 
 
@@ -96,6 +96,7 @@ def load():
     for nm in ("hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual", "hpg_tet_set_residual"):
         getattr(L, nm).argtypes = [vp, dp, dp]
     L.hpg_tet_get_preconditioner.argtypes = [vp, dp, dp, dp, dp]
+    L.hpg_tet_get_cns_blocks.argtypes = [vp, dp, dp]
     L.hpg_tet_rsdl.argtypes = [vp, C.c_int]
     for nm in ("hpg_tet_setup_preconditioner", "hpg_tet_minvrt", "hpg_tet_update", "hpg_tet_mg_restrict",
                "hpg_tet_mg_prolongate", "hpg_tet_synchronize"):

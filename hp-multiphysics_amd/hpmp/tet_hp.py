@@ -1,10 +1,11 @@
-"""Host-side mirror of tet_hp / tet_hp_ins (tet_hp/tet_hp.h:229-273,
-tet_hp/ins/tet_hp_ins.h) over the hpg_tet_* C-ABI of libhpgpu.so.
+"""Host-side mirror of tet_hp / tet_hp_ins / tet_hp_cns (tet_hp/tet_hp.h:229-273,
+tet_hp/ins/tet_hp_ins.h, tet_hp/cns/tet_hp_cns.h) over the hpg_tet_* C-ABI of
+libhpgpu.so.
 
 Same method names and argument meaning as the reference's virtual interface:
 rsdl(stage), setup_preconditioner() -> error count, minvrt(), update(), maxres(),
 mg_restrict(), mg_prolongate().  State is exchanged in the reference layouts
-v(npnt, 4) and e(nseg, em0, 4) (tet_hp/tet_hp.h:89-94).  There is no CPU path:
+v(npnt, NV) and e(nseg, em0, NV) (tet_hp/tet_hp.h:89-94).  There is no CPU path:
 the library must be built.
 """
 from __future__ import annotations
@@ -21,37 +22,34 @@ NSTAGE_BETA = (1.0, 0.0, 5.0 / 9.0, 0.0, 4.0 / 9.0)
 TET_CFL_DEFAULT = (2.2, 1.5, 1.0, 0.5)   # tet_hp/allocate.cpp:234
 
 
-class TetHpIns:
+class _TetHp:
     NV = 4
 
-    def __init__(self, mesh: TetMesh, log2p: int = 1, rho: float = 1.0, mu: float = 0.0, adis: float = 1.0,
-                 fadd: float = 1.0, alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA, cfl=TET_CFL_DEFAULT):
+    def _create(self, mesh: TetMesh, prm, log2p: int):
         self.L = _lib.load()
         self.mesh = mesh
         self.log2pmax = int(log2p)
         self.em0 = (1 << self.log2pmax) - 1
-        self.nstage = len(alpha)
         m = mesh
         self._keep = [np.ascontiguousarray(a) for a in (
             m.pnts.astype(np.float64), m.seg_pnt.astype(np.int32), m.tet_pnt.astype(np.int32),
             m.tet_seg.astype(np.int32), m.tet_sgn.astype(np.int32), m.bf_pnt.astype(np.int32),
             m.bf_seg.astype(np.int32), m.bf_sgn.astype(np.int32))]
         cm = _lib.TetMeshC(m.npnt, m.nseg, m.ntet, m.nbf, *(C.c_void_p(a.ctypes.data) for a in self._keep))
-        prm = _lib.ParamsC()
-        prm.physics = _lib.HPG_PHYSICS_INS
-        prm.log2p = self.log2pmax
-        prm.rho, prm.mu, prm.adis, prm.fadd = rho, mu, adis, fadd
-        prm.nstage = self.nstage
-        for s in range(self.nstage):
-            prm.alpha[s] = alpha[s]
-            prm.beta[s] = beta[s]
-        for lvl in range(4):
-            prm.cfl[lvl] = cfl[lvl] if lvl < len(cfl) else 0.5
         h = C.c_void_p()
         rc = self.L.hpg_tet_create(C.byref(cm), C.byref(prm), C.byref(h))
         if rc != 0 or not h.value:
             raise _lib.HpgError(f"hpg_tet_create failed ({rc})")
         self.h = h
+
+    @staticmethod
+    def _stages(prm, alpha, beta, cfl):
+        prm.nstage = len(alpha)
+        for s in range(len(alpha)):
+            prm.alpha[s] = alpha[s]
+            prm.beta[s] = beta[s]
+        for lvl in range(4):
+            prm.cfl[lvl] = cfl[lvl] if lvl < len(cfl) else 0.5
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -94,8 +92,10 @@ class TetHpIns:
         self._check(self.L.hpg_tet_set_residual(self.h, *(_lib.ptr(x) for x in a)), "set_residual")
 
     def get_preconditioner(self):
-        m = self.mesh
-        out = tuple(np.zeros((n, 4)) for n in (m.npnt, m.nseg, m.ntet, m.ntet))
+        """vprcn, eprcn, iprcn (NV values each) and tau: (ntet, 4) for ins, (ntet, 5, 5) for cns"""
+        m, nv = self.mesh, self.NV
+        out = (np.zeros((m.npnt, nv)), np.zeros((m.nseg, nv)), np.zeros((m.ntet, nv)),
+               np.zeros((m.ntet, 4)) if nv == 4 else np.zeros((m.ntet, 5, 5)))
         self._check(self.L.hpg_tet_get_preconditioner(self.h, *(_lib.ptr(x) for x in out)), "get_preconditioner")
         return out
 
@@ -160,3 +160,47 @@ class TetHpIns:
             for _ in range(iterrfne):
                 self.update()
         return error
+
+
+class TetHpIns(_TetHp):
+    NV = 4
+
+    def __init__(self, mesh: TetMesh, log2p: int = 1, rho: float = 1.0, mu: float = 0.0, adis: float = 1.0,
+                 fadd: float = 1.0, alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA, cfl=TET_CFL_DEFAULT):
+        self.nstage = len(alpha)
+        prm = _lib.ParamsC()
+        prm.physics = _lib.HPG_PHYSICS_INS
+        prm.log2p = int(log2p)
+        prm.rho, prm.mu, prm.adis, prm.fadd = rho, mu, adis, fadd
+        self._stages(prm, alpha, beta, cfl)
+        self._create(mesh, prm, log2p)
+
+
+class TetHpCns(_TetHp):
+    """tet_hp_cns: unknowns (p, u, v, w, RT), every boundary face an inflow face with the
+    boundary state bc = (u, v, w, RT); Weiss-Smith preconditioner."""
+    NV = 5
+
+    def __init__(self, mesh: TetMesh, log2p: int = 1, mu: float = 0.0, adis: float = 1.0, fadd: float = 1.0,
+                 gamma: float = 1.4, prandtl: float = 0.713, R: float = 287.058, bc=(0.0, 0.0, 0.0, 1.0),
+                 alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA, cfl=TET_CFL_DEFAULT):
+        if len(bc) != 4:
+            raise ValueError("bc is the inflow state (u, v, w, RT)")
+        self.nstage = len(alpha)
+        prm = _lib.ParamsC()
+        prm.physics = _lib.HPG_PHYSICS_CNS
+        prm.log2p = int(log2p)
+        prm.rho, prm.mu, prm.adis, prm.fadd = 1.0, mu, adis, fadd
+        prm.gamma, prm.prandtl, prm.Rgas = gamma, prandtl, R
+        prm.bc[0] = 0.0
+        for k in range(4):
+            prm.bc[k + 1] = bc[k]
+        self._stages(prm, alpha, beta, cfl)
+        self._create(mesh, prm, log2p)
+
+    def get_cns_blocks(self):
+        """vpreconditioner (npnt, 5, 5) and epreconditioner (nseg, 5, 5)"""
+        m = self.mesh
+        vpc, epc = np.zeros((m.npnt, 5, 5)), np.zeros((m.nseg, 5, 5))
+        self._check(self.L.hpg_tet_get_cns_blocks(self.h, _lib.ptr(vpc), _lib.ptr(epc)), "get_cns_blocks")
+        return vpc, epc

@@ -61,10 +61,11 @@ typedef struct {
     int nstage;              /* <= 7 */
     double alpha[8], beta[8];/* stage coefficients; entry nstage is set to 1 */
     double cfl[4];           /* per log2p */
-    /* cns only (tri_hp/cns/allocate.cpp:32-45): gamma, prandtl, Rgas, and the
-     * inflow / boundary-flux state (p, u, v, RT) of the ibc */
+    /* cns only (tri_hp/cns/allocate.cpp:32-45, tet_hp/cns/allocate.cpp:29-50): gamma,
+     * prandtl, Rgas, and the inflow / boundary-flux state of the ibc: tri (p, u, v, RT)
+     * in bc[0..3]; tet (unused, u, v, w, RT) in bc[0..4] */
     double gamma, prandtl, Rgas;
-    double bc[4];
+    double bc[5];
 } hpg_params;
 
 /* replaces tri_hp::init / tri_hp_ins::init (tri_hp/allocate.cpp, tri_hp/ins/allocate.cpp:13-44) */
@@ -192,14 +193,18 @@ int hpg_selftest_mfma_bf16(const float *A, const float *B, float *out);
  * traffic counters of the other kernels are scaled against. */
 int hpg_selftest_copy(long long nbytes, int reps, double *ms_per_copy);
 
-/* ---- tet_hp_ins (straight tetrahedra, p = 1, 2, NV = 4: u, v, w, p) ------------------
+/* ---- tet_hp_ins (straight tetrahedra, p = 1, 2, NV = 4: u, v, w, p) and tet_hp_cns
+ * (NV = 5: p, u, v, w, RT; tet_hp/cns/tet_hp_cns.h:19-66) ----------------------------
  * One context per tetrahedral block.  Mesh arrays of tet_mesh (tet_mesh/tet_mesh.h:
  * 73-137): edge i of a tet joins its local vertices (2,3), (3,1), (2,1), (1,0), (2,0),
  * (3,0) with sign +1 when the edge's first vertex is the first of that pair
  * (tet_mesh/setup.cpp:230-275); tets positively oriented; boundary faces ordered so
  * that (p2 - p1) x (p0 - p1) points out, edge j of a face opposite its vertex j.
- * Every boundary face is an inflow face (tet_hp/ins/bdry_ins.h:118-183): u, v, w
- * essential on its vertices and edges, continuity flux rho u.n through it. */
+ * Every boundary face is an inflow face: ins (tet_hp/ins/bdry_ins.h:118-183) u, v, w
+ * essential on its vertices and edges, continuity flux rho u.n through it; cns
+ * (tet_hp/cns/bdry_cns.h:70-98, bdry.cpp:129-150) u, v, w, RT essential, the five
+ * fluxes with the solution's pressure and the boundary state bc[1..4].  The cns path
+ * uses the Weiss-Smith preconditioner (preconditioner = 1, the reference's default). */
 typedef struct hpg_tet_ctx hpg_tet_ctx;
 typedef struct {
     int npnt, nseg, ntet, nbf;
@@ -212,28 +217,38 @@ typedef struct {
     const int *bf_seg;       /* nbf x 3 */
     const int *bf_sgn;       /* nbf x 3 */
 } hpg_tet_mesh;
-/* replaces tet_hp::init / tet_hp_ins::init (tet_hp/allocate.cpp, tet_hp/ins/allocate.cpp);
- * prm: physics HPG_PHYSICS_INS, log2p 0 or 1, rho, mu, adis, fadd, nstage, alpha, beta, cfl */
+/* replaces tet_hp::init / tet_hp_ins::init / tet_hp_cns::init (tet_hp/allocate.cpp,
+ * tet_hp/ins/allocate.cpp, tet_hp/cns/allocate.cpp); prm: physics HPG_PHYSICS_INS or
+ * HPG_PHYSICS_CNS, log2p 0 or 1, rho (ins), mu, adis, fadd, nstage, alpha, beta, cfl;
+ * cns also gamma, prandtl, Rgas, bc[1..4] */
 int hpg_tet_create(const hpg_tet_mesh *mesh, const hpg_params *prm, hpg_tet_ctx **out);
 int hpg_tet_destroy(hpg_tet_ctx *ctx);
-/* tet_hp::ug (tet_hp/tet_hp.h:89-94): v(npnt,4), e(nseg,em0,4), em0 = p - 1 at log2p */
+/* tet_hp::ug (tet_hp/tet_hp.h:89-94): v(npnt,NV), e(nseg,em0,NV), em0 = p - 1 at log2p */
 int hpg_tet_set_state(hpg_tet_ctx *ctx, const double *ugv, const double *uge);
 int hpg_tet_get_state(hpg_tet_ctx *ctx, double *ugv, double *uge);
 /* tet_hp::hp_global::res (tet_hp/tet_hp.h:120-130) */
 int hpg_tet_get_residual(hpg_tet_ctx *ctx, double *resv, double *rese);
 int hpg_tet_set_residual(hpg_tet_ctx *ctx, const double *resv, const double *rese);
-/* vprcn, eprcn (inverted), iprcn (per element) and tau (tet_hp_ins::tau) */
+/* vprcn, eprcn (inverted), iprcn (per element), NV values each, and tau: ins
+ * tet_hp_ins::tau (ntet x 4), cns tet_hp_cns::tau (ntet x 5 x 5, adis Tau / jcb) */
 int hpg_tet_get_preconditioner(hpg_tet_ctx *ctx, double *vprcn, double *eprcn, double *iprcn, double *tau);
+/* cns: hp_cns_gbl->vpreconditioner (npnt x 5 x 5) and epreconditioner (nseg x 5 x 5)
+ * (tet_hp/cns/setup_preconditioner.cpp:152-183) */
+int hpg_tet_get_cns_blocks(hpg_tet_ctx *ctx, double *vpc, double *epc);
 /* replaces tet_hp::rsdl(stage) (tet_hp/update.cpp:9-131, tet_hp.h:229) with
- * tet_hp_ins::element_rsdl (tet_hp/ins/rsdl.cpp:15-785, straight, steady) */
+ * tet_hp_ins::element_rsdl (tet_hp/ins/rsdl.cpp:15-785) or tet_hp_cns::element_rsdl
+ * (tet_hp/cns/rsdl.cpp:16-843), straight, steady */
 int hpg_tet_rsdl(hpg_tet_ctx *ctx, int stage);
 /* replaces tet_hp_ins::setup_preconditioner (tet_hp/ins/setup_preconditioner.cpp:7-225)
- * and tet_hp::setup_preconditioner (tet_hp/setup_preconditioner.cpp:428-525);
- * returns the error count */
+ * or tet_hp_cns::setup_preconditioner (tet_hp/cns/setup_preconditioner.cpp:9-489), then
+ * tet_hp::setup_preconditioner (tet_hp/setup_preconditioner.cpp:428-525); returns the
+ * error count (cns: the reference aborts on a NaN or a non-positive element size) */
 int hpg_tet_setup_preconditioner(hpg_tet_ctx *ctx);
-/* replaces tet_hp::minvrt (tet_hp/setup_preconditioner.cpp:11-351), p <= 2 */
+/* replaces tet_hp::minvrt (tet_hp/setup_preconditioner.cpp:11-351) or
+ * tet_hp_cns::minvrt (tet_hp/cns/update.cpp:146-347), p <= 2 */
 int hpg_tet_minvrt(hpg_tet_ctx *ctx);
-/* replaces tet_hp::update (tet_hp/update.cpp:181-417) */
+/* replaces tet_hp::update (tet_hp/update.cpp:181-417) or tet_hp_cns::update
+ * (tet_hp/cns/update.cpp:18-143) */
 int hpg_tet_update(hpg_tet_ctx *ctx);
 /* replaces tet_hp::maxres (tet_hp/allocate.cpp:530-559) */
 int hpg_tet_maxres(hpg_tet_ctx *ctx, double *per_var, double *out);

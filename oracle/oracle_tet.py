@@ -1,8 +1,10 @@
-"""ctypes front end of the tet_hp_ins part of the CPU oracle (test infrastructure only).
+"""ctypes front end of the tet_hp_ins and tet_hp_cns part of the CPU oracle (test
+infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
 this module.  It drives the same liboracle.so as oracle.py; the C++ restatement
-is oracle/otet_basis.cpp (tet_basis) and oracle/otet_hp.cpp (tet_hp, tet_hp_ins).
+is oracle/otet_basis.cpp (tet_basis), oracle/otet_hp.cpp (tet_hp, tet_hp_ins) and
+oracle/otet_cns.cpp (tet_hp_cns).
 """
 from __future__ import annotations
 
@@ -55,6 +57,25 @@ def lib():
         L.otet_log2p.argtypes = [C.c_void_p]
         L.otet_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.otet_face_rsdl.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.otetc_create.restype = C.c_void_p
+        L.otetc_create.argtypes = L.otet_create.argtypes
+        L.otetc_destroy.argtypes = [C.c_void_p]
+        for nm in ("otetc_set_state", "otetc_get_state", "otetc_get_res", "otetc_set_res"):
+            getattr(L, nm).argtypes = [C.c_void_p, _dp, _dp]
+        L.otetc_get_prcn.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.otetc_set_tau.argtypes = [C.c_void_p, _dp]
+        L.otetc_rsdl.argtypes = [C.c_void_p, C.c_int]
+        L.otetc_setup_preconditioner.argtypes = [C.c_void_p]
+        L.otetc_setup_preconditioner.restype = C.c_int
+        for nm in ("otetc_minvrt", "otetc_update", "otetc_mg_restrict", "otetc_mg_prolongate"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+        L.otetc_maxres.argtypes = [C.c_void_p, _dp]
+        L.otetc_maxres.restype = C.c_double
+        L.otetc_log2p.argtypes = [C.c_void_p]
+        L.otetc_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.otetc_face_rsdl.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.otetc_tau_timestep.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.otetc_switch_variables.argtypes = [C.c_void_p, _dp, _dp]
         _done = True
     return L
 
@@ -214,3 +235,118 @@ class TetHpIns:
         lf = np.zeros((4, tf))
         lib().otet_face_rsdl(self.h, f, lf)
         return lf
+
+
+class TetHpCns:
+    """tet_hp_cns (p, u, v, w, RT) on a straight tetrahedral mesh, every boundary face an
+    inflow face with the boundary values bc = (unused, u, v, w, RT)."""
+    NV = 5
+
+    def __init__(self, mesh, log2p=1, mu=0.0, adis=1.0, fadd=1.0, gamma=1.4, prandtl=0.713, R=287.058,
+                 bc=(0.0, 0.0, 0.0, 0.0, 1.0), preconditioner=1, alpha=NSTAGE_ALPHA, beta=NSTAGE_BETA,
+                 cfl=TET_CFL_DEFAULT):
+        L = lib()
+        self.mesh = mesh
+        self.log2pmax = int(log2p)
+        self.em0 = (1 << self.log2pmax) - 1
+        m = mesh
+        self._keep = [np.ascontiguousarray(a) for a in (
+            m.pnts.astype(np.float64), m.seg_pnt.astype(np.int32), m.tet_pnt.astype(np.int32),
+            m.tet_seg.astype(np.int32), m.tet_sgn.astype(np.int32), m.bf_pnt.astype(np.int32),
+            m.bf_seg.astype(np.int32), m.bf_sgn.astype(np.int32))]
+        k = self._keep
+        prm = np.array([mu, adis, fadd, gamma, prandtl, R, *bc, preconditioner], dtype=np.float64)
+        cf = np.array(list(cfl) + [0.5] * 4, dtype=np.float64)[:4]
+        self.h = L.otetc_create(m.npnt, k[0].ravel(), m.nseg, k[1].ravel(), m.ntet, k[2].ravel(), k[3].ravel(),
+                                k[4].ravel(), m.nbf, k[5].ravel(), k[6].ravel(), k[7].ravel(), prm, self.log2pmax,
+                                len(alpha), np.array(alpha, dtype=np.float64), np.array(beta, dtype=np.float64), cf)
+        if not self.h:
+            raise RuntimeError("otetc_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().otetc_destroy(self.h)
+            self.h = None
+
+    def shapes(self):
+        return (self.mesh.npnt, self.NV), (self.mesh.nseg, self.em0, self.NV)
+
+    def zeros(self):
+        return tuple(np.zeros(s) for s in self.shapes())
+
+    def set_state(self, v, e):
+        lib().otetc_set_state(self.h, *(np.ascontiguousarray(a, dtype=np.float64) for a in (v, e)))
+
+    def get_state(self):
+        v, e = self.zeros()
+        lib().otetc_get_state(self.h, v, e)
+        return v, e
+
+    def get_res(self):
+        v, e = self.zeros()
+        lib().otetc_get_res(self.h, v, e)
+        return v, e
+
+    def set_res(self, v, e):
+        lib().otetc_set_res(self.h, *(np.ascontiguousarray(a, dtype=np.float64) for a in (v, e)))
+
+    def get_prcn(self):
+        """vprcn (npnt, 5), eprcn (nseg, 5), tau (ntet, 5, 5), vpc (npnt, 5, 5),
+        epc (nseg, 5, 5), tstep (ntet,)"""
+        m = self.mesh
+        vp, ep = np.zeros((m.npnt, 5)), np.zeros((m.nseg, 5))
+        tau, vpc, epc = np.zeros((m.ntet, 5, 5)), np.zeros((m.npnt, 5, 5)), np.zeros((m.nseg, 5, 5))
+        ts = np.zeros(m.ntet)
+        lib().otetc_get_prcn(self.h, vp, ep, tau, vpc, epc, ts)
+        return vp, ep, tau, vpc, epc, ts
+
+    def set_tau(self, tau):
+        lib().otetc_set_tau(self.h, np.ascontiguousarray(tau, dtype=np.float64))
+
+    def rsdl(self, stage=None):
+        lib().otetc_rsdl(self.h, len(NSTAGE_ALPHA) if stage is None else stage)
+
+    def setup_preconditioner(self) -> int:
+        return lib().otetc_setup_preconditioner(self.h)
+
+    def minvrt(self):
+        lib().otetc_minvrt(self.h)
+
+    def update(self):
+        lib().otetc_update(self.h)
+
+    def maxres(self):
+        pv = np.zeros(5)
+        return lib().otetc_maxres(self.h, pv), pv
+
+    def mg_restrict(self):
+        lib().otetc_mg_restrict(self.h)
+
+    def mg_prolongate(self):
+        lib().otetc_mg_prolongate(self.h)
+
+    @property
+    def log2p(self):
+        return lib().otetc_log2p(self.h)
+
+    def element_rsdl(self, tind, stage, uht):
+        tm = uht.shape[1]
+        lre, lim = np.zeros((5, tm)), np.zeros((5, tm))
+        lib().otetc_element_rsdl(self.h, tind, stage, np.ascontiguousarray(uht, dtype=np.float64), lre, lim)
+        return lre, lim
+
+    def face_rsdl(self, f):
+        tf = 3 * (1 + ((1 << self.log2p) - 1))
+        lf = np.zeros((5, tf))
+        lib().otetc_face_rsdl(self.h, f, lf)
+        return lf
+
+    def tau_timestep(self, pvu, hmin, hmax):
+        pinv, tau, ts = np.zeros(25), np.zeros(25), np.zeros(1)
+        lib().otetc_tau_timestep(self.h, np.ascontiguousarray(pvu, dtype=np.float64), hmin, hmax, pinv, tau, ts)
+        return pinv.reshape(5, 5), tau.reshape(5, 5), float(ts[0])
+
+    def switch_variables(self, pvu, a):
+        x = np.array(a, dtype=np.float64)
+        lib().otetc_switch_variables(self.h, np.ascontiguousarray(pvu, dtype=np.float64), x)
+        return x

@@ -1,5 +1,5 @@
-// Oracle (test infrastructure only): C entry points of the tet_hp_ins restatement,
-// bound by oracle/oracle_tet.py.
+// Oracle (test infrastructure only): C entry points of the tet_hp_ins and tet_hp_cns
+// restatements, bound by oracle/oracle_tet.py.
 #include <cstring>
 #include <memory>
 #include <string>
@@ -111,5 +111,77 @@ void otet_face_rsdl(void* h, int f, double* lf) {
     for (int n = 0; n < 4; ++n)
         for (int i = 0; i < TF; ++i) lf[n * TF + i] = l[n][i];
 }
+
+// ---- tet_hp_cns ----------------------------------------------------------------------
+// prm: mu, adis, fadd, gamma, prandtl, R, bc[5] (p unused, u, v, w, RT), preconditioner
+void* otetc_create(int npnt, const double* pnts, int nseg, const int* seg_pnt, int ntet, const int* tet_pnt,
+                   const int* tet_seg, const int* tet_sgn, int nbf, const int* bf_pnt, const int* bf_seg,
+                   const int* bf_sgn, const double* prm, int log2p, int nstage, const double* alpha,
+                   const double* beta, const double* cfl) {
+    try {
+        TetMesh m;
+        m.npnt = npnt; m.nseg = nseg; m.ntet = ntet; m.nbf = nbf;
+        m.pnts.assign(pnts, pnts + 3 * npnt);
+        m.seg_pnt.assign(seg_pnt, seg_pnt + 2 * nseg);
+        m.tet_pnt.assign(tet_pnt, tet_pnt + 4 * ntet);
+        m.tet_seg.assign(tet_seg, tet_seg + 6 * ntet);
+        m.tet_sgn.assign(tet_sgn, tet_sgn + 6 * ntet);
+        m.bf_pnt.assign(bf_pnt, bf_pnt + 3 * nbf);
+        m.bf_seg.assign(bf_seg, bf_seg + 3 * nbf);
+        m.bf_sgn.assign(bf_sgn, bf_sgn + 3 * nbf);
+        TetParams p;
+        p.mu = prm[0]; p.adis = prm[1]; p.fadd = prm[2]; p.gamma = prm[3]; p.prandtl = prm[4]; p.R = prm[5];
+        for (int k = 0; k < 5; ++k) p.bc[k] = prm[6 + k];
+        p.preconditioner = (int)prm[11];
+        p.log2pmax = log2p;
+        p.nstage = nstage;
+        for (int s = 0; s < nstage && s < 8; ++s) { p.alpha[s] = alpha[s]; p.beta[s] = beta[s]; }
+        for (int l = 0; l <= log2p; ++l) p.cfl[l] = cfl[l];
+        return new TetHpCns(m, p);
+    } catch (...) {
+        return nullptr;
+    }
+}
+void otetc_destroy(void* h) { delete static_cast<TetHpCns*>(h); }
+static TetHpCns& XC(void* h) { return *static_cast<TetHpCns*>(h); }
+void otetc_set_state(void* h, const double* v, const double* e) { cp_in(XC(h).ugv, v); cp_in(XC(h).uge, e); }
+void otetc_get_state(void* h, double* v, double* e) { cp_out(XC(h).ugv, v); cp_out(XC(h).uge, e); }
+void otetc_get_res(void* h, double* v, double* e) { cp_out(XC(h).resv, v); cp_out(XC(h).rese, e); }
+void otetc_set_res(void* h, const double* v, const double* e) { cp_in(XC(h).resv, v); cp_in(XC(h).rese, e); }
+// vprcn, eprcn (npnt/nseg x 5), tau (ntet x 25), vpc (npnt x 25), epc (nseg x 25), tstep (ntet)
+void otetc_get_prcn(void* h, double* vp, double* ep, double* tau, double* vpc, double* epc, double* tstep) {
+    cp_out(XC(h).vprcn, vp); cp_out(XC(h).eprcn, ep); cp_out(XC(h).tau, tau);
+    cp_out(XC(h).vpc, vpc); cp_out(XC(h).epc, epc); cp_out(XC(h).tstep, tstep);
+}
+void otetc_set_tau(void* h, const double* tau) { cp_in(XC(h).tau, tau); }
+void otetc_rsdl(void* h, int stage) { XC(h).rsdl(stage); }
+int otetc_setup_preconditioner(void* h) { return XC(h).setup_preconditioner(); }
+void otetc_minvrt(void* h) { XC(h).minvrt(); }
+void otetc_update(void* h) { XC(h).update(); }
+double otetc_maxres(void* h, double* per_var) { return XC(h).maxres(per_var); }
+void otetc_mg_restrict(void* h) { XC(h).mg_restrict(); }
+void otetc_mg_prolongate(void* h) { XC(h).mg_prolongate(); }
+int otetc_log2p(void* h) { return XC(h).log2p; }
+void otetc_element_rsdl(void* h, int tind, int stage, const double* uh, double* lre, double* lim) {
+    const int T = XC(h).b().tm;
+    double u[5][10], a[5][10], b[5][10];
+    for (int n = 0; n < 5; ++n)
+        for (int i = 0; i < T; ++i) u[n][i] = uh[n * T + i];
+    XC(h).element_rsdl(tind, stage, u, a, b);
+    for (int n = 0; n < 5; ++n)
+        for (int i = 0; i < T; ++i) { lre[n * T + i] = a[n][i]; lim[n * T + i] = b[n][i]; }
+}
+void otetc_face_rsdl(void* h, int f, double* lf) {
+    double l[5][6];
+    XC(h).face_rsdl(f, l);
+    const int TF = XC(h).b().tm2d;
+    for (int n = 0; n < 5; ++n)
+        for (int i = 0; i < TF; ++i) lf[n * TF + i] = l[n][i];
+}
+// Tau and Pinv of one element state (test support)
+void otetc_tau_timestep(void* h, const double* pvu, double hmin, double hmax, double* pinv, double* tau, double* ts) {
+    XC(h).preconditioner_tau_timestep(pvu, hmin, hmax, pinv, tau, *ts);
+}
+void otetc_switch_variables(void* h, const double* pvu, double* a) { XC(h).switch_variables(pvu, a); }
 
 }  // extern "C"

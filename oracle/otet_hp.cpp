@@ -1,4 +1,4 @@
-// Oracle (test infrastructure only): tet_hp / tet_hp_ins restatement.  See otet_hp.h.
+// Oracle (test infrastructure only): the generic tet_hp part and tet_hp_ins.  See otet_hp.h.
 #include "otet_hp.h"
 
 #include <algorithm>
@@ -14,8 +14,9 @@ const int kVS[6][2] = {{2, 3}, {3, 1}, {2, 1}, {1, 0}, {2, 0}, {3, 0}};  // tet_
 const int kVF[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
 }  // namespace
 
-TetHpIns::TetHpIns(const TetMesh& mesh, const TetParams& p) : m(mesh), prm(p) {
-    if (prm.log2pmax < 0 || prm.log2pmax > 1) throw std::runtime_error("oracle TetHpIns: log2p must be 0 or 1");
+template <int NV_>
+TetHp<NV_>::TetHp(const TetMesh& mesh, const TetParams& p) : m(mesh), prm(p) {
+    if (prm.log2pmax < 0 || prm.log2pmax > 1) throw std::runtime_error("oracle TetHp: log2p must be 0 or 1");
     for (int l = 0; l <= prm.log2pmax; ++l) bases.emplace_back(1 << l);
     log2p = prm.log2pmax;
     em0 = bases[log2p].em;
@@ -27,7 +28,6 @@ TetHpIns::TetHpIns(const TetMesh& mesh, const TetParams& p) : m(mesh), prm(p) {
     vprcn.assign(nv, 0.0);
     eprcn.assign(m.nseg * NV, 0.0);
     iprcn.assign(m.ntet * NV, 0.0);
-    tau.assign(m.ntet * NV, 0.0);
     vol.assign(m.ntet, 0.0);
     // tet(tind).vol: |triple product| (tet_mesh/setup.cpp:58-90, tet_prims.cpp:180-191)
     for (int t = 0; t < m.ntet; ++t) {
@@ -37,7 +37,7 @@ TetHpIns::TetHpIns(const TetMesh& mesh, const TetParams& p) : m(mesh), prm(p) {
         for (int n = 0; n < 3; ++n) { d1[n] = P[3][n] - P[2][n]; d2[n] = P[1][n] - P[2][n]; d3[n] = P[0][n] - P[2][n]; }
         const double c[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
         const double v = c[0] * d3[0] + c[1] * d3[1] + c[2] * d3[2];
-        if (!(v > 0.0)) throw std::runtime_error("oracle TetHpIns: tet not positively oriented");
+        if (!(v > 0.0)) throw std::runtime_error("oracle TetHp: tet not positively oriented");
         vol[t] = std::fabs(v);
     }
     vess.assign(m.npnt, 0);
@@ -49,8 +49,11 @@ TetHpIns::TetHpIns(const TetMesh& mesh, const TetParams& p) : m(mesh), prm(p) {
         }
 }
 
+TetHpIns::TetHpIns(const TetMesh& mesh, const TetParams& p) : TetHp<4>(mesh, p) { tau.assign(m.ntet * NV, 0.0); }
+
 // tet_hp/gtol.cpp:17-85 (vertices, edges with sgn^m)
-void TetHpIns::ugtouht(int t, double uht[NV][10]) const {
+template <int NV_>
+void TetHp<NV_>::ugtouht(int t, double uht[NV][10]) const {
     const TetBasis& B = b();
     for (int i = 0; i < 4; ++i)
         for (int n = 0; n < NV; ++n) uht[n][i] = ugv[m.tet_pnt[4 * t + i] * NV + n];
@@ -220,7 +223,8 @@ void TetHpIns::face_rsdl(int f, double lf[NV][6]) const {
 }
 
 // tet_hp/update.cpp:9-131
-void TetHpIns::rsdl(int stage) {
+template <int NV_>
+void TetHp<NV_>::rsdl(int stage) {
     const TetBasis& B = b();
     const double omb = 1.0 - (stage < prm.nstage ? prm.beta[stage] : 1.0);
     for (int i = 0; i < m.npnt * NV; ++i) { resv[i] = 0.0; res_rv[i] *= omb; }
@@ -424,8 +428,10 @@ void TetHpIns::update() {
 }
 
 // tet_hp/allocate.cpp:530-559
-double TetHpIns::maxres(double* per_var) const {
-    double mx[NV] = {0, 0, 0, 0};
+template <int NV_>
+double TetHp<NV_>::maxres(double* per_var) const {
+    double mx[NV];
+    for (int n = 0; n < NV; ++n) mx[n] = 0.0;
     for (int i = 0; i < m.npnt; ++i)
         for (int n = 0; n < NV; ++n) mx[n] = std::max(mx[n], std::fabs(resv[i * NV + n]));
     double e = 0.0;
@@ -434,10 +440,11 @@ double TetHpIns::maxres(double* per_var) const {
 }
 
 // tet_hp/mg_restrict.cpp:4-25 (p-levels)
-void TetHpIns::mg_restrict() {
+template <int NV_>
+void TetHp<NV_>::mg_restrict() {
     isfrst = true;
     coarse_flag = true;
-    if (log2p == 0) throw std::runtime_error("oracle TetHpIns: no coarser p-level");
+    if (log2p == 0) throw std::runtime_error("oracle TetHp: no coarser p-level");
     --log2p;
     res0v = resv;
     const TetBasis& B = b();
@@ -447,9 +454,13 @@ void TetHpIns::mg_restrict() {
 }
 
 // tet_hp/mg_prolongate.cpp:7-14
-void TetHpIns::mg_prolongate() {
+template <int NV_>
+void TetHp<NV_>::mg_prolongate() {
     ++log2p;
     if (log2p == prm.log2pmax) coarse_flag = false;
 }
+
+template class TetHp<4>;
+template class TetHp<5>;
 
 }  // namespace oracle

@@ -31,6 +31,6 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     # hpg_tri_mesh: 3 ints, 5 pointers, 1 int, 3 pointers (natural alignment)
     assert C.sizeof(_lib.TriMeshC) == 8 + 8 + 5 * 8 + 8 + 3 * 8
-    # hpg_params: physics, log2p, 5 doubles, nstage, alpha[8], beta[8], cfl[4], gamma, prandtl, Rgas, bc[4]
-    assert C.sizeof(_lib.ParamsC) == 8 + 5 * 8 + 8 + 20 * 8 + 3 * 8 + 4 * 8
-    assert _lib.ParamsC.bc.offset == C.sizeof(_lib.ParamsC) - 32
+    # hpg_params: physics, log2p, 5 doubles, nstage, alpha[8], beta[8], cfl[4], gamma, prandtl, Rgas, bc[5]
+    assert C.sizeof(_lib.ParamsC) == 8 + 5 * 8 + 8 + 20 * 8 + 3 * 8 + 5 * 8
+    assert _lib.ParamsC.bc.offset == C.sizeof(_lib.ParamsC) - 40
