@@ -76,6 +76,8 @@ def lib():
         L.otetc_face_rsdl.argtypes = [C.c_void_p, C.c_int, _dp]
         L.otetc_tau_timestep.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
         L.otetc_switch_variables.argtypes = [C.c_void_p, _dp, _dp]
+        L.otetc_abs_jacobians.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp, _dp]
+        L.otetc_flux_jacobians.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         _done = True
     return L
 
@@ -350,3 +352,13 @@ class TetHpCns:
         x = np.array(a, dtype=np.float64)
         lib().otetc_switch_variables(self.h, np.ascontiguousarray(pvu, dtype=np.float64), x)
         return x
+
+    def abs_jacobians(self, pvu, b2):
+        A, B, Cm = np.zeros(25), np.zeros(25), np.zeros(25)
+        lib().otetc_abs_jacobians(self.h, np.ascontiguousarray(pvu, dtype=np.float64), float(b2), A, B, Cm)
+        return A.reshape(5, 5), B.reshape(5, 5), Cm.reshape(5, 5)
+
+    def flux_jacobians(self, w):
+        A, B, Cm = np.zeros(25), np.zeros(25), np.zeros(25)
+        lib().otetc_flux_jacobians(self.h, np.ascontiguousarray(w, dtype=np.float64), A, B, Cm)
+        return A.reshape(5, 5), B.reshape(5, 5), Cm.reshape(5, 5)

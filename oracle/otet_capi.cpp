@@ -183,5 +183,11 @@ void otetc_tau_timestep(void* h, const double* pvu, double hmin, double hmax, do
     XC(h).preconditioner_tau_timestep(pvu, hmin, hmax, pinv, tau, *ts);
 }
 void otetc_switch_variables(void* h, const double* pvu, double* a) { XC(h).switch_variables(pvu, a); }
+void otetc_flux_jacobians(void* h, const double* w, double* A, double* B, double* C) {
+    XC(h).flux_jacobians(w, 1, A, B, C);
+}
+void otetc_abs_jacobians(void* h, const double* pvu, double b2, double* A, double* B, double* C) {
+    XC(h).abs_flux_jacobians(pvu, b2, A, B, C);
+}
 
 }  // extern "C"

@@ -52,54 +52,17 @@ void TetHpCns::switch_variables(const double pvu[5], double a[5]) const {
     for (int i = 0; i < 5; ++i) a[i] = t[i];
 }
 
-// tet_hp/cns/setup_preconditioner.cpp:206-489 (steady: bd(0) = 0)
-void TetHpCns::preconditioner_tau_timestep(const double pvu[5], double h, double hmax, double Pinv[25],
-                                           double Tau[25], double& timestep) const {
-    const double gam = prm.gamma, gm1 = gam - 1.0, gogm1 = gam / gm1;
+// |P dpdc A|, |P dpdc B|, |P dpdc C| of setup_preconditioner.cpp:289-400 from their
+// eigen-decompositions V diag(|eig|) VINV
+void TetHpCns::abs_flux_jacobians(const double pvu[5], double b2, double A[25], double Bm[25], double C[25]) const {
+    const double gam = prm.gamma, gm1 = gam - 1.0;
     const double pr = pvu[0], u = pvu[1], v = pvu[2], w = pvu[3], rt = pvu[4];
     const double rho = pr / rt;
-    const double ke = 0.5 * (u * u + v * v + w * w);
-    const double c2 = gam * rt, c = std::sqrt(c2);
-    const double nu = prm.mu / rho;
-    const double kcond = prm.R * prm.mu / prm.prandtl * prm.gamma / (prm.gamma - 1.0);
-    const double cp = gogm1 * prm.R;
-    const double alpha = kcond / (rho * cp);
-    const double bd0 = 0.0;
-    const double hdt = 0.5 * hmax * bd0 / c;
-    const double umag = std::sqrt(u * u + v * v + w * w);
-    const double M = pmax(umag / c, 1.0e-5);
-    const double nuh = 3.0 * pmax(4.0 * nu / (3.0 * hmax * c), alpha / (hmax * c));
-    double b2 = 1.0;
-    if (M < 0.8 && prm.preconditioner > 0) b2 = pmin(3.0 * M * M + (hdt + nuh) * (hdt + nuh), 1.0);
-
-    double P[25] = {b2, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, (b2 - 1.0) / (gogm1 * rho), 0, 0, 0, 1};
-    const double pinv[25] = {1.0 / b2, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0,
-                             -(b2 - 1.0) / (gogm1 * rho * b2), 0, 0, 0, 1};
-    for (int k = 0; k < 25; ++k) Pinv[k] = pinv[k];
-    const double dpdc[25] = {ke * gm1,          -u * gm1,      -v * gm1,      -w * gm1,      gm1,
-                             -u / rho,          1.0 / rho,     0.0,           0.0,           0.0,
-                             -v / rho,          0.0,           1.0 / rho,     0.0,           0.0,
-                             -w / rho,          0.0,           0.0,           1.0 / rho,     0.0,
-                             (gm1 * ke - rt) / rho, -u * gm1 / rho, -v * gm1 / rho, -w * gm1 / rho, gm1 / rho};
-    const double dcdp[25] = {1.0 / rt, 0, 0, 0, -rho / rt,
-                             u / rt, rho, 0, 0, -rho * u / rt,
-                             v / rt, 0, rho, 0, -rho * v / rt,
-                             w / rt, 0, 0, rho, -rho * w / rt,
-                             (rt + gm1 * ke) / (gm1 * rt), rho * u, rho * v, rho * w, -rho * ke / rt};
-    double tmp[25];
-    matmul5(P, dpdc, tmp);   // P <- P dpdc
-    for (int k = 0; k < 25; ++k) P[k] = tmp[k];
-    if (prm.preconditioner == 2) {
-        matmul5(dcdp, pinv, tmp);
-        for (int k = 0; k < 25; ++k) Pinv[k] = tmp[k];
-    }
-
-    // |P dpdc A|, |P dpdc B|, |P dpdc C| from their eigen-decompositions
+    const double c2 = gam * rt;
     const double bb = 1.0 - 2.0 * b2 + b2 * b2;
     const double t1 = std::sqrt(u * u * bb + 4.0 * b2 * c2);
     const double t2 = std::sqrt(v * v * bb + 4.0 * b2 * c2);
     const double t3 = std::sqrt(w * w * bb + 4.0 * b2 * c2);
-    double A[25], Bm[25], C[25];
     {
         const double V[25] = {0.5 * (u * (b2 - 1.0) + t1) * rho, 0.5 * (u * (b2 - 1.0) - t1) * rho, 0, 0, 0,
                               1.0, 1.0, 0, 0, 0,
@@ -152,6 +115,53 @@ void TetHpCns::preconditioner_tau_timestep(const double pvu[5], double h, double
             for (int j = 0; j < 5; ++j) VI[i * 5 + j] *= eig[i];
         matmul5(V, VI, C);
     }
+}
+
+// tet_hp/cns/setup_preconditioner.cpp:206-489 (steady: bd(0) = 0)
+void TetHpCns::preconditioner_tau_timestep(const double pvu[5], double h, double hmax, double Pinv[25],
+                                           double Tau[25], double& timestep) const {
+    const double gam = prm.gamma, gm1 = gam - 1.0, gogm1 = gam / gm1;
+    const double pr = pvu[0], u = pvu[1], v = pvu[2], w = pvu[3], rt = pvu[4];
+    const double rho = pr / rt;
+    const double ke = 0.5 * (u * u + v * v + w * w);
+    const double c2 = gam * rt, c = std::sqrt(c2);
+    const double nu = prm.mu / rho;
+    const double kcond = prm.R * prm.mu / prm.prandtl * prm.gamma / (prm.gamma - 1.0);
+    const double cp = gogm1 * prm.R;
+    const double alpha = kcond / (rho * cp);
+    const double bd0 = 0.0;
+    const double hdt = 0.5 * hmax * bd0 / c;
+    const double umag = std::sqrt(u * u + v * v + w * w);
+    const double M = pmax(umag / c, 1.0e-5);
+    const double nuh = 3.0 * pmax(4.0 * nu / (3.0 * hmax * c), alpha / (hmax * c));
+    double b2 = 1.0;
+    if (M < 0.8 && prm.preconditioner > 0) b2 = pmin(3.0 * M * M + (hdt + nuh) * (hdt + nuh), 1.0);
+
+    double P[25] = {b2, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, (b2 - 1.0) / (gogm1 * rho), 0, 0, 0, 1};
+    const double pinv[25] = {1.0 / b2, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0,
+                             -(b2 - 1.0) / (gogm1 * rho * b2), 0, 0, 0, 1};
+    for (int k = 0; k < 25; ++k) Pinv[k] = pinv[k];
+    const double dpdc[25] = {ke * gm1,          -u * gm1,      -v * gm1,      -w * gm1,      gm1,
+                             -u / rho,          1.0 / rho,     0.0,           0.0,           0.0,
+                             -v / rho,          0.0,           1.0 / rho,     0.0,           0.0,
+                             -w / rho,          0.0,           0.0,           1.0 / rho,     0.0,
+                             (gm1 * ke - rt) / rho, -u * gm1 / rho, -v * gm1 / rho, -w * gm1 / rho, gm1 / rho};
+    const double dcdp[25] = {1.0 / rt, 0, 0, 0, -rho / rt,
+                             u / rt, rho, 0, 0, -rho * u / rt,
+                             v / rt, 0, rho, 0, -rho * v / rt,
+                             w / rt, 0, 0, rho, -rho * w / rt,
+                             (rt + gm1 * ke) / (gm1 * rt), rho * u, rho * v, rho * w, -rho * ke / rt};
+    double tmp[25];
+    matmul5(P, dpdc, tmp);   // P <- P dpdc
+    for (int k = 0; k < 25; ++k) P[k] = tmp[k];
+    if (prm.preconditioner == 2) {
+        matmul5(dcdp, pinv, tmp);
+        for (int k = 0; k < 25; ++k) Pinv[k] = tmp[k];
+    }
+
+    const double bb = 1.0 - 2.0 * b2 + b2 * b2;
+    double A[25], Bm[25], C[25];
+    abs_flux_jacobians(pvu, b2, A, Bm, C);
     double S[5] = {0.0, 4.0 * nu / (3.0 * h * h), 4.0 * nu / (3.0 * h * h), 4.0 * nu / (3.0 * h * h), alpha / (h * h)};
     for (int i = 0; i < 5; ++i) S[i] = 3.0 * S[i] + 0.5 * bd0;
     const double maxeig = 0.5 * (umag + umag * b2 + std::sqrt(umag * umag * bb + 4.0 * b2 * c2));
@@ -168,6 +178,31 @@ void TetHpCns::preconditioner_tau_timestep(const double pvu[5], double h, double
         lu_solve_inplace(Tinv, 5, 5, x);
         for (int i = 0; i < 5; ++i) Tau[i * 5 + col] = x[i];
     }
+}
+
+// flux Jacobians dF/dw, dG/dw, dH/dw with respect to the primitive unknowns
+// (p, u, v, w, RT) at one point (tet_hp/cns/rsdl.cpp:799-826); u[n * stride]
+void TetHpCns::flux_jacobians(const double* u, int stride, double A[25], double Bq[25], double Cq[25]) const {
+    const double gogm1 = prm.gamma / (prm.gamma - 1.0);
+    const double pr = u[0], uv = u[stride], vv = u[2 * stride], wv = u[3 * stride], rt = u[4 * stride];
+    const double ke = 0.5 * (uv * uv + vv * vv + wv * wv);
+    const double rho = (pr + prm.atm_pressure) / rt;
+    const double a[25] = {uv / rt, rho, 0.0, 0.0, -rho * uv / rt,
+                          uv * uv / rt + 1.0, 2.0 * rho * uv, 0.0, 0.0, -rho * uv * uv / rt,
+                          uv * vv / rt, rho * vv, rho * uv, 0.0, -rho * uv * vv / rt,
+                          uv * wv / rt, rho * wv, 0.0, rho * uv, -rho * uv * wv / rt,
+                          uv * (gogm1 * rt + ke) / rt, rho * (gogm1 * rt + ke) + rho * uv * uv, rho * uv * vv, rho * uv * wv, -rho * uv * ke / rt};
+    const double b[25] = {vv / rt, 0.0, rho, 0.0, -rho * vv / rt,
+                          uv * vv / rt, rho * vv, rho * uv, 0.0, -rho * uv * vv / rt,
+                           vv * vv / rt + 1.0, 0.0, 2.0 * rho * vv, 0.0, -rho * vv * vv / rt,
+                           vv * wv / rt, 0.0, rho * wv, rho * vv, -rho * vv * wv / rt,
+                           vv * (gogm1 * rt + ke) / rt, rho * uv * vv, rho * (gogm1 * rt + ke) + rho * vv * vv, rho * vv * wv, -rho * vv * ke / rt};
+    const double c[25] = {wv / rt, 0.0, 0.0, rho, -rho * wv / rt,
+                           uv * wv / rt, rho * wv, 0.0, rho * uv, -rho * uv * wv / rt,
+                           vv * wv / rt, 0.0, rho * wv, rho * vv, -rho * vv * wv / rt,
+                           wv * wv / rt + 1.0, 0.0, 0.0, 2.0 * rho * wv, -rho * wv * wv / rt,
+                           wv * (gogm1 * rt + ke) / rt, rho * uv * wv, rho * vv * wv, rho * (gogm1 * rt + ke) + rho * wv * wv, -rho * wv * ke / rt};
+    for (int k = 0; k < 25; ++k) { A[k] = a[k]; Bq[k] = b[k]; Cq[k] = c[k]; }
 }
 
 // tet_hp/cns/rsdl.cpp:16-87 and the straight branch :472-843; steady (mvel = 0,
@@ -277,24 +312,8 @@ void TetHpCns::element_rsdl(int t, int stage, const double uht[NV][10], double l
         double tres[5] = {0, 0, 0, 0, 0};
         for (int mm = 0; mm < 5; ++mm)
             for (int n = 0; n < 5; ++n) tres[mm] += tt[mm * 5 + n] * res[n * G + g];
-        const double pr = U(0, g), uv = U(1, g), vv = U(2, g), wv = U(3, g), rt = U(4, g);
-        const double ke = 0.5 * (uv * uv + vv * vv + wv * wv);
-        const double rho = (pr + prm.atm_pressure) / rt;
-        const double A[25] = {uv / rt, rho, 0.0, 0.0, -rho * uv / rt,
-                              uv * uv / rt + 1.0, 2.0 * rho * uv, 0.0, 0.0, -rho * uv * uv / rt,
-                              uv * vv / rt, rho * vv, rho * uv, 0.0, -rho * uv * vv / rt,
-                              uv * wv / rt, rho * wv, 0.0, rho * uv, -rho * uv * wv / rt,
-                              uv * (gogm1 * rt + ke) / rt, rho * (gogm1 * rt + ke) + rho * uv * uv, rho * uv * vv, rho * uv * wv, -rho * uv * ke / rt};
-        const double Bq[25] = {vv / rt, 0.0, rho, 0.0, -rho * vv / rt,
-                               uv * vv / rt, rho * vv, rho * uv, 0.0, -rho * uv * vv / rt,
-                               vv * vv / rt + 1.0, 0.0, 2.0 * rho * vv, 0.0, -rho * vv * vv / rt,
-                               vv * wv / rt, 0.0, rho * wv, rho * vv, -rho * vv * wv / rt,
-                               vv * (gogm1 * rt + ke) / rt, rho * uv * vv, rho * (gogm1 * rt + ke) + rho * vv * vv, rho * vv * wv, -rho * vv * ke / rt};
-        const double Cq[25] = {wv / rt, 0.0, 0.0, rho, -rho * wv / rt,
-                               uv * wv / rt, rho * wv, 0.0, rho * uv, -rho * uv * wv / rt,
-                               vv * wv / rt, 0.0, rho * wv, rho * vv, -rho * vv * wv / rt,
-                               wv * wv / rt + 1.0, 0.0, 0.0, 2.0 * rho * wv, -rho * wv * wv / rt,
-                               wv * (gogm1 * rt + ke) / rt, rho * uv * wv, rho * vv * wv, rho * (gogm1 * rt + ke) + rho * wv * wv, -rho * wv * ke / rt};
+        double A[25], Bq[25], Cq[25];
+        flux_jacobians(&U(0, g), G, A, Bq, Cq);
         for (int mm = 0; mm < 5; ++mm)
             for (int n = 0; n < 5; ++n) {
                 const int k = mm * 5 + n;

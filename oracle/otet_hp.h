@@ -113,6 +113,10 @@ class TetHpCns : public TetHp<5> {
     // tet_hp/cns/setup_preconditioner.cpp:206-489 for one element
     void preconditioner_tau_timestep(const double pvu[5], double h, double hmax, double Pinv[25], double Tau[25],
                                      double& timestep) const;
+    // dF/dw, dG/dw, dH/dw at one point, u[n * stride] (tet_hp/cns/rsdl.cpp:799-826)
+    void flux_jacobians(const double* u, int stride, double A[25], double B[25], double C[25]) const;
+    // |P dpdc A|, |P dpdc B|, |P dpdc C| (setup_preconditioner.cpp:289-400)
+    void abs_flux_jacobians(const double pvu[5], double b2, double A[25], double B[25], double C[25]) const;
     // tet_hp/cns/update.cpp:349-375: a <- dpdc(pvu) a
     void switch_variables(const double pvu[5], double a[5]) const;
 };
