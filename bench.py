@@ -15,6 +15,8 @@ Also reported (same JSON line):
   throughput and the element kernel's rate);
 * ``tet``: the tetrahedral ins path on 64^3 cubes -> 1 572 864 tetrahedra, p = 2
   (step throughput and the element kernel's rate);
+* ``tet_cns``: the tetrahedral cns path on the same mesh, p = 2 (step throughput and
+  the element kernel's rate);
 * ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual with the
   h+p multigrid cycle on the same 512 x 512 p = 4 mesh family at mu = 1e-2, for FP64
   plain, mixed, mixed + learned preconditioner and FP64 + learned (training time
@@ -107,6 +109,10 @@ def cns_throughput(args) -> dict:
 # (4 fields x proj with r, s, t derivatives, intgrtrst x 8, intgrt x 4, deriv r/s/t),
 # plus about 150 pointwise flop at each of the 27 Gauss points
 FLOP_TET_FULL_P2 = 30000.0
+# k_elem_tetc<2, full>: transforms 23 625 multiply-adds (values 5 x 27 x 10, derivatives
+# 4 x 3 x 27 x 10, two intgrtrst 5 x 3 x 27 x 10, strong residual 5 x 3 x 27 x 27) = 47 250
+# flop, plus about 770 flop of fluxes, stress, tau and stabilisation per Gauss point
+FLOP_TETC_FULL_P2 = 68000.0
 
 
 def tet_throughput(args) -> dict:
@@ -148,6 +154,53 @@ def tet_throughput(args) -> dict:
                         "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
                         "avg_launch_ms": tot.value / max(1, nl.value),
                         "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_TET_FULL_P2}}
+    g.close()
+    return out
+
+
+def tetc_throughput(args) -> dict:
+    """tet_hp cns on the 64^3 p = 2 mesh: p = RT = 1, solenoidal Fourier velocity scaled to
+    Mach 0.25, inflow on every face; FP64 step throughput and the element kernel's rate."""
+    from hpmp import tetmesh as TM, workloads as W
+    from hpmp.tet_hp import TetHpCns
+
+    n = args.tet_n
+    mesh = TM.unit_cube(n)
+    v4, e4 = W.tet_solenoidal_state(mesh, 2, seed=4)
+    sc = 0.25 * np.sqrt(1.4) / np.abs(v4[:, :3]).max()
+    v = np.zeros((mesh.npnt, 5))
+    v[:, 0], v[:, 4] = 1.0, 1.0
+    v[:, 1:4] = sc * v4[:, :3]
+    e = np.zeros((mesh.nseg, 1, 5))
+    e[:, :, 1:4] = sc * e4[:, :, :3]
+    g = TetHpCns(mesh, log2p=1, mu=1e-2, gamma=1.4, prandtl=0.72, R=1.0, bc=(0.0, 0.0, 0.0, 1.0), cfl=(0.0, 0.0))
+    g.set_state(v, e)
+    unknowns = 5 * (mesh.npnt + mesh.nseg)
+    g.setup_preconditioner()
+    g.update()
+    g.synchronize()
+    g.L.hpg_tet_kernel_timing(g.h, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.tet_steps):
+        if g.setup_preconditioner():
+            raise SystemExit("tet cns setup_preconditioner error")
+        g.update()
+    g.synchronize()
+    t1 = time.perf_counter()
+    tot = C.c_double(); nl = C.c_longlong(); ne = C.c_longlong()
+    g.L.hpg_tet_kernel_timing_read(g.h, 1, C.byref(tot), C.byref(nl), C.byref(ne))
+    g.L.hpg_tet_kernel_timing(g.h, 0)
+    ach = FLOP_TETC_FULL_P2 * ne.value / (tot.value * 1e-3) / 1e12 if tot.value > 0 else 0.0
+    out = {"workload": f"tet_hp cns {n}^3 cubes -> {mesh.ntet} tetrahedra, p=2, mu=1e-2, gamma 1.4, Pr 0.72, "
+                       "p = RT = 1, u = curl psi scaled to Mach 0.25, inflow on every face; step = "
+                       "setup_preconditioner + 5-stage update at CFL 0 (full work, state held)",
+           "unknowns": unknowns, "steps": args.tet_steps,
+           "ms_per_step": 1e3 * (t1 - t0) / args.tet_steps,
+           "dof_evals_per_s": unknowns * 5 * args.tet_steps / (t1 - t0),
+           "roofline": {"bound": "fp64 valu", "kernel": "k_elem_tetc<2,full>", "achieved": ach,
+                        "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
+                        "avg_launch_ms": tot.value / max(1, nl.value),
+                        "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_TETC_FULL_P2}}
     g.close()
     return out
 
@@ -360,6 +413,7 @@ def main() -> int:
         out["cns"] = cns_throughput(args)
     if not args.no_tet:
         out["tet"] = tet_throughput(args)
+        out["tet_cns"] = tetc_throughput(args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
