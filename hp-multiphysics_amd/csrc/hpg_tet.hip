@@ -27,9 +27,10 @@ std::vector<double> build_tet_pack(const TetOps& o) {
         for (int m = 0; m < D::TM; ++m)
             for (int g = 0; g < D::G; ++g) pk[op * D::TM * 32 + m * 32 + g] = (*proj[op])[g * D::TM + m];
     const std::vector<double>* integ[3] = {&o.Ir, &o.Is, &o.It};
+    const int ioff[3] = {D::OP_IR, D::OP_IS, D::OP_IT};
     for (int op = 0; op < 3; ++op)
         for (int g = 0; g < D::G; ++g)
-            for (int m = 0; m < D::TM; ++m) pk[D::OP_IR + op * D::G * 16 + g * 16 + m] = (*integ[op])[m * D::G + g];
+            for (int m = 0; m < D::TM; ++m) pk[ioff[op] + g * 16 + m] = (*integ[op])[m * D::G + g];
     const std::vector<double>* der[3] = {&o.Dr, &o.Ds, &o.Dt};
     for (int op = 0; op < 3; ++op)
         for (int q = 0; q < D::G; ++q)

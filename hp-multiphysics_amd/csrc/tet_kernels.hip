@@ -4,7 +4,7 @@
 // Element residual: one element per half-wave, one Gauss point per lane (27 points
 // at p = 2, 8 at p = 1).  The transform operators sit in LDS, lane-contiguous; the
 // point fields a reduction needs are exchanged through a half-wave's private LDS
-// rows.  Element residuals go to an element-major buffer lf[t][re/im][n][m] and the
+// rows.  Element residuals go to an element-major buffer lf[t][m][im/re][n] and the
 // gather kernels sum them per vertex / edge over the incident elements in
 // ascending element order, the order of the reference's element loop, so the
 // assembly is deterministic without colouring.
@@ -23,9 +23,12 @@ struct TDims {
     static constexpr int TM2 = 3 + 3 * EM;
     static constexpr int GF = GP * GP;
     // operator pack (doubles): P, Pr, Ps, Pt as [m][32] (lane = Gauss point);
-    // Ir, Is, It as [g][16] (lane = mode); Dr, Ds, Dt as [g'][32] (lane g reads D[g][g'])
+    // Ir, Is, It as [g][16] (lane 3m + d reads direction d of mode m), each block padded
+    // so that the three directions start 20 LDS banks apart; Dr, Ds, Dt as [g'][32]
+    // (lane g reads D[g][g'])
+    static constexpr int IPAD = ((20 - (2 * G * 16) % 64) % 64 + 64) % 64 / 2;
     static constexpr int OP_P = 0, OP_PR = TM * 32, OP_PS = 2 * TM * 32, OP_PT = 3 * TM * 32;
-    static constexpr int OP_IR = 4 * TM * 32, OP_IS = OP_IR + G * 16, OP_IT = OP_IS + G * 16;
+    static constexpr int OP_IR = 4 * TM * 32, OP_IS = OP_IR + G * 16 + IPAD, OP_IT = OP_IS + G * 16 + IPAD;
     static constexpr int OP_DR = OP_IT + G * 16, OP_DS = OP_DR + G * 32, OP_DT = OP_DS + G * 32;
     static constexpr int SIZE = OP_DT + G * 32;
 };
@@ -39,10 +42,39 @@ struct TetElemArgs {
     const double* ugv;
     const double* uge;
     const double* tau;  // ntet x 4 (0: momentum, 3: continuity)
-    double* lf;         // ntet x 2 x 4 x TM: [t][0 = imaginary, 1 = real][n][m]
+    double* lf;         // ntet x TM x 2 x 4: [t][m][0 = imaginary, 1 = real][n]
     const double* ops;
     double rho, mu, beta;
 };
+
+// intgrtrst of a half-wave's point fluxes sf[3n + d][q] (d = r, s, t directions): lane
+// 3m + d sums direction d of mode m over the Gauss points, and lane 3m adds the three
+// partial sums (two shuffles inside the half-wave); lanes from 3 TM on carry zeros.
+// Every lane of the half-wave takes part.  One operator read and NVn flux reads per
+// point and lane, a third of the LDS traffic of a lane-per-mode sum.
+template <class D, int NVn, int ROW>
+__device__ __forceinline__ void tet_intgrtrst(const double* __restrict__ sops, const double (*sf)[ROW], int g,
+                                              double out[NVn]) {
+    const int m = g / 3, d = g - 3 * (g / 3);
+    const bool act = g < 3 * D::TM;
+    const double* op = sops + (d == 0 ? D::OP_IR : d == 1 ? D::OP_IS : D::OP_IT) + (act ? m : 0);
+    double s[NVn];
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) s[n] = 0.0;
+    if (act) {
+#pragma unroll 3
+        for (int q = 0; q < D::G; ++q) {
+            const double iq = op[q * 16];
+#pragma unroll
+            for (int n = 0; n < NVn; ++n) s[n] += iq * sf[3 * n + d][q];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NVn; ++n) {
+        const double t1 = __shfl_down(s[n], 1, 64), t2 = __shfl_down(s[n], 2, 64);
+        out[n] = s[n] + t1 + t2;
+    }
+}
 
 // tet_hp_ins::element_rsdl, straight branch (tet_hp/ins/rsdl.cpp:15-87, 442-785),
 // steady, tau frozen by setup_preconditioner
@@ -52,7 +84,7 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
     constexpr int TM = D::TM, G = D::G;
     __shared__ double sops[D::SIZE];
     __shared__ double suh[8][TNV][16];
-    __shared__ double sf[8][12][32];
+    __shared__ double sf[8][12][33];   // rows padded: the three direction rows of a variable sit in different banks
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     __syncthreads();  // operators staged
@@ -154,18 +186,15 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) sf[hw][9 + c][g] = c3[c];
         wave_lds_sync();
-        // lf_im = intgrtrst of the fluxes (rsdl.cpp:509-511): lane = mode
+        // lf_im = intgrtrst of the fluxes (rsdl.cpp:509-511); mode g / 3 lands in lane g
         double* lfo = a.lf + (size_t)tt * (2 * TNV * TM);
-        if (valid && g < TM) {
+        const bool owner = valid && g < 3 * TM && g % 3 == 0;
+        {
+            double li[TNV];
+            tet_intgrtrst<D, TNV, 33>(sops, sf[hw], g, li);
+            if (owner)
 #pragma unroll
-            for (int n = 0; n < TNV; ++n) {
-                double s = 0.0;
-#pragma unroll 3
-                for (int q = 0; q < G; ++q)
-                    s += sops[D::OP_IR + q * 16 + g] * sf[hw][3 * n + 0][q] + sops[D::OP_IS + q * 16 + g] * sf[hw][3 * n + 1][q] +
-                         sops[D::OP_IT + q * 16 + g] * sf[hw][3 * n + 2][q];
-                lfo[n * TM + g] = s;
-            }
+                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + n] = li[n];
         }
         if constexpr (FULL) {
             // viscous fluxes (rsdl.cpp:518-674): visc(n,m)(a,b) = -mu/cjcb (delta_nm
@@ -225,22 +254,15 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
             for (int c = 0; c < 3; ++c) sf[hw][9 + c][g] = g < G ? c3[c] : 0.0;
             wave_lds_sync();
             // lf_re = beta * intgrtrst (rsdl.cpp:768-776)
-            if (valid && g < TM) {
+            double lr[TNV];
+            tet_intgrtrst<D, TNV, 33>(sops, sf[hw], g, lr);
+            if (owner)
 #pragma unroll
-                for (int n = 0; n < TNV; ++n) {
-                    double s = 0.0;
-#pragma unroll 3
-                for (int q = 0; q < G; ++q)
-                        s += sops[D::OP_IR + q * 16 + g] * sf[hw][3 * n + 0][q] +
-                             sops[D::OP_IS + q * 16 + g] * sf[hw][3 * n + 1][q] +
-                             sops[D::OP_IT + q * 16 + g] * sf[hw][3 * n + 2][q];
-                    lfo[(TNV + n) * TM + g] = s * a.beta;
-                }
-            }
+                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = lr[n] * a.beta;
         } else {
-            if (valid && g < TM)
+            if (owner)
 #pragma unroll
-                for (int n = 0; n < TNV; ++n) lfo[(TNV + n) * TM + g] = 0.0;
+                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = 0.0;
         }
     }
 }
@@ -334,11 +356,12 @@ __global__ __launch_bounds__(256) void k_gather_tet(TetGatherArgs a) {
     for (int n = 0; n < NV; ++n) sr[n] = a.omb * RR[n];
     for (int k = a.ptr[v]; k < a.ptr[v + 1]; ++k) {
         const int c = a.code[k], t = c >> 4, i = c & 15;
-        const double* L = a.lf + (size_t)t * (2 * NV * a.tm) + a.mbase + i;
+        // one contiguous record of 2 NV doubles per (element, mode)
+        const double* L = a.lf + (size_t)t * (2 * NV * a.tm) + (size_t)(a.mbase + i) * (2 * NV);
 #pragma unroll
         for (int n = 0; n < NV; ++n) {
-            si[n] += L[n * a.tm];
-            sr[n] += L[(NV + n) * a.tm];
+            si[n] += L[n];
+            sr[n] += L[NV + n];
         }
     }
 #pragma unroll

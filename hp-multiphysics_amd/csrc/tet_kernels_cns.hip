@@ -1,7 +1,7 @@
 // HIP kernels of the tet_hp_cns residual/relaxation path (gfx950), p in {1, 2},
 // NV = 5 (p, u, v, w, RT).  Same shapes as the tet_hp_ins kernels: one element per
 // half-wave with a lane per Gauss point and the transform operators in LDS; element
-// residuals go to the element-major buffer lf[t][im/re][n][m] and are gathered per
+// residuals go to the element-major buffer lf[t][m][im/re][n] and are gathered per
 // vertex / edge in ascending element order (k_gather_tet<5>).  The 5x5 algebra of the
 // Weiss-Smith preconditioner and the element tau runs one element per lane.
 #include <hip/hip_runtime.h>
@@ -361,7 +361,7 @@ struct TetcElemArgs {
     const double* ugv;
     const double* uge;
     const double* tau;  // ntet x 25
-    double* lf;         // ntet x 2 x 5 x TM: [t][0 = imaginary, 1 = real][n][m]
+    double* lf;         // ntet x TM x 2 x 5: [t][m][0 = imaginary, 1 = real][n]
     const double* ops;
     double mu, kcond, R, gamma, atm, beta;
 };
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
     constexpr int TM = D::TM, G = D::G;
     __shared__ double sops[D::SIZE];
     __shared__ double suh[8][CNV][16];
-    __shared__ double sf[8][15][32];
+    __shared__ double sf[8][15][33];   // rows padded as in k_elem_tet
     __shared__ double stau[8][25];
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
@@ -514,18 +514,15 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? cv[n][c] : 0.0;
         wave_lds_sync();
-        // lf_im = intgrtrst of the fluxes (rsdl.cpp:542-543): lane = mode
+        // lf_im = intgrtrst of the fluxes (rsdl.cpp:542-543); mode g / 3 lands in lane g
         double* lfo = a.lf + (size_t)tt * (2 * CNV * TM);
-        if (valid && g < TM) {
+        const bool owner = valid && g < 3 * TM && g % 3 == 0;
+        {
+            double li[CNV];
+            tet_intgrtrst<D, CNV, 33>(sops, sf[hw], g, li);
+            if (owner)
 #pragma unroll
-            for (int n = 0; n < CNV; ++n) {
-                double s = 0.0;
-#pragma unroll 3
-                for (int q = 0; q < G; ++q)
-                    s += sops[D::OP_IR + q * 16 + g] * sf[hw][3 * n + 0][q] + sops[D::OP_IS + q * 16 + g] * sf[hw][3 * n + 1][q] +
-                         sops[D::OP_IT + q * 16 + g] * sf[hw][3 * n + 2][q];
-                lfo[n * TM + g] = s;
-            }
+                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + n] = li[n];
         }
         if constexpr (FULL) {
             // viscous stress and heat flux (rsdl.cpp:584-755): visc(n,m)(a,b) applied to
@@ -599,22 +596,15 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
                 for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? df[n][c] : 0.0;
             wave_lds_sync();
             // lf_re = beta * intgrtrst (rsdl.cpp:829-835)
-            if (valid && g < TM) {
+            double lr[CNV];
+            tet_intgrtrst<D, CNV, 33>(sops, sf[hw], g, lr);
+            if (owner)
 #pragma unroll
-                for (int n = 0; n < CNV; ++n) {
-                    double s = 0.0;
-#pragma unroll 3
-                    for (int q = 0; q < G; ++q)
-                        s += sops[D::OP_IR + q * 16 + g] * sf[hw][3 * n + 0][q] +
-                             sops[D::OP_IS + q * 16 + g] * sf[hw][3 * n + 1][q] +
-                             sops[D::OP_IT + q * 16 + g] * sf[hw][3 * n + 2][q];
-                    lfo[(CNV + n) * TM + g] = s * a.beta;
-                }
-            }
+                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = lr[n] * a.beta;
         } else {
-            if (valid && g < TM)
+            if (owner)
 #pragma unroll
-                for (int n = 0; n < CNV; ++n) lfo[(CNV + n) * TM + g] = 0.0;
+                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = 0.0;
         }
     }
 }
