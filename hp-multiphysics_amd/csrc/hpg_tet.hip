@@ -34,7 +34,7 @@ std::vector<double> build_tet_pack(const TetOps& o) {
     const std::vector<double>* der[3] = {&o.Dr, &o.Ds, &o.Dt};
     for (int op = 0; op < 3; ++op)
         for (int q = 0; q < D::G; ++q)
-            for (int g = 0; g < D::G; ++g) pk[D::OP_DR + op * D::G * 32 + q * 32 + g] = (*der[op])[g * D::G + q];
+            for (int g = 0; g < D::G; ++g) pk[D::OP_DR + op * D::G * D::DQ + g * D::DQ + q] = (*der[op])[g * D::G + q];
     return pk;
 }
 

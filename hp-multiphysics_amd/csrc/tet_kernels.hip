@@ -24,13 +24,17 @@ struct TDims {
     static constexpr int GF = GP * GP;
     // operator pack (doubles): P, Pr, Ps, Pt as [m][32] (lane = Gauss point);
     // Ir, Is, It as [g][16] (lane 3m + d reads direction d of mode m), each block padded
-    // so that the three directions start 20 LDS banks apart; Dr, Ds, Dt as [g'][32]
-    // (lane g reads D[g][g'])
+    // so that the three directions start 20 LDS banks apart; Dr, Ds, Dt as [g][DQ]: lane
+    // g reads its own row D[g][g'] two points per 16-byte load, rows zero-padded to an
+    // even length DQ > G whose 2 DQ dwords stride the banks with period 16 lanes
     static constexpr int IPAD = ((20 - (2 * G * 16) % 64) % 64 + 64) % 64 / 2;
+    static constexpr int DQ = P == 2 ? 30 : 10;
+    static constexpr int FROW = 34;   // flux row length: even (16-byte pairs), 4 banks apart
     static constexpr int OP_P = 0, OP_PR = TM * 32, OP_PS = 2 * TM * 32, OP_PT = 3 * TM * 32;
     static constexpr int OP_IR = 4 * TM * 32, OP_IS = OP_IR + G * 16 + IPAD, OP_IT = OP_IS + G * 16 + IPAD;
-    static constexpr int OP_DR = OP_IT + G * 16, OP_DS = OP_DR + G * 32, OP_DT = OP_DS + G * 32;
-    static constexpr int SIZE = OP_DT + G * 32;
+    static constexpr int OP_DR = OP_IT + G * 16, OP_DS = OP_DR + G * DQ, OP_DT = OP_DS + G * DQ;
+    static constexpr int SIZE = OP_DT + G * DQ;
+    static_assert(OP_DR % 2 == 0 && DQ % 2 == 0 && DQ > G, "16-byte pairs of the derivative rows");
 };
 
 struct TetElemArgs {
@@ -76,15 +80,39 @@ __device__ __forceinline__ void tet_intgrtrst(const double* __restrict__ sops, c
     }
 }
 
+// strong residual res[n] += sum_q (Dr[g][q] F[3n][q] + Ds[g][q] F[3n+1][q] + Dt[g][q] F[3n+2][q])
+// of the Gauss point g, two points per 16-byte LDS load of the operator rows and of
+// the (broadcast) flux rows; the zero pads at q = G close the odd count
+template <class D, int NVn>
+__device__ __forceinline__ void tet_strong_residual(const double* __restrict__ sops, const double (*sf)[D::FROW], int g,
+                                                    double res[NVn]) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2* dr = reinterpret_cast<const d2*>(sops + D::OP_DR + g * D::DQ);
+    const d2* ds = reinterpret_cast<const d2*>(sops + D::OP_DS + g * D::DQ);
+    const d2* dt = reinterpret_cast<const d2*>(sops + D::OP_DT + g * D::DQ);
+#pragma unroll 2
+    for (int h = 0; h < (D::G + 1) / 2; ++h) {
+        const d2 a = dr[h], b = ds[h], c = dt[h];
+#pragma unroll
+        for (int n = 0; n < NVn; ++n) {
+            const d2 f0 = reinterpret_cast<const d2*>(sf[3 * n + 0])[h];
+            const d2 f1 = reinterpret_cast<const d2*>(sf[3 * n + 1])[h];
+            const d2 f2 = reinterpret_cast<const d2*>(sf[3 * n + 2])[h];
+            res[n] += a.x * f0.x + b.x * f1.x + c.x * f2.x;
+            res[n] += a.y * f0.y + b.y * f1.y + c.y * f2.y;
+        }
+    }
+}
+
 // tet_hp_ins::element_rsdl, straight branch (tet_hp/ins/rsdl.cpp:15-87, 442-785),
 // steady, tau frozen by setup_preconditioner
 template <int P, bool FULL>
 __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
     using D = TDims<P>;
     constexpr int TM = D::TM, G = D::G;
-    __shared__ double sops[D::SIZE];
+    __shared__ __attribute__((aligned(16))) double sops[D::SIZE];
     __shared__ double suh[8][TNV][16];
-    __shared__ double sf[8][12][33];   // rows padded: the three direction rows of a variable sit in different banks
+    __shared__ __attribute__((aligned(16))) double sf[8][12][D::FROW];  // point flux rows, padded
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     __syncthreads();  // operators staged
@@ -191,7 +219,7 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
         const bool owner = valid && g < 3 * TM && g % 3 == 0;
         {
             double li[TNV];
-            tet_intgrtrst<D, TNV, 33>(sops, sf[hw], g, li);
+            tet_intgrtrst<D, TNV, D::FROW>(sops, sf[hw], g, li);
             if (owner)
 #pragma unroll
                 for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + n] = li[n];
@@ -222,16 +250,7 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
             wave_lds_sync();
             // strong residual: derivr/derivs/derivt of the total fluxes (rsdl.cpp:683-692)
             double res[TNV] = {0.0, 0.0, 0.0, 0.0};
-            if (g < G) {
-#pragma unroll 3
-                for (int q = 0; q < G; ++q) {
-                    const double dr = sops[D::OP_DR + q * 32 + g], ds = sops[D::OP_DS + q * 32 + g],
-                                 dt = sops[D::OP_DT + q * 32 + g];
-#pragma unroll
-                    for (int n = 0; n < TNV; ++n)
-                        res[n] += dr * sf[hw][3 * n + 0][q] + ds * sf[hw][3 * n + 1][q] + dt * sf[hw][3 * n + 2][q];
-                }
-            }
+            if (g < G) tet_strong_residual<D, TNV>(sops, sf[hw], g, res);
             // stabilisation (rsdl.cpp:695-767)
             const double t0 = a.tau[4 * tt] * res[0], t1 = a.tau[4 * tt] * res[1], t2 = a.tau[4 * tt] * res[2],
                          t3 = a.tau[4 * tt + 3] * res[3];
@@ -255,7 +274,7 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
             wave_lds_sync();
             // lf_re = beta * intgrtrst (rsdl.cpp:768-776)
             double lr[TNV];
-            tet_intgrtrst<D, TNV, 33>(sops, sf[hw], g, lr);
+            tet_intgrtrst<D, TNV, D::FROW>(sops, sf[hw], g, lr);
             if (owner)
 #pragma unroll
                 for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = lr[n] * a.beta;

@@ -404,9 +404,9 @@ template <int P, bool FULL>
 __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
     using D = TDims<P>;
     constexpr int TM = D::TM, G = D::G;
-    __shared__ double sops[D::SIZE];
+    __shared__ __attribute__((aligned(16))) double sops[D::SIZE];
     __shared__ double suh[8][CNV][16];
-    __shared__ double sf[8][15][33];   // rows padded as in k_elem_tet
+    __shared__ __attribute__((aligned(16))) double sf[8][15][D::FROW];  // point flux rows, padded
     __shared__ double stau[8][25];
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
         const bool owner = valid && g < 3 * TM && g % 3 == 0;
         {
             double li[CNV];
-            tet_intgrtrst<D, CNV, 33>(sops, sf[hw], g, li);
+            tet_intgrtrst<D, CNV, D::FROW>(sops, sf[hw], g, li);
             if (owner)
 #pragma unroll
                 for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + n] = li[n];
@@ -569,16 +569,7 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
             wave_lds_sync();
             // strong residual: derivr/derivs/derivt of the total fluxes (rsdl.cpp:768-773)
             double res[CNV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            if (g < G) {
-#pragma unroll 3
-                for (int q = 0; q < G; ++q) {
-                    const double dr = sops[D::OP_DR + q * 32 + g], ds = sops[D::OP_DS + q * 32 + g],
-                                 dt = sops[D::OP_DT + q * 32 + g];
-#pragma unroll
-                    for (int n = 0; n < CNV; ++n)
-                        res[n] += dr * sf[hw][3 * n + 0][q] + ds * sf[hw][3 * n + 1][q] + dt * sf[hw][3 * n + 2][q];
-                }
-            }
+            if (g < G) tet_strong_residual<D, CNV>(sops, sf[hw], g, res);
             // stabilisation with the element tau (rsdl.cpp:776-827)
             double tres[CNV];
 #pragma unroll
@@ -597,7 +588,7 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
             wave_lds_sync();
             // lf_re = beta * intgrtrst (rsdl.cpp:829-835)
             double lr[CNV];
-            tet_intgrtrst<D, CNV, 33>(sops, sf[hw], g, lr);
+            tet_intgrtrst<D, CNV, D::FROW>(sops, sf[hw], g, lr);
             if (owner)
 #pragma unroll
                 for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = lr[n] * a.beta;
