@@ -397,7 +397,12 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
         e.begin = c->eoff[col];
         e.end = c->eoff[col + 1];
-        if (e.end > e.begin) hipLaunchKernelGGL(k_edge_ins, dim3(nblk(e.end - e.begin, 64)), dim3(64), 0, c->stream, e);
+        if (e.end > e.begin) {
+            const dim3 be(nblk(e.end - e.begin, 64)), te(64);
+            if (L.p == 1) hipLaunchKernelGGL(k_edge_ins<1>, be, te, 0, c->stream, e);
+            else if (L.p == 2) hipLaunchKernelGGL(k_edge_ins<2>, be, te, 0, c->stream, e);
+            else hipLaunchKernelGGL(k_edge_ins<4>, be, te, 0, c->stream, e);
+        }
     }
     if (c->coarse_flag) {
         const int l = c->log2p;
@@ -448,7 +453,12 @@ void do_lin_rsdl(hpg_ctx* c, int stage, bool with_source) {
     for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
         e.begin = c->eoff[col];
         e.end = c->eoff[col + 1];
-        if (e.end > e.begin) hipLaunchKernelGGL(k_edge_lin, dim3(nblk(e.end - e.begin, 64)), dim3(64), 0, c->stream, e);
+        if (e.end > e.begin) {
+            const dim3 be(nblk(e.end - e.begin, 64)), te(64);
+            if (L.p == 1) hipLaunchKernelGGL(k_edge_lin<1>, be, te, 0, c->stream, e);
+            else if (L.p == 2) hipLaunchKernelGGL(k_edge_lin<2>, be, te, 0, c->stream, e);
+            else hipLaunchKernelGGL(k_edge_lin<4>, be, te, 0, c->stream, e);
+        }
     }
     check(hipGetLastError(), "lin rsdl launch");
 }

@@ -435,31 +435,41 @@ struct EdgeArgs {
     double rho, bc_pressure;
 };
 
-__global__ void k_edge_ins(EdgeArgs a) {
+// P fixes the edge's mode count (p + 1) and Gauss points (p + 1), so the per-lane
+// arrays stay in registers
+template <int P>
+__global__ __launch_bounds__(64) void k_edge_ins(EdgeArgs a) {
+    constexpr int SM = P - 1, N1 = SM + 2, GPX = P + 1;
     const int idx = a.begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.end) return;
     const int sind = a.eseg[idx];
     const int v0 = a.seg_pnt[2 * sind], v1 = a.seg_pnt[2 * sind + 1];
-    const int n1 = a.sm + 2;
-    double uh[NV][6];
+    double uh[NV][N1];
+#pragma unroll
     for (int n = 0; n < NV; ++n) {
         uh[n][0] = a.ugv[v0 * NV + n];
         uh[n][1] = a.ugv[v1 * NV + n];
-        for (int m = 0; m < a.sm; ++m) uh[n][m + 2] = a.ugs[(sind * a.sm0 + m) * NV + n];
+#pragma unroll
+        for (int m = 0; m < SM; ++m) uh[n][m + 2] = a.ugs[(sind * a.sm0 + m) * NV + n];
     }
     const double X0 = a.pnts[2 * v0], Y0 = a.pnts[2 * v0 + 1], X1 = a.pnts[2 * v1], Y1 = a.pnts[2 * v1 + 1];
-    double lf[NV][6];
+    double lf[NV][N1];
+#pragma unroll
     for (int n = 0; n < NV; ++n)
-        for (int k = 0; k < n1; ++k) lf[n][k] = 0.0;
-    for (int ip = 0; ip < a.gpx; ++ip) {
-        const double* g = a.g1 + ip * n1;
-        const double* dg = a.dg1 + ip * n1;
+#pragma unroll
+        for (int k = 0; k < N1; ++k) lf[n][k] = 0.0;
+#pragma unroll
+    for (int ip = 0; ip < GPX; ++ip) {
+        const double* g = a.g1 + ip * N1;
+        const double* dg = a.dg1 + ip * N1;
         const double dxc = X0 * dg[0] + X1 * dg[1];
         const double dyc = Y0 * dg[0] + Y1 * dg[1];
         double u[NV];
+#pragma unroll
         for (int n = 0; n < NV; ++n) {
             double s = 0.0;
-            for (int k = 0; k < n1; ++k) s += uh[n][k] * g[k];
+#pragma unroll
+            for (int k = 0; k < N1; ++k) s += uh[n][k] * g[k];
             u[n] = s;
         }
         double nx = dyc, ny = -dxc;
@@ -467,15 +477,19 @@ __global__ void k_edge_ins(EdgeArgs a) {
         nx /= jcb; ny /= jcb;
         const double f2 = a.rho * (u[0] * nx + u[1] * ny);
         const double fl[NV] = {f2 * u[0] + a.bc_pressure * nx, f2 * u[1] + a.bc_pressure * ny, f2};
+#pragma unroll
         for (int n = 0; n < NV; ++n) {
             const double r = fl[n] * jcb * a.w1[ip];
-            for (int k = 0; k < n1; ++k) lf[n][k] += g[k] * r;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) lf[n][k] += g[k] * r;
         }
     }
+#pragma unroll
     for (int n = 0; n < NV; ++n) {
         a.resv[v0 * NV + n] += lf[n][0];
         a.resv[v1 * NV + n] += lf[n][1];
-        for (int m = 0; m < a.sm; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += lf[n][m + 2];
+#pragma unroll
+        for (int m = 0; m < SM; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += lf[n][m + 2];
     }
 }
 

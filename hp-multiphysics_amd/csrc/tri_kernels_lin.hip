@@ -309,34 +309,42 @@ struct EdgeLinArgs {
     double rho;
 };
 
-__global__ void k_edge_lin(EdgeLinArgs a) {
+template <int P>
+__global__ __launch_bounds__(64) void k_edge_lin(EdgeLinArgs a) {
+    constexpr int SM = P - 1, N1 = SM + 2, GPX = P + 1;
     const int idx = a.begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.end) return;
     const int sind = a.eseg[idx];
     const int v0 = a.seg_pnt[2 * sind], v1 = a.seg_pnt[2 * sind + 1];
-    const int n1 = a.sm + 2;
-    double uh[2][6], dh[2][6];
+    double uh[2][N1], dh[2][N1];
+#pragma unroll
     for (int n = 0; n < 2; ++n) {
         uh[n][0] = a.ugv[v0 * NV + n]; uh[n][1] = a.ugv[v1 * NV + n];
         dh[n][0] = a.dv[v0 * NV + n]; dh[n][1] = a.dv[v1 * NV + n];
-        for (int m = 0; m < a.sm; ++m) {
+#pragma unroll
+        for (int m = 0; m < SM; ++m) {
             uh[n][m + 2] = a.ugs[(sind * a.sm0 + m) * NV + n];
             dh[n][m + 2] = a.ds[(sind * a.sm0 + m) * NV + n];
         }
     }
     const double X0 = a.pnts[2 * v0], Y0 = a.pnts[2 * v0 + 1], X1 = a.pnts[2 * v1], Y1 = a.pnts[2 * v1 + 1];
-    double lf[NV][6];
+    double lf[NV][N1];
+#pragma unroll
     for (int n = 0; n < NV; ++n)
-        for (int k = 0; k < n1; ++k) lf[n][k] = 0.0;
-    for (int ip = 0; ip < a.gpx; ++ip) {
-        const double* g = a.g1 + ip * n1;
-        const double* dg = a.dg1 + ip * n1;
+#pragma unroll
+        for (int k = 0; k < N1; ++k) lf[n][k] = 0.0;
+#pragma unroll
+    for (int ip = 0; ip < GPX; ++ip) {
+        const double* g = a.g1 + ip * N1;
+        const double* dg = a.dg1 + ip * N1;
         const double dxc = X0 * dg[0] + X1 * dg[1];
         const double dyc = Y0 * dg[0] + Y1 * dg[1];
         double u[2], du[2];
+#pragma unroll
         for (int n = 0; n < 2; ++n) {
             double s = 0.0, ds = 0.0;
-            for (int k = 0; k < n1; ++k) { s += uh[n][k] * g[k]; ds += dh[n][k] * g[k]; }
+#pragma unroll
+            for (int k = 0; k < N1; ++k) { s += uh[n][k] * g[k]; ds += dh[n][k] * g[k]; }
             u[n] = s; du[n] = ds;
         }
         double nx = dyc, ny = -dxc;
@@ -345,15 +353,19 @@ __global__ void k_edge_lin(EdgeLinArgs a) {
         const double f2 = a.rho * (u[0] * nx + u[1] * ny);
         const double d2 = a.rho * (du[0] * nx + du[1] * ny);
         const double fl[NV] = {d2 * u[0] + f2 * du[0], d2 * u[1] + f2 * du[1], d2};
+#pragma unroll
         for (int n = 0; n < NV; ++n) {
             const double r = fl[n] * jcb * a.w1[ip];
-            for (int k = 0; k < n1; ++k) lf[n][k] += g[k] * r;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) lf[n][k] += g[k] * r;
         }
     }
+#pragma unroll
     for (int n = 0; n < NV; ++n) {
         a.resv[v0 * NV + n] += (float)lf[n][0];
         a.resv[v1 * NV + n] += (float)lf[n][1];
-        for (int m = 0; m < a.sm; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += (float)lf[n][m + 2];
+#pragma unroll
+        for (int m = 0; m < SM; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += (float)lf[n][m + 2];
     }
 }
 
