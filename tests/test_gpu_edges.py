@@ -55,6 +55,30 @@ def test_single_cube_parity(oracle_mod, log2p):
     assert err < 1e-12
 
 
+@pytest.mark.parametrize("log2p", [0, 1])
+def test_single_cube_cns_parity(oracle_mod, log2p):
+    _case(f"tet cns 1 cube (6 tetrahedra) log2p={log2p}")
+    import oracle_tet as OT
+    from hpmp import tetmesh as TM
+    from hpmp.tet_hp import TetHpCns
+    m = TM.unit_cube(1)
+    st = np.array([1.0, 0.3, 0.1, -0.05, 1.0])
+    rng = np.random.default_rng(6)
+    v = np.tile(st, (m.npnt, 1)) + 0.02 * rng.standard_normal((m.npnt, 5))
+    e = 0.02 * rng.standard_normal((m.nseg, (1 << log2p) - 1, 5))
+    kw = dict(gamma=1.4, prandtl=0.72, R=1.0)
+    G = TetHpCns(m, log2p=log2p, mu=1e-2, bc=tuple(st[1:]), **kw)
+    O = OT.TetHpCns(m, log2p=log2p, mu=1e-2, bc=(0.0, *st[1:]), **kw)
+    for X in (G, O):
+        X.set_state(v, e)
+        assert X.setup_preconditioner() == 0
+        X.update()
+        X.rsdl()
+    err = rel_err(G.get_residual(), O.get_res())
+    print("rel err", err, flush=True)
+    assert err < 1e-12
+
+
 def test_out_of_range_indices_are_rejected():
     _case("index arrays out of range")
     from hpmp import _lib, mesh as M, tetmesh as TM
