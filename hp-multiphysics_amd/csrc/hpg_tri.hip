@@ -17,6 +17,7 @@
 #include "tri_kernels.hip"
 #include "tri_kernels_lin.hip"
 #include "tri_kernels_cns.hip"
+#include "tri_jacobian.hip"
 #include "tri_ops.h"
 #include "mlp.h"
 
@@ -166,6 +167,7 @@ struct hpg_ctx {
     int *tri_pnt = nullptr, *tri_seg = nullptr, *tri_sgn = nullptr, *seg_pnt = nullptr;
     int* order = nullptr;
     std::vector<int> coff;
+    std::vector<int> tcol;  // colour of each element (host)
     int* eseg = nullptr;
     std::vector<int> eoff;
     int *vs_ptr, *vs_side, *vs_end, *vt_ptr, *vt_tri, *vt_loc, *st_tri, *st_loc;
@@ -282,13 +284,15 @@ static int resident_blocks(const void* kernel) {
 static int persistent_grid(int want, int cap) { return std::min((want + 7) / 8 * 8, cap); }
 
 template <int P>
-void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full) {
-    for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full, const std::vector<int>* offs = nullptr) {
+    // offs: colour offsets into a0.order (default: every element, c->order)
+    const std::vector<int>& off = offs ? *offs : c->coff;
+    for (size_t col = 0; col + 1 < off.size(); ++col) {
         ElemArgs a = a0;
         a.colour = (int)col;
-        a.begin = c->coff[col];
-        a.end = c->coff[col + 1];
-        a.gbase = c->gbase[col];
+        a.begin = off[col];
+        a.end = off[col + 1];
+        a.gbase = offs ? 0 : c->gbase[col];
         const int nlogical = (a.end - a.begin + 63) / 64;
         if (nlogical == 0) continue;
         const size_t r = c->recs.size();
@@ -520,12 +524,13 @@ void do_minvrt32(hpg_ctx* c, float cfa) {
 
 // ---- tri_hp_cns drivers ------------------------------------------------------------
 template <int P>
-void launch_elem_cns(hpg_ctx* c, const CnsElemArgs& a0, bool full) {
-    for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+void launch_elem_cns(hpg_ctx* c, const CnsElemArgs& a0, bool full, const std::vector<int>* offs = nullptr) {
+    const std::vector<int>& off = offs ? *offs : c->coff;
+    for (size_t col = 0; col + 1 < off.size(); ++col) {
         CnsElemArgs a = a0;
         a.colour = (int)col;
-        a.begin = c->coff[col];
-        a.end = c->coff[col + 1];
+        a.begin = off[col];
+        a.end = off[col + 1];
         const int noct = (a.end - a.begin + 7) / 8;
         if (noct == 0) continue;
         const int nb = std::min(((noct + 7) / 8) * 8, 1024);
@@ -727,6 +732,71 @@ void absmax(hpg_ctx* c, long nent, int nm, int mstride, const double* x, const u
 
 }  // namespace
 
+// device buffers of one call, released when the call ends
+struct CallBuffers {
+    hipStream_t stream;
+    std::vector<void*> p;
+    explicit CallBuffers(hipStream_t s) : stream(s) {}
+    template <class T>
+    T* get(size_t n, bool zero = false) {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+        p.push_back(q);
+        if (zero) check(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), stream), "memset");
+        return static_cast<T*>(q);
+    }
+    ~CallBuffers() {
+        hipStreamSynchronize(stream);
+        for (void* q : p) hipFree(q);
+    }
+};
+
+template <int P, int NVn>
+void jacobian_colour(hpg_ctx* c, const JacArgs& ja, int N) {
+    const int nel = ja.end - ja.begin;
+    const dim3 b1(nblk(nel, 64)), t1(64), bN(nblk((long)nel * N)), tN(256);
+    const Level& L = c->lv[c->log2p];
+    const double beta = c->prm.beta[0];
+    const bool full = beta > 0.0;
+    // one launch over the colour's batch elements.  The kernel's first-visit flags are
+    // keyed to colour 0 here, not to the elements' own colour; that does not matter:
+    // every scratch entry starts at zero, so both lftog paths store lf_re + lf_im
+    const std::vector<int> offs = {ja.begin, ja.end};
+    hipLaunchKernelGGL(k_jac_dw<NVn>, b1, t1, 0, c->stream, ja);
+    for (int col = -1; col < N; ++col) {
+        JacArgs a = ja;
+        a.mode = col < 0 ? -1 : col / NVn;
+        a.var = col < 0 ? 0 : col % NVn;
+        if (col >= 0) hipLaunchKernelGGL((k_jac_perturb<P, NVn>), b1, t1, 0, c->stream, a);
+        if (c->physics == HPG_PHYSICS_CNS) {
+            CnsElemArgs e;
+            e.pnts = c->pnts; e.tri_pnt = c->tri_pnt; e.tri_seg = c->tri_seg; e.tri_sgn = c->tri_sgn;
+            e.order = ja.jel; e.begin = 0; e.end = 0;
+            e.ugv = c->ugv; e.ugs = c->ugs; e.ugi = c->ugi;
+            e.resv = ja.jrv; e.ress = ja.jrs; e.resi = ja.jri;
+            e.rrv = ja.jqv; e.rrs = ja.jqs; e.rri = ja.jqi;
+            e.ops = L.cpack; e.sm0 = c->sm0; e.im0 = c->im0;
+            e.ph = c->cph; e.beta = beta; e.keep = 0.0;
+            e.fcv = c->fcv; e.fcs = c->fcs; e.colour = 0;
+            launch_elem_cns<P>(c, e, full, &offs);
+        } else {
+            ElemArgs e;
+            e.pnts = c->pnts; e.tri_pnt = c->tri_pnt; e.tri_seg = c->tri_seg; e.tri_sgn = c->tri_sgn;
+            e.order = ja.jel; e.begin = 0; e.end = 0;
+            e.ugv = c->ugv; e.ugs = c->ugs; e.ugi = c->ugi;
+            e.resv = ja.jrv; e.ress = ja.jrs; e.resi = ja.jri;
+            e.rrv = ja.jqv; e.rrs = ja.jqs; e.rri = ja.jqi;
+            e.ops = L.pack; e.sm0 = c->sm0; e.im0 = c->im0;
+            e.rho = c->prm.rho; e.mu = c->prm.mu; e.adis = c->prm.adis; e.beta = beta;
+            e.fcv = c->fcv; e.fcs = c->fcs; e.colour = 0; e.keep = 0.0;
+            e.lin = nullptr; e.gbase = 0;
+            launch_elem<P>(c, e, full, &offs);
+        }
+        hipLaunchKernelGGL((k_jac_collect<P, NVn>), bN, tN, 0, c->stream, a);
+    }
+    check(hipGetLastError(), "jacobian launch");
+}
+
 extern "C" {
 
 int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
@@ -797,6 +867,7 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->fcs = c->upload(fcs.data(), fcs.size());
         c->order = c->upload(order.data(), order.size());
         c->coff = off;
+        c->tcol = tcol;
         const int neseg = m->ebdry_start[m->nebd];
         std::vector<int> ev(2 * (size_t)neseg);
         for (int j = 0; j < neseg; ++j) {
@@ -1170,6 +1241,68 @@ int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
             }
         check(hipGetLastError(), "probe launch");
         check(hipMemcpyAsync(out, c->jdiag, (size_t)c->ntri * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+
+int hpg_element_jacobian(hpg_ctx* c, const int* tind, int nel, double eps_r, double eps_a, double* K) {
+    if (!c || nel < 0) return -2;
+    if (nel == 0) return 0;
+    if (!tind || !K) return -2;
+    HPG_TRY(c, {
+        const int nvv = c->nv;
+        const Level& L = c->lv[c->log2p];
+        const LevelDims d(L.p);
+        const int N = d.tm * nvv;
+        std::vector<char> seen(c->ntri, 0);
+        for (int j = 0; j < nel; ++j) {
+            if (tind[j] < 0 || tind[j] >= c->ntri) throw std::runtime_error("element index out of range");
+            if (seen[tind[j]]) throw std::runtime_error("element listed twice");
+            seen[tind[j]] = 1;
+        }
+        // group the batch by colour (stable in the caller's order)
+        const int ncol = (int)c->coff.size() - 1;
+        std::vector<int> cnt(ncol + 1, 0), jel(nel), jpos(nel);
+        for (int j = 0; j < nel; ++j) ++cnt[c->tcol[tind[j]] + 1];
+        for (int k = 0; k < ncol; ++k) cnt[k + 1] += cnt[k];
+        const std::vector<int> joff = cnt;
+        for (int j = 0; j < nel; ++j) {
+            const int q = cnt[c->tcol[tind[j]]]++;
+            jel[q] = tind[j];
+            jpos[q] = j;
+        }
+        CallBuffers B(c->stream);
+        JacArgs ja{};
+        ja.jel = B.get<int>(nel);
+        ja.jpos = B.get<int>(nel);
+        check(hipMemcpyAsync((void*)ja.jel, jel.data(), (size_t)nel * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        check(hipMemcpyAsync((void*)ja.jpos, jpos.data(), (size_t)nel * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        ja.tri_pnt = c->tri_pnt; ja.tri_seg = c->tri_seg; ja.tri_sgn = c->tri_sgn;
+        ja.sm0 = c->sm0; ja.im0 = c->im0;
+        ja.ugv = c->ugv; ja.ugs = c->ugs; ja.ugi = c->ugi;
+        const size_t nv = (size_t)c->npnt * nvv, ns = (size_t)c->nseg * c->sm0 * nvv, ni = (size_t)c->ntri * c->im0 * nvv;
+        ja.jrv = B.get<double>(nv, true); ja.jrs = B.get<double>(ns, true); ja.jri = B.get<double>(ni, true);
+        ja.jqv = B.get<double>(nv, true); ja.jqs = B.get<double>(ns, true); ja.jqi = B.get<double>(ni, true);
+        ja.dw = B.get<double>((size_t)nel * nvv);
+        ja.save = B.get<double>(nel);
+        ja.base = B.get<double>((size_t)nel * N);
+        ja.K = B.get<double>((size_t)nel * N * N);
+        ja.eps_r = eps_r; ja.eps_a = eps_a;
+        for (int col = 0; col < ncol; ++col) {
+            if (joff[col + 1] == joff[col]) continue;
+            JacArgs a = ja;
+            a.begin = joff[col];
+            a.end = joff[col + 1];
+            const bool cns = c->physics == HPG_PHYSICS_CNS;
+            switch (L.p) {
+                case 1: cns ? jacobian_colour<1, 4>(c, a, N) : jacobian_colour<1, 3>(c, a, N); break;
+                case 2: cns ? jacobian_colour<2, 4>(c, a, N) : jacobian_colour<2, 3>(c, a, N); break;
+                case 4: cns ? jacobian_colour<4, 4>(c, a, N) : jacobian_colour<4, 3>(c, a, N); break;
+                default: throw std::runtime_error("unsupported p");
+            }
+        }
+        check(hipMemcpyAsync(K, ja.K, (size_t)nel * N * N * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
         check(hipStreamSynchronize(c->stream), "sync");
     });
     return 0;

@@ -27,7 +27,7 @@ EXPORTS = (
     "hpg_update", "hpg_maxres", "hpg_resnorm", "hpg_mg_restrict", "hpg_mg_prolongate", "hpg_get_log2p",
     "hpg_set_precision", "hpg_set_preconditioner", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
     "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32", "hpg_selftest_mfma_bf16",
-    "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
+    "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_element_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
     "hpg_get_block_preconditioner", "hpg_link_fine",
@@ -128,6 +128,7 @@ def load():
     L.hpg_selftest_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.hpg_set_defect_inner.argtypes = [vp, C.c_int]
     L.hpg_apply_jacobian.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.hpg_element_jacobian.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double, dp]
     L.hpg_mlp_create.argtypes = [C.c_uint, C.POINTER(vp)]
     L.hpg_mlp_destroy.argtypes = [vp]
     L.hpg_mlp_num_params.argtypes = []

@@ -206,6 +206,19 @@ class TriHpIns:
                     "apply_jacobian")
         return out
 
+    def element_jacobian(self, tinds, eps_r: float = 1e-8, eps_a: float = 1e-8):
+        """tri_hp::element_jacobian (tri_hp/nstage.cpp:131-165) for the distinct elements
+        ``tinds`` on the current level: array (len, N, N), N = tm * NV, with
+        K[i*NV + n, mode*NV + var] = d(lf_re + lf_im)(n)(i) / d uht(var)(mode) by forward
+        differences (the reference's eps_r = eps_a = 1e-8)."""
+        t = np.ascontiguousarray(np.atleast_1d(tinds), dtype=np.int32)
+        p = 1 << self.log2p
+        n = (p + 1) * (p + 2) // 2 * self.NV
+        K = np.zeros((t.size, n, n))
+        self._check(self.L.hpg_element_jacobian(self.h, t.ctypes.data_as(C.POINTER(C.c_int)), int(t.size),
+                                                float(eps_r), float(eps_a), _lib.ptr(K)), "element_jacobian")
+        return K
+
     def synchronize(self):
         self._check(self.L.hpg_synchronize(self.h), "synchronize")
 

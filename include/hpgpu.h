@@ -138,6 +138,15 @@ int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
  * the directional derivative of tri_hp::rsdl(nstage). */
 int hpg_apply_jacobian(hpg_ctx *ctx, const double *dv, const double *ds, const double *di,
                        double *outv, double *outs, double *outi);
+/* Replaces tri_hp::element_jacobian (tri_hp/nstage.cpp:131-165, fixed mesh) for a batch
+ * of `nel` distinct elements `tind` on the current level, ins or cns: forward
+ * differences of element_rsdl(tind, 0) with dw(var) = eps_r * sum of |uht(var)| over
+ * the vertex modes + eps_a (the reference's eps_r = eps_a = 1e-8,
+ * tri_hp/hp_boundary.h:19-23).  K (host, nel x N x N, N = tm * NV, row-major) holds
+ * K(i*NV + n, mode*NV + var) = d(lf_re + lf_im)(n)(i) / d uht(var)(mode).  The state
+ * is left bit-identical; the residual arrays are untouched.  Returns -1 with an
+ * error for an index out of range or listed twice. */
+int hpg_element_jacobian(hpg_ctx *ctx, const int *tind, int nel, double eps_r, double eps_a, double *K);
 
 /* Learned preconditioner (no reference counterpart).  An MLP 11 -> 64 -> 64 -> 64 -> 3
  * (GELU, bfloat16 weights, FP32 accumulation) maps element features (vertex offsets,
