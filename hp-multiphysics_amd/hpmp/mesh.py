@@ -230,6 +230,6 @@ def check_mesh(m: TriMesh) -> None:
     # boundary segments owned with sgn = +1 (domain on the left)
     owner_sgn = np.zeros(m.nseg, dtype=np.int64)
     for k in range(3):
-        owner_sgn[m.tri_seg[:, k]] += m.tri_sgn[:, k]
+        np.add.at(owner_sgn, m.tri_seg[:, k], m.tri_sgn[:, k])
     assert np.all(owner_sgn[bmask] == 1)
     assert np.all(owner_sgn[~bmask] == 0)

@@ -29,12 +29,15 @@ def hpmp_mod():
 
 
 def rel_err(a, b):
-    """max |a - b| / max |b| over a list of array pairs (0/0 -> 0)."""
+    """max |a - b| / max |b| over a list of array pairs (0/0 -> 0; inf when either
+    side holds a NaN or an infinity, so a non-finite result never passes)."""
     num = 0.0
     den = 0.0
     for x, y in zip(a, b):
         x = np.asarray(x)
         y = np.asarray(y)
+        if not (np.isfinite(x).all() and np.isfinite(y).all()):
+            return float("inf")
         if x.size:
             num = max(num, float(np.abs(x - y).max()))
             den = max(den, float(np.abs(y).max()))
