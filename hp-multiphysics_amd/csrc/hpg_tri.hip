@@ -322,13 +322,16 @@ void launch_lin(hpg_ctx* c, const LinArgs& a0, bool full) {
         a.gbase = c->gbase[col];
         const int nlogical = (a.end - a.begin + 63) / 64;
         if (nlogical == 0) continue;
-        const int nb = ((nlogical + 7) / 8) * 8;
         const size_t r = c->recs.size();
         if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
-        if (full)
-            hipLaunchKernelGGL((k_elem_lin<P, true>), dim3(nb), dim3(256), 0, c->stream, a, nlogical);
-        else
-            hipLaunchKernelGGL((k_elem_lin<P, false>), dim3(nb), dim3(256), 0, c->stream, a, nlogical);
+        if (full) {
+            static const int cap = resident_blocks((const void*)k_elem_lin<P, true>);
+            hipLaunchKernelGGL((k_elem_lin<P, true>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream, a);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_lin<P, false>);
+            hipLaunchKernelGGL((k_elem_lin<P, false>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream,
+                               a);
+        }
         if (c->timing) {
             check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
             c->recs.push_back({full ? 3 : 2, a.end - a.begin});
@@ -1233,9 +1236,9 @@ int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
                     if (nlogical == 0) continue;
                     const int nb = ((nlogical + 7) / 8) * 8;
                     switch (L.p) {
-                        case 1: hipLaunchKernelGGL((k_elem_lin<1, true, true>), dim3(nb), dim3(256), 0, c->stream, b, nlogical); break;
-                        case 2: hipLaunchKernelGGL((k_elem_lin<2, true, true>), dim3(nb), dim3(256), 0, c->stream, b, nlogical); break;
-                        default: hipLaunchKernelGGL((k_elem_lin<4, true, true>), dim3(nb), dim3(256), 0, c->stream, b, nlogical); break;
+                        case 1: hipLaunchKernelGGL((k_elem_lin<1, true, true>), dim3(nb), dim3(256), 0, c->stream, b); break;
+                        case 2: hipLaunchKernelGGL((k_elem_lin<2, true, true>), dim3(nb), dim3(256), 0, c->stream, b); break;
+                        default: hipLaunchKernelGGL((k_elem_lin<4, true, true>), dim3(nb), dim3(256), 0, c->stream, b); break;
                     }
                 }
             }

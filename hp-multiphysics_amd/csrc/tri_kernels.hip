@@ -66,12 +66,14 @@ struct ElemSlots {
 template <int P>
 __device__ __forceinline__ int mode_kind(int k) { return k < 3 ? 0 : (k < Dims<P>::BM ? 1 : 2); }
 
-__device__ __forceinline__ int elem_of_group(const ElemArgs& a, int g, int gend, int e) {
+template <class A>
+__device__ __forceinline__ int elem_of_group(const A& a, int g, int gend, int e) {
     const int idx = a.begin + g * 16 + e;
     return (g < gend && idx < a.end) ? a.order[idx] : -1;
 }
 
-__device__ __forceinline__ ElemConn elem_conn(const ElemArgs& a, int t) {
+template <class A>
+__device__ __forceinline__ ElemConn elem_conn(const A& a, int t) {
     ElemConn c;
     c.t = t;
 #pragma unroll
@@ -83,8 +85,8 @@ __device__ __forceinline__ ElemConn elem_conn(const ElemArgs& a, int t) {
     return c;
 }
 
-template <int P>
-__device__ __forceinline__ ElemSlots<P> elem_slots(const ElemArgs& a, const ElemConn& c, int q) {
+template <int P, class A>
+__device__ __forceinline__ ElemSlots<P> elem_slots(const A& a, const ElemConn& c, int q) {
     using D = Dims<P>;
     constexpr int SMD = D::SM > 0 ? D::SM : 1;
     ElemSlots<P> S;

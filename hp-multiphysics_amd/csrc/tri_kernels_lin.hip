@@ -26,208 +26,114 @@ __device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
 __host__ __device__ constexpr int f32_row(int i, int q) { return HPG_F32_ROW_4I_PLUS_Q ? 4 * i + q : 4 * q + i; }
 
 template <int P, bool FULL, bool PROBE = false>
-__global__ __launch_bounds__(256) void k_elem_lin(LinArgs a, int nlogical) {
+__global__ __launch_bounds__(256) void k_elem_lin(LinArgs a) {
     using D = Dims<P>;
     constexpr int GT = D::GT, KT = D::KT;
     constexpr int SMD = D::SM > 0 ? D::SM : 1;
     __shared__ float sops[D::PACK];
-    const int per = gridDim.x >> 3;
-    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (lb >= nlogical) return;
     for (int k = threadIdx.x; k < D::PACK; k += blockDim.x) sops[k] = a.ops[k];
     __syncthreads();
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int e = lane & 15, q = lane >> 4;
-    const int idx = a.begin + (lb * 4 + w) * 16 + e;
-    const bool valid = idx < a.end;
-    if (__builtin_amdgcn_ballot_w64(valid) == 0) return;
-    const int t = valid ? a.order[idx] : a.order[a.begin];
+    // Persistent and XCD-aware like k_elem_ins: XCD x walks the groups
+    // [x*gpx, (x+1)*gpx) of the colour, each wave every WX-th one, with element ids
+    // three groups ahead, connectivity two ahead and the mode offsets, geometry and
+    // delta coefficients one group ahead of their use.
+    const int WX = (gridDim.x >> 3) * 4;
+    const int rank = (blockIdx.x >> 3) * 4 + w;
+    const int xcd = blockIdx.x & 7;
+    const int ng = (a.end - a.begin + 15) >> 4;
+    const int gpx = (ng + 7) >> 3;
+    const int g0 = xcd * gpx + rank;
+    const int gend = min(ng, (xcd + 1) * gpx);
+    if (g0 >= gend) return;
 
-    int vtx[3], seg[3], sgn[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        vtx[k] = a.tri_pnt[3 * t + k];
-        seg[k] = a.tri_seg[3 * t + k];
-        sgn[k] = a.tri_sgn[3 * t + k];
-    }
-    const double X0 = a.pnts[2 * vtx[0]], Y0 = a.pnts[2 * vtx[0] + 1];
-    const double X1 = a.pnts[2 * vtx[1]], Y1 = a.pnts[2 * vtx[1] + 1];
-    const double X2 = a.pnts[2 * vtx[2]], Y2 = a.pnts[2 * vtx[2] + 1];
-    const float l00 = (float)(0.5 * (X2 - X1)), l01 = (float)(0.5 * (X0 - X1));
-    const float l10 = (float)(0.5 * (Y2 - Y1)), l11 = (float)(0.5 * (Y0 - Y1));
-
-    float b[NV][KT];
-#pragma unroll
-    for (int s = 0; s < KT; ++s) {
-        const int k = 4 * s + q;
-        int kind;
-        double sg;
-        const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sg);
-        const float* src = kind == 0 ? a.dv : (kind == 1 ? a.ds : a.di);
-#pragma unroll
-        for (int n = 0; n < NV; ++n) b[n][s] = 0.f;
-        if constexpr (PROBE) {
-#pragma unroll
-            for (int n = 0; n < NV; ++n) b[n][s] = (k == a.probe_m && n == a.probe_n) ? 1.f : 0.f;
-        } else if (valid && k < D::TM) {
-#pragma unroll
-            for (int n = 0; n < NV; ++n) b[n][s] = (float)sg * src[off + n];
-        }
-    }
-
-    // frozen data of this group: the FP64 kernel stored row 16mt + 4i' + q' at
-    // (mt*4 + i')*64 + e + 16q'
-    const float* L = a.lin + (size_t)(a.gbase + lb * 4 + w) * (5 * GT * 4 * 64);
-    auto frozen = [&](int f, int mt, int i) {
-        const int r = f32_row(i, q);
-        return L[f * GT * 256 + (mt * 4 + (r >> 2)) * 64 + e + 16 * (r & 3)];
-    };
-
-    const f4 z = {0.f, 0.f, 0.f, 0.f};
-    f4 dU[GT], dV[GT], dP[GT];
-#pragma unroll
-    for (int mt = 0; mt < GT; ++mt) {
-        f4 cu = z, cv = z, cp = z;
+    // K-step s of lane l is mode 4s + l/16: the delta coefficients of a group
+    auto gather = [&](const ElemSlots<P>& S, float (&bb)[NV][KT]) {
 #pragma unroll
         for (int s = 0; s < KT; ++s) {
-            const float av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
-            cu = mfma32(av, b[0][s], cu);
-            cv = mfma32(av, b[1][s], cv);
-            cp = mfma32(av, b[2][s], cp);
+            const int k = 4 * s + q, kind = mode_kind<P>(k);
+            const bool live = S.t >= 0 && k < D::TM;
+            const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
+            const float sg = S.off[s] < 0 ? -1.f : 1.f;
+            const float* src = kind == 0 ? a.dv : (kind == 1 ? a.ds : a.di);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                if constexpr (PROBE)
+                    bb[n][s] = (k == a.probe_m && n == a.probe_n) ? 1.f : 0.f;
+                else
+                    bb[n][s] = live ? sg * src[off + n] : 0.f;
+            }
         }
-        dU[mt] = cu; dV[mt] = cv; dP[mt] = cp;
-    }
+    };
 
-    const float rho = a.rho;
-    f4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
-    f4 U0[GT], V0[GT];
-#pragma unroll
-    for (int mt = 0; mt < GT; ++mt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float u = frozen(0, mt, i), v = frozen(1, mt, i);
-            U0[mt][i] = u; V0[mt][i] = v;
-            const float du = dU[mt][i], dv = dV[mt][i], dp = dP[mt][i];
-            const float c0 = rho * (l11 * u - l01 * v), c1 = rho * (-l10 * u + l00 * v);
-            const float dc0 = rho * (l11 * du - l01 * dv), dc1 = rho * (-l10 * du + l00 * dv);
-            C0[mt][i] = dc0; C1[mt][i] = dc1;
-            A00[mt][i] = du * c0 + u * dc0 + l11 * dp;
-            A01[mt][i] = du * c1 + u * dc1 - l10 * dp;
-            A10[mt][i] = dv * c0 + v * dc0 - l01 * dp;
-            A11[mt][i] = dv * c1 + v * dc1 + l00 * dp;
-        }
-    f4 LI0 = z, LI1 = z, LI2 = z;
-#pragma unroll
-    for (int kt = 0; kt < GT; ++kt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int kk = kt * 4 + i;
-            const float ar = sops[D::OFF_IR + kk * 64 + lane];
-            const float as = sops[D::OFF_IS + kk * 64 + lane];
-            LI0 = mfma32(ar, A00[kt][i], LI0); LI0 = mfma32(as, A01[kt][i], LI0);
-            LI1 = mfma32(ar, A10[kt][i], LI1); LI1 = mfma32(as, A11[kt][i], LI1);
-            LI2 = mfma32(ar, C0[kt][i], LI2);  LI2 = mfma32(as, C1[kt][i], LI2);
-        }
+    ElemConn cur = elem_conn(a, elem_of_group(a, g0, gend, e));
+    ElemSlots<P> S = elem_slots<P>(a, cur, q);
+    float b[NV][KT];
+    gather(S, b);
+    ElemConn cnext = elem_conn(a, elem_of_group(a, g0 + WX, gend, e));
+    int tnext2 = elem_of_group(a, g0 + 2 * WX, gend, e);
 
-    f4 LR0 = z, LR1 = z, LR2 = z;
-    if constexpr (FULL) {
-        f4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
+    for (int g = g0; g < gend; g += WX) {
+        const ElemSlots<P> Sn = elem_slots<P>(a, cnext, q);
+        float bn[NV][KT];
+        gather(Sn, bn);
+        const ElemConn cn2 = elem_conn(a, tnext2);
+        const int tn3 = elem_of_group(a, g + 3 * WX, gend, e);
+
+        const bool valid = cur.t >= 0;
+        const int t = cur.t;
+        const int* vtx = cur.vtx;
+        const int* seg = cur.seg;
+        const int* sgn = cur.sgn;
+        // padding lanes run on the reference triangle with a zero delta
+        const double X0 = S.x[0], Y0 = S.y[0], X1 = S.x[1], Y1 = S.y[1], X2 = S.x[2], Y2 = S.y[2];
+        const float l00 = (float)(0.5 * (X2 - X1)), l01 = (float)(0.5 * (X0 - X1));
+        const float l10 = (float)(0.5 * (Y2 - Y1)), l11 = (float)(0.5 * (Y0 - Y1));
+
+        // frozen data of this group: the FP64 kernel stored row 16mt + 4i' + q' at
+        // (mt*4 + i')*64 + e + 16q'
+        const float* L = a.lin + (size_t)(a.gbase + g) * (5 * GT * 4 * 64);
+        auto frozen = [&](int f, int mt, int i) {
+            const int r = f32_row(i, q);
+            return L[f * GT * 256 + (mt * 4 + (r >> 2)) * 64 + e + 16 * (r & 3)];
+        };
+
+        const f4 z = {0.f, 0.f, 0.f, 0.f};
+        f4 dU[GT], dV[GT], dP[GT];
 #pragma unroll
         for (int mt = 0; mt < GT; ++mt) {
-            f4 c1 = z, c2 = z, c3 = z, c4 = z;
+            f4 cu = z, cv = z, cp = z;
 #pragma unroll
             for (int s = 0; s < KT; ++s) {
-                const float ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
-                const float as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
-                c1 = mfma32(ar, b[0][s], c1); c2 = mfma32(as, b[0][s], c2);
-                c3 = mfma32(ar, b[1][s], c3); c4 = mfma32(as, b[1][s], c4);
+                const float av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
+                cu = mfma32(av, b[0][s], cu);
+                cv = mfma32(av, b[1][s], cv);
+                cp = mfma32(av, b[2][s], cp);
             }
-            Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
+            dU[mt] = cu; dV[mt] = cv; dP[mt] = cp;
         }
-        const double cjcbd = (0.5 * (X2 - X1)) * (0.5 * (Y0 - Y1)) - (0.5 * (Y2 - Y1)) * (0.5 * (X0 - X1));
-        const float cjcb = (float)cjcbd;
-        const float cjcbi = a.mu / cjcb;
-        const float v0000 = -cjcbi * (2.f * l11 * l11 + l01 * l01);
-        const float v0011 = -cjcbi * (2.f * l10 * l10 + l00 * l00);
-        const float v0001 = cjcbi * (2.f * l11 * l10 + l01 * l00);
-        const float v1100 = -cjcbi * (l11 * l11 + 2.f * l01 * l01);
-        const float v1111 = -cjcbi * (l10 * l10 + 2.f * l00 * l00);
-        const float v1101 = cjcbi * (l11 * l10 + 2.f * l01 * l00);
-        const float v0100 = cjcbi * l01 * l11;
-        const float v0111 = cjcbi * l00 * l10;
-        const float v0101 = -cjcbi * l01 * l10;
-        const float v0110 = -cjcbi * l00 * l11;
-        f4 F00[GT], F01[GT], F10[GT], F11[GT];
+
+        const float rho = a.rho;
+        f4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
+        f4 U0[GT], V0[GT];
 #pragma unroll
         for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
-                const float f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
-                const float f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
-                const float f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
-                const float f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
-                F00[mt][i] = f00; F01[mt][i] = f01; F10[mt][i] = f10; F11[mt][i] = f11;
-                A00[mt][i] += f00; A01[mt][i] += f01; A10[mt][i] += f10; A11[mt][i] += f11;
+                const float u = frozen(0, mt, i), v = frozen(1, mt, i);
+                U0[mt][i] = u; V0[mt][i] = v;
+                const float du = dU[mt][i], dv = dV[mt][i], dp = dP[mt][i];
+                const float c0 = rho * (l11 * u - l01 * v), c1 = rho * (-l10 * u + l00 * v);
+                const float dc0 = rho * (l11 * du - l01 * dv), dc1 = rho * (-l10 * du + l00 * dv);
+                C0[mt][i] = dc0; C1[mt][i] = dc1;
+                A00[mt][i] = du * c0 + u * dc0 + l11 * dp;
+                A01[mt][i] = du * c1 + u * dc1 - l10 * dp;
+                A10[mt][i] = dv * c0 + v * dc0 - l01 * dp;
+                A11[mt][i] = dv * c1 + v * dc1 + l00 * dp;
             }
-        f4 R0[GT], R1[GT], R2[GT];
-#pragma unroll
-        for (int mt = 0; mt < GT; ++mt) {
-            f4 r0 = z, r1 = z, r2 = z;
-#pragma unroll
-            for (int kt = 0; kt < GT; ++kt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int kk = (mt * GT + kt) * 4 + i;
-                    const float dr = sops[D::OFF_DR + kk * 64 + lane];
-                    const float ds = sops[D::OFF_DS + kk * 64 + lane];
-                    r0 = mfma32(dr, A00[kt][i], r0); r0 = mfma32(ds, A01[kt][i], r0);
-                    r1 = mfma32(dr, A10[kt][i], r1); r1 = mfma32(ds, A11[kt][i], r1);
-                    r2 = mfma32(dr, C0[kt][i], r2);  r2 = mfma32(ds, C1[kt][i], r2);
-                }
-            R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
-        }
-        const double dx1 = X0 - X2, dy1 = Y0 - Y2, dx2 = X1 - X0, dy2 = Y1 - Y0;
-        const double ar2 = dx1 * dy2 - dy1 * dx2;
-        const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
-                             sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
-        const double hd = (ar2 / perim) / (0.25 * (P + 1) * (P + 1));
-        const float gv = (float)(2.0 * ((double)a.mu / (double)rho) / hd);
-        const float gvis = gv * gv;
-        const float tfac = (float)(a.adis * hd / cjcbd);
-#pragma unroll
-        for (int mt = 0; mt < GT; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float u = U0[mt][i], v = V0[mt][i];
-                const float du = dU[mt][i], dv = dV[mt][i];
-                const float s0 = frozen(2, mt, i), s1 = frozen(3, mt, i), s2 = frozen(4, mt, i);
-                const float qq = u * u + v * v;
-                float gam = 3.f * qq + gvis;
-                float dq = 2.f * (u * du + v * dv);
-                float dgam = 3.f * dq;
-                if (a.mu == 0.f && gam < 0.1f) { gam = 0.1f; dgam = 0.f; }
-                const float tau0 = tfac / sqrtf(gam);
-                const float dtau0 = -0.5f * tau0 * dgam / gam;
-                const float sq = sqrtf(qq);
-                const float tau2 = sq * tau0;
-                const float dtau2 = (qq > 0.f ? dq / (2.f * sq) : 0.f) * tau0 + sq * dtau0;
-                const float t0 = tau0 * s0, t1 = tau0 * s1, t2 = tau2 * s2;
-                const float dt0 = dtau0 * s0 + tau0 * R0[mt][i];
-                const float dt1 = dtau0 * s1 + tau0 * R1[mt][i];
-                const float dt2 = dtau2 * s2 + tau2 * R2[mt][i];
-                F00[mt][i] -= (2.f * l11 * du - l01 * dv) * t0 + (2.f * l11 * u - l01 * v) * dt0 -
-                              l01 * (du * t1 + u * dt1) + l11 * dt2;
-                F01[mt][i] -= (-2.f * l10 * du + l00 * dv) * t0 + (-2.f * l10 * u + l00 * v) * dt0 +
-                              l00 * (du * t1 + u * dt1) - l10 * dt2;
-                F10[mt][i] -= l11 * (dv * t0 + v * dt0) + (l11 * du - 2.f * l01 * dv) * t1 +
-                              (l11 * u - 2.f * l01 * v) * dt1 - l01 * dt2;
-                F11[mt][i] -= -l10 * (dv * t0 + v * dt0) + (-l10 * du + 2.f * l00 * dv) * t1 +
-                              (-l10 * u + 2.f * l00 * v) * dt1 + l00 * dt2;
-                C0[mt][i] = -(l11 * dt0 - l01 * dt1);
-                C1[mt][i] = -(-l10 * dt0 + l00 * dt1);
-            }
+        f4 LI0 = z, LI1 = z, LI2 = z;
 #pragma unroll
         for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
@@ -235,58 +141,177 @@ __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a, int nlogical) {
                 const int kk = kt * 4 + i;
                 const float ar = sops[D::OFF_IR + kk * 64 + lane];
                 const float as = sops[D::OFF_IS + kk * 64 + lane];
-                LR0 = mfma32(ar, F00[kt][i], LR0); LR0 = mfma32(as, F01[kt][i], LR0);
-                LR1 = mfma32(ar, F10[kt][i], LR1); LR1 = mfma32(as, F11[kt][i], LR1);
-                LR2 = mfma32(ar, C0[kt][i], LR2);  LR2 = mfma32(as, C1[kt][i], LR2);
+                LI0 = mfma32(ar, A00[kt][i], LI0); LI0 = mfma32(as, A01[kt][i], LI0);
+                LI1 = mfma32(ar, A10[kt][i], LI1); LI1 = mfma32(as, A11[kt][i], LI1);
+                LI2 = mfma32(ar, C0[kt][i], LI2);  LI2 = mfma32(as, C1[kt][i], LI2);
             }
-    }
 
-    if (!valid) return;
-    if constexpr (PROBE) {
-        // element-local diagonal entry J_e(m n, m n) of the full residual (beta = 1)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (f32_row(i, q) != a.probe_m) continue;
-            const float li[3] = {LI0[i], LI1[i], LI2[i]};
-            const float lr[3] = {LR0[i], LR1[i], LR2[i]};
-            a.diag[(size_t)t * NV + a.probe_n] += fabs((double)li[a.probe_n] + (double)lr[a.probe_n]);
-        }
-        return;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int k = f32_row(i, q);
-        if (k >= D::TM) continue;
-        int kind;
-        double sgd;
-        const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sgd);
-        const float sg = (float)sgd;
-        float* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
-        float* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
-        const double* sr = kind == 0 ? a.srcv : (kind == 1 ? a.srcs : a.srci);
-        const bool first = kind == 2 || (kind == 0 ? a.fcv[sel3(vtx, k)] : a.fcs[sel3(seg, (k - 3) / SMD)]) == a.colour;
-        const float li[3] = {sg * LI0[i], sg * LI1[i], sg * LI2[i]};
-        float lr[3] = {0.f, 0.f, 0.f};
+        f4 LR0 = z, LR1 = z, LR2 = z;
         if constexpr (FULL) {
-            lr[0] = sg * (LR0[i] * a.beta);
-            lr[1] = sg * (LR1[i] * a.beta);
-            lr[2] = sg * (LR2[i] * a.beta);
-        }
-        if (first) {
+            f4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
 #pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                const float r0 = a.keep == 0.f ? 0.f : a.keep * dr[off + n];
-                if (FULL || a.keep != 1.f) dr[off + n] = r0 + lr[n];
-                const float src = sr ? (float)sr[off + n] : 0.f;
-                di[off + n] = r0 + src + lr[n] + li[n];
-            }
-        } else {
+            for (int mt = 0; mt < GT; ++mt) {
+                f4 c1 = z, c2 = z, c3 = z, c4 = z;
 #pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                if constexpr (FULL) dr[off + n] += lr[n];
-                di[off + n] += lr[n] + li[n];
+                for (int s = 0; s < KT; ++s) {
+                    const float ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
+                    const float as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
+                    c1 = mfma32(ar, b[0][s], c1); c2 = mfma32(as, b[0][s], c2);
+                    c3 = mfma32(ar, b[1][s], c3); c4 = mfma32(as, b[1][s], c4);
+                }
+                Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
+            }
+            const double cjcbd = (0.5 * (X2 - X1)) * (0.5 * (Y0 - Y1)) - (0.5 * (Y2 - Y1)) * (0.5 * (X0 - X1));
+            const float cjcb = (float)cjcbd;
+            const float cjcbi = a.mu / cjcb;
+            const float v0000 = -cjcbi * (2.f * l11 * l11 + l01 * l01);
+            const float v0011 = -cjcbi * (2.f * l10 * l10 + l00 * l00);
+            const float v0001 = cjcbi * (2.f * l11 * l10 + l01 * l00);
+            const float v1100 = -cjcbi * (l11 * l11 + 2.f * l01 * l01);
+            const float v1111 = -cjcbi * (l10 * l10 + 2.f * l00 * l00);
+            const float v1101 = cjcbi * (l11 * l10 + 2.f * l01 * l00);
+            const float v0100 = cjcbi * l01 * l11;
+            const float v0111 = cjcbi * l00 * l10;
+            const float v0101 = -cjcbi * l01 * l10;
+            const float v0110 = -cjcbi * l00 * l11;
+            f4 F00[GT], F01[GT], F10[GT], F11[GT];
+#pragma unroll
+            for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
+                    const float f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
+                    const float f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
+                    const float f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
+                    const float f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
+                    F00[mt][i] = f00; F01[mt][i] = f01; F10[mt][i] = f10; F11[mt][i] = f11;
+                    A00[mt][i] += f00; A01[mt][i] += f01; A10[mt][i] += f10; A11[mt][i] += f11;
+                }
+            f4 R0[GT], R1[GT], R2[GT];
+#pragma unroll
+            for (int mt = 0; mt < GT; ++mt) {
+                f4 r0 = z, r1 = z, r2 = z;
+#pragma unroll
+                for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int kk = (mt * GT + kt) * 4 + i;
+                        const float dr = sops[D::OFF_DR + kk * 64 + lane];
+                        const float ds = sops[D::OFF_DS + kk * 64 + lane];
+                        r0 = mfma32(dr, A00[kt][i], r0); r0 = mfma32(ds, A01[kt][i], r0);
+                        r1 = mfma32(dr, A10[kt][i], r1); r1 = mfma32(ds, A11[kt][i], r1);
+                        r2 = mfma32(dr, C0[kt][i], r2);  r2 = mfma32(ds, C1[kt][i], r2);
+                    }
+                R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
+            }
+            const double dx1 = X0 - X2, dy1 = Y0 - Y2, dx2 = X1 - X0, dy2 = Y1 - Y0;
+            const double ar2 = dx1 * dy2 - dy1 * dx2;
+            const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
+                                 sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
+            const double hd = (ar2 / perim) / (0.25 * (P + 1) * (P + 1));
+            const float gv = (float)(2.0 * ((double)a.mu / (double)rho) / hd);
+            const float gvis = gv * gv;
+            const float tfac = (float)(a.adis * hd / cjcbd);
+#pragma unroll
+            for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float u = U0[mt][i], v = V0[mt][i];
+                    const float du = dU[mt][i], dv = dV[mt][i];
+                    const float s0 = frozen(2, mt, i), s1 = frozen(3, mt, i), s2 = frozen(4, mt, i);
+                    const float qq = u * u + v * v;
+                    float gam = 3.f * qq + gvis;
+                    float dq = 2.f * (u * du + v * dv);
+                    float dgam = 3.f * dq;
+                    if (a.mu == 0.f && gam < 0.1f) { gam = 0.1f; dgam = 0.f; }
+                    const float tau0 = tfac / sqrtf(gam);
+                    const float dtau0 = -0.5f * tau0 * dgam / gam;
+                    const float sq = sqrtf(qq);
+                    const float tau2 = sq * tau0;
+                    const float dtau2 = (qq > 0.f ? dq / (2.f * sq) : 0.f) * tau0 + sq * dtau0;
+                    const float t0 = tau0 * s0, t1 = tau0 * s1, t2 = tau2 * s2;
+                    const float dt0 = dtau0 * s0 + tau0 * R0[mt][i];
+                    const float dt1 = dtau0 * s1 + tau0 * R1[mt][i];
+                    const float dt2 = dtau2 * s2 + tau2 * R2[mt][i];
+                    F00[mt][i] -= (2.f * l11 * du - l01 * dv) * t0 + (2.f * l11 * u - l01 * v) * dt0 -
+                                  l01 * (du * t1 + u * dt1) + l11 * dt2;
+                    F01[mt][i] -= (-2.f * l10 * du + l00 * dv) * t0 + (-2.f * l10 * u + l00 * v) * dt0 +
+                                  l00 * (du * t1 + u * dt1) - l10 * dt2;
+                    F10[mt][i] -= l11 * (dv * t0 + v * dt0) + (l11 * du - 2.f * l01 * dv) * t1 +
+                                  (l11 * u - 2.f * l01 * v) * dt1 - l01 * dt2;
+                    F11[mt][i] -= -l10 * (dv * t0 + v * dt0) + (-l10 * du + 2.f * l00 * dv) * t1 +
+                                  (-l10 * u + 2.f * l00 * v) * dt1 + l00 * dt2;
+                    C0[mt][i] = -(l11 * dt0 - l01 * dt1);
+                    C1[mt][i] = -(-l10 * dt0 + l00 * dt1);
+                }
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = kt * 4 + i;
+                    const float ar = sops[D::OFF_IR + kk * 64 + lane];
+                    const float as = sops[D::OFF_IS + kk * 64 + lane];
+                    LR0 = mfma32(ar, F00[kt][i], LR0); LR0 = mfma32(as, F01[kt][i], LR0);
+                    LR1 = mfma32(ar, F10[kt][i], LR1); LR1 = mfma32(as, F11[kt][i], LR1);
+                    LR2 = mfma32(ar, C0[kt][i], LR2);  LR2 = mfma32(as, C1[kt][i], LR2);
+                }
+        }
+
+        if (valid && PROBE) {
+            // element-local diagonal entry J_e(m n, m n) of the full residual (beta = 1)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (f32_row(i, q) != a.probe_m) continue;
+                const float li[3] = {LI0[i], LI1[i], LI2[i]};
+                const float lr[3] = {LR0[i], LR1[i], LR2[i]};
+                a.diag[(size_t)t * NV + a.probe_n] += fabs((double)li[a.probe_n] + (double)lr[a.probe_n]);
+            }
+        } else if (valid) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = f32_row(i, q);
+                if (k >= D::TM) continue;
+                int kind;
+                double sgd;
+                const int off = mode_offset<P>(k, t, vtx, seg, sgn, a.sm0, a.im0, kind, sgd);
+                const float sg = (float)sgd;
+                float* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
+                float* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+                const double* sr = kind == 0 ? a.srcv : (kind == 1 ? a.srcs : a.srci);
+                const bool first =
+                    kind == 2 || (kind == 0 ? a.fcv[sel3(vtx, k)] : a.fcs[sel3(seg, (k - 3) / SMD)]) == a.colour;
+                const float li[3] = {sg * LI0[i], sg * LI1[i], sg * LI2[i]};
+                float lr[3] = {0.f, 0.f, 0.f};
+                if constexpr (FULL) {
+                    lr[0] = sg * (LR0[i] * a.beta);
+                    lr[1] = sg * (LR1[i] * a.beta);
+                    lr[2] = sg * (LR2[i] * a.beta);
+                }
+                if (first) {
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) {
+                        const float r0 = a.keep == 0.f ? 0.f : a.keep * dr[off + n];
+                        if (FULL || a.keep != 1.f) dr[off + n] = r0 + lr[n];
+                        const float src = sr ? (float)sr[off + n] : 0.f;
+                        di[off + n] = r0 + src + lr[n] + li[n];
+                    }
+                } else {
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) {
+                        if constexpr (FULL) dr[off + n] += lr[n];
+                        di[off + n] += lr[n] + li[n];
+                    }
+                }
             }
         }
+        S = Sn;
+        cur = cnext;
+        cnext = cn2;
+        tnext2 = tn3;
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+#pragma unroll
+            for (int s = 0; s < KT; ++s) b[n][s] = bn[n][s];
     }
 }
 
@@ -381,15 +406,15 @@ __global__ void k_to_f32(long n, const double* __restrict__ x, float* __restrict
     if (k < n) y[k] = (float)x[k];
 }
 
-template __global__ void k_elem_lin<1, true>(LinArgs, int);
-template __global__ void k_elem_lin<1, false>(LinArgs, int);
-template __global__ void k_elem_lin<2, true>(LinArgs, int);
-template __global__ void k_elem_lin<2, false>(LinArgs, int);
-template __global__ void k_elem_lin<4, true>(LinArgs, int);
-template __global__ void k_elem_lin<4, false>(LinArgs, int);
-template __global__ void k_elem_lin<1, true, true>(LinArgs, int);
-template __global__ void k_elem_lin<2, true, true>(LinArgs, int);
-template __global__ void k_elem_lin<4, true, true>(LinArgs, int);
+template __global__ void k_elem_lin<1, true>(LinArgs);
+template __global__ void k_elem_lin<1, false>(LinArgs);
+template __global__ void k_elem_lin<2, true>(LinArgs);
+template __global__ void k_elem_lin<2, false>(LinArgs);
+template __global__ void k_elem_lin<4, true>(LinArgs);
+template __global__ void k_elem_lin<4, false>(LinArgs);
+template __global__ void k_elem_lin<1, true, true>(LinArgs);
+template __global__ void k_elem_lin<2, true, true>(LinArgs);
+template __global__ void k_elem_lin<4, true, true>(LinArgs);
 
 // learned-preconditioner inputs per element (11): vertex offsets from the centroid
 // scaled by sqrt(area) (6), vertex-mean u, v, p (3), log(mu / (rho sqrt(area))),
