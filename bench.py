@@ -17,6 +17,9 @@ Also reported (same JSON line):
   (step throughput and the element kernel's rate);
 * ``tet_cns``: the tetrahedral cns path on the same mesh, p = 2 (step throughput and
   the element kernel's rate);
+* ``element_jacobian``: batched forward-difference element Jacobians of 32 768
+  distinct elements of the 512 x 512 p = 4 mesh (elements per second, K copied to
+  host included), with the oracle's rate on 1024 elements beside it;
 * ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual with the
   h+p multigrid cycle on the same 512 x 512 p = 4 mesh family at mu = 1e-2, for FP64
   plain, mixed, mixed + learned preconditioner and FP64 + learned (training time
@@ -291,6 +294,8 @@ def main() -> int:
     ap.add_argument("--no-tet", action="store_true")
     ap.add_argument("--tet-n", type=int, default=64)
     ap.add_argument("--tet-steps", type=int, default=3)
+    ap.add_argument("--no-jac", action="store_true")
+    ap.add_argument("--jac-batch", type=int, default=32768)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
@@ -368,6 +373,30 @@ def main() -> int:
             "fp64_plain_judge_after_steps": judge,
         }
         g.set_precision(_lib.HPG_FP64)
+    jac = None
+    if not args.no_jac:
+        # batched forward-difference element Jacobians (tri_hp::element_jacobian) of a
+        # random batch of distinct elements; the call returns K to host memory, so the
+        # rate includes the device-to-host copy of K
+        g.set_state(*st)
+        rng = np.random.default_rng(0)
+        batch = rng.choice(mesh.ntri, size=min(args.jac_batch, mesh.ntri), replace=False)
+        g.element_jacobian(batch[:64])
+        g.synchronize()
+        j0 = time.perf_counter()
+        K = g.element_jacobian(batch)
+        j1 = time.perf_counter()
+        jac = {
+            "workload": f"element_jacobian of {batch.size} distinct random elements of the {n}x{n} p=4 ins mesh, "
+                        "forward differences (eps_r = eps_a = 1e-8), K returned to host",
+            "elements": int(batch.size),
+            "N": int(K.shape[1]),
+            "seconds": j1 - j0,
+            "elements_per_s": batch.size / (j1 - j0),
+            "element_residual_evals_per_s": batch.size * (K.shape[1] + 1) / (j1 - j0),
+            "K_bytes": int(K.nbytes),
+        }
+        del K
     g.close()
 
     out = {
@@ -407,6 +436,8 @@ def main() -> int:
     }
     if mixed is not None:
         out["mixed"] = mixed
+    if jac is not None:
+        out["element_jacobian"] = jac
     if not args.no_tts:
         out["time_to_tol"] = time_to_tol(args)
     if not args.no_cns:
@@ -428,6 +459,15 @@ def main() -> int:
             o.setup_preconditioner()
             o.update()
         c1 = time.perf_counter()
+        if jac is not None:
+            # the oracle's element_jacobian beside it, on a bounded sample of the batch
+            jb = np.random.default_rng(0).choice(cm.ntri, size=1024, replace=False)
+            k0 = time.perf_counter()
+            for t in jb:
+                o.element_jacobian(int(t))
+            k1 = time.perf_counter()
+            out["element_jacobian"]["cpu_elements_per_s"] = jb.size / (k1 - k0)
+            out["element_jacobian"]["cpu_sample"] = f"oracle, 1 thread, {jb.size} elements, {k1 - k0:.2f} s"
         out["cpu_baseline"] = {
             "value": cunk * 5 * args.cpu_steps / (c1 - c0),
             "unit": "DOF-evals/s",
