@@ -113,16 +113,16 @@ def read_grd(path: str, essential=0b011) -> TriMesh:
         ebdry_seg=np.array(segs, dtype=np.int32),
         ebdry_essential=np.array(ess, dtype=np.int32),
         ebdry_names=[f"b{i}" for i in ids],
+        ebdry_ids=ids,
+        vbdry=vbdry,
     )
-    m.ebdry_ids = ids
-    m.vbdry = vbdry
     check_mesh(m)
     return m
 
 
 def write_grd(path: str, m: TriMesh) -> None:
     """Write ``m`` in the ``.grd`` layout read_grd (and tri_mesh::input) reads."""
-    ids = getattr(m, "ebdry_ids", list(range(1, m.nebd + 1)))
+    ids = m.ebdry_ids or list(range(1, m.nebd + 1))
     with open(path, "w") as f:
         f.write(f"npnt: {m.npnt} nseg: {m.nseg} ntri: {m.ntri}\n")
         for i, (x, y) in enumerate(m.pnts):
@@ -137,7 +137,7 @@ def write_grd(path: str, m: TriMesh) -> None:
             f.write(f"idnum: {ids[k]}\nnumber: {len(s)}\n")
             for j, sj in enumerate(s):
                 f.write(f"{j}: {sj}\n")
-        vb = getattr(m, "vbdry", [])
+        vb = m.vbdry
         f.write(f"nvbd: {len(vb)}\n")
         for vid, v in vb:
             f.write(f"idnum: {vid}\npoint: {v}\n")

@@ -31,6 +31,8 @@ class TriMesh:
     ebdry_seg: np.ndarray     # ordered boundary segments, counter-clockwise
     ebdry_essential: np.ndarray  # (nebd,) bit mask of essential variables
     ebdry_names: list = field(default_factory=list)
+    ebdry_ids: list = field(default_factory=list)   # boundary idnum of each edge boundary (.grd files)
+    vbdry: list = field(default_factory=list)       # (idnum, vertex) vertex boundaries (.grd files)
 
     @property
     def npnt(self) -> int:
