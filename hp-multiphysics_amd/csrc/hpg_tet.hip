@@ -4,6 +4,7 @@
 // and shared kernels (k_modes, k_absmax) it uses.
 #include "tet_kernels.hip"
 #include "tet_kernels_cns.hip"
+#include "tet_jacobian.hip"
 #include "tet_ops.h"
 
 namespace {
@@ -562,6 +563,91 @@ int hpg_tet_update(hpg_tet_ctx* c) {
     });
     return 0;
 }
+int hpg_tet_element_jacobian(hpg_tet_ctx* c, const int* tind, int nel, double eps_r, double eps_a, double* K) {
+    if (!c || nel < 0) return -2;
+    if (nel == 0) return 0;
+    if (!tind || !K) return -2;
+    HPG_TRY(c, {
+        const TetLevel& L = c->lv[c->log2p];
+        const int tm = L.p == 1 ? 4 : 10, nvv = c->nv, N = tm * nvv;
+        std::vector<int> tp((size_t)c->ntet * 4);
+        check(hipMemcpyAsync(tp.data(), c->tet_pnt, tp.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+        std::vector<char> seen(c->ntet, 0);
+        for (int j = 0; j < nel; ++j) {
+            if (tind[j] < 0 || tind[j] >= c->ntet) throw std::runtime_error("element index out of range");
+            if (seen[tind[j]]) throw std::runtime_error("element listed twice");
+            seen[tind[j]] = 1;
+        }
+        // greedy split of the batch into vertex-disjoint sets
+        std::vector<std::vector<int>> members;
+        for (int j = 0; j < nel; ++j) {
+            const int* v = &tp[4 * (size_t)tind[j]];
+            int k = 0;
+            for (;; ++k) {
+                if (k == (int)members.size()) members.emplace_back();
+                bool clash = false;
+                for (int q : members[k])
+                    for (int x = 0; x < 4 && !clash; ++x)
+                        for (int y = 0; y < 4; ++y)
+                            if (tp[4 * (size_t)tind[q] + x] == v[y]) { clash = true; break; }
+                if (!clash) break;
+            }
+            members[k].push_back(j);
+        }
+        std::vector<int> jel, jpos, off(1, 0);
+        for (auto& mset : members) {
+            for (int j : mset) { jel.push_back(tind[j]); jpos.push_back(j); }
+            off.push_back((int)jel.size());
+        }
+        std::vector<void*> bufs;
+        auto get = [&](size_t bytes) {
+            void* q = nullptr;
+            if (hipMalloc(&q, std::max<size_t>(bytes, 8)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+            bufs.push_back(q);
+            return q;
+        };
+        struct Release {
+            hpg_tet_ctx* c; std::vector<void*>& b;
+            ~Release() { hipStreamSynchronize(c->stream); for (void* q : b) hipFree(q); }
+        } release{c, bufs};
+        TetJacArgs a{};
+        a.jel = (const int*)get((size_t)nel * 4);
+        a.jpos = (const int*)get((size_t)nel * 4);
+        check(hipMemcpyAsync((void*)a.jel, jel.data(), (size_t)nel * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        check(hipMemcpyAsync((void*)a.jpos, jpos.data(), (size_t)nel * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.em0 = c->em0; a.tm = tm; a.nv = nvv;
+        a.ugv = c->ugv; a.uge = c->uge; a.lf = c->lf;
+        a.dw = (double*)get((size_t)nel * 8);
+        a.save = (double*)get((size_t)nel * 8);
+        a.base = (double*)get((size_t)nel * N * 8);
+        a.K = (double*)get((size_t)nel * N * N * 8);
+        a.eps_r = eps_r; a.eps_a = eps_a;
+        auto elem = [&]() { if (L.p == 1) launch_elem_tet<1>(c, 0); else launch_elem_tet<2>(c, 0); };
+        // dw and Rbar of every batch element from the unperturbed state
+        TetJacArgs all = a;
+        all.begin = 0; all.end = nel; all.mode = -1; all.var = 0;
+        hipLaunchKernelGGL(k_tjac_dw, dim3(nblk(nel, 64)), dim3(64), 0, c->stream, all);
+        elem();
+        hipLaunchKernelGGL(k_tjac_collect, dim3(nblk((long)nel * N)), dim3(256), 0, c->stream, all);
+        for (size_t k = 0; k + 1 < off.size(); ++k) {
+            TetJacArgs b = a;
+            b.begin = off[k]; b.end = off[k + 1];
+            const int ns = b.end - b.begin;
+            for (int col = 0; col < N; ++col) {
+                b.mode = col / nvv; b.var = col % nvv;
+                hipLaunchKernelGGL(k_tjac_perturb, dim3(nblk(ns, 64)), dim3(64), 0, c->stream, b);
+                elem();
+                hipLaunchKernelGGL(k_tjac_collect, dim3(nblk((long)ns * N)), dim3(256), 0, c->stream, b);
+            }
+        }
+        check(hipGetLastError(), "tet jacobian launch");
+        check(hipMemcpyAsync(K, a.K, (size_t)nel * N * N * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+    return 0;
+}
+
 int hpg_tet_maxres(hpg_tet_ctx* c, double* per_var, double* out) {
     if (!c || !out) return -2;
     HPG_TRY(c, {

@@ -35,7 +35,7 @@ EXPORTS = (
     "hpg_tet_set_residual", "hpg_tet_get_preconditioner", "hpg_tet_get_cns_blocks", "hpg_tet_rsdl", "hpg_tet_setup_preconditioner",
     "hpg_tet_minvrt", "hpg_tet_update", "hpg_tet_maxres", "hpg_tet_resnorm", "hpg_tet_mg_restrict",
     "hpg_tet_mg_prolongate", "hpg_tet_get_log2p", "hpg_tet_synchronize", "hpg_tet_last_error",
-    "hpg_tet_kernel_timing", "hpg_tet_kernel_timing_read",
+    "hpg_tet_kernel_timing", "hpg_tet_kernel_timing_read", "hpg_tet_element_jacobian",
 )
 
 
@@ -129,6 +129,7 @@ def load():
     L.hpg_set_defect_inner.argtypes = [vp, C.c_int]
     L.hpg_apply_jacobian.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.hpg_element_jacobian.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double, dp]
+    L.hpg_tet_element_jacobian.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double, dp]
     L.hpg_mlp_create.argtypes = [C.c_uint, C.POINTER(vp)]
     L.hpg_mlp_destroy.argtypes = [vp]
     L.hpg_mlp_num_params.argtypes = []

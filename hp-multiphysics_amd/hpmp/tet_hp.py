@@ -134,6 +134,19 @@ class _TetHp:
         self._check(self.L.hpg_tet_get_log2p(self.h, C.byref(v)), "get_log2p")
         return v.value
 
+    def element_jacobian(self, tinds, eps_r: float = 1e-6, eps_a: float = 1e-10):
+        """tet_hp::element_jacobian (tet_hp/update.cpp:134-178) for the distinct
+        tetrahedra ``tinds`` on the current level: array (len, N, N), N = tm * NV, with
+        K[i*NV + n, mode*NV + var] by forward differences (one dw for all variables,
+        the reference's eps_r = 1e-6, eps_a = 1e-10).  Needs setup_preconditioner first
+        (the element residual reads its tau)."""
+        t = np.ascontiguousarray(np.atleast_1d(tinds), dtype=np.int32)
+        n = (4 if self.log2p == 0 else 10) * self.NV
+        K = np.zeros((t.size, n, n))
+        self._check(self.L.hpg_tet_element_jacobian(self.h, t.ctypes.data_as(C.POINTER(C.c_int)), int(t.size),
+                                                    float(eps_r), float(eps_a), _lib.ptr(K)), "element_jacobian")
+        return K
+
     def synchronize(self):
         self._check(self.L.hpg_tet_synchronize(self.h), "synchronize")
 

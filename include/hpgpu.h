@@ -260,6 +260,14 @@ int hpg_tet_minvrt(hpg_tet_ctx *ctx);
  * (tet_hp/cns/update.cpp:18-143) */
 int hpg_tet_update(hpg_tet_ctx *ctx);
 /* replaces tet_hp::maxres (tet_hp/allocate.cpp:530-559) */
+/* Replaces tet_hp::element_jacobian (tet_hp/update.cpp:134-178) for a batch of `nel`
+ * distinct tetrahedra on the current level (ins or cns, p <= 2): forward differences
+ * of element_rsdl(tind, 0) with one dw for all variables, eps_r * (sum of |uht(n)(i)|
+ * over the four vertices and all variables) + eps_a (the reference's eps_r = 1e-6,
+ * eps_a = 1e-10).  K (host, nel x N x N, N = tm * NV) holds
+ * K(i*NV + n, mode*NV + var).  The state is left bit-identical; the residual arrays
+ * are untouched (the element records are overwritten). */
+int hpg_tet_element_jacobian(hpg_tet_ctx *ctx, const int *tind, int nel, double eps_r, double eps_a, double *K);
 int hpg_tet_maxres(hpg_tet_ctx *ctx, double *per_var, double *out);
 /* max |res| over vertex and edge modes, essential components excluded */
 int hpg_tet_resnorm(hpg_tet_ctx *ctx, double *out);

@@ -54,6 +54,7 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p]
         L.otet_maxres.argtypes = [C.c_void_p, _dp]
         L.otet_maxres.restype = C.c_double
+        L.otet_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.otet_log2p.argtypes = [C.c_void_p]
         L.otet_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.otet_face_rsdl.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -71,6 +72,7 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p]
         L.otetc_maxres.argtypes = [C.c_void_p, _dp]
         L.otetc_maxres.restype = C.c_double
+        L.otetc_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.otetc_log2p.argtypes = [C.c_void_p]
         L.otetc_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.otetc_face_rsdl.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -216,6 +218,13 @@ class TetHpIns:
         pv = np.zeros(4)
         return lib().otet_maxres(self.h, pv), pv
 
+    def element_jacobian(self, tind, eps_r=1e-6, eps_a=1e-10):
+        """tet_hp::element_jacobian (tet_hp/update.cpp:134-178), current level."""
+        tm = 4 if self.log2p == 0 else 10
+        K = np.zeros((tm * 4, tm * 4))
+        lib().otet_element_jacobian(self.h, int(tind), eps_r, eps_a, K)
+        return K
+
     def mg_restrict(self):
         lib().otet_mg_restrict(self.h)
 
@@ -320,6 +329,13 @@ class TetHpCns:
     def maxres(self):
         pv = np.zeros(5)
         return lib().otetc_maxres(self.h, pv), pv
+
+    def element_jacobian(self, tind, eps_r=1e-6, eps_a=1e-10):
+        """tet_hp::element_jacobian (tet_hp/update.cpp:134-178), current level."""
+        tm = 4 if self.log2p == 0 else 10
+        K = np.zeros((tm * 5, tm * 5))
+        lib().otetc_element_jacobian(self.h, int(tind), eps_r, eps_a, K)
+        return K
 
     def mg_restrict(self):
         lib().otetc_mg_restrict(self.h)

@@ -92,6 +92,7 @@ int otet_setup_preconditioner(void* h) { return X(h).setup_preconditioner(); }
 void otet_minvrt(void* h) { X(h).minvrt(); }
 void otet_update(void* h) { X(h).update(); }
 double otet_maxres(void* h, double* per_var) { return X(h).maxres(per_var); }
+void otet_element_jacobian(void* h, int t, double er, double ea, double* K) { X(h).element_jacobian(t, er, ea, K); }
 void otet_mg_restrict(void* h) { X(h).mg_restrict(); }
 void otet_mg_prolongate(void* h) { X(h).mg_prolongate(); }
 int otet_log2p(void* h) { return X(h).log2p; }
@@ -159,6 +160,7 @@ int otetc_setup_preconditioner(void* h) { return XC(h).setup_preconditioner(); }
 void otetc_minvrt(void* h) { XC(h).minvrt(); }
 void otetc_update(void* h) { XC(h).update(); }
 double otetc_maxres(void* h, double* per_var) { return XC(h).maxres(per_var); }
+void otetc_element_jacobian(void* h, int t, double er, double ea, double* K) { XC(h).element_jacobian(t, er, ea, K); }
 void otetc_mg_restrict(void* h) { XC(h).mg_restrict(); }
 void otetc_mg_prolongate(void* h) { XC(h).mg_prolongate(); }
 int otetc_log2p(void* h) { return XC(h).log2p; }

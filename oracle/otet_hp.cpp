@@ -460,6 +460,35 @@ void TetHp<NV_>::mg_prolongate() {
     if (log2p == prm.log2pmax) coarse_flag = false;
 }
 
+// tet_hp/update.cpp:134-178.  dw is one value for every variable: the reference adds
+// each |uht(n)(i)| of the four vertices to the whole dw array (a Blitz array plus a
+// scalar), then scales by eps_r and adds eps_a.
+template <int NV_>
+void TetHp<NV_>::element_jacobian(int tind, double eps_r, double eps_a, double* K) {
+    const int T = b().tm, N = T * NV;
+    double uht[NV][10], lre[NV][10], lim[NV][10], Rbar[NV][10];
+    ugtouht(tind, uht);
+    double dw = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int n = 0; n < NV; ++n) dw = dw + std::fabs(uht[n][i]);
+    dw = dw * eps_r;
+    dw = dw + eps_a;
+    element_rsdl(tind, 0, uht, lre, lim);
+    for (int i = 0; i < T; ++i)
+        for (int n = 0; n < NV; ++n) Rbar[n][i] = lre[n][i] + lim[n][i];
+    int kcol = 0;
+    for (int mode = 0; mode < T; ++mode)
+        for (int var = 0; var < NV; ++var) {
+            uht[var][mode] += dw;
+            element_rsdl(tind, 0, uht, lre, lim);
+            int krow = 0;
+            for (int i = 0; i < T; ++i)
+                for (int n = 0; n < NV; ++n) K[(krow++) * N + kcol] = (lre[n][i] + lim[n][i] - Rbar[n][i]) / dw;
+            ++kcol;
+            uht[var][mode] -= dw;
+        }
+}
+
 template class TetHp<4>;
 template class TetHp<5>;
 

@@ -1,7 +1,7 @@
 // Oracle (test infrastructure only): CPU restatement of tet_hp with the tet_hp_ins and
 // tet_hp_cns physics for straight tetrahedra, steady, p in {1, 2} with p-multigrid
 // 2 -> 1, inflow on every boundary face.
-// Generic part (TetHp<NV>): tet_hp/update.cpp:9-131 (rsdl), tet_hp/gtol.cpp:17-85,
+// Generic part (TetHp<NV>): tet_hp/update.cpp:9-131 (rsdl), :134-178 (element_jacobian), tet_hp/gtol.cpp:17-85,
 // 815-876 (ugtouht / lftog), tet_hp/allocate.cpp:530-559 (maxres),
 // tet_hp/mg_restrict.cpp:4-25, tet_hp/mg_prolongate.cpp:7-14.
 // tet_hp_ins (TetHpIns): tet_hp/update.cpp:181-417 (update),
@@ -77,6 +77,8 @@ class TetHp {
     virtual void minvrt() = 0;
     virtual void update() = 0;
     double maxres(double* per_var) const;
+    // tet_hp/update.cpp:134-178: forward-difference element Jacobian, K[(i*NV+n)*N + mode*NV+var]
+    void element_jacobian(int tind, double eps_r, double eps_a, double* K);
     void mg_restrict();
     void mg_prolongate();
 };

@@ -266,3 +266,44 @@ def test_nan_state_reports_setup_error_in_max_order(OT, oracle_mod):
         w[0][pos, 1] = np.nan
         Ot.set_state(*w)
         assert Ot.setup_preconditioner() == 1, pos
+
+
+@pytest.mark.parametrize("cns", [False, True])
+def test_tet_element_jacobian_matches_central_differences(cns):
+    """The forward-difference tet element Jacobian (tet_hp/update.cpp:134-178, one dw for
+    every variable) against central differences of element_rsdl built here from the
+    state, and its dw convention: with eps_r = 0 every column uses dw = eps_a."""
+    import oracle_tet as OT
+    from hpmp import tetmesh as TM
+    m = TM.unit_cube(2, 0.15, 3)
+    rng = np.random.default_rng(12)
+    if cns:
+        st = np.array([1.0, 0.3, 0.1, -0.05, 1.0])
+        X = OT.TetHpCns(m, log2p=1, mu=1e-2, bc=(0.0, *st[1:]), gamma=1.4, prandtl=0.72, R=1.0)
+    else:
+        st = np.array([1.0, 0.3, -0.2, 0.5])
+        X = OT.TetHpIns(m, log2p=1, mu=1e-2)
+    nv = st.size
+    v = np.tile(st, (m.npnt, 1)) + 0.02 * rng.standard_normal((m.npnt, nv))
+    e = 0.02 * rng.standard_normal((m.nseg, 1, nv))
+    X.set_state(v, e)
+    assert X.setup_preconditioner() == 0
+    t = 5
+    K = X.element_jacobian(t)
+    uht = np.zeros((nv, 10))
+    uht[:, :4] = v[m.tet_pnt[t]].T
+    uht[:, 4:] = e[m.tet_seg[t], 0, :].T   # edge mode 0 carries sgn^0 = 1
+    h = 1e-6
+    Kc = np.zeros_like(K)
+    for mode in range(10):
+        for var in range(nv):
+            up = uht.copy(); up[var, mode] += h
+            dn = uht.copy(); dn[var, mode] -= h
+            rp = sum(X.element_rsdl(t, 0, up)); rm = sum(X.element_rsdl(t, 0, dn))
+            Kc[:, mode * nv + var] = ((rp - rm) / (2 * h)).T.ravel()
+    assert np.abs(K - Kc).max() < 1e-4 * np.abs(Kc).max()
+    # dw = eps_r * sum |vertex coefficients| + eps_a, the same for all variables: a
+    # column formed with eps_r = 0 and eps_a = dw matches the default one exactly
+    dw = 1e-6 * np.abs(uht[:, :4]).T.sum() + 1e-10
+    K2 = X.element_jacobian(t, 0.0, dw)
+    assert np.abs(K2 - K).max() <= 1e-9 * np.abs(K).max()
