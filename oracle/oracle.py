@@ -79,6 +79,7 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.ocns_get_prcn.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ocns_rsdl.argtypes = [C.c_void_p, C.c_int]
+        L.ocns_set_vpc.argtypes = [C.c_void_p, _dp]
         L.ocns_setup_preconditioner.argtypes = [C.c_void_p]
         L.ocns_setup_preconditioner.restype = C.c_int
         for nm in ("ocns_minvrt", "ocns_update", "ocns_mg_restrict", "ocns_mg_prolongate",
@@ -389,6 +390,10 @@ class TriHpCns:
                np.zeros((m.npnt, 4, 4)), np.zeros((m.nseg, 4, 4)), np.zeros((m.ntri, 4, 4))]
         lib().ocns_get_prcn(self.h, *out)
         return out
+
+    def set_vpc(self, vpc):
+        """Overwrite the vertex 4x4 preconditioner blocks (npnt x 4 x 4, row-major)."""
+        lib().ocns_set_vpc(self.h, np.ascontiguousarray(vpc, dtype=np.float64).reshape(-1))
 
     def rsdl(self, stage=None):
         lib().ocns_rsdl(self.h, len(NSTAGE_ALPHA) if stage is None else stage)

@@ -208,6 +208,9 @@ void ocns_get_prcn(void* h, double* vp, double* sp, double* tp, double* vpc, dou
     auto& x = Cn(h);
     take(x.vprcn, vp); take(x.sprcn, sp); take(x.tprcn, tp); take(x.vpc, vpc); take(x.spc, spc); take(x.tpc, tpc);
 }
+// vertex 4x4 blocks (row-major), as left by setup_preconditioner; lets the projection of
+// project_new_variables be checked with chosen blocks
+void ocns_set_vpc(void* h, const double* vpc) { put(Cn(h).vpc, vpc); }
 void ocns_rsdl(void* h, int stage) { Cn(h).rsdl(stage); }
 int ocns_setup_preconditioner(void* h) { return Cn(h).setup_preconditioner(); }
 void ocns_minvrt(void* h) { Cn(h).minvrt(); }

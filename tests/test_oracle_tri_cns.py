@@ -222,3 +222,195 @@ def test_switch_variables_is_the_primitive_jacobian(oracle_mod):
         if bv[k]:
             ref[1:] = 0.0
         assert np.abs(out[k] - ref).max() < 1e-7 * max(1.0, np.abs(ref).max())
+
+
+def _dpdc_independent(w, gam):
+    """d(primitive)/d(conservative) at primitive state w = (p, u, v, RT), as the
+    inverse of the complex-step Jacobian (exact to rounding) of the primitive ->
+    conservative map (rho = p/RT, rho u, rho v, rho (RT/(gam-1) + |u|^2/2))."""
+    gm1 = gam - 1.0
+
+    def cons(x):
+        rho = x[0] / x[3]
+        return np.array([rho, rho * x[1], rho * x[2], rho * (x[3] / gm1 + 0.5 * (x[1] ** 2 + x[2] ** 2))])
+
+    h = 1e-30
+    w = np.asarray(w, dtype=complex)
+    J = np.stack([cons(w + 1j * h * e).imag / h for e in np.eye(4)], 1)
+    return np.linalg.inv(J)
+
+
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_project_new_variables_is_exact_for_polynomial_fields(oracle_mod, log2p):
+    """project_new_variables (tri_hp/cns/update.cpp:5-245) maps the residual field r(x)
+    to L(x)^-1 dpdc(u(x)) r(x): exactly at the vertices, by 1-D L2 projection on the
+    sides (vertex values held) and by 2-D L2 projection on the interior modes.  With a
+    uniform state and the same lower-triangular block L at every vertex the target is
+    C r(x), C = L^-1 dpdc, a polynomial of the element's own degree, so both
+    projections must reproduce it: every coefficient (vertex, side mode, interior mode)
+    becomes C times its old value.  A wrong sign of the odd side modes, a wrong mode
+    index, or a side / interior mass matrix that is not the exact one all fail this."""
+    p = 1 << log2p
+    m = _mesh(4, 0.25, 11, essential=0)   # no essential boundaries: no Dirichlet zeroing
+    w = np.array([1.3, 0.4, -0.25, 0.9])
+    T = _cns(oracle_mod, m, log2p, (1.0, 0.5, 0.0, 1.0))
+    st = [np.zeros(sh) for sh in T.shapes()]
+    st[0][:] = w
+    T.set_state(*st)
+    rng = np.random.default_rng(17 + log2p)
+    Lfull = rng.standard_normal((4, 4)) * 0.3 + np.diag([2.0, 1.5, 1.8, 2.2])
+    T.set_vpc(np.broadcast_to(Lfull, (m.npnt, 4, 4)).copy())   # only the lower triangle may be used
+    r = [rng.standard_normal(sh) for sh in T.shapes()]
+    T.set_res(*r)
+    T.project_new_variables()
+    out = T.get_res()
+    C = np.linalg.solve(np.tril(Lfull), _dpdc_independent(w, 1.4))
+    for a, b in zip(out, r):
+        if a.size:
+            ref = b @ C.T
+            assert np.abs(a - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
+def test_inflow_modify_boundary_residual_closed_form(oracle_mod, log2p):
+    """inflow::modify_boundary_residual (tri_hp/cns/bdry.cpp:247-340): with u, v, RT
+    prescribed, the conservative momentum and energy residuals follow from the mass
+    residual as rho u = u rho, rho v = v rho, rho E = (RT/(gam-1) + |u|^2/2) rho.  On
+    the inflow vertices that is pointwise; on the inflow sides the side modes are the
+    1-D L2 projection of b_n r_mass(x) with the vertex values held, which for a
+    constant boundary state is exactly b_n times the mass residual's side modes.
+    Nothing off the inflow boundary changes."""
+    p = 1 << log2p
+    gam = 1.4
+    m = _mesh(4, 0.25, 12)               # every boundary edge is an inflow edge
+    bc = (1.1, 0.6, -0.2, 0.8)
+    T = _cns(oracle_mod, m, log2p, bc, gamma=gam)
+    rng = np.random.default_rng(23 + log2p)
+    r = [rng.standard_normal(sh) for sh in T.shapes()]
+    T.set_res(*r)
+    T.modify_boundary_residual()
+    ov, os_, oi = T.get_res()
+    b = np.array([1.0, bc[1], bc[2], bc[3] / (gam - 1) + 0.5 * (bc[1] ** 2 + bc[2] ** 2)])
+    bv, bs = m.boundary_vertex_mask(), m.boundary_segment_mask()
+    ev = r[0].copy()
+    ev[bv, 1:] = r[0][bv, :1] * b[1:]
+    assert np.abs(ov - ev).max() < 1e-14 * np.abs(ev).max()
+    if p > 1:
+        es = r[1].copy()
+        es[bs, :, 1:] = r[1][bs, :, :1] * b[1:]
+        assert np.abs(os_ - es).max() < 1e-12 * np.abs(es).max()
+    if oi.size:
+        assert np.array_equal(oi, r[2])
+
+
+@pytest.mark.parametrize("mu", [2e-2, 1e-3])
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_residual_mesh_scaling_law(oracle_mod, log2p, mu):
+    """Scaling the mesh by s at fixed Reynolds and Peclet numbers (mu -> s mu, so kcond
+    -> s kcond) and unchanged state multiplies every residual coefficient by s
+    (fluxes times d(area)/d(length) = s), stabilisation included: the per-point tau
+    (Tinv = 2(|A| + |B| + h S)/h with S ~ mu/h^2) must scale as a length.  A wrong power
+    of h in Tinv, in S, in the low-Mach parameter b2 (live at mu = 1e-3, saturated at 1
+    at mu = 2e-2) or in the element size fails this; the constants do not."""
+    from hpmp import workloads as W
+    from hpmp.mesh import TriMesh
+    p = 1 << log2p
+    m = _mesh(4, 0.2, 13)
+    st, bc = W.cns_gaussian_state(m, p, seed=4, amp=0.05, width=0.2, noise=1e-2)
+    out = []
+    for s in (1.0, 3.0):
+        ms = TriMesh(**{**m.__dict__, "pnts": m.pnts * s})
+        T = _cns(oracle_mod, ms, log2p, bc, mu=mu * s)
+        T.set_state(*st)
+        T.rsdl()
+        out.append(T.get_res())
+    for a, b in zip(out[1], out[0]):
+        if a.size:
+            assert np.abs(a - 3.0 * b).max() < 1e-12 * np.abs(3.0 * b).max()
+
+
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_project_new_variables_brute_force(oracle_mod, log2p):
+    """project_new_variables (tri_hp/cns/update.cpp:5-245) on a non-uniform state with
+    a different block L at every vertex, against an independent restatement: the
+    target field L(x)^-1 dpdc(u(x)) r(x) (L interpolated linearly, only its lower
+    triangle used, dpdc by complex step) evaluated with the closed-form basis at the
+    basis' own Gauss points (Gauss-Legendre on sides and in xi, Gauss-Jacobi(1,0) in
+    eta, from scipy), then the discrete L2 projection onto the side modes (vertex
+    values held) and onto the interior modes (vertex and side values held).  It pins
+    the sign of the odd side modes in the element gather, the mode ordering, and
+    which vertex state and block the side and interior targets see."""
+    from scipy.special import eval_jacobi, roots_jacobi
+    from hpmp import workloads as W
+    from test_oracle_basis import _basis_closed_form
+    p = 1 << log2p
+    sm, im = p - 1, (p - 1) * (p - 2) // 2
+    gam = 1.4
+    m = _mesh(3, 0.25, 31, essential=0)
+    st, bc = W.cns_gaussian_state(m, p, seed=6, amp=0.1, width=0.3, noise=3e-2)
+    T = _cns(oracle_mod, m, log2p, bc, gamma=gam)
+    T.set_state(*st)
+    rng = np.random.default_rng(41 + log2p)
+    vpc = rng.standard_normal((m.npnt, 4, 4)) * 0.2 + np.diag([2.0, 1.5, 1.8, 2.2])
+    T.set_vpc(vpc)
+    r = [rng.standard_normal(sh) for sh in T.shapes()]
+    T.set_res(*r)
+    T.project_new_variables()
+    ov, os_, oi = T.get_res()
+
+    def dpdc(w):
+        return _dpdc_independent(w, gam)
+
+    def target(Lq, wq, rq):
+        return np.linalg.solve(np.tril(Lq), dpdc(wq) @ rq)
+
+    # vertices: pointwise
+    for k in range(m.npnt):
+        assert np.allclose(ov[k], target(vpc[k], st[0][k], r[0][k]), rtol=0, atol=1e-12 * np.abs(ov).max())
+
+    # sides: 1-D projection with the vertex values held
+    xs, ws = np.polynomial.legendre.leggauss(p + 1)
+
+    def side1d(k, t):
+        return 0.25 * (1 - t) * (1 + t) * eval_jacobi(k, 2, 2, t)
+
+    nrm = [1.0 / np.sqrt((ws * side1d(k, xs) ** 2).sum()) for k in range(sm)]
+    psi = np.array([side1d(k, xs) * nrm[k] for k in range(sm)])          # sm x q
+    l0, l1 = 0.5 * (1 - xs), 0.5 * (1 + xs)
+    Mss = (psi * ws) @ psi.T
+    for sd in range(m.nseg):
+        a, b = m.seg_pnt[sd]
+        u = np.outer(l0, st[0][a]) + np.outer(l1, st[0][b]) + psi.T @ st[1][sd]
+        rr = np.outer(l0, r[0][a]) + np.outer(l1, r[0][b]) + psi.T @ r[1][sd]
+        Lq = l0[:, None, None] * vpc[a] + l1[:, None, None] * vpc[b]
+        tq = np.array([target(Lq[q], u[q], rr[q]) for q in range(p + 1)])
+        held = np.outer(l0, ov[a]) + np.outer(l1, ov[b])
+        c = np.linalg.solve(Mss, (psi * ws) @ (tq - held))
+        assert np.abs(os_[sd] - c).max() < 1e-12 * np.abs(os_).max()
+
+    if im == 0:
+        return
+    # interior: 2-D projection with vertex and side values held
+    xg, wx = np.polynomial.legendre.leggauss(p + 1)
+    eg, we = roots_jacobi(p + 1, 1.0, 0.0)
+    X, E = np.meshgrid(xg, eg, indexing="ij")
+    Wq = np.outer(wx, 0.5 * we).ravel()
+    phi = _basis_closed_form(p, X.ravel(), E.ravel())                     # tm x q
+    nb = 3 + 3 * sm
+    MII = (phi[nb:] * Wq) @ phi[nb:].T
+    for t in range(m.ntri):
+        vs, sg, ss = m.tri_pnt[t], m.tri_sgn[t], m.tri_seg[t]
+
+        def gather(v, s, i):
+            rows = [v[vs[k]] for k in range(3)]
+            for k in range(3):
+                rows += [float(sg[k]) ** mm * s[ss[k], mm] for mm in range(sm)]
+            return np.array(rows + [i[t, k] for k in range(im)])
+
+        u = phi.T @ gather(*st)
+        rr = phi.T @ gather(*r)
+        Lq = np.einsum("kq,kab->qab", phi[:3], vpc[vs])
+        tq = np.array([target(Lq[q], u[q], rr[q]) for q in range(phi.shape[1])])
+        held = phi[:nb].T @ gather(ov, os_, np.zeros_like(oi))[:nb]
+        c = np.linalg.solve(MII, (phi[nb:] * Wq) @ (tq - held))
+        assert np.abs(oi[t] - c).max() < 1e-11 * np.abs(oi).max()

@@ -276,3 +276,130 @@ def test_element_jacobian_matches_central_differences(oracle_mod):
             rp = sum(T.element_rsdl(4, 0, up)); rm = sum(T.element_rsdl(4, 0, dn))
             Kc[:, mode * 3 + var] = ((rp - rm) / (2 * h)).T.ravel()
     assert np.abs(K - Kc).max() < 1e-5 * np.abs(Kc).max()
+
+
+def _scaled(m, s):
+    from hpmp.mesh import TriMesh
+    return TriMesh(**{**m.__dict__, "pnts": m.pnts * s})
+
+
+def _scaled_state(st, c, k=1.0):
+    """velocity x c, pressure x k c^2 (coefficient arrays; the basis is linear)."""
+    out = []
+    for a in st:
+        b = a.copy()
+        if b.size:
+            b[..., :2] *= c
+            b[..., 2] *= k * c * c
+        out.append(b)
+    return out
+
+
+def _rsdl(oracle_mod, m, st, **kw):
+    T = oracle_mod.TriHpIns(m, **kw)
+    T.set_state(*st)
+    T.rsdl()
+    return T.get_res()
+
+
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_residual_mesh_scaling_law(oracle_mod, log2p):
+    """x -> s x with the Reynolds number held (mu -> s mu) and the state unchanged
+    multiplies every residual coefficient by s: Galerkin fluxes carry d(area)/d(length),
+    and the stabilisation (tri_hp/ins/rsdl.cpp:377-395) only keeps that if tau scales as
+    length/velocity, i.e. h/(cjcb sqrt(3q + (2 nu/h)^2)) with h a length and cjcb an
+    area.  A tau built on h^2, on area instead of cjcb, or on nu/h^2 fails; the
+    constants 3, 2 and 0.25(p+1)^2 do not (they stay unpinned)."""
+    from hpmp import workloads as W
+    m = _mesh(5, 0.2, 21)
+    st = W.fourier_state(m, 1 << log2p, seed=5)
+    r1 = _rsdl(oracle_mod, m, st, log2p=log2p, mu=3e-2)
+    r3 = _rsdl(oracle_mod, _scaled(m, 3.0), st, log2p=log2p, mu=9e-2)
+    for a, b in zip(r3, r1):
+        if a.size:
+            assert np.abs(a - 3.0 * b).max() < 1e-12 * np.abs(3.0 * b).max()
+
+
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_residual_density_scaling_law(oracle_mod, log2p):
+    """rho -> k rho, mu -> k mu, p -> k p at fixed velocity multiplies every residual
+    row by k, stabilisation included: tau depends on the kinematic viscosity
+    nu = mu/rho only.  A tau on mu instead of nu fails."""
+    from hpmp import workloads as W
+    m = _mesh(5, 0.2, 22)
+    st = W.fourier_state(m, 1 << log2p, seed=6)
+    r1 = _rsdl(oracle_mod, m, st, log2p=log2p, rho=1.0, mu=2e-2)
+    r2 = _rsdl(oracle_mod, m, _scaled_state(st, 1.0, 2.5), log2p=log2p, rho=2.5, mu=5e-2)
+    for a, b in zip(r2, r1):
+        if a.size:
+            assert np.abs(a - 2.5 * b).max() < 1e-12 * np.abs(2.5 * b).max()
+
+
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_residual_velocity_scaling_law(oracle_mod, log2p):
+    """u -> c u, p -> c^2 p, mu -> c mu on a fixed mesh.  The continuity residual
+    (Galerkin and the tau_0 pressure stabilisation) scales as c.  Momentum splits
+    into c^2 (fluxes and the tau_0 = h/(cjcb sqrt(gam)) streamline term) plus c times
+    the grad-div term, whose tau = sqrt(q) tau_0 (tri_hp/ins/rsdl.cpp:392) is
+    velocity-independent.  So R_mom(c) = c^2 A + c B for three values of c, with B != 0
+    when the state has divergence and B = 0 when the discrete divergence vanishes
+    (a linear solenoidal field).  tau = q tau_0 would put the grad-div term into c^2
+    (B = 0 always); tau independent of q would put it into c^0."""
+    from hpmp import workloads as W
+    p = 1 << log2p
+    m = _mesh(5, 0.2, 23, essential=0)
+    kw = dict(log2p=log2p, bc_pressure=0.0)
+    st = W.fourier_state(m, p, seed=7)
+    R = {c: _rsdl(oracle_mod, m, _scaled_state(st, c), mu=2e-2 * c, **kw) for c in (1.0, 2.0, 3.0)}
+    for k in range(3):
+        if not R[1.0][k].size:
+            continue
+        r1, r2, r3 = (R[c][k] for c in (1.0, 2.0, 3.0))
+        scale = np.abs(r3).max()
+        assert np.abs(r3[..., 2] - 3.0 * r1[..., 2]).max() < 1e-12 * scale
+        B = (4.0 * r1[..., :2] - r2[..., :2]) / 2.0
+        A = r1[..., :2] - B
+        assert np.abs(r3[..., :2] - (9.0 * A + 3.0 * B)).max() < 1e-11 * scale
+        if k == 0:
+            assert np.abs(B).max() > 1e-6 * scale
+    # linear solenoidal velocity: u = (a + b x + c y, d + e x - b y), linear pressure
+    from hpmp.workloads import mode_orders  # noqa: F401
+    v = np.zeros((m.npnt, 3))
+    x, y = m.pnts[:, 0], m.pnts[:, 1]
+    v[:, 0] = 0.7 + 0.3 * x - 0.4 * y
+    v[:, 1] = -0.2 + 0.5 * x - 0.3 * y
+    v[:, 2] = 0.1 * x + 0.2 * y
+    lin = [v, np.zeros((m.nseg, p - 1, 3)), np.zeros((m.ntri, (p - 1) * (p - 2) // 2, 3))]
+    R = {c: _rsdl(oracle_mod, m, _scaled_state(lin, c), mu=2e-2 * c, **kw) for c in (1.0, 3.0)}
+    for a, b in zip(R[3.0], R[1.0]):
+        if a.size:
+            scale = np.abs(a).max()
+            assert np.abs(a[..., :2] - 9.0 * b[..., :2]).max() < 1e-12 * scale
+
+
+@pytest.mark.parametrize("log2p", [1, 2])
+def test_setup_preconditioner_scaling_laws(oracle_mod, log2p):
+    """tprcn of tri_hp_ins::setup_preconditioner (tri_hp/ins/setup_preconditioner.cpp:
+    166-204) is rho * area * (viscous rate + wave rate) for u, v and that over gam for p:
+    under x -> s x, mu -> s mu it scales as s (area s^2 times rates 1/s); under
+    u -> c u, mu -> c mu as (c, c, 1/c); under rho -> k rho, mu -> k mu as (k, k, 1)."""
+    from hpmp import workloads as W
+    p = 1 << log2p
+    m = _mesh(5, 0.2, 24)
+    st = W.fourier_state(m, p, seed=8)
+
+    def tp(mesh, state, **kw):
+        T = oracle_mod.TriHpIns(mesh, log2p=log2p, **kw)
+        T.set_state(*state)
+        assert T.setup_preconditioner() == 0
+        return T.get_prcn()[2]
+
+    t1 = tp(m, st, mu=2e-2)
+    ts = tp(_scaled(m, 2.0), st, mu=4e-2)
+    assert np.abs(ts - 2.0 * t1).max() < 1e-12 * np.abs(ts).max()
+    tc = tp(m, _scaled_state(st, 3.0), mu=6e-2)
+    ref = t1 * np.array([3.0, 3.0, 1.0 / 3.0])
+    assert np.abs(tc - ref).max() < 1e-12 * np.abs(ref).max()
+    tk = tp(m, _scaled_state(st, 1.0, 2.0), rho=2.0, mu=4e-2)
+    ref = t1 * np.array([2.0, 2.0, 1.0])
+    assert np.abs(tk - ref).max() < 1e-12 * np.abs(ref).max()
