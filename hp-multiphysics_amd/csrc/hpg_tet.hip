@@ -114,14 +114,15 @@ struct hpg_tet_ctx {
 
 namespace {
 
+// elist: optional device list of `count` elements to evaluate (default: all ntet)
 template <int P>
-void launch_elem_tetc(hpg_tet_ctx* c, double beta) {
+void launch_elem_tetc(hpg_tet_ctx* c, double beta, const int* elist, int count) {
     TetcElemArgs a;
     a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg;
-    a.ntet = c->ntet; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
+    a.ntet = count; a.elist = elist; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
     a.ops = c->lv[c->log2p].ops; a.mu = c->prm.mu; a.kcond = c->kcond; a.R = c->prm.Rgas; a.gamma = c->prm.gamma;
     a.atm = 0.0; a.beta = beta;
-    const int noct = (c->ntet + 7) / 8;
+    const int noct = (count + 7) / 8;
     if (beta > 0.0) {
         static const int cap = resident_blocks((const void*)k_elem_tetc<P, true>);
         hipLaunchKernelGGL((k_elem_tetc<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
@@ -132,24 +133,25 @@ void launch_elem_tetc(hpg_tet_ctx* c, double beta) {
 }
 
 template <int P>
-void launch_elem_tet(hpg_tet_ctx* c, int stage) {
+void launch_elem_tet(hpg_tet_ctx* c, int stage, const int* elist = nullptr, int count = -1) {
+    if (count < 0) count = c->ntet;
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     if (c->cns()) {
         const size_t r = c->recs.size();
         if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
-        launch_elem_tetc<P>(c, beta);
+        launch_elem_tetc<P>(c, beta, elist, count);
         if (c->timing) {
             check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
-            c->recs.push_back({beta > 0.0 ? 1 : 0, c->ntet});
+            c->recs.push_back({beta > 0.0 ? 1 : 0, count});
         }
         return;
     }
     TetElemArgs a;
     a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.tet_sgn = nullptr;
-    a.ntet = c->ntet; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
+    a.ntet = count; a.elist = elist; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
     a.ops = c->lv[c->log2p].ops; a.rho = c->prm.rho; a.mu = c->prm.mu; a.beta = beta;
-    const int noct = (c->ntet + 7) / 8;
+    const int noct = (count + 7) / 8;
     const size_t r = c->recs.size();
     if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
     if (beta > 0.0) {
@@ -161,7 +163,7 @@ void launch_elem_tet(hpg_tet_ctx* c, int stage) {
     }
     if (c->timing) {
         check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
-        c->recs.push_back({beta > 0.0 ? 1 : 0, c->ntet});
+        c->recs.push_back({beta > 0.0 ? 1 : 0, count});
     }
 }
 
@@ -579,27 +581,14 @@ int hpg_tet_element_jacobian(hpg_tet_ctx* c, const int* tind, int nel, double ep
             if (seen[tind[j]]) throw std::runtime_error("element listed twice");
             seen[tind[j]] = 1;
         }
-        // greedy split of the batch into vertex-disjoint sets
-        std::vector<std::vector<int>> members;
-        for (int j = 0; j < nel; ++j) {
-            const int* v = &tp[4 * (size_t)tind[j]];
-            int k = 0;
-            for (;; ++k) {
-                if (k == (int)members.size()) members.emplace_back();
-                bool clash = false;
-                for (int q : members[k])
-                    for (int x = 0; x < 4 && !clash; ++x)
-                        for (int y = 0; y < 4; ++y)
-                            if (tp[4 * (size_t)tind[q] + x] == v[y]) { clash = true; break; }
-                if (!clash) break;
-            }
-            members[k].push_back(j);
-        }
-        std::vector<int> jel, jpos, off(1, 0);
-        for (auto& mset : members) {
-            for (int j : mset) { jel.push_back(tind[j]); jpos.push_back(j); }
-            off.push_back((int)jel.size());
-        }
+        // split of the batch into vertex-disjoint sets: the greedy colouring of its
+        // vertex lists (at p <= 2 elements sharing no vertex share no edge either)
+        std::vector<int> bv((size_t)nel * 4), order, off, colr;
+        for (int j = 0; j < nel; ++j)
+            for (int k = 0; k < 4; ++k) bv[4 * (size_t)j + k] = tp[4 * (size_t)tind[j] + k];
+        colour(nel, 4, bv.data(), c->npnt, order, off, colr);
+        std::vector<int> jel(nel), jpos(nel);
+        for (int i = 0; i < nel; ++i) { jel[i] = tind[order[i]]; jpos[i] = order[i]; }
         std::vector<void*> bufs;
         auto get = [&](size_t bytes) {
             void* q = nullptr;
@@ -623,12 +612,16 @@ int hpg_tet_element_jacobian(hpg_tet_ctx* c, const int* tind, int nel, double ep
         a.base = (double*)get((size_t)nel * N * 8);
         a.K = (double*)get((size_t)nel * N * N * 8);
         a.eps_r = eps_r; a.eps_a = eps_a;
-        auto elem = [&]() { if (L.p == 1) launch_elem_tet<1>(c, 0); else launch_elem_tet<2>(c, 0); };
+        // element residuals of batch positions [b0, b1) only
+        auto elem = [&](int b0, int b1) {
+            if (L.p == 1) launch_elem_tet<1>(c, 0, a.jel + b0, b1 - b0);
+            else launch_elem_tet<2>(c, 0, a.jel + b0, b1 - b0);
+        };
         // dw and Rbar of every batch element from the unperturbed state
         TetJacArgs all = a;
         all.begin = 0; all.end = nel; all.mode = -1; all.var = 0;
         hipLaunchKernelGGL(k_tjac_dw, dim3(nblk(nel, 64)), dim3(64), 0, c->stream, all);
-        elem();
+        elem(0, nel);
         hipLaunchKernelGGL(k_tjac_collect, dim3(nblk((long)nel * N)), dim3(256), 0, c->stream, all);
         for (size_t k = 0; k + 1 < off.size(); ++k) {
             TetJacArgs b = a;
@@ -637,7 +630,7 @@ int hpg_tet_element_jacobian(hpg_tet_ctx* c, const int* tind, int nel, double ep
             for (int col = 0; col < N; ++col) {
                 b.mode = col / nvv; b.var = col % nvv;
                 hipLaunchKernelGGL(k_tjac_perturb, dim3(nblk(ns, 64)), dim3(64), 0, c->stream, b);
-                elem();
+                elem(b.begin, b.end);
                 hipLaunchKernelGGL(k_tjac_collect, dim3(nblk((long)ns * N)), dim3(256), 0, c->stream, b);
             }
         }

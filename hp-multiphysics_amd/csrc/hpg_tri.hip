@@ -72,22 +72,25 @@ std::vector<double> build_pack(const TriOps& o, RowFn row) {
     return pk;
 }
 
-// Greedy colouring: items whose vertex sets intersect get different colours.
-// Returns the item order grouped by colour (ascending inside a colour) and offsets.
+// Greedy colouring: items whose vertex sets intersect get different colours (at most
+// 128, a bit mask of two words per vertex).  Returns the item order grouped by colour
+// (ascending inside a colour) and offsets.  O(nitem * per).
 void colour(int nitem, int per, const int* verts, int npnt, std::vector<int>& order, std::vector<int>& off,
             std::vector<int>& col) {
-    std::vector<uint64_t> used(npnt, 0);
+    constexpr int WORDS = 2;
+    std::vector<uint64_t> used((size_t)WORDS * npnt, 0);
     col.assign(nitem, 0);
     int ncol = 0;
     for (int t = 0; t < nitem; ++t) {
-        uint64_t m = 0;
-        for (int k = 0; k < per; ++k) m |= used[verts[per * t + k]];
+        uint64_t m[WORDS] = {0, 0};
+        for (int k = 0; k < per; ++k)
+            for (int w = 0; w < WORDS; ++w) m[w] |= used[(size_t)WORDS * verts[per * t + k] + w];
         int c = 0;
-        while (c < 64 && ((m >> c) & 1)) ++c;
-        if (c == 64) throw std::runtime_error("colouring needs more than 64 colours");
+        while (c < 64 * WORDS && ((m[c >> 6] >> (c & 63)) & 1)) ++c;
+        if (c == 64 * WORDS) throw std::runtime_error("colouring needs more than 128 colours");
         col[t] = c;
         ncol = std::max(ncol, c + 1);
-        for (int k = 0; k < per; ++k) used[verts[per * t + k]] |= (uint64_t(1) << c);
+        for (int k = 0; k < per; ++k) used[(size_t)WORDS * verts[per * t + k] + (c >> 6)] |= (uint64_t(1) << (c & 63));
     }
     off.assign(ncol + 1, 0);
     for (int t = 0; t < nitem; ++t) off[col[t] + 1]++;
@@ -842,11 +845,15 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         }
         for (long k = 0; k < 2L * ns; ++k)
             if (m->seg_pnt[k] < 0 || m->seg_pnt[k] >= np) throw std::runtime_error("seg_pnt out of range");
+        // a mesh without edge boundaries may pass nebd = 0 and null boundary arrays
+        int neseg = 0;
         if (m->nebd > 0) {
             if (!m->ebdry_start || !m->ebdry_essential || m->ebdry_start[0] != 0) throw std::runtime_error("ebdry_start");
             for (int b = 0; b < m->nebd; ++b)
                 if (m->ebdry_start[b + 1] < m->ebdry_start[b]) throw std::runtime_error("ebdry_start not ascending");
-            for (int k = 0; k < m->ebdry_start[m->nebd]; ++k)
+            neseg = m->ebdry_start[m->nebd];
+            if (neseg > 0 && !m->ebdry_seg) throw std::runtime_error("null ebdry_seg");
+            for (int k = 0; k < neseg; ++k)
                 if (m->ebdry_seg[k] < 0 || m->ebdry_seg[k] >= ns) throw std::runtime_error("ebdry_seg out of range");
         }
 
@@ -871,7 +878,6 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->order = c->upload(order.data(), order.size());
         c->coff = off;
         c->tcol = tcol;
-        const int neseg = m->ebdry_start[m->nebd];
         std::vector<int> ev(2 * (size_t)neseg);
         for (int j = 0; j < neseg; ++j) {
             ev[2 * j] = m->seg_pnt[2 * m->ebdry_seg[j]];

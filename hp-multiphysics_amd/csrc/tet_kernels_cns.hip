@@ -357,7 +357,8 @@ struct TetcElemArgs {
     const double* pnts;
     const int* tet_pnt;
     const int* tet_seg;
-    int ntet, em0;
+    int ntet, em0;      // ntet: number of elements to evaluate (of elist when set)
+    const int* elist;   // optional list of elements; null: elements 0 .. ntet - 1
     const double* ugv;
     const double* uge;
     const double* tau;  // ntet x 25
@@ -414,9 +415,9 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
     __syncthreads();  // operators staged
     HPG_OCTET_LOOP(noct, oc) {
         wave_lds_sync();  // previous octet done with this half-wave's LDS rows
-        const int t = oc * 8 + hw;
-        const bool valid = t < a.ntet;
-        const int tt = valid ? t : 0;
+        const int slot = oc * 8 + hw;
+        const bool valid = slot < a.ntet;
+        const int tt = valid ? (a.elist ? a.elist[slot] : slot) : 0;
         int vtx[4], seg[6];
 #pragma unroll
         for (int k = 0; k < 4; ++k) vtx[k] = a.tet_pnt[4 * tt + k];

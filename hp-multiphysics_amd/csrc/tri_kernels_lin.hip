@@ -18,12 +18,9 @@ __device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Accumulator row held by register i of lane group q for v_mfma_f32_16x16x4f32
-// (checked on the device by hpg_selftest_mfma_f32).
-#ifndef HPG_F32_ROW_4I_PLUS_Q
-#define HPG_F32_ROW_4I_PLUS_Q 0
-#endif
-__host__ __device__ constexpr int f32_row(int i, int q) { return HPG_F32_ROW_4I_PLUS_Q ? 4 * i + q : 4 * q + i; }
+// Accumulator row held by register i of lane group q for v_mfma_f32_16x16x4f32:
+// 4q + i (checked on the device by hpg_selftest_mfma_f32).
+__host__ __device__ constexpr int f32_row(int i, int q) { return 4 * q + i; }
 
 template <int P, bool FULL, bool PROBE = false>
 __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a) {

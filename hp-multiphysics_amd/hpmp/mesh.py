@@ -213,25 +213,32 @@ def connect(fine: TriMesh, n_fine: int, coarse: TriMesh):
     return cc_tri, cc_wt, fc_tri, fc_wt
 
 
+def _require(ok, what: str) -> None:
+    if not bool(np.all(ok)):
+        raise ValueError(f"inconsistent mesh: {what}")
+
+
 def check_mesh(m: TriMesh) -> None:
-    """Structural invariants every mesh handed to the solver must satisfy."""
-    assert np.all(m.area2() > 0.0), "triangles must be counter-clockwise"
+    """Structural invariants every mesh handed to the solver must satisfy; raises
+    ValueError naming the first one that fails (the C-ABI checks index ranges only)."""
+    _require(m.area2() > 0.0, "triangles must be counter-clockwise")
     # every segment used by one (boundary) or two (interior) triangles, opposite signs
     cnt = np.bincount(m.tri_seg.ravel(), minlength=m.nseg)
-    assert np.all((cnt == 1) | (cnt == 2))
+    _require((cnt == 1) | (cnt == 2), "a segment is used by no triangle or by more than two")
     bmask = m.boundary_segment_mask()
-    assert np.all(cnt[bmask] == 1) and np.all(cnt[~bmask] == 2)
+    _require(cnt[bmask] == 1, "a boundary segment is shared by two triangles")
+    _require(cnt[~bmask] == 2, "an interior segment belongs to one triangle only")
     for k in range(3):
         s = m.tri_seg[:, k]
         a = m.tri_pnt[:, (k + 1) % 3]
         b = m.tri_pnt[:, (k + 2) % 3]
         fwd = (m.seg_pnt[s, 0] == a) & (m.seg_pnt[s, 1] == b)
         rev = (m.seg_pnt[s, 0] == b) & (m.seg_pnt[s, 1] == a)
-        assert np.all(fwd | rev)
-        assert np.all(np.where(fwd, 1, -1) == m.tri_sgn[:, k])
+        _require(fwd | rev, f"side {k} of a triangle does not join its vertices")
+        _require(np.where(fwd, 1, -1) == m.tri_sgn[:, k], f"tri_sgn of side {k} disagrees with seg_pnt")
     # boundary segments owned with sgn = +1 (domain on the left)
     owner_sgn = np.zeros(m.nseg, dtype=np.int64)
     for k in range(3):
         np.add.at(owner_sgn, m.tri_seg[:, k], m.tri_sgn[:, k])
-    assert np.all(owner_sgn[bmask] == 1)
-    assert np.all(owner_sgn[~bmask] == 0)
+    _require(owner_sgn[bmask] == 1, "a boundary segment is not stored counter-clockwise")
+    _require(owner_sgn[~bmask] == 0, "an interior segment has equal signs in both triangles")

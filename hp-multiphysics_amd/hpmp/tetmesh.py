@@ -135,15 +135,21 @@ def unit_cube(n: int, jitter: float = 0.0, seed: int = 0) -> TetMesh:
 
 def check_mesh(m: TetMesh) -> None:
     """Invariants the reference relies on: positive orientation, consistent edge
-    signs, outward boundary-face ordering, closed boundary."""
-    assert np.all(m.volumes() > 0)
+    signs, outward boundary-face ordering, closed boundary.  Raises ValueError naming
+    the first one that fails."""
+    def require(ok, what):
+        if not bool(np.all(ok)):
+            raise ValueError(f"inconsistent tet mesh: {what}")
+
+    require(m.volumes() > 0, "tetrahedra must be positively oriented")
     ends = m.tet_pnt[:, VS]
     sp = m.seg_pnt[m.tet_seg]
     fwd = (ends[:, :, 0] == sp[:, :, 0]) & (ends[:, :, 1] == sp[:, :, 1])
     bwd = (ends[:, :, 0] == sp[:, :, 1]) & (ends[:, :, 1] == sp[:, :, 0])
-    assert np.all(fwd | bwd) and np.all(np.where(fwd, 1, -1) == m.tet_sgn)
+    require(fwd | bwd, "an edge of a tet does not join its vertices")
+    require(np.where(fwd, 1, -1) == m.tet_sgn, "tet_sgn disagrees with seg_pnt")
     p = m.pnts[m.bf_pnt]
     nrm = np.cross(p[:, 2] - p[:, 1], p[:, 0] - p[:, 1])
     c = p.mean(axis=1) - m.pnts.mean(axis=0)
-    assert np.all(np.einsum("ij,ij->i", nrm, c) > 0)
-    assert np.allclose(nrm.sum(axis=0), 0.0, atol=1e-12)
+    require(np.einsum("ij,ij->i", nrm, c) > 0, "a boundary face is not ordered outward")
+    require(np.allclose(nrm.sum(axis=0), 0.0, atol=1e-12), "the boundary is not closed")

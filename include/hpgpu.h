@@ -145,7 +145,9 @@ int hpg_apply_jacobian(hpg_ctx *ctx, const double *dv, const double *ds, const d
  * tri_hp/hp_boundary.h:19-23).  K (host, nel x N x N, N = tm * NV, row-major) holds
  * K(i*NV + n, mode*NV + var) = d(lf_re + lf_im)(n)(i) / d uht(var)(mode).  The state
  * is left bit-identical; the residual arrays are untouched.  Returns -1 with an
- * error for an index out of range or listed twice. */
+ * error for an index out of range or listed twice.  After any other negative return
+ * (a HIP failure in the middle of the column loop) the state may still hold one
+ * perturbed coefficient per batch element: set it again before further use. */
 int hpg_element_jacobian(hpg_ctx *ctx, const int *tind, int nel, double eps_r, double eps_a, double *K);
 
 /* Learned preconditioner (no reference counterpart).  An MLP 11 -> 64 -> 64 -> 64 -> 3
@@ -259,15 +261,19 @@ int hpg_tet_minvrt(hpg_tet_ctx *ctx);
 /* replaces tet_hp::update (tet_hp/update.cpp:181-417) or tet_hp_cns::update
  * (tet_hp/cns/update.cpp:18-143) */
 int hpg_tet_update(hpg_tet_ctx *ctx);
-/* replaces tet_hp::maxres (tet_hp/allocate.cpp:530-559) */
 /* Replaces tet_hp::element_jacobian (tet_hp/update.cpp:134-178) for a batch of `nel`
  * distinct tetrahedra on the current level (ins or cns, p <= 2): forward differences
  * of element_rsdl(tind, 0) with one dw for all variables, eps_r * (sum of |uht(n)(i)|
  * over the four vertices and all variables) + eps_a (the reference's eps_r = 1e-6,
  * eps_a = 1e-10).  K (host, nel x N x N, N = tm * NV) holds
  * K(i*NV + n, mode*NV + var).  The state is left bit-identical; the residual arrays
- * are untouched (the element records are overwritten). */
+ * are untouched (the element records are overwritten).  The batch is split into
+ * vertex-disjoint sets by colouring; each column evaluates only the set's elements,
+ * so every batch element is evaluated tm * NV + 1 times, whatever the mesh size.  After a
+ * negative return other than a rejected batch the state may hold perturbed
+ * coefficients: set it again before further use. */
 int hpg_tet_element_jacobian(hpg_tet_ctx *ctx, const int *tind, int nel, double eps_r, double eps_a, double *K);
+/* replaces tet_hp::maxres (tet_hp/allocate.cpp:530-559) */
 int hpg_tet_maxres(hpg_tet_ctx *ctx, double *per_var, double *out);
 /* max |res| over vertex and edge modes, essential components excluded */
 int hpg_tet_resnorm(hpg_tet_ctx *ctx, double *out);

@@ -101,6 +101,35 @@ def test_out_of_range_indices_are_rejected():
         TetHpIns(t, log2p=1)
 
 
+def test_mesh_without_edge_boundaries_and_null_arrays():
+    """nebd = 0 with NULL boundary arrays is a valid mesh (no edge boundaries); a
+    non-empty boundary with a NULL segment list is rejected on the host."""
+    _case("nebd = 0 with null boundary arrays")
+    import ctypes as C
+    from hpmp import _lib, mesh as M
+    L = _lib.load()
+    m = M.unit_square(2, 0.0, 0)
+    k = [np.ascontiguousarray(a) for a in (m.pnts.astype(np.float64), m.seg_pnt.astype(np.int32),
+                                           m.tri_pnt.astype(np.int32), m.tri_seg.astype(np.int32),
+                                           m.tri_sgn.astype(np.int32))]
+    prm = _lib.ParamsC()
+    prm.physics, prm.log2p, prm.rho, prm.mu, prm.adis, prm.fadd, prm.nstage = 0, 1, 1.0, 1e-2, 1.0, 1.0, 1
+    prm.alpha[0] = prm.beta[0] = 1.0
+    prm.cfl[0] = prm.cfl[1] = 1.0
+    ptrs = [C.c_void_p(a.ctypes.data) for a in k]
+    cm = _lib.TriMeshC(m.npnt, m.nseg, m.ntri, *ptrs, 0, None, None, None)
+    h = C.c_void_p()
+    assert L.hpg_create(C.byref(cm), C.byref(prm), C.byref(h)) == 0 and h.value
+    assert L.hpg_rsdl(h, 1) == 0
+    assert L.hpg_destroy(h) == 0
+    start = np.array([0, 2], dtype=np.int32)
+    ess = np.array([3], dtype=np.int32)
+    cm = _lib.TriMeshC(m.npnt, m.nseg, m.ntri, *ptrs, 1, C.c_void_p(start.ctypes.data), None,
+                       C.c_void_p(ess.ctypes.data))
+    h = C.c_void_p()
+    assert L.hpg_create(C.byref(cm), C.byref(prm), C.byref(h)) < 0
+
+
 def test_nan_state_reports_setup_error(oracle_mod):
     _case("NaN state -> setup_preconditioner error count")
     from hpmp import mesh as M, tetmesh as TM, workloads as W

@@ -51,3 +51,36 @@ def test_tet_element_jacobian_parity(oracle_mod, cns, log2p):
     assert worst < 1e-7
     for a, b in zip(G.get_state(), s0):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("cns", [False, True])
+def test_tet_element_jacobian_on_coarse_p_level(oracle_mod, cns):
+    """The p = 1 level of a p = 2 context (tm = 4 with em0 = 1 edge slots per edge)
+    after mg_restrict, against the oracle at the same level."""
+    _case(f"tet element_jacobian coarse level cns={cns}")
+    m, G, O = _pair(cns, 1)
+    for X in (G, O):
+        X.rsdl()
+        X.mg_restrict()
+        assert X.log2p == 0
+    batch = np.random.default_rng(7).choice(m.ntet, size=10, replace=False)
+    s0 = [a.copy() for a in G.get_state()]
+    K = G.element_jacobian(batch)
+    worst = max(np.abs(K[j] - O.element_jacobian(int(t))).max() / np.abs(O.element_jacobian(int(t))).max()
+                for j, t in enumerate(batch))
+    print("max rel err", worst, flush=True)
+    assert worst < 1e-7
+    for a, b in zip(G.get_state(), s0):
+        assert np.array_equal(a, b)
+
+
+def test_tet_element_jacobian_whole_mesh_batch(oracle_mod):
+    """Every element of the mesh in one batch (split into vertex-disjoint sets by the
+    colouring; each set's columns evaluate that set's elements only)."""
+    _case("tet element_jacobian whole mesh")
+    m, G, O = _pair(False, 1)
+    batch = np.arange(m.ntet)
+    K = G.element_jacobian(batch)
+    for t in (0, m.ntet // 2, m.ntet - 1):
+        Ko = O.element_jacobian(int(t))
+        assert np.abs(K[t] - Ko).max() < 1e-7 * np.abs(Ko).max()

@@ -119,3 +119,20 @@ def test_free_stream_residual_vanishes_on_reference_mesh(oracle_mod):
     assert np.abs(rv[interior]).max() < 1e-13
     assert np.abs(ri).max() < 1e-13
     assert np.abs(rs[~m.boundary_segment_mask()]).max() < 1e-13
+
+
+@pytest.mark.parametrize("edit,what", [
+    (("0: 0 1", "0: 1 0"), "not stored counter-clockwise"),     # boundary side 0 reversed
+    (("0: 3 0 1", "0: 3 1 0"), "counter-clockwise"),            # triangle 0 clockwise
+])
+def test_inconsistent_grd_is_rejected(tmp_path, edit, what):
+    """A .grd whose side or triangle orientation contradicts the rest raises ValueError
+    naming the invariant (the check survives python -O: no assert)."""
+    src = open(os.path.join(HERE, "square1.grd")).read().splitlines()
+    old, new = edit
+    k = next(i for i, ln in enumerate(src) if ln.strip() == old)
+    src[k] = new
+    p = tmp_path / "bad.grd"
+    p.write_text("\n".join(src) + "\n")
+    with pytest.raises(ValueError, match=what):
+        grd.read_grd(str(p))
