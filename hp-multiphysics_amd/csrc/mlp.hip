@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -320,6 +321,26 @@ __global__ void k_mlp_reduce_adam(int nblocks, const float* __restrict__ partial
     wq[p] = is_bias ? w : bf16r(w);
 }
 
+// mean-square loss from the block losses: the same double sum, in block order, and
+// the same final division as the host side of hpg_mlp_loss_grad
+__global__ void k_mlp_loss_sum(int nblocks, const float* __restrict__ lossp, int nscale, float* __restrict__ loss) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += lossp[b];
+    loss[0] = (float)(s / (double)nscale);
+}
+
+// Scratch of the allocation-free loss/gradient entry point: the per-block partial
+// gradients (NPARAM floats per 64-sample block) followed by the block losses.  One
+// copy per GPU, held until the process ends; host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the symbol itself.
+// Every call overwrites what it reads (each block writes all of its partials and
+// its loss), and calls that overlap on one GPU race on it.
+constexpr int NOALLOC_FLOATS = 4 << 20;  // 16 MiB
+constexpr int NOALLOC_MAX_BLOCKS = NOALLOC_FLOATS / (NPARAM + 1);
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+static_assert(NOALLOC_MAX_BLOCKS * 64 == HPG_MLP_NOALLOC_MAX_SAMPLES, "header limit out of step with the scratch size");
+
 }  // namespace hpg
 
 using namespace hpg;
@@ -329,6 +350,7 @@ struct hpg_mlp {
     float *master = nullptr, *wq = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
     float *partial = nullptr, *lossp = nullptr, *X = nullptr, *T = nullptr, *Y = nullptr;
     int cap_blocks = 0, cap_n = 0, step = 0;
+    int dev = 0;  // device the MLP was created on
     std::string err;
     ~hpg_mlp() {
         for (float* p : {master, wq, m1, m2, grad, partial, lossp, X, T, Y})
@@ -360,6 +382,32 @@ void ensure_n(hpg_mlp* m, int n) {
     ck(hipMalloc(&m->Y, (size_t)n * MO * 4), "hipMalloc");
     m->cap_n = n;
 }
+// device address of `scratch` per GPU, resolved once per device with that device
+// current (each GPU holds its own copy of the array at its own address)
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[MAX_DEVICES] = {};
+
+void resolve_scratch(int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return;
+    void* p = nullptr;
+    ck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned null");
+    g_scratch[dev] = static_cast<float*>(p);
+}
+
+// enqueue the backward pass of n samples (block partials and block losses into
+// partial / lossp) and the ordered gradient sum into grad, no weight update
+void enqueue_loss_grad(hpg_mlp* m, const float* X, const float* T, int n, float* partial, float* lossp,
+                       float* grad) {
+    const int nb = (n + 63) / 64;
+    hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(64), 0, m->stream, n, n * MO, X, T, m->wq, partial, lossp);
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, nb, partial, grad,
+                       m->master, m->wq, m->m1, m->m2, 0.f, 1.f, 1.f, 0);
+}
+
 void ensure_blocks(hpg_mlp* m, int nb) {
     if (nb <= m->cap_blocks) return;
     if (m->partial) hipFree(m->partial);
@@ -386,6 +434,8 @@ int hpg_mlp_create(unsigned seed, hpg_mlp** out) {
     if (!out) return -2;
     std::unique_ptr<hpg_mlp> m(new hpg_mlp());
     try {
+        ck(hipGetDevice(&m->dev), "hipGetDevice");
+        resolve_scratch(m->dev);
         ck(hipStreamCreate(&m->stream), "hipStreamCreate");
         for (float** p : {&m->master, &m->wq, &m->m1, &m->m2, &m->grad}) ck(hipMalloc(p, NPARAM * 4), "hipMalloc");
         ck(hipMemsetAsync(m->m1, 0, NPARAM * 4, m->stream), "memset");
@@ -475,10 +525,7 @@ int hpg_mlp_loss_grad(hpg_mlp* m, const float* X, const float* T, int n, float* 
         ensure_blocks(m, nb);
         ck(hipMemcpyAsync(m->X, X, (size_t)n * MI * 4, hipMemcpyHostToDevice, m->stream), "H2D");
         ck(hipMemcpyAsync(m->T, T, (size_t)n * MO * 4, hipMemcpyHostToDevice, m->stream), "H2D");
-        hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(64), 0, m->stream, n, n * MO, m->X, m->T, m->wq, m->partial,
-                           m->lossp);
-        hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, nb, m->partial,
-                           m->grad, m->master, m->wq, m->m1, m->m2, 0.f, 1.f, 1.f, 0);
+        enqueue_loss_grad(m, m->X, m->T, n, m->partial, m->lossp, m->grad);
         std::vector<float> lp(nb);
         ck(hipMemcpyAsync(lp.data(), m->lossp, nb * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
         if (grad) ck(hipMemcpyAsync(grad, m->grad, NPARAM * 4, hipMemcpyDeviceToHost, m->stream), "D2H");
@@ -486,6 +533,30 @@ int hpg_mlp_loss_grad(hpg_mlp* m, const float* X, const float* T, int n, float* 
         double s = 0.0;
         for (float v : lp) s += v;
         if (loss) *loss = (float)(s / ((double)n * MO));
+    } catch (const std::exception& e) { m->err = e.what(); return -1; }
+    return 0;
+}
+
+// Allocation-free form of hpg_mlp_loss_grad on device buffers: X (n x 11), T (n x 3),
+// loss (1 float) and grad (NPARAM floats) are device pointers of the caller; the
+// block partials live in the library's static scratch array.  Same kernels and
+// summation order as hpg_mlp_loss_grad, so loss and gradient are bitwise equal.
+int hpg_mlp_loss_grad_noalloc(hpg_mlp* m, const float* X, const float* T, int n, float* loss, float* grad) {
+    if (!m || !X || !T || !loss || !grad || n <= 0) return -2;
+    const int nb = (n + 63) / 64;
+    if (nb > NOALLOC_MAX_BLOCKS) {
+        m->err = "hpg_mlp_loss_grad_noalloc: n exceeds the scratch capacity of " +
+                 std::to_string(HPG_MLP_NOALLOC_MAX_SAMPLES) + " samples";
+        return -2;
+    }
+    try {
+        float* sp = (m->dev >= 0 && m->dev < MAX_DEVICES) ? g_scratch[m->dev] : nullptr;
+        if (!sp) throw std::runtime_error("scratch address not resolved for this device");
+        float* lossp = sp + (size_t)nb * NPARAM;
+        enqueue_loss_grad(m, X, T, n, sp, lossp, grad);
+        hipLaunchKernelGGL(k_mlp_loss_sum, dim3(1), dim3(64), 0, m->stream, nb, lossp, n * MO, loss);
+        ck(hipGetLastError(), "loss_grad_noalloc launch");
+        ck(hipStreamSynchronize(m->stream), "sync");
     } catch (const std::exception& e) { m->err = e.what(); return -1; }
     return 0;
 }

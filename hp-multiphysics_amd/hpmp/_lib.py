@@ -19,6 +19,7 @@ HPG_FP64 = 0
 HPG_FP32_DEFECT = 1
 HPG_PRCN_ANALYTIC = 0
 HPG_PRCN_LEARNED = 1
+HPG_MLP_NOALLOC_MAX_SAMPLES = 28864
 
 # every entry point include/hpgpu.h declares
 EXPORTS = (
@@ -29,6 +30,7 @@ EXPORTS = (
     "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32", "hpg_selftest_mfma_bf16",
     "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_element_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
+    "hpg_mlp_loss_grad_noalloc",
     "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
     "hpg_get_block_preconditioner", "hpg_link_fine",
     "hpg_tet_create", "hpg_tet_destroy", "hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual",
@@ -137,6 +139,7 @@ def load():
     L.hpg_mlp_set_weights.argtypes = [vp, dp]
     L.hpg_mlp_forward.argtypes = [vp, dp, C.c_int, dp]
     L.hpg_mlp_loss_grad.argtypes = [vp, dp, dp, C.c_int, C.POINTER(C.c_float), dp]
+    L.hpg_mlp_loss_grad_noalloc.argtypes = [vp, dp, dp, C.c_int, dp, dp]
     L.hpg_mlp_train.argtypes = [vp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
     L.hpg_set_mlp.argtypes = [vp, vp]
     L.hpg_mlp_features.argtypes = [vp, dp]

@@ -168,6 +168,17 @@ int hpg_mlp_set_weights(hpg_mlp *m, const float *w);
 int hpg_mlp_forward(hpg_mlp *m, const float *X, int n, float *Y);
 int hpg_mlp_loss_grad(hpg_mlp *m, const float *X, const float *T, int n, float *loss, float *grad);
 int hpg_mlp_train(hpg_mlp *m, const float *X, const float *T, int n, int epochs, int batch, float lr, float *last_loss);
+/* No counterpart in the reference: an allocation-free form of hpg_mlp_loss_grad for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * training loops, harnesses that forbid hipMalloc).  X (n x 11), T (n x 3), loss (1)
+ * and grad (hpg_mlp_num_params()) are DEVICE pointers of the caller; the per-block
+ * partial gradients live in a 16 MiB static device array that the library holds on
+ * each GPU until the process ends.  Returns -2 with nothing enqueued when
+ * n > HPG_MLP_NOALLOC_MAX_SAMPLES.  Loss and gradient are bitwise those of
+ * hpg_mlp_loss_grad; the call returns after its work is done.  Calls that overlap on
+ * one GPU (from several threads or MLPs) share the array and race. */
+#define HPG_MLP_NOALLOC_MAX_SAMPLES 28864
+int hpg_mlp_loss_grad_noalloc(hpg_mlp *m, const float *X, const float *T, int n, float *loss, float *grad);
 int hpg_set_mlp(hpg_ctx *ctx, hpg_mlp *m);
 int hpg_mlp_features(hpg_ctx *ctx, float *out);             /* ntri x 11 */
 int hpg_measure_jacobian_diag(hpg_ctx *ctx, double *out);   /* ntri x 3, finest level */
