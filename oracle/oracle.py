@@ -25,12 +25,18 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's element loops (results do not depend on it)."""
+    return int(lib().oracle_set_threads(int(n)))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
         L = C.CDLL(LIB_PATH)
+        L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_recur.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.oracle_gauss.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.obasis_info.argtypes = [C.c_int, _ip]
@@ -95,6 +101,7 @@ def lib():
         L.ocns_calculate_tau.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp, _dp]
         L.ocns_spectral_radius.argtypes = [_dp]
         L.ocns_spectral_radius.restype = C.c_double
+        L.oracle_set_threads(1)  # one thread unless a caller asks for more
         _lib = L
     return _lib
 

@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): C entry points over the CPU restatement so
 // that tests/ (ctypes) and bench.py's cpu_baseline leg can drive it.  The product
 // library never links this file.
+#include <omp.h>
 #include "otri_basis.h"
 #include "otri_hp_ins.h"
 #include "otri_hp_cns.h"
@@ -21,6 +22,13 @@ static const TriBasis& basis_for(int p) {
 }
 
 extern "C" {
+
+// OpenMP threads of the element loops (rsdl, minvrt's interior sweep); returns the
+// number in effect.  Results do not depend on it.
+int oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 
 void oracle_recur(int n, int ipoly, double al, double be, double* a, double* b) { recur_coeffs(n, ipoly, al, be, a, b); }
 void oracle_gauss(int n, const double* a, const double* b, double* x, double* w) { gauss_rule(n, a, b, x, w); }

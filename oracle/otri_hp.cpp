@@ -47,7 +47,10 @@ double TriHp<NV>::inscribedradius(int t) const {
 
 // tri_hp/gtol.cpp:14-55
 template <int NV>
-void TriHp<NV>::ugtouht(int tind) {
+void TriHp<NV>::ugtouht(int tind) { ugtouht_to(tind, uht); }
+
+template <int NV>
+void TriHp<NV>::ugtouht_to(int tind, double uht[NV][15]) const {
     const TriBasis& B = b();
     for (int i = 0; i < 3; ++i) {
         const int indx = m.tri_pnt[3 * tind + i];
@@ -125,12 +128,30 @@ void TriHp<NV>::rsdl(int stage) {
                     resi[ix] = 0.0;
                     res_ri[ix] = (stage < nst) ? res_ri[ix] * f : 0.0;
                 }
-    double lf_re[NV][15], lf_im[NV][15];
-    for (int tind = 0; tind < m.ntri; ++tind) {
-        ugtouht(tind);
-        element_rsdl(tind, stage, uht, lf_re, lf_im);
-        lftog(tind, resv.data(), ress.data(), resi.data(), lf_im);
-        lftog(tind, res_rv.data(), res_rs.data(), res_ri.data(), lf_re);
+    // Element residuals of a chunk are computed in parallel (OpenMP; each is a pure
+    // function of the element's coefficients), then scattered by one thread in element
+    // order, the order of the reference's loop: the sums are bitwise those of a serial run.
+    const int CH = std::min(m.ntri, 8192);
+    if (bre.size() < (size_t)CH * NV * 15) {
+        bre.resize((size_t)CH * NV * 15);
+        bim.resize((size_t)CH * NV * 15);
+    }
+    for (int t0 = 0; t0 < m.ntri; t0 += CH) {
+        const int t1 = std::min(m.ntri, t0 + CH);
+#pragma omp parallel for schedule(static)
+        for (int tind = t0; tind < t1; ++tind) {
+            double uh[NV][15];
+            ugtouht_to(tind, uh);
+            double(*lre)[15] = reinterpret_cast<double(*)[15]>(&bre[(size_t)(tind - t0) * NV * 15]);
+            double(*lim)[15] = reinterpret_cast<double(*)[15]>(&bim[(size_t)(tind - t0) * NV * 15]);
+            element_rsdl(tind, stage, uh, lre, lim);
+        }
+        for (int tind = t0; tind < t1; ++tind) {
+            const double(*lre)[15] = reinterpret_cast<const double(*)[15]>(&bre[(size_t)(tind - t0) * NV * 15]);
+            const double(*lim)[15] = reinterpret_cast<const double(*)[15]>(&bim[(size_t)(tind - t0) * NV * 15]);
+            lftog(tind, resv.data(), ress.data(), resi.data(), lim);
+            lftog(tind, res_rv.data(), res_rs.data(), res_ri.data(), lre);
+        }
     }
     for (int i = 0; i < m.npnt * NV; ++i) resv[i] += res_rv[i];
     if (B.sm > 0) {
@@ -320,6 +341,9 @@ void TriHp<NV>::minvrt() {
     sdirichlet(mode);
     if (im == 0) return;
 
+    // each element reads the solved vertex and side values and writes only its own
+    // interior modes: independent, so OpenMP threads share the loop
+#pragma omp parallel for schedule(static)
     for (int tind = 0; tind < m.ntri; ++tind) {
         dpbtrsnu2_literal(B.idiag.data(), B.ibwth + 1, im, B.ibwth, &resi[tind * imf * NV], NV);
         // restouht_bdry (tri_hp/setup_preconditioner.cpp:375-399)

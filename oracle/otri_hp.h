@@ -81,11 +81,13 @@ class TriHp {
     std::vector<std::vector<double>> dresv, dress, dresi;
     std::vector<double> vprcn, sprcn, tprcn;  // (npnt|nseg|ntri) x NV
     double uht[NV][15], lf[NV][15];
+    std::vector<double> bre, bim;  // element residuals of one chunk of rsdl's parallel loop
 
     const TriBasis& b() const { return *bases[log2p]; }
     double area(int t) const;
     double inscribedradius(int t) const;
     void ugtouht(int tind);
+    void ugtouht_to(int tind, double uht[NV][15]) const;
     void lftog(int tind, double* gv, double* gs, double* gi, const double lfl[NV][15]);
     virtual void element_rsdl(int tind, int stage, const double u_ht[NV][15], double lf_re[NV][15],
                               double lf_im[NV][15]) = 0;
