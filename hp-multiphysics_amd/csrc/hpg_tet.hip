@@ -12,6 +12,7 @@ namespace {
 struct TetLevel {
     int p = 0;
     double* ops = nullptr;   // TDims<p>::SIZE
+    float* ops32 = nullptr;  // the same operators in FP32 (defect-correction inner solve)
     double* P2 = nullptr;    // face values GF x tm2d
     double* W2 = nullptr;    // face weights GF
     double* vfms = nullptr;  // 4 x 10
@@ -74,6 +75,13 @@ struct hpg_tet_ctx {
     int* d_err = nullptr;
     unsigned long long* d_max = nullptr;
     TetLevel lv[2];
+    // FP32 defect correction (hpg_tet_set_precision): device scale scalars and maxima,
+    // the FP64 residual R64(u0) and state u0 of the outer iteration (allocated on first use)
+    int precision = HPG_FP64, ninner = 1;
+    double* fd_scal = nullptr;
+    unsigned* fd_max = nullptr;
+    bool fd_reset = true;
+    double *srcv = nullptr, *srce = nullptr, *u0cv = nullptr, *u0ce = nullptr;
     bool timing = false;
     std::vector<hipEvent_t> events;
     struct Rec { int full, nelem; };
@@ -167,6 +175,45 @@ void launch_elem_tet(hpg_tet_ctx* c, int stage, const int* elist = nullptr, int 
     }
 }
 
+// FP32 element residual of the defect correction's inner solve (timing records 3 / 2)
+template <int P>
+void launch_elem_tet32(hpg_tet_ctx* c, int stage) {
+    const int nst = c->prm.nstage, count = c->ntet, noct = (count + 7) / 8;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const size_t r = c->recs.size();
+    if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+    if (c->cns()) {
+        TetcElemArgs32 a;
+        a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg;
+        a.ntet = count; a.elist = nullptr; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
+        a.ops = c->lv[c->log2p].ops32; a.mu = c->prm.mu; a.kcond = c->kcond; a.R = c->prm.Rgas;
+        a.gamma = c->prm.gamma; a.atm = 0.0; a.beta = beta;
+        if (beta > 0.0) {
+            static const int cap = resident_blocks((const void*)k_elem_tetc32<P, true>);
+            hipLaunchKernelGGL((k_elem_tetc32<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_tetc32<P, false>);
+            hipLaunchKernelGGL((k_elem_tetc32<P, false>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+        }
+    } else {
+        TetElemArgs32 a;
+        a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.tet_sgn = nullptr;
+        a.ntet = count; a.elist = nullptr; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
+        a.ops = c->lv[c->log2p].ops32; a.rho = c->prm.rho; a.mu = c->prm.mu; a.beta = beta;
+        if (beta > 0.0) {
+            static const int cap = resident_blocks((const void*)k_elem_tet32<P, true>);
+            hipLaunchKernelGGL((k_elem_tet32<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_tet32<P, false>);
+            hipLaunchKernelGGL((k_elem_tet32<P, false>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
+        }
+    }
+    if (c->timing) {
+        check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+        c->recs.push_back({beta > 0.0 ? 3 : 2, count});
+    }
+}
+
 template <int NVn>
 void tet_modes_nv(hpg_tet_ctx* c, int op, double* xv, double* xe, double* yv, double* ye, const double* zv,
                   const double* ze, double k) {
@@ -186,12 +233,14 @@ void tet_modes(hpg_tet_ctx* c, int op, double* xv, double* xe, double* yv, doubl
     else tet_modes_nv<TNV>(c, op, xv, xe, yv, ye, zv, ze, k);
 }
 
-// tet_hp::rsdl (tet_hp/update.cpp:9-131)
-void do_rsdl_tet(hpg_tet_ctx* c, int stage) {
+// tet_hp::rsdl (tet_hp/update.cpp:9-131).  fp32: the element residual in FP32 arithmetic
+// (the inner solve of the defect correction; the face fluxes and the assembly stay FP64)
+void do_rsdl_tet(hpg_tet_ctx* c, int stage, bool fp32 = false) {
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     const TetLevel& L = c->lv[c->log2p];
-    if (L.p == 1) launch_elem_tet<1>(c, stage); else launch_elem_tet<2>(c, stage);
+    if (fp32) { if (L.p == 1) launch_elem_tet32<1>(c, stage); else launch_elem_tet32<2>(c, stage); }
+    else if (L.p == 1) launch_elem_tet<1>(c, stage); else launch_elem_tet<2>(c, stage);
     if (c->nbf > 0 && c->cns()) {
         const double* b = c->prm.bc;
         if (L.p == 1)
@@ -334,6 +383,49 @@ int do_setup_tet(hpg_tet_ctx* c) {
     return h;
 }
 
+// y += sfac[0] x (the scaled defect source of the inner solve)
+__global__ void k_axpy_dev(long n, const double* __restrict__ x, const double* __restrict__ sfac, double* __restrict__ y) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] += sfac[0] * x[k];
+}
+
+// FP32 defect correction of tet_hp::update / tet_hp_cns::update with FP64 storage: the
+// outer iteration of the tri cns form (mixed_update_cns).  R64(u0) is kept in src; the
+// inner state v = fl32(u0) + dhat lives in ug, so minvrt and the stage update run as
+// in the FP64 update; stage 0 is R64(u0)/c, the later stages evaluate the element
+// residual in FP32 and add (1/c - 1) R64(u0); finally u = u0 + c (v - fl32(u0)).
+void mixed_update_tet(hpg_tet_ctx* c) {
+    const long nv = (long)c->npnt * c->nv, ne = (long)c->nseg * c->em0 * c->nv;
+    do_rsdl_tet(c, c->prm.nstage);
+    auto copy = [&](double* dst, const double* src, long n) {
+        if (n) check(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    };
+    auto grid = [](long n) { return dim3(std::min<unsigned>(nblk(n), RED_BLOCKS)); };
+    copy(c->srcv, c->resv, nv); copy(c->srce, c->rese, ne);
+    copy(c->u0cv, c->ugv, nv); copy(c->u0ce, c->uge, ne);
+    hipLaunchKernelGGL(k_round32, grid(nv), dim3(256), 0, c->stream, nv, c->ugv, c->fd_max);
+    if (ne) hipLaunchKernelGGL(k_round32, grid(ne), dim3(256), 0, c->stream, ne, c->uge, c->fd_max);
+    hipLaunchKernelGGL(k_fd_scale, dim3(1), dim3(1), 0, c->stream, c->fd_scal, c->fd_max, c->fd_reset ? 1 : 0, FD_H);
+    c->fd_reset = false;
+    for (int it = 0; it < c->ninner; ++it) {
+        copy(c->ug0v, c->ugv, nv); copy(c->ug0e, c->uge, ne);
+        for (int s = 0; s < c->prm.nstage; ++s) {
+            if (it == 0 && s == 0) {
+                hipLaunchKernelGGL(k_scale64, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->resv, c->fd_scal);
+                if (ne) hipLaunchKernelGGL(k_scale64, dim3(nblk(ne)), dim3(256), 0, c->stream, ne, c->rese, c->fd_scal);
+            } else {
+                do_rsdl_tet(c, s, true);
+                hipLaunchKernelGGL(k_axpy_dev, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->srcv, c->fd_scal + 1, c->resv);
+                if (ne) hipLaunchKernelGGL(k_axpy_dev, dim3(nblk(ne)), dim3(256), 0, c->stream, ne, c->srce, c->fd_scal + 1, c->rese);
+            }
+            do_minvrt_tet(c, !c->cns(), c->prm.alpha[s] * c->prm.cfl[c->log2p]);
+        }
+    }
+    hipLaunchKernelGGL(k_fd_final64, grid(nv), dim3(256), 0, c->stream, nv, c->ugv, c->u0cv, c->fd_scal, c->fd_max + 1);
+    if (ne) hipLaunchKernelGGL(k_fd_final64, grid(ne), dim3(256), 0, c->stream, ne, c->uge, c->u0ce, c->fd_scal, c->fd_max + 1);
+    check(hipGetLastError(), "tet mixed update launch");
+}
+
 double tet_absmax(hpg_tet_ctx* c, long nent, int nm, int mstride, const double* x, const unsigned char* mask,
                   double out[CNV]) {
     unsigned long long h[CNV] = {0, 0, 0, 0, 0};
@@ -451,12 +543,16 @@ int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** o
         }
         c->d_err = c->alloc<int>(1);
         c->d_max = c->alloc<unsigned long long>(CNV);
+        c->fd_scal = c->alloc<double>(2);
+        c->fd_max = c->alloc<unsigned>(2);
         for (int l = 0; l <= c->log2pmax; ++l) {
             const TetOps o(1 << l);
             TetLevel& L = c->lv[l];
             L.p = o.p;
             const std::vector<double> pk = o.p == 1 ? build_tet_pack<1>(o) : build_tet_pack<2>(o);
             L.ops = c->upload(pk.data(), pk.size());
+            const std::vector<float> pk32(pk.begin(), pk.end());
+            L.ops32 = c->upload(pk32.data(), pk32.size());
             L.P2 = c->upload(o.P2.data(), o.P2.size());
             L.W2 = c->upload(o.W2.data(), o.W2.size());
             L.Pg = c->upload(o.P.data(), o.P.size());
@@ -497,6 +593,7 @@ static int tet_copy(hpg_tet_ctx* c, bool h2d, double* dv, double* de, const doub
 
 int hpg_tet_set_state(hpg_tet_ctx* c, const double* v, const double* e) {
     if (!c || !v || (c->em0 && !e)) return -2;
+    c->fd_reset = true;
     return tet_copy(c, true, c->ugv, c->uge, v, e, nullptr, nullptr);
 }
 int hpg_tet_get_state(hpg_tet_ctx* c, double* v, double* e) {
@@ -551,6 +648,10 @@ int hpg_tet_minvrt(hpg_tet_ctx* c) {
 }
 int hpg_tet_update(hpg_tet_ctx* c) {
     if (!c) return -2;
+    if (c->precision == HPG_FP32_DEFECT && !c->coarse_flag) {
+        HPG_TRY(c, mixed_update_tet(c));
+        return 0;
+    }
     HPG_TRY(c, {
         // tet_hp::update (update.cpp:181-417) and tet_hp_cns::update (cns/update.cpp:18-143,
         // no Dirichlet pass before minvrt)
@@ -678,6 +779,29 @@ int hpg_tet_mg_prolongate(hpg_tet_ctx* c) {
     if (c->log2p == c->log2pmax) c->coarse_flag = false;
     return 0;
 }
+int hpg_tet_set_precision(hpg_tet_ctx* c, int mode, int ninner) {
+    if (!c || (mode != HPG_FP64 && mode != HPG_FP32_DEFECT) || ninner < 1 || ninner > 1000) return -2;
+    if (mode == HPG_FP32_DEFECT && !c->srcv) {
+        HPG_TRY(c, {
+            const size_t nv = (size_t)c->npnt * c->nv, ne = (size_t)c->nseg * c->em0 * c->nv;
+            c->srcv = c->alloc<double>(nv); c->srce = c->alloc<double>(ne);
+            c->u0cv = c->alloc<double>(nv); c->u0ce = c->alloc<double>(ne);
+            check(hipStreamSynchronize(c->stream), "sync");
+        });
+    }
+    c->precision = mode;
+    c->ninner = ninner;
+    c->fd_reset = true;
+    return 0;
+}
+
+int hpg_tet_residual_fp32(hpg_tet_ctx* c, double* resv, double* rese) {
+    if (!c || !resv) return -2;
+    if (c->log2p != c->log2pmax) { c->err = "finest level only"; return -2; }
+    HPG_TRY(c, do_rsdl_tet(c, c->prm.nstage, true));
+    return hpg_tet_get_residual(c, resv, rese);
+}
+
 int hpg_tet_get_log2p(hpg_tet_ctx* c, int* l) {
     if (!c || !l) return -2;
     *l = c->log2p;

@@ -38,8 +38,12 @@ struct LevelDims {
 // Lane-ready operator pack: A operand of every MFMA K-step, 64 values per step.
 // `row(i, q)` is the accumulator row held by register i of lane group q (FP64:
 // 4i + q; FP32: f32_row), which fixes the K permutation of the chained operators.
+// `moderow(r)` is the mode that A row r of Ir/Is produces (identity unless the
+// accumulator layout of the output tile differs from the FP64 one: the FP32 element
+// kernel maps row 4q + i to mode 4i + q, so that its register s of lane group q holds
+// mode 4s + q as in the FP64 kernel).
 template <class RowFn>
-std::vector<double> build_pack(const TriOps& o, RowFn row) {
+std::vector<double> build_pack(const TriOps& o, RowFn row, bool transpose_modes = false) {
     const LevelDims d(o.p);
     std::vector<double> pk(d.PACK, 0.0);
     const std::vector<double>* proj[3] = {&o.P, &o.Pr, &o.Ps};
@@ -65,7 +69,8 @@ std::vector<double> build_pack(const TriOps& o, RowFn row) {
         for (int kt = 0; kt < d.GT; ++kt)
             for (int i = 0; i < 4; ++i)
                 for (int l = 0; l < 64; ++l) {
-                    const int m = l & 15, k = 16 * kt + row(i, l >> 4);
+                    const int r = l & 15, m = transpose_modes ? 4 * (r & 3) + (r >> 2) : r;
+                    const int k = 16 * kt + row(i, l >> 4);
                     pk[3 * d.PROJ + 2 * d.DER + op * d.INT + (kt * 4 + i) * 64 + l] =
                         (m < d.tm && k < d.G) ? (*in[op])[m * d.G + k] : 0.0;
                 }
@@ -100,6 +105,11 @@ void colour(int nitem, int per, const int* verts, int npnt, std::vector<int>& or
     for (int t = 0; t < nitem; ++t) order[fill[col[t]]++] = t;
 }
 
+// relative size |dhat| / |u0| that the scale of the FP32 defect correction aims at:
+// FP32 residual differences then resolve J(u0) dhat to about 1e-3 (rounding ~ eps32 / h,
+// nonlinearity ~ h)
+constexpr double FD_H = 1e-3;
+
 struct Level {
     int p = 0;
     std::unique_ptr<TriOps> ops;
@@ -107,11 +117,13 @@ struct Level {
     double* Pmat = nullptr;
     double* lump = nullptr;
     double* edge = nullptr;  // g1 | dg1 | w1
-    float* pack32 = nullptr;
+    float* pack32 = nullptr;   // FP32 linearised kernel (output rows in the FP32 layout)
+    float* packn32 = nullptr;  // FP32 element residual (output rows permuted to the FP64 layout)
     float* lump32 = nullptr;
     int o_sfmv, o_ifmb, o_vfms, o_sdiag, o_sfms, o_bfmi, o_idiag;
     // cns: point-per-lane operator pack (CDims layout), edge side-mass and interior band factors
     double* cpack = nullptr;
+    float* cpack32 = nullptr;  // the same operators in FP32 (defect-correction inner solve)
     double* sdiag1d = nullptr;
     double* idiag = nullptr;
 };
@@ -173,6 +185,11 @@ struct hpg_ctx {
     std::vector<int> tcol;  // colour of each element (host)
     int* eseg = nullptr;
     std::vector<int> eoff;
+    int neseg = 0;
+    // p = 1 vertex path (ins): element-major element and boundary-edge results and the
+    // boundary-edge endpoints at each vertex (entries 2*idx + k into eout)
+    double *eres = nullptr, *eout = nullptr;
+    int *vb_ptr = nullptr, *vb_ent = nullptr;
     int *vs_ptr, *vs_side, *vs_end, *vt_ptr, *vt_tri, *vt_loc, *st_tri, *st_loc;
     unsigned char *vess, *sess;
     unsigned char *fcv, *fcs;  // first colour touching each vertex / side
@@ -189,11 +206,25 @@ struct hpg_ctx {
     float *dlv = nullptr, *dls = nullptr, *dli = nullptr, *d0v = nullptr, *d0s = nullptr, *d0i = nullptr;
     float *r32v = nullptr, *r32s = nullptr, *r32i = nullptr, *rr32v = nullptr, *rr32s = nullptr, *rr32i = nullptr;
     float *vp32 = nullptr, *sp32 = nullptr, *tp32 = nullptr;
+    // FP32 defect correction: base state fl32(u0) of the outer iteration; device scalars
+    // (scale c of the inner correction, 1/c - 1) and maxima (|u0|, |v - b|) as float bits
+    float *b32v = nullptr, *b32s = nullptr, *b32i = nullptr;
+    double* fd_scal = nullptr;
+    unsigned* fd_max = nullptr;
+    bool fd_reset = true;
+    // cns: the FP64 residual R64(u0) and the state u0 of the outer iteration
+    double *srcv = nullptr, *srcs = nullptr, *srci = nullptr, *u0cv = nullptr, *u0cs = nullptr, *u0ci = nullptr;
     // learned preconditioner (the MLP is owned by the caller)
     hpg_mlp* mlp = nullptr;
     float* featX = nullptr;
     float* mlpY = nullptr;
     double* jdiag = nullptr;
+    // element scalings of the vertex, side and interior rows when the learned
+    // preconditioner is in effect on the finest level (typed_prcn), FP64 and FP32
+    double *tprcn_vx = nullptr, *tprcn_sd = nullptr, *tprcn_in = nullptr;
+    float *tp32s = nullptr, *tp32i = nullptr;
+    bool typed_prcn = false;
+    bool user_y = false;  // mlpY holds log scalings set by the caller (hpg_set_prcn_log_scalings)
     unsigned long long* d_max;
     Level lv[3];
     std::string err;
@@ -315,6 +346,36 @@ void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full, const std::vector<in
     }
 }
 
+// FP32 element residual of the defect-correction inner solve, colour by colour (timing
+// records 3 / 2 like the linearised kernel)
+template <int P>
+void launch_elem32(hpg_ctx* c, const ElemArgs32& a0, bool full) {
+    for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+        ElemArgs32 a = a0;
+        a.colour = (int)col;
+        a.begin = c->coff[col];
+        a.end = c->coff[col + 1];
+        a.gbase = 0;
+        const int nlogical = (a.end - a.begin + 63) / 64;
+        if (nlogical == 0) continue;
+        const size_t r = c->recs.size();
+        if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+        if (full) {
+            static const int cap = resident_blocks((const void*)k_elem_ins32<P, true>);
+            hipLaunchKernelGGL((k_elem_ins32<P, true>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream,
+                               a);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_ins32<P, false>);
+            hipLaunchKernelGGL((k_elem_ins32<P, false>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0,
+                               c->stream, a);
+        }
+        if (c->timing) {
+            check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+            c->recs.push_back({full ? 3 : 2, a.end - a.begin});
+        }
+    }
+}
+
 template <int P>
 void launch_lin(hpg_ctx* c, const LinArgs& a0, bool full) {
     for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
@@ -355,11 +416,15 @@ MinvArgsT<T> minv_args(hpg_ctx* c) {
     a.vess = c->vess; a.sess = c->sess;
     if constexpr (sizeof(T) == 8) {
         a.resv = c->resv; a.ress = c->ress; a.resi = c->resi;
-        a.vprcn = c->vprcn; a.sprcn = c->sprcn; a.tprcn = c->tprcn;
+        a.vprcn = c->vprcn; a.sprcn = c->sprcn;
+        a.tprcn_s = c->typed_prcn ? c->tprcn_sd : c->tprcn;
+        a.tprcn_i = c->typed_prcn ? c->tprcn_in : c->tprcn;
         a.lump = L.lump;
     } else {
         a.resv = c->r32v; a.ress = c->r32s; a.resi = c->r32i;
-        a.vprcn = c->vp32; a.sprcn = c->sp32; a.tprcn = c->tp32;
+        a.vprcn = c->vp32; a.sprcn = c->sp32;
+        a.tprcn_s = c->typed_prcn ? c->tp32s : c->tp32;
+        a.tprcn_i = c->typed_prcn ? c->tp32i : c->tp32;
         a.lump = L.lump32;
     }
     a.npnt = c->npnt; a.nseg = c->nseg; a.ntri = c->ntri;
@@ -374,13 +439,75 @@ MinvArgsT<T> minv_args(hpg_ctx* c) {
 
 void do_rsdl_cns(hpg_ctx* c, int stage);
 
+// ins stage at p = 1 in three launches: element residuals and boundary-edge fluxes in
+// element-major form (no colouring), then one lane per (vertex, variable) sums them in
+// a fixed order, adds the multigrid source and, with cfa >= 0, applies the vertex
+// minvrt and the stage update (save_u0: first stage, ug0 = ug)
+void vstage_p1(hpg_ctx* c, int stage, double cfa, bool save_u0) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const double f = stage < nst ? 1.0 - beta : 0.0;
+    const bool full = beta > 0.0;
+    const Level& L = c->lv[c->log2p];
+    ElemArgs a{};
+    a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+    a.order = c->order; a.begin = 0; a.end = c->ntri;
+    a.ugv = c->ugv; a.ugs = c->ugs; a.ugi = c->ugi;
+    a.ops = L.pack; a.sm0 = c->sm0; a.im0 = c->im0;
+    a.rho = c->prm.rho; a.mu = c->prm.mu; a.adis = c->prm.adis; a.beta = beta;
+    a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0; a.keep = f;
+    const int nlogical = (c->ntri + 63) / 64;
+    const size_t r = c->recs.size();
+    if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+    if (full) {
+        static const int cap = resident_blocks((const void*)k_elem_ins1_em<true>);
+        hipLaunchKernelGGL(k_elem_ins1_em<true>, dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream, a,
+                           c->eres);
+    } else {
+        static const int cap = resident_blocks((const void*)k_elem_ins1_em<false>);
+        hipLaunchKernelGGL(k_elem_ins1_em<false>, dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream, a,
+                           c->eres);
+    }
+    if (c->timing) {
+        check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+        c->recs.push_back({full ? 1 : 0, c->ntri});
+    }
+    if (c->neseg > 0) {
+        const LevelDims d(1);
+        EdgeArgs e{};
+        e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
+        e.resv = c->resv; e.ress = c->ress;
+        e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+        e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
+        e.begin = 0; e.end = c->neseg;
+        hipLaunchKernelGGL(k_edge_ins1_em, dim3(nblk(c->neseg, 64)), dim3(64), 0, c->stream, e, c->eout);
+    }
+    VStageArgs v{};
+    v.npnt = c->npnt; v.vt_ptr = c->vt_ptr; v.vt_tri = c->vt_tri; v.vt_loc = c->vt_loc;
+    v.vb_ptr = c->vb_ptr; v.vb_ent = c->vb_ent; v.eres = c->eres; v.eout = c->eout;
+    v.resv = c->resv; v.rrv = c->rrv; v.keep = f; v.full = full ? 1 : 0;
+    if (c->coarse_flag) {
+        const int l = c->log2p;
+        v.mg = c->isfrst ? 2 : 1;
+        v.dres = c->dresv[l]; v.r0 = c->r0v; v.fadd = c->prm.fadd;
+        c->isfrst = false;
+    }
+    if (cfa >= 0.0) {
+        v.vess = c->vess; v.vprcn = c->vprcn; v.ug = c->ugv; v.ug0 = c->ug0v; v.cfa = cfa;
+        v.save_u0 = save_u0 ? 1 : 0;
+    }
+    hipLaunchKernelGGL(k_vstage1<NV>, dim3(nblk8((long)c->npnt * NV)), dim3(256), 0, c->stream, v);
+    check(hipGetLastError(), "p1 stage launch");
+}
+
 void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     if (c->physics == HPG_PHYSICS_CNS) { do_rsdl_cns(c, stage); return; }
+    if (c->lv[c->log2p].p == 1 && !store_lin) { vstage_p1(c, stage, -1.0, false); return; }
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     const double f = stage < nst ? 1.0 - beta : 0.0;
     const Level& L = c->lv[c->log2p];
-    ElemArgs a;
+    ElemArgs a{};
     a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
     a.order = c->order; a.begin = 0; a.end = 0;
     a.ugv = c->ugv; a.ugs = c->ugs; a.ugi = c->ugi;
@@ -426,6 +553,50 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
         }
     }
     check(hipGetLastError(), "rsdl launch");
+}
+
+// FP32 residual of the defect-correction inner solve at the FP32 state v (dlv/dls/dli):
+// r32 = R32(v) + (1/c - 1) R64(u0), multistage blended as rsdl(stage) blends R
+void do_rsdl32(hpg_ctx* c, int stage) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const double f = stage < nst ? 1.0 - beta : 0.0;
+    const Level& L = c->lv[c->log2p];
+    ElemArgs32 a{};
+    a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+    a.order = c->order; a.begin = 0; a.end = 0;
+    a.ugv = c->dlv; a.ugs = c->dls; a.ugi = c->dli;
+    a.resv = c->r32v; a.ress = c->r32s; a.resi = c->r32i;
+    a.rrv = c->rr32v; a.rrs = c->rr32s; a.rri = c->rr32i;
+    a.ops = L.packn32; a.sm0 = c->sm0; a.im0 = c->im0;
+    a.rho = (float)c->prm.rho; a.mu = (float)c->prm.mu; a.adis = (float)c->prm.adis; a.beta = (float)beta;
+    a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0; a.keep = (float)f;
+    a.lin = nullptr; a.gbase = 0;
+    a.srcv = c->resv; a.srcs = c->ress; a.srci = c->resi; a.sfac = c->fd_scal + 1;
+    const bool full = beta > 0.0;
+    switch (L.p) {
+        case 1: launch_elem32<1>(c, a, full); break;
+        case 2: launch_elem32<2>(c, a, full); break;
+        case 4: launch_elem32<4>(c, a, full); break;
+        default: throw std::runtime_error("unsupported p");
+    }
+    const LevelDims d(L.p);
+    EdgeArgsT<float> e;
+    e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->dlv; e.ugs = c->dls;
+    e.resv = c->r32v; e.ress = c->r32s;
+    e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+    e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
+    for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
+        e.begin = c->eoff[col];
+        e.end = c->eoff[col + 1];
+        if (e.end > e.begin) {
+            const dim3 be(nblk(e.end - e.begin, 64)), te(64);
+            if (L.p == 1) hipLaunchKernelGGL((k_edge_ins<1, float>), be, te, 0, c->stream, e);
+            else if (L.p == 2) hipLaunchKernelGGL((k_edge_ins<2, float>), be, te, 0, c->stream, e);
+            else hipLaunchKernelGGL((k_edge_ins<4, float>), be, te, 0, c->stream, e);
+        }
+    }
+    check(hipGetLastError(), "rsdl32 launch");
 }
 
 // FP32 linearised residual of the inner solve: r32 = src + J(u0) delta (multistage blended)
@@ -553,6 +724,29 @@ void launch_elem_cns(hpg_ctx* c, const CnsElemArgs& a0, bool full, const std::ve
     }
 }
 
+template <int P>
+void launch_elem_cns32(hpg_ctx* c, const CnsElemArgs32& a0, bool full) {
+    for (size_t col = 0; col + 1 < c->coff.size(); ++col) {
+        CnsElemArgs32 a = a0;
+        a.colour = (int)col;
+        a.begin = c->coff[col];
+        a.end = c->coff[col + 1];
+        const int noct = (a.end - a.begin + 7) / 8;
+        if (noct == 0) continue;
+        const int nb = std::min(((noct + 7) / 8) * 8, 1024);
+        const size_t r = c->recs.size();
+        if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+        if (full)
+            hipLaunchKernelGGL((k_elem_cns32<P, true>), dim3(nb), dim3(256), 0, c->stream, a, noct);
+        else
+            hipLaunchKernelGGL((k_elem_cns32<P, false>), dim3(nb), dim3(256), 0, c->stream, a, noct);
+        if (c->timing) {
+            check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+            c->recs.push_back({full ? 3 : 2, a.end - a.begin});
+        }
+    }
+}
+
 // coarse p-level source (tri_hp/nstage.cpp:113-129)
 void mg_source(hpg_ctx* c) {
     const int l = c->log2p;
@@ -604,10 +798,53 @@ void do_rsdl_cns(hpg_ctx* c, int stage) {
     check(hipGetLastError(), "cns rsdl launch");
 }
 
-int do_setup_cns(hpg_ctx* c) {
+// cns residual of the defect correction's inner solve at the FP64 state v (ug): the
+// element residual in FP32 plus the scaled defect source (1/c - 1) R64(u0) (srcv...),
+// the boundary fluxes in FP64
+void do_rsdl_cns32(hpg_ctx* c, int stage) {
+    const int nst = c->prm.nstage;
+    const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
+    const double f = stage < nst ? 1.0 - beta : 0.0;
+    const Level& L = c->lv[c->log2p];
+    CnsElemArgs32 a{};
+    a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
+    a.order = c->order; a.begin = 0; a.end = 0;
+    a.ugv = c->ugv; a.ugs = c->ugs; a.ugi = c->ugi;
+    a.resv = c->resv; a.ress = c->ress; a.resi = c->resi;
+    a.rrv = c->rrv; a.rrs = c->rrs; a.rri = c->rri;
+    a.ops = L.cpack32; a.sm0 = c->sm0; a.im0 = c->im0;
+    a.ph = c->cph; a.beta = beta; a.keep = f;
+    a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0;
+    a.srcv = c->srcv; a.srcs = c->srcs; a.srci = c->srci; a.sfac = c->fd_scal + 1;
+    const bool full = beta > 0.0;
+    const LevelDims d(L.p);
+    CnsEdgeArgs e;
+    e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
+    e.resv = c->resv; e.ress = c->ress;
+    e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+    e.sm0 = c->sm0; e.gamma = c->cph.gamma; e.pbc = c->cbc[0];
+    auto edges = [&](auto kern) {
+        for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
+            e.begin = c->eoff[col];
+            e.end = c->eoff[col + 1];
+            if (e.end > e.begin) hipLaunchKernelGGL(kern, dim3(nblk(e.end - e.begin, 64)), dim3(64), 0, c->stream, e);
+        }
+    };
+    switch (L.p) {
+        case 1: launch_elem_cns32<1>(c, a, full); edges(k_edge_cns<1>); break;
+        case 2: launch_elem_cns32<2>(c, a, full); edges(k_edge_cns<2>); break;
+        case 4: launch_elem_cns32<4>(c, a, full); edges(k_edge_cns<4>); break;
+        default: throw std::runtime_error("unsupported p");
+    }
+    check(hipGetLastError(), "cns rsdl32 launch");
+}
+
+// sync = false (the fused multigrid cycle): the error flag accumulates on the device and
+// is read once after the cycle; nothing waits for the device here
+int do_setup_cns(hpg_ctx* c, bool sync = true) {
     const Level& L = c->lv[c->log2p];
     const LevelDims d(L.p);
-    check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
+    if (sync) check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
     CnsSetupArgs s;
     s.pnts = c->pnts; s.tri_pnt = c->tri_pnt; s.tri_seg = c->tri_seg; s.tri_sgn = c->tri_sgn;
     s.ugv = c->ugv; s.ugs = c->ugs; s.ugi = c->ugi; s.Pmat = L.Pmat; s.tprcn = c->tprcn; s.tpc = c->tpc;
@@ -623,6 +860,7 @@ int do_setup_cns(hpg_ctx* c) {
     if (d.sm > 0)
         hipLaunchKernelGGL(k_setup_side_cns, dim3(nblk8(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->st_tri,
                            c->tprcn, c->tpc, c->sprcn, c->spc);
+    if (!sync) return 0;
     int herr = 0;
     check(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "memcpy err");
     check(hipStreamSynchronize(c->stream), "sync");
@@ -681,11 +919,52 @@ void update_cns(hpg_ctx* c) {
     check(hipGetLastError(), "cns update launch");
 }
 
-int do_setup(hpg_ctx* c) {
-    if (c->physics == HPG_PHYSICS_CNS) return do_setup_cns(c);
+// FP32 defect correction of tri_hp_cns::update: the outer iteration of mixed_update with
+// FP64 storage.  R64(u0) (judge and defect) is kept in src; the inner state v = fl32(u0)
+// + dhat lives in ug, so minvrt, the inflow boundary residual, the primitive projection
+// and the stage update run as in update_cns; stage 0 is F(b) = R64(u0)/c and the later
+// stages evaluate the element residual in FP32; finally u = u0 + c (v - fl32(u0)).
+void mixed_update_cns(hpg_ctx* c) {
+    const long nv = (long)c->npnt * c->nv, ns = (long)c->nseg * c->sm0 * c->nv, ni = (long)c->ntri * c->im0 * c->nv;
+    do_rsdl_cns(c, c->prm.nstage);
+    auto copy = [&](double* dst, const double* src, long n) {
+        if (n) check(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    };
+    copy(c->srcv, c->resv, nv); copy(c->srcs, c->ress, ns); copy(c->srci, c->resi, ni);
+    copy(c->u0cv, c->ugv, nv); copy(c->u0cs, c->ugs, ns); copy(c->u0ci, c->ugi, ni);
+    hipLaunchKernelGGL(k_round32, dim3(std::min<unsigned>(nblk(nv), RED_BLOCKS)), dim3(256), 0, c->stream, nv, c->ugv, c->fd_max);
+    if (ns) hipLaunchKernelGGL(k_round32, dim3(std::min<unsigned>(nblk(ns), RED_BLOCKS)), dim3(256), 0, c->stream, ns, c->ugs, c->fd_max);
+    if (ni) hipLaunchKernelGGL(k_round32, dim3(std::min<unsigned>(nblk(ni), RED_BLOCKS)), dim3(256), 0, c->stream, ni, c->ugi, c->fd_max);
+    hipLaunchKernelGGL(k_fd_scale, dim3(1), dim3(1), 0, c->stream, c->fd_scal, c->fd_max, c->fd_reset ? 1 : 0, FD_H);
+    c->fd_reset = false;
+    for (int it = 0; it < c->ninner; ++it) {
+        copy(c->ug0v, c->ugv, nv); copy(c->ug0s, c->ugs, ns); copy(c->ug0i, c->ugi, ni);
+        for (int stage = 0; stage < c->prm.nstage; ++stage) {
+            if (it == 0 && stage == 0) {
+                // res and res_r hold R64(u0) and its viscous part: the residual at v = b
+                hipLaunchKernelGGL(k_scale64, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->resv, c->fd_scal);
+                if (ns) hipLaunchKernelGGL(k_scale64, dim3(nblk(ns)), dim3(256), 0, c->stream, ns, c->ress, c->fd_scal);
+                if (ni) hipLaunchKernelGGL(k_scale64, dim3(nblk(ni)), dim3(256), 0, c->stream, ni, c->resi, c->fd_scal);
+            } else {
+                do_rsdl_cns32(c, stage);
+            }
+            do_minvrt(c);
+            cns_boundary_and_projection(c);
+            modes_vsi(c, 4, c->ugv, c->ugs, c->ugi, c->ug0v, c->ug0s, c->ug0i, c->resv, c->ress, c->resi,
+                      c->prm.alpha[stage] * c->prm.cfl[c->log2p]);
+        }
+    }
+    hipLaunchKernelGGL(k_fd_final64, dim3(std::min<unsigned>(nblk(nv), RED_BLOCKS)), dim3(256), 0, c->stream, nv, c->ugv, c->u0cv, c->fd_scal, c->fd_max + 1);
+    if (ns) hipLaunchKernelGGL(k_fd_final64, dim3(std::min<unsigned>(nblk(ns), RED_BLOCKS)), dim3(256), 0, c->stream, ns, c->ugs, c->u0cs, c->fd_scal, c->fd_max + 1);
+    if (ni) hipLaunchKernelGGL(k_fd_final64, dim3(std::min<unsigned>(nblk(ni), RED_BLOCKS)), dim3(256), 0, c->stream, ni, c->ugi, c->u0ci, c->fd_scal, c->fd_max + 1);
+    check(hipGetLastError(), "cns mixed update launch");
+}
+
+int do_setup(hpg_ctx* c, bool sync = true) {
+    if (c->physics == HPG_PHYSICS_CNS) return do_setup_cns(c, sync);
     const Level& L = c->lv[c->log2p];
     const LevelDims d(L.p);
-    check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
+    if (sync) check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
     SetupArgs s;
     s.pnts = c->pnts; s.tri_pnt = c->tri_pnt; s.tri_seg = c->tri_seg; s.tri_sgn = c->tri_sgn;
     s.ugv = c->ugv; s.ugs = c->ugs; s.ugi = c->ugi; s.Pmat = L.Pmat; s.tprcn = c->tprcn; s.err = c->d_err;
@@ -696,19 +975,24 @@ int do_setup(hpg_ctx* c) {
         case 2: hipLaunchKernelGGL(k_setup_elem<2>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
         default: hipLaunchKernelGGL(k_setup_elem<4>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
     }
-    if (c->prcn_mode == HPG_PRCN_LEARNED && c->mlp) {
-        // learned scalings of the analytic tprcn (element features -> MLP -> exp)
-        hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts, c->tri_pnt,
-                           c->ugv, c->tprcn, c->prm.mu, c->prm.rho, c->featX);
-        mlp_forward_device(c->mlp, c->featX, c->ntri, c->mlpY, c->stream);
+    // learned preconditioner on the finest level: element features -> MLP (or the
+    // caller's log scalings) -> separate scalings of the vertex, side and interior rows
+    c->typed_prcn = c->prcn_mode == HPG_PRCN_LEARNED && (c->mlp || c->user_y) && c->log2p == c->log2pmax;
+    if (c->typed_prcn) {
+        if (!c->user_y) {
+            hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts,
+                               c->tri_pnt, c->ugv, c->tprcn, c->prm.mu, c->prm.rho, c->featX);
+            mlp_forward_device(c->mlp, c->featX, c->ntri, c->mlpY, c->stream);
+        }
         hipLaunchKernelGGL(k_mlp_apply, dim3(nblk((long)c->ntri * NV)), dim3(256), 0, c->stream, c->ntri, c->mlpY,
-                           c->tprcn);
+                           c->tprcn, c->tprcn_vx, c->tprcn_sd, c->tprcn_in);
     }
     hipLaunchKernelGGL(k_setup_vrtx, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vt_ptr, c->vt_tri,
-                       c->tprcn, c->vprcn, L.ops->vdiag);
+                       c->typed_prcn ? c->tprcn_vx : c->tprcn, c->vprcn, L.ops->vdiag);
     if (d.sm > 0)
-        hipLaunchKernelGGL(k_setup_side, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->st_tri, c->tprcn,
-                           c->sprcn);
+        hipLaunchKernelGGL(k_setup_side, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->st_tri,
+                           c->typed_prcn ? c->tprcn_sd : c->tprcn, c->sprcn);
+    if (!sync) return 0;
     int herr = 0;
     check(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "memcpy err");
     check(hipStreamSynchronize(c->stream), "sync");
@@ -786,7 +1070,7 @@ void jacobian_colour(hpg_ctx* c, const JacArgs& ja, int N) {
             e.fcv = c->fcv; e.fcs = c->fcs; e.colour = 0;
             launch_elem_cns<P>(c, e, full, &offs);
         } else {
-            ElemArgs e;
+            ElemArgs e{};
             e.pnts = c->pnts; e.tri_pnt = c->tri_pnt; e.tri_seg = c->tri_seg; e.tri_sgn = c->tri_sgn;
             e.order = ja.jel; e.begin = 0; e.end = 0;
             e.ugv = c->ugv; e.ugs = c->ugs; e.ugi = c->ugi;
@@ -888,6 +1172,19 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         for (int j = 0; j < neseg; ++j) eseg[j] = m->ebdry_seg[eorder[j]];
         c->eseg = c->upload(eseg.data(), eseg.size());
         c->eoff = eoffv;
+        c->neseg = neseg;
+        {
+            // endpoints of the boundary edges at each vertex, in the edge list's order
+            std::vector<int> vb_ptr(np + 1, 0), vb_ent(2 * (size_t)neseg);
+            for (int j = 0; j < neseg; ++j)
+                for (int k = 0; k < 2; ++k) vb_ptr[m->seg_pnt[2 * eseg[j] + k] + 1]++;
+            for (int v = 0; v < np; ++v) vb_ptr[v + 1] += vb_ptr[v];
+            std::vector<int> f(vb_ptr.begin(), vb_ptr.end() - 1);
+            for (int j = 0; j < neseg; ++j)
+                for (int k = 0; k < 2; ++k) vb_ent[f[m->seg_pnt[2 * eseg[j] + k]]++] = 2 * j + k;
+            c->vb_ptr = c->upload(vb_ptr.data(), vb_ptr.size());
+            c->vb_ent = c->upload(vb_ent.data(), vb_ent.size());
+        }
 
         // adjacency in ascending order (the reference's accumulation order)
         std::vector<int> vs_ptr(np + 1, 0), vt_ptr(np + 1, 0);
@@ -995,12 +1292,26 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         for (auto p : f32v) *p = c->alloc<float>(nv);
         for (auto p : f32s) *p = c->alloc<float>(nsz);
         for (auto p : f32i) *p = c->alloc<float>(ni);
+        c->b32v = c->alloc<float>(nv);
+        c->b32s = c->alloc<float>(nsz);
+        c->b32i = c->alloc<float>(ni);
+        c->fd_scal = c->alloc<double>(2);
+        c->fd_max = c->alloc<unsigned>(2);
         c->vp32 = c->alloc<float>(nv);
         c->sp32 = c->alloc<float>((size_t)ns * NV);
         c->tp32 = c->alloc<float>((size_t)nt * NV);
         c->featX = c->alloc<float>((size_t)nt * MLP_IN);
         c->mlpY = c->alloc<float>((size_t)nt * MLP_OUT);
-        c->jdiag = c->alloc<double>((size_t)nt * NV);
+        c->jdiag = c->alloc<double>((size_t)nt * 3 * NV);
+        if (c->physics == HPG_PHYSICS_INS) {
+            c->eres = c->alloc<double>((size_t)nt * 3 * 2 * NV);
+            c->eout = c->alloc<double>((size_t)neseg * 2 * NV);
+            c->tprcn_vx = c->alloc<double>((size_t)nt * NV);
+            c->tprcn_sd = c->alloc<double>((size_t)nt * NV);
+            c->tprcn_in = c->alloc<double>((size_t)nt * NV);
+            c->tp32s = c->alloc<float>((size_t)nt * NV);
+            c->tp32i = c->alloc<float>((size_t)nt * NV);
+        }
 
         for (int l = 0; l <= c->log2pmax; ++l) {
             Level& L = c->lv[l];
@@ -1012,6 +1323,9 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             const std::vector<double> pk32d = build_pack(o, [](int i, int q) { return f32_row(i, q); });
             const std::vector<float> pk32(pk32d.begin(), pk32d.end());
             L.pack32 = c->upload(pk32.data(), pk32.size());
+            const std::vector<double> pkn32d = build_pack(o, [](int i, int q) { return f32_row(i, q); }, true);
+            const std::vector<float> pkn32(pkn32d.begin(), pkn32d.end());
+            L.packn32 = c->upload(pkn32.data(), pkn32.size());
             L.Pmat = c->upload(o.P.data(), o.P.size());
             std::vector<double> lump;
             auto app = [&](const std::vector<double>& v) { const int o2 = (int)lump.size(); lump.insert(lump.end(), v.begin(), v.end()); return o2; };
@@ -1028,6 +1342,8 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             if (c->physics == HPG_PHYSICS_CNS) {
                 const std::vector<double> cp = build_cns_pack(o);
                 L.cpack = c->upload(cp.data(), cp.size());
+                const std::vector<float> cp32(cp.begin(), cp.end());
+                L.cpack32 = c->upload(cp32.data(), cp32.size());
                 L.sdiag1d = c->upload(o.sdiag1d.data(), o.sdiag1d.size());
                 const std::vector<double> idg = o.idiag.empty() ? std::vector<double>(2, 1.0) : o.idiag;
                 L.idiag = c->upload(idg.data(), idg.size());
@@ -1067,6 +1383,7 @@ static int copy3(hpg_ctx* c, bool h2d, double* dv, double* ds, double* di, const
 
 int hpg_set_state(hpg_ctx* c, const double* v, const double* s, const double* i) {
     if (!c) return -2;
+    c->fd_reset = true;
     return copy3(c, true, c->ugv, c->ugs, c->ugi, v, s, i, nullptr, nullptr, nullptr);
 }
 int hpg_get_state(hpg_ctx* c, double* v, double* s, double* i) {
@@ -1135,38 +1452,97 @@ namespace {
 void launch_f32(hpg_ctx* c, long n, const double* x, float* y) {
     if (n > 0) hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, y);
 }
-void launch_add(hpg_ctx* c, long n, double* ug, const float* d) {
-    if (n > 0) hipLaunchKernelGGL(k_add_delta, dim3(nblk(n)), dim3(256), 0, c->stream, n, ug, d);
-}
 
-// One outer defect-correction iteration: FP64 residual R(u0) (the judge) with the
-// linearisation data, then `ninner` multistage iterations in FP32 on
-// R(u0) + J(u0) delta = 0, then u0 += delta.
+// One outer defect-correction iteration: the FP64 residual R64(u0) (the judge and the
+// defect), then `ninner` multistage iterations of the reference's scheme in FP32 on
+// F(v) = R32(v) + (1/c - 1) R64(u0) from v = b = fl32(u0) with the FP32 minvrt, then
+// u0 += c (v - b) in FP64.  F(b + dhat) ~ R64(u0)/c + J(u0) dhat, so the inner solve
+// is the scaled defect equation; c keeps |dhat| near FD_H |u0| (k_fd_scale).  Stage 0
+// of the first inner iteration is the full residual at v = b, i.e. R64(u0)/c with the
+// FP64 viscous part carried into the stage blend, so it costs no FP32 evaluation: an
+// update evaluates one FP64 and nstage - 1 FP32 residuals, as many as the FP64 scheme.
 void mixed_update(hpg_ctx* c) {
     const long nv = (long)c->npnt * NV, nsz = (long)c->nseg * c->sm0 * NV, ni = (long)c->ntri * c->im0 * NV;
-    do_rsdl(c, c->prm.nstage, true);
+    do_rsdl(c, c->prm.nstage);
     launch_f32(c, nv, c->vprcn, c->vp32);
     launch_f32(c, (long)c->nseg * NV, c->sprcn, c->sp32);
     launch_f32(c, (long)c->ntri * NV, c->tprcn, c->tp32);
-    check(hipMemsetAsync(c->dlv, 0, nv * 4, c->stream), "memset");
-    if (nsz) check(hipMemsetAsync(c->dls, 0, nsz * 4, c->stream), "memset");
-    if (ni) check(hipMemsetAsync(c->dli, 0, ni * 4, c->stream), "memset");
+    if (c->typed_prcn) {
+        launch_f32(c, (long)c->ntri * NV, c->tprcn_sd, c->tp32s);
+        launch_f32(c, (long)c->ntri * NV, c->tprcn_in, c->tp32i);
+    }
+    hipLaunchKernelGGL(k_base32, dim3(std::min<unsigned>(nblk(nv), RED_BLOCKS)), dim3(256), 0, c->stream, nv, c->ugv, c->b32v, c->dlv, c->fd_max);
+    if (nsz) hipLaunchKernelGGL(k_base32, dim3(std::min<unsigned>(nblk(nsz), RED_BLOCKS)), dim3(256), 0, c->stream, nsz, c->ugs, c->b32s, c->dls, c->fd_max);
+    if (ni) hipLaunchKernelGGL(k_base32, dim3(std::min<unsigned>(nblk(ni), RED_BLOCKS)), dim3(256), 0, c->stream, ni, c->ugi, c->b32i, c->dli, c->fd_max);
+    hipLaunchKernelGGL(k_fd_scale, dim3(1), dim3(1), 0, c->stream, c->fd_scal, c->fd_max, c->fd_reset ? 1 : 0, FD_H);
+    c->fd_reset = false;
     for (int it = 0; it < c->ninner; ++it) {
         check(hipMemcpyAsync(c->d0v, c->dlv, nv * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
         if (nsz) check(hipMemcpyAsync(c->d0s, c->dls, nsz * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
         if (ni) check(hipMemcpyAsync(c->d0i, c->dli, ni * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
         for (int stage = 0; stage < c->prm.nstage; ++stage) {
-            do_lin_rsdl(c, stage, true);
+            if (it == 0 && stage == 0) {
+                hipLaunchKernelGGL(k_scale_f32, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->resv, c->fd_scal, c->r32v);
+                if (nsz) hipLaunchKernelGGL(k_scale_f32, dim3(nblk(nsz)), dim3(256), 0, c->stream, nsz, c->ress, c->fd_scal, c->r32s);
+                if (ni) hipLaunchKernelGGL(k_scale_f32, dim3(nblk(ni)), dim3(256), 0, c->stream, ni, c->resi, c->fd_scal, c->r32i);
+                launch_f32(c, nv, c->rrv, c->rr32v);
+                launch_f32(c, nsz, c->rrs, c->rr32s);
+                launch_f32(c, ni, c->rri, c->rr32i);
+            } else {
+                do_rsdl32(c, stage);
+            }
             do_minvrt32(c, (float)(c->prm.alpha[stage] * c->prm.cfl[c->log2p]));
         }
     }
-    launch_add(c, nv, c->ugv, c->dlv);
-    launch_add(c, nsz, c->ugs, c->dls);
-    launch_add(c, ni, c->ugi, c->dli);
+    // |dhat| of this iteration (read by the next k_fd_scale), then the FP64 update
+    hipLaunchKernelGGL(k_diffmax32, dim3(std::min<unsigned>(nblk(nv), RED_BLOCKS)), dim3(256), 0, c->stream, nv, c->dlv, c->b32v, c->fd_max + 1);
+    if (nsz) hipLaunchKernelGGL(k_diffmax32, dim3(std::min<unsigned>(nblk(nsz), RED_BLOCKS)), dim3(256), 0, c->stream, nsz, c->dls, c->b32s, c->fd_max + 1);
+    if (ni) hipLaunchKernelGGL(k_diffmax32, dim3(std::min<unsigned>(nblk(ni), RED_BLOCKS)), dim3(256), 0, c->stream, ni, c->dli, c->b32i, c->fd_max + 1);
+    hipLaunchKernelGGL(k_add_scaled, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->ugv, c->dlv, c->b32v, c->fd_scal);
+    if (nsz) hipLaunchKernelGGL(k_add_scaled, dim3(nblk(nsz)), dim3(256), 0, c->stream, nsz, c->ugs, c->dls, c->b32s, c->fd_scal);
+    if (ni) hipLaunchKernelGGL(k_add_scaled, dim3(nblk(ni)), dim3(256), 0, c->stream, ni, c->ugi, c->dli, c->b32i, c->fd_scal);
     check(hipGetLastError(), "mixed update launch");
 }
 
 }  // namespace
+
+// FP32 residual R32(fl32(u)) of the current state (finest level, no defect source):
+// the kernels of the defect correction's inner solve, for tests
+int hpg_residual_fp32(hpg_ctx* c, double* ov, double* os, double* oi) {
+    if (!c || !ov) return -2;
+    if (c->log2p != c->log2pmax) { c->err = "finest level only"; return -2; }
+    if (c->physics == HPG_PHYSICS_CNS) {
+        // FP64 state and residual arrays, FP32 element arithmetic; the residual arrays
+        // are overwritten
+        HPG_TRY(c, {
+            hipLaunchKernelGGL(k_fd_scale, dim3(1), dim3(1), 0, c->stream, c->fd_scal, c->fd_max, 1, 1.0);
+            c->fd_reset = true;
+            do_rsdl_cns32(c, c->prm.nstage);
+        });
+        return hpg_get_residual(c, ov, os, oi);
+    }
+    HPG_TRY(c, {
+        const long nv = (long)c->npnt * NV, nsz = (long)c->nseg * c->sm0 * NV, ni = (long)c->ntri * c->im0 * NV;
+        hipLaunchKernelGGL(k_base32, dim3(std::min<unsigned>(nblk(nv), RED_BLOCKS)), dim3(256), 0, c->stream, nv, c->ugv, c->b32v, c->dlv, c->fd_max);
+        if (nsz) hipLaunchKernelGGL(k_base32, dim3(std::min<unsigned>(nblk(nsz), RED_BLOCKS)), dim3(256), 0, c->stream, nsz, c->ugs, c->b32s, c->dls, c->fd_max);
+        if (ni) hipLaunchKernelGGL(k_base32, dim3(std::min<unsigned>(nblk(ni), RED_BLOCKS)), dim3(256), 0, c->stream, ni, c->ugi, c->b32i, c->dli, c->fd_max);
+        // scale 1: the source factor 1/c - 1 is zero
+        hipLaunchKernelGGL(k_fd_scale, dim3(1), dim3(1), 0, c->stream, c->fd_scal, c->fd_max, 1, 1.0);
+        c->fd_reset = true;
+        do_rsdl32(c, c->prm.nstage);
+        std::vector<float> h((size_t)std::max(nv, std::max(nsz, ni)));
+        auto get = [&](const float* d, long n, double* out) {
+            if (n == 0) return;
+            check(hipMemcpyAsync(h.data(), d, n * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+            check(hipStreamSynchronize(c->stream), "sync");
+            for (long k = 0; k < n; ++k) out[k] = h[k];
+        };
+        get(c->r32v, nv, ov);
+        get(c->r32s, nsz, os);
+        get(c->r32i, ni, oi);
+    });
+    return 0;
+}
 
 int hpg_set_defect_inner(hpg_ctx* c, int ninner) {
     if (!c || ninner < 1 || ninner > 1000) return -2;
@@ -1206,6 +1582,17 @@ int hpg_set_mlp(hpg_ctx* c, hpg_mlp* m) {
     return 0;
 }
 
+int hpg_set_prcn_log_scalings(hpg_ctx* c, const float* y) {
+    if (!c || c->physics != HPG_PHYSICS_INS) return -2;
+    if (!y) { c->user_y = false; return 0; }
+    HPG_TRY(c, {
+        check(hipMemcpyAsync(c->mlpY, y, (size_t)c->ntri * MLP_OUT * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        check(hipStreamSynchronize(c->stream), "sync");
+        c->user_y = true;
+    });
+    return 0;
+}
+
 int hpg_mlp_features(hpg_ctx* c, float* out) {
     if (!c || !out) return -2;
     if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
@@ -1224,7 +1611,7 @@ int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
     if (c->log2p != c->log2pmax) { c->err = "measure_jacobian_diag runs on the finest level"; return -2; }
     HPG_TRY(c, {
         do_rsdl(c, c->prm.nstage, true);
-        check(hipMemsetAsync(c->jdiag, 0, (size_t)c->ntri * NV * 8, c->stream), "memset");
+        check(hipMemsetAsync(c->jdiag, 0, (size_t)c->ntri * 3 * NV * 8, c->stream), "memset");
         const Level& L = c->lv[c->log2p];
         const LevelDims d(L.p);
         LinArgs a{};
@@ -1249,7 +1636,7 @@ int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
                 }
             }
         check(hipGetLastError(), "probe launch");
-        check(hipMemcpyAsync(out, c->jdiag, (size_t)c->ntri * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipMemcpyAsync(out, c->jdiag, (size_t)c->ntri * 3 * NV * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
         check(hipStreamSynchronize(c->stream), "sync");
     });
     return 0;
@@ -1317,28 +1704,41 @@ int hpg_element_jacobian(hpg_ctx* c, const int* tind, int nel, double eps_r, dou
     return 0;
 }
 
-int hpg_update(hpg_ctx* c) {
-    if (!c) return -2;
+namespace {
+// tri_hp::update / tri_hp_cns::update, or their FP32 defect-correction forms on the
+// finest level; enqueued on the context's stream, no host synchronisation
+void do_update(hpg_ctx* c) {
     if (c->physics == HPG_PHYSICS_CNS) {
-        HPG_TRY(c, update_cns(c));
-        return 0;
+        if (c->precision == HPG_FP32_DEFECT && !c->coarse_flag) mixed_update_cns(c);
+        else update_cns(c);
+        return;
     }
     if (c->precision == HPG_FP32_DEFECT && !c->coarse_flag) {
-        HPG_TRY(c, mixed_update(c));
-        return 0;
+        mixed_update(c);
+        return;
     }
-    HPG_TRY(c, {
-        const size_t nv = (size_t)c->npnt * NV * 8, ns = (size_t)c->nseg * c->sm0 * NV * 8,
-                     ni = (size_t)c->ntri * c->im0 * NV * 8;
-        check(hipMemcpyAsync(c->ug0v, c->ugv, nv, hipMemcpyDeviceToDevice, c->stream), "D2D");
-        if (ns) check(hipMemcpyAsync(c->ug0s, c->ugs, ns, hipMemcpyDeviceToDevice, c->stream), "D2D");
-        if (ni) check(hipMemcpyAsync(c->ug0i, c->ugi, ni, hipMemcpyDeviceToDevice, c->stream), "D2D");
-        for (int stage = 0; stage < c->prm.nstage; ++stage) {
-            do_rsdl(c, stage);
-            do_minvrt(c, c->prm.alpha[stage] * c->prm.cfl[c->log2p]);
-        }
-        check(hipGetLastError(), "update launch");
-    });
+    if (c->lv[c->log2p].p == 1) {
+        // vertex modes only: every stage is one vstage_p1 (ug0 saved by the first)
+        for (int stage = 0; stage < c->prm.nstage; ++stage)
+            vstage_p1(c, stage, c->prm.alpha[stage] * c->prm.cfl[c->log2p], stage == 0);
+        return;
+    }
+    const size_t nv = (size_t)c->npnt * NV * 8, ns = (size_t)c->nseg * c->sm0 * NV * 8,
+                 ni = (size_t)c->ntri * c->im0 * NV * 8;
+    check(hipMemcpyAsync(c->ug0v, c->ugv, nv, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    if (ns) check(hipMemcpyAsync(c->ug0s, c->ugs, ns, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    if (ni) check(hipMemcpyAsync(c->ug0i, c->ugi, ni, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    for (int stage = 0; stage < c->prm.nstage; ++stage) {
+        do_rsdl(c, stage);
+        do_minvrt(c, c->prm.alpha[stage] * c->prm.cfl[c->log2p]);
+    }
+    check(hipGetLastError(), "update launch");
+}
+}  // namespace
+
+int hpg_update(hpg_ctx* c) {
+    if (!c) return -2;
+    HPG_TRY(c, do_update(c));
     return 0;
 }
 
@@ -1490,6 +1890,80 @@ int hpg_mg_prolongate(hpg_ctx* c) {
     return 0;
 }
 
+namespace {
+struct CycleRun {
+    hpg_ctx* const* grids;
+    int ngrid, extra, mglvls, vw, itercrsn, iterrfne;
+};
+
+// block::cycle (tri_mesh/blocks.cpp:1267-1348): level lvl uses grid
+// clamp(lvl - extra, 0, ngrid - 1) (:1274-1275); p-levels of the fine grid first
+void cycle_level(const CycleRun& r, int lvl) {
+    const int gl = std::min(std::max(lvl - r.extra, 0), r.ngrid - 1);
+    const int glp = std::min(std::max(lvl - r.extra + 1, 0), r.ngrid - 1);
+    hpg_ctx* g = r.grids[gl];
+    hpg_ctx* gp = r.grids[glp];
+    auto ok = [](hpg_ctx* x, int rc, const char* what) {
+        if (rc < 0) throw std::runtime_error(std::string(what) + ": " + x->err);
+    };
+    for (int vcount = 0; vcount < r.vw; ++vcount) {
+        do_setup(g, false);
+        for (int k = 0; k < r.itercrsn; ++k) do_update(g);
+        if (lvl + vcount == 0) {
+            // maxres of the entry level after its first smoothing (tri_hp/allocate.cpp:496-525),
+            // read after the cycle
+            hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, g->stream, 2 * NVC, (unsigned*)g->d_max);
+            const dim3 nb(std::min<unsigned>(nblk((long)g->npnt), 2048u));
+            if (g->nv == NVC)
+                hipLaunchKernelGGL(k_absmax<NVC>, nb, dim3(256), 0, g->stream, (long)g->npnt, 1, 1, g->resv, nullptr, g->d_max);
+            else
+                hipLaunchKernelGGL(k_absmax<NV>, nb, dim3(256), 0, g->stream, (long)g->npnt, 1, 1, g->resv, nullptr, g->d_max);
+        }
+        if (lvl == r.mglvls - 1) return;
+        do_rsdl(g, g->prm.nstage);
+        ok(gp, hpg_mg_restrict(gp), "mg_restrict");
+        cycle_level(r, lvl + 1);
+        ok(gp, hpg_mg_prolongate(gp), "mg_prolongate");
+        if (r.iterrfne) do_setup(g, false);
+        for (int k = 0; k < r.iterrfne; ++k) do_update(g);
+    }
+}
+}  // namespace
+
+// block::cycle over a stack of grids, enqueued without host round trips inside the cycle
+int hpg_mg_cycle(hpg_ctx* const* grids, int ngrid, int extra_finest_levels, int vw, int itercrsn, int iterrfne,
+                 double* maxres) {
+    if (!grids || ngrid < 1 || extra_finest_levels < 0 || vw < 1 || itercrsn < 0 || iterrfne < 0) return -2;
+    for (int k = 0; k < ngrid; ++k)
+        if (!grids[k] || (k > 0 && (!grids[k]->coarse_level || grids[k]->fine != grids[k - 1]))) return -2;
+    hpg_ctx* f = grids[0];
+    if (f->log2p != f->log2pmax || extra_finest_levels > f->log2pmax) { f->err = "cycle starts on the finest p-level"; return -2; }
+    int nerr = 0;
+    HPG_TRY(f, {
+        for (int k = 0; k < ngrid; ++k)
+            hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, grids[k]->stream, 1, (unsigned*)grids[k]->d_err);
+        CycleRun r{grids, ngrid, extra_finest_levels, ngrid + extra_finest_levels, vw, itercrsn, iterrfne};
+        cycle_level(r, 0);
+        check(hipGetLastError(), "cycle launch");
+        unsigned long long mx[NVC] = {0, 0, 0, 0};
+        check(hipMemcpyAsync(mx, f->d_max, sizeof(mx), hipMemcpyDeviceToHost, f->stream), "D2H");
+        for (int k = 0; k < ngrid; ++k) {
+            int e = 0;
+            check(hipMemcpyAsync(&e, grids[k]->d_err, sizeof(int), hipMemcpyDeviceToHost, grids[k]->stream), "D2H");
+            check(hipStreamSynchronize(grids[k]->stream), "sync");
+            nerr += e != 0;
+        }
+        double m = 0.0;
+        for (int n = 0; n < f->nv; ++n) {
+            double v;
+            std::memcpy(&v, &mx[n], sizeof(double));
+            m = std::max(m, v);
+        }
+        if (maxres) *maxres = m;
+    });
+    return nerr;
+}
+
 int hpg_get_log2p(hpg_ctx* c, int* l) {
     if (!c || !l) return -2;
     *l = c->log2p;
@@ -1498,8 +1972,18 @@ int hpg_get_log2p(hpg_ctx* c, int* l) {
 
 int hpg_set_precision(hpg_ctx* c, int mode) {
     if (!c || (mode != HPG_FP64 && mode != HPG_FP32_DEFECT)) return -2;
-    if (c->physics != HPG_PHYSICS_INS && mode != HPG_FP64) { c->err = "the FP32 defect-correction path is built for ins"; return -2; }
+    if (c->physics == HPG_PHYSICS_CNS && mode == HPG_FP32_DEFECT && !c->srcv) {
+        // the FP64 defect and u0 copies of the cns outer iteration, allocated on first use
+        HPG_TRY(c, {
+            const size_t nv = (size_t)c->npnt * c->nv, ns = (size_t)c->nseg * c->sm0 * c->nv,
+                         ni = (size_t)c->ntri * c->im0 * c->nv;
+            c->srcv = c->alloc<double>(nv); c->srcs = c->alloc<double>(ns); c->srci = c->alloc<double>(ni);
+            c->u0cv = c->alloc<double>(nv); c->u0cs = c->alloc<double>(ns); c->u0ci = c->alloc<double>(ni);
+            check(hipStreamSynchronize(c->stream), "sync");
+        });
+    }
     c->precision = mode;
+    c->fd_reset = true;
     return 0;
 }
 
@@ -1521,13 +2005,13 @@ int hpg_kernel_timing(hpg_ctx* c, int enable) {
 }
 
 int hpg_kernel_timing_read(hpg_ctx* c, int full, double* total_ms, long long* nlaunch, long long* nelem) {
-    if (!c || !total_ms || !nlaunch || !nelem) return -2;
+    if (!c || !total_ms || !nlaunch || !nelem || full < 0 || full > 3) return -2;
     HPG_TRY(c, {
         check(hipStreamSynchronize(c->stream), "sync");
         double tot = 0.0;
         long long nl = 0, ne = 0;
         for (size_t r = 0; r < c->recs.size(); ++r) {
-            if (c->recs[r].full != (full ? 1 : 0)) continue;
+            if (c->recs[r].full != full) continue;
             float ms = 0.f;
             check(hipEventElapsedTime(&ms, c->events[2 * r], c->events[2 * r + 1]), "hipEventElapsedTime");
             tot += ms;

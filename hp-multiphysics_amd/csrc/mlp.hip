@@ -1,6 +1,6 @@
 // Preconditioner MLP: training and inference kernels for gfx950.
 //
-// Network 11 -> 64 -> 64 -> 64 -> 3, GELU (tanh form) on the hidden layers.  The
+// Network 11 -> 64 -> 64 -> 64 -> 9, GELU (tanh form) on the hidden layers.  The
 // FP32 master weights are rounded to bfloat16 (nearest even) for every matrix
 // product and activations are rounded to bfloat16 on input to a product;
 // accumulation is FP32.  Inference runs on the matrix cores (bf16 MFMA, 16 samples
@@ -24,7 +24,8 @@
 
 namespace hpg {
 
-constexpr int MI = 11, MH = 64, MO = 3;
+constexpr int MI = MLP_IN, MH = 64, MO = MLP_OUT;
+static_assert(MO <= 16, "the inference head is one 16-row MFMA tile");
 constexpr int O_W1 = 0, O_B1 = O_W1 + MI * MH, O_W2 = O_B1 + MH, O_B2 = O_W2 + MH * MH, O_W3 = O_B2 + MH,
               O_B3 = O_W3 + MH * MH, O_W4 = O_B3 + MH, O_B4 = O_W4 + MH * MO, NPARAM = O_B4 + MO;
 constexpr float C_GELU = 0.7978845608028654f;  // sqrt(2/pi)
@@ -176,9 +177,10 @@ __global__ __launch_bounds__(256) void k_mlp_forward_mfma(int n, const float* __
         f32x4_t y = b4;
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) y = mfma_bf16(A4[kt], h[kt], y);
-        if (q == 0 && s < n)
+        // register i of lane group q holds output 4q + i of sample column c
 #pragma unroll
-            for (int i = 0; i < MO; ++i) Y[s * MO + i] = y[i];
+        for (int i = 0; i < 4; ++i)
+            if (4 * q + i < MO && s < n) Y[s * MO + 4 * q + i] = y[i];
     }
 }
 

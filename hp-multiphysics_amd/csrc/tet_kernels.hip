@@ -37,7 +37,10 @@ struct TDims {
     static_assert(OP_DR % 2 == 0 && DQ % 2 == 0 && DQ > G, "16-byte pairs of the derivative rows");
 };
 
-struct TetElemArgs {
+// T: arithmetic of the element kernel (double, or float in the FP32 inner solve of the
+// defect correction); state, tau and the element records are FP64 in both
+template <typename T>
+struct TetElemArgsT {
     const double* pnts;
     const int* tet_pnt;
     const int* tet_seg;
@@ -48,35 +51,37 @@ struct TetElemArgs {
     const double* uge;
     const double* tau;  // ntet x 4 (0: momentum, 3: continuity)
     double* lf;         // ntet x TM x 2 x 4: [t][m][0 = imaginary, 1 = real][n]
-    const double* ops;
+    const T* ops;
     double rho, mu, beta;
 };
+typedef TetElemArgsT<double> TetElemArgs;
+typedef TetElemArgsT<float> TetElemArgs32;
 
 // intgrtrst of a half-wave's point fluxes sf[3n + d][q] (d = r, s, t directions): lane
 // 3m + d sums direction d of mode m over the Gauss points, and lane 3m adds the three
 // partial sums (two shuffles inside the half-wave); lanes from 3 TM on carry zeros.
 // Every lane of the half-wave takes part.  One operator read and NVn flux reads per
 // point and lane, a third of the LDS traffic of a lane-per-mode sum.
-template <class D, int NVn, int ROW>
-__device__ __forceinline__ void tet_intgrtrst(const double* __restrict__ sops, const double (*sf)[ROW], int g,
-                                              double out[NVn]) {
+template <class D, int NVn, int ROW, typename T>
+__device__ __forceinline__ void tet_intgrtrst(const T* __restrict__ sops, const T (*sf)[ROW], int g,
+                                              T out[NVn]) {
     const int m = g / 3, d = g - 3 * (g / 3);
     const bool act = g < 3 * D::TM;
-    const double* op = sops + (d == 0 ? D::OP_IR : d == 1 ? D::OP_IS : D::OP_IT) + (act ? m : 0);
-    double s[NVn];
+    const T* op = sops + (d == 0 ? D::OP_IR : d == 1 ? D::OP_IS : D::OP_IT) + (act ? m : 0);
+    T s[NVn];
 #pragma unroll
-    for (int n = 0; n < NVn; ++n) s[n] = 0.0;
+    for (int n = 0; n < NVn; ++n) s[n] = T(0.0);
     if (act) {
 #pragma unroll 3
         for (int q = 0; q < D::G; ++q) {
-            const double iq = op[q * 16];
+            const T iq = op[q * 16];
 #pragma unroll
             for (int n = 0; n < NVn; ++n) s[n] += iq * sf[3 * n + d][q];
         }
     }
 #pragma unroll
     for (int n = 0; n < NVn; ++n) {
-        const double t1 = __shfl_down(s[n], 1, 64), t2 = __shfl_down(s[n], 2, 64);
+        const T t1 = __shfl_down(s[n], 1, 64), t2 = __shfl_down(s[n], 2, 64);
         out[n] = s[n] + t1 + t2;
     }
 }
@@ -84,10 +89,10 @@ __device__ __forceinline__ void tet_intgrtrst(const double* __restrict__ sops, c
 // strong residual res[n] += sum_q (Dr[g][q] F[3n][q] + Ds[g][q] F[3n+1][q] + Dt[g][q] F[3n+2][q])
 // of the Gauss point g, two points per 16-byte LDS load of the operator rows and of
 // the (broadcast) flux rows; the zero pads at q = G close the odd count
-template <class D, int NVn>
-__device__ __forceinline__ void tet_strong_residual(const double* __restrict__ sops, const double (*sf)[D::FROW], int g,
-                                                    double res[NVn]) {
-    typedef double d2 __attribute__((ext_vector_type(2)));
+template <class D, int NVn, typename T>
+__device__ __forceinline__ void tet_strong_residual(const T* __restrict__ sops, const T (*sf)[D::FROW], int g,
+                                                    T res[NVn]) {
+    typedef T d2 __attribute__((ext_vector_type(2)));
     const d2* dr = reinterpret_cast<const d2*>(sops + D::OP_DR + g * D::DQ);
     const d2* ds = reinterpret_cast<const d2*>(sops + D::OP_DS + g * D::DQ);
     const d2* dt = reinterpret_cast<const d2*>(sops + D::OP_DT + g * D::DQ);
@@ -107,13 +112,13 @@ __device__ __forceinline__ void tet_strong_residual(const double* __restrict__ s
 
 // tet_hp_ins::element_rsdl, straight branch (tet_hp/ins/rsdl.cpp:15-87, 442-785),
 // steady, tau frozen by setup_preconditioner
-template <int P, bool FULL>
-__global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
+template <typename T, int P, bool FULL>
+__device__ __forceinline__ void elem_tet_body(const TetElemArgsT<T>& a, int noct) {
     using D = TDims<P>;
     constexpr int TM = D::TM, G = D::G;
-    __shared__ __attribute__((aligned(16))) double sops[D::SIZE];
-    __shared__ double suh[8][TNV][16];
-    __shared__ __attribute__((aligned(16))) double sf[8][12][D::FROW];  // point flux rows, padded
+    __shared__ __attribute__((aligned(16))) T sops[D::SIZE];
+    __shared__ T suh[8][TNV][16];
+    __shared__ __attribute__((aligned(16))) T sf[8][12][D::FROW];  // point flux rows, padded
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     __syncthreads();  // operators staged
@@ -140,24 +145,24 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
                     const int s = e == 0 ? seg[0] : e == 1 ? seg[1] : e == 2 ? seg[2] : e == 3 ? seg[3] : e == 4 ? seg[4] : seg[5];
                     v = a.uge[(s * a.em0) * TNV + n];
                 }
-                suh[hw][n][g] = valid ? v : (n == 0 && g < 4 ? 1.0 : 0.0);
+                suh[hw][n][g] = valid ? T(v) : (n == 0 && g < 4 ? T(1.0) : T(0.0));
             }
         }
         // coordinate derivatives (rsdl.cpp:58-62) and cofactors (:449-458); padding
         // half-waves use the reference tet
-        double X[4][3];
+        T X[4][3];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) X[k][n] = valid ? a.pnts[3 * vtx[k] + n] : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? 1.0 : -1.0);
-        double ld[3][3];
+            for (int n = 0; n < 3; ++n) X[k][n] = valid ? T(a.pnts[3 * vtx[k] + n]) : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? T(1.0) : -T(1.0));
+        T ld[3][3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
-            ld[n][0] = 0.5 * (X[3][n] - X[2][n]);
-            ld[n][1] = 0.5 * (X[1][n] - X[2][n]);
-            ld[n][2] = 0.5 * (X[0][n] - X[2][n]);
+            ld[n][0] = T(0.5) * (X[3][n] - X[2][n]);
+            ld[n][1] = T(0.5) * (X[1][n] - X[2][n]);
+            ld[n][2] = T(0.5) * (X[0][n] - X[2][n]);
         }
-        double d[3][3];
+        T d[3][3];
         d[0][0] = ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1];
         d[0][1] = -ld[0][1] * ld[2][2] + ld[0][2] * ld[2][1];
         d[0][2] = ld[0][1] * ld[1][2] - ld[0][2] * ld[1][1];
@@ -167,26 +172,26 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
         d[2][0] = ld[1][0] * ld[2][1] - ld[1][1] * ld[2][0];
         d[2][1] = -ld[0][0] * ld[2][1] + ld[0][1] * ld[2][0];
         d[2][2] = ld[0][0] * ld[1][1] - ld[0][1] * ld[1][0];
-        const double cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
+        const T cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
                             ld[0][1] * (ld[1][0] * ld[2][2] - ld[2][0] * ld[1][2]) +
                             ld[0][2] * (ld[1][0] * ld[2][1] - ld[2][0] * ld[1][1]);
         wave_lds_sync();
         // values (and velocity r, s, t derivatives) at this lane's Gauss point
-        double u[TNV], du[3][3];
+        T u[TNV], du[3][3];
 #pragma unroll
-        for (int n = 0; n < TNV; ++n) u[n] = 0.0;
+        for (int n = 0; n < TNV; ++n) u[n] = T(0.0);
 #pragma unroll
         for (int n = 0; n < 3; ++n)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) du[n][c] = 0.0;
+            for (int c = 0; c < 3; ++c) du[n][c] = T(0.0);
         if (g < G) {
 #pragma unroll
             for (int k = 0; k < TM; ++k) {
-                const double pk = sops[D::OP_P + k * 32 + g];
+                const T pk = sops[D::OP_P + k * 32 + g];
 #pragma unroll
                 for (int n = 0; n < TNV; ++n) u[n] += pk * suh[hw][n][k];
                 if constexpr (FULL) {
-                    const double pr = sops[D::OP_PR + k * 32 + g], ps = sops[D::OP_PS + k * 32 + g],
+                    const T pr = sops[D::OP_PR + k * 32 + g], ps = sops[D::OP_PS + k * 32 + g],
                                  pt = sops[D::OP_PT + k * 32 + g];
 #pragma unroll
                     for (int n = 0; n < 3; ++n) {
@@ -198,9 +203,9 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
             }
         }
         // convective and pressure fluxes (rsdl.cpp:461-508)
-        double cv[3][3], c3[3];
+        T cv[3][3], c3[3];
         {
-            const double fx = a.rho * u[0], fy = a.rho * u[1], fz = a.rho * u[2];
+            const T fx = T(a.rho) * u[0], fy = T(a.rho) * u[1], fz = T(a.rho) * u[2];
 #pragma unroll
             for (int c = 0; c < 3; ++c) c3[c] = d[c][0] * fx + d[c][1] * fy + d[c][2] * fz;
 #pragma unroll
@@ -219,8 +224,8 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
         double* lfo = a.lf + (size_t)tt * (2 * TNV * TM);
         const bool owner = valid && g < 3 * TM && g % 3 == 0;
         {
-            double li[TNV];
-            tet_intgrtrst<D, TNV, D::FROW>(sops, sf[hw], g, li);
+            T li[TNV];
+            tet_intgrtrst<D, TNV, D::FROW, T>(sops, sf[hw], g, li);
             if (owner)
 #pragma unroll
                 for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + n] = li[n];
@@ -228,18 +233,18 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
         if constexpr (FULL) {
             // viscous fluxes (rsdl.cpp:518-674): visc(n,m)(a,b) = -mu/cjcb (delta_nm
             // sum_c d(a)(c) d(b)(c) + d(a)(m) d(b)(n)) applied to du(m,b)
-            const double cjcbi = a.mu / cjcb;
-            double df[3][3];
+            const T cjcbi = T(a.mu) / cjcb;
+            T df[3][3];
 #pragma unroll
             for (int n = 0; n < 3; ++n)
 #pragma unroll
                 for (int aa = 0; aa < 3; ++aa) {
-                    double s = 0.0;
+                    T s = T(0.0);
 #pragma unroll
                     for (int bb = 0; bb < 3; ++bb) {
-                        const double dd = d[aa][0] * d[bb][0] + d[aa][1] * d[bb][1] + d[aa][2] * d[bb][2];
+                        const T dd = d[aa][0] * d[bb][0] + d[aa][1] * d[bb][1] + d[aa][2] * d[bb][2];
 #pragma unroll
-                        for (int m = 0; m < 3; ++m) s += (d[aa][m] * d[bb][n] + (m == n ? dd : 0.0)) * du[m][bb];
+                        for (int m = 0; m < 3; ++m) s += (d[aa][m] * d[bb][n] + (m == n ? dd : T(0.0))) * du[m][bb];
                     }
                     df[n][aa] = -cjcbi * s;
                 }
@@ -250,41 +255,52 @@ __global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
                 for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = cv[n][c] + df[n][c];
             wave_lds_sync();
             // strong residual: derivr/derivs/derivt of the total fluxes (rsdl.cpp:683-692)
-            double res[TNV] = {0.0, 0.0, 0.0, 0.0};
-            if (g < G) tet_strong_residual<D, TNV>(sops, sf[hw], g, res);
+            T res[TNV] = {T(0.0), T(0.0), T(0.0), T(0.0)};
+            if (g < G) tet_strong_residual<D, TNV, T>(sops, sf[hw], g, res);
             // stabilisation (rsdl.cpp:695-767)
-            const double t0 = a.tau[4 * tt] * res[0], t1 = a.tau[4 * tt] * res[1], t2 = a.tau[4 * tt] * res[2],
-                         t3 = a.tau[4 * tt + 3] * res[3];
+            const T t0 = T(a.tau[4 * tt]) * res[0], t1 = T(a.tau[4 * tt]) * res[1], t2 = T(a.tau[4 * tt]) * res[2],
+                         t3 = T(a.tau[4 * tt + 3]) * res[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                df[0][c] -= (d[c][0] * (2. * u[0]) + d[c][1] * u[1] + d[c][2] * u[2]) * t0 + d[c][1] * u[0] * t1 +
+                df[0][c] -= (d[c][0] * (T(2.) * u[0]) + d[c][1] * u[1] + d[c][2] * u[2]) * t0 + d[c][1] * u[0] * t1 +
                             d[c][2] * u[0] * t2 + d[c][0] * t3;
-                df[1][c] -= d[c][0] * u[1] * t0 + (d[c][0] * u[0] + d[c][1] * (2. * u[1]) + d[c][2] * u[2]) * t1 +
+                df[1][c] -= d[c][0] * u[1] * t0 + (d[c][0] * u[0] + d[c][1] * (T(2.) * u[1]) + d[c][2] * u[2]) * t1 +
                             d[c][2] * u[1] * t2 + d[c][1] * t3;
                 df[2][c] -= d[c][0] * u[2] * t0 + d[c][1] * u[2] * t1 +
-                            (d[c][0] * u[0] + d[c][1] * u[1] + d[c][2] * (2. * u[2])) * t2 + d[c][2] * t3;
+                            (d[c][0] * u[0] + d[c][1] * u[1] + d[c][2] * (T(2.) * u[2])) * t2 + d[c][2] * t3;
                 c3[c] = -d[c][0] * t0 - d[c][1] * t1 - d[c][2] * t2;
             }
             wave_lds_sync();  // the half-wave is done reading the total fluxes
 #pragma unroll
             for (int n = 0; n < 3; ++n)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? df[n][c] : 0.0;
+                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? df[n][c] : T(0.0);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) sf[hw][9 + c][g] = g < G ? c3[c] : 0.0;
+            for (int c = 0; c < 3; ++c) sf[hw][9 + c][g] = g < G ? c3[c] : T(0.0);
             wave_lds_sync();
             // lf_re = beta * intgrtrst (rsdl.cpp:768-776)
-            double lr[TNV];
-            tet_intgrtrst<D, TNV, D::FROW>(sops, sf[hw], g, lr);
+            T lr[TNV];
+            tet_intgrtrst<D, TNV, D::FROW, T>(sops, sf[hw], g, lr);
             if (owner)
 #pragma unroll
-                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = lr[n] * a.beta;
+                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = lr[n] * T(a.beta);
         } else {
             if (owner)
 #pragma unroll
-                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = 0.0;
+                for (int n = 0; n < TNV; ++n) lfo[(g / 3) * (2 * TNV) + TNV + n] = T(0.0);
         }
     }
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tet(TetElemArgs a, int noct) {
+    elem_tet_body<double, P, FULL>(a, noct);
+}
+
+// FP32 arithmetic (the inner solve of the defect correction); FP64 state and records
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tet32(TetElemArgs32 a, int noct) {
+    elem_tet_body<float, P, FULL>(a, noct);
 }
 
 // inflow face flux, continuity only (tet_hp/ins/bdry.cpp:59-102, bdry_ins.h:122-134):

@@ -353,7 +353,10 @@ __global__ __launch_bounds__(256) void k_setup_gather_tetc(int nent, const int* 
     for (int n = 0; n < CNV; ++n) prcn[CNV * e + n] = 1.0 / sj;
 }
 
-struct TetcElemArgs {
+// T: arithmetic of the element kernel (double, or float in the FP32 inner solve of the
+// defect correction); state, tau and the element records are FP64 in both
+template <typename T>
+struct TetcElemArgsT {
     const double* pnts;
     const int* tet_pnt;
     const int* tet_seg;
@@ -363,31 +366,34 @@ struct TetcElemArgs {
     const double* uge;
     const double* tau;  // ntet x 25
     double* lf;         // ntet x TM x 2 x 5: [t][m][0 = imaginary, 1 = real][n]
-    const double* ops;
+    const T* ops;
     double mu, kcond, R, gamma, atm, beta;
 };
+typedef TetcElemArgsT<double> TetcElemArgs;
+typedef TetcElemArgsT<float> TetcElemArgs32;
 
 // df(m, a) -= sum_n (d(a)(0) A + d(a)(1) B + d(a)(2) C)(m, n) tres(n) with the
 // conservative flux Jacobians of rsdl.cpp:799-826
-__device__ __forceinline__ void tetc_stabilise(const double u[5], const double tres[5], const double d[3][3],
-                                               double gogm1, double atm, double df[5][3]) {
-    const double pr = u[0], uv = u[1], vv = u[2], wv = u[3], rt = u[4];
-    const double ke = 0.5 * (uv * uv + vv * vv + wv * wv);
-    const double rho = (pr + atm) / rt;
-    const double A[25] = {uv / rt, rho, 0.0, 0.0, -rho * uv / rt,
-                          uv * uv / rt + 1.0, 2.0 * rho * uv, 0.0, 0.0, -rho * uv * uv / rt,
-                          uv * vv / rt, rho * vv, rho * uv, 0.0, -rho * uv * vv / rt,
-                          uv * wv / rt, rho * wv, 0.0, rho * uv, -rho * uv * wv / rt,
+template <typename T>
+__device__ __forceinline__ void tetc_stabilise(const T u[5], const T tres[5], const T d[3][3],
+                                               T gogm1, T atm, T df[5][3]) {
+    const T pr = u[0], uv = u[1], vv = u[2], wv = u[3], rt = u[4];
+    const T ke = T(0.5) * (uv * uv + vv * vv + wv * wv);
+    const T rho = (pr + atm) / rt;
+    const T A[25] = {uv / rt, rho, T(0.0), T(0.0), -rho * uv / rt,
+                          uv * uv / rt + T(1.0), T(2.0) * rho * uv, T(0.0), T(0.0), -rho * uv * uv / rt,
+                          uv * vv / rt, rho * vv, rho * uv, T(0.0), -rho * uv * vv / rt,
+                          uv * wv / rt, rho * wv, T(0.0), rho * uv, -rho * uv * wv / rt,
                           uv * (gogm1 * rt + ke) / rt, rho * (gogm1 * rt + ke) + rho * uv * uv, rho * uv * vv, rho * uv * wv, -rho * uv * ke / rt};
-    const double B[25] = {vv / rt, 0.0, rho, 0.0, -rho * vv / rt,
-                          uv * vv / rt, rho * vv, rho * uv, 0.0, -rho * uv * vv / rt,
-                          vv * vv / rt + 1.0, 0.0, 2.0 * rho * vv, 0.0, -rho * vv * vv / rt,
-                          vv * wv / rt, 0.0, rho * wv, rho * vv, -rho * vv * wv / rt,
+    const T B[25] = {vv / rt, T(0.0), rho, T(0.0), -rho * vv / rt,
+                          uv * vv / rt, rho * vv, rho * uv, T(0.0), -rho * uv * vv / rt,
+                          vv * vv / rt + T(1.0), T(0.0), T(2.0) * rho * vv, T(0.0), -rho * vv * vv / rt,
+                          vv * wv / rt, T(0.0), rho * wv, rho * vv, -rho * vv * wv / rt,
                           vv * (gogm1 * rt + ke) / rt, rho * uv * vv, rho * (gogm1 * rt + ke) + rho * vv * vv, rho * vv * wv, -rho * vv * ke / rt};
-    const double C[25] = {wv / rt, 0.0, 0.0, rho, -rho * wv / rt,
-                          uv * wv / rt, rho * wv, 0.0, rho * uv, -rho * uv * wv / rt,
-                          vv * wv / rt, 0.0, rho * wv, rho * vv, -rho * vv * wv / rt,
-                          wv * wv / rt + 1.0, 0.0, 0.0, 2.0 * rho * wv, -rho * wv * wv / rt,
+    const T C[25] = {wv / rt, T(0.0), T(0.0), rho, -rho * wv / rt,
+                          uv * wv / rt, rho * wv, T(0.0), rho * uv, -rho * uv * wv / rt,
+                          vv * wv / rt, T(0.0), rho * wv, rho * vv, -rho * vv * wv / rt,
+                          wv * wv / rt + T(1.0), T(0.0), T(0.0), T(2.0) * rho * wv, -rho * wv * wv / rt,
                           wv * (gogm1 * rt + ke) / rt, rho * uv * wv, rho * vv * wv, rho * (gogm1 * rt + ke) + rho * wv * wv, -rho * wv * ke / rt};
 #pragma unroll
     for (int m = 0; m < 5; ++m)
@@ -401,17 +407,17 @@ __device__ __forceinline__ void tetc_stabilise(const double u[5], const double t
 
 // tet_hp_cns::element_rsdl, straight branch (tet_hp/cns/rsdl.cpp:16-87, 472-843), steady,
 // tau frozen by setup_preconditioner, body force and reference density zero
-template <int P, bool FULL>
-__global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
+template <typename T, int P, bool FULL>
+__device__ __forceinline__ void elem_tetc_body(const TetcElemArgsT<T>& a, int noct) {
     using D = TDims<P>;
     constexpr int TM = D::TM, G = D::G;
-    __shared__ __attribute__((aligned(16))) double sops[D::SIZE];
-    __shared__ double suh[8][CNV][16];
-    __shared__ __attribute__((aligned(16))) double sf[8][15][D::FROW];  // point flux rows, padded
-    __shared__ double stau[8][25];
+    __shared__ __attribute__((aligned(16))) T sops[D::SIZE];
+    __shared__ T suh[8][CNV][16];
+    __shared__ __attribute__((aligned(16))) T sf[8][15][D::FROW];  // point flux rows, padded
+    __shared__ T stau[8][25];
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
-    const double gm1 = a.gamma - 1.0, gogm1 = a.gamma / gm1;
+    const T gm1 = T(a.gamma) - T(1.0), gogm1 = T(a.gamma) / gm1;
     __syncthreads();  // operators staged
     HPG_OCTET_LOOP(noct, oc) {
         wave_lds_sync();  // previous octet done with this half-wave's LDS rows
@@ -436,23 +442,23 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
                     const int s = e == 0 ? seg[0] : e == 1 ? seg[1] : e == 2 ? seg[2] : e == 3 ? seg[3] : e == 4 ? seg[4] : seg[5];
                     v = a.uge[(s * a.em0) * CNV + n];
                 }
-                suh[hw][n][g] = valid ? v : ((n == 0 || n == 4) && g < 4 ? 1.0 : 0.0);
+                suh[hw][n][g] = valid ? T(v) : ((n == 0 || n == 4) && g < 4 ? T(1.0) : T(0.0));
             }
         }
-        if (FULL && g < 25) stau[hw][g] = valid ? a.tau[25L * tt + g] : 0.0;
-        double X[4][3];
+        if (FULL && g < 25) stau[hw][g] = valid ? T(a.tau[25L * tt + g]) : T(0.0);
+        T X[4][3];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) X[k][n] = valid ? a.pnts[3 * vtx[k] + n] : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? 1.0 : -1.0);
-        double ld[3][3];
+            for (int n = 0; n < 3; ++n) X[k][n] = valid ? T(a.pnts[3 * vtx[k] + n]) : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? T(1.0) : -T(1.0));
+        T ld[3][3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
-            ld[n][0] = 0.5 * (X[3][n] - X[2][n]);
-            ld[n][1] = 0.5 * (X[1][n] - X[2][n]);
-            ld[n][2] = 0.5 * (X[0][n] - X[2][n]);
+            ld[n][0] = T(0.5) * (X[3][n] - X[2][n]);
+            ld[n][1] = T(0.5) * (X[1][n] - X[2][n]);
+            ld[n][2] = T(0.5) * (X[0][n] - X[2][n]);
         }
-        double d[3][3];
+        T d[3][3];
         d[0][0] = ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1];
         d[0][1] = -ld[0][1] * ld[2][2] + ld[0][2] * ld[2][1];
         d[0][2] = ld[0][1] * ld[1][2] - ld[0][2] * ld[1][1];
@@ -462,26 +468,26 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
         d[2][0] = ld[1][0] * ld[2][1] - ld[1][1] * ld[2][0];
         d[2][1] = -ld[0][0] * ld[2][1] + ld[0][1] * ld[2][0];
         d[2][2] = ld[0][0] * ld[1][1] - ld[0][1] * ld[1][0];
-        const double cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
+        const T cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
                             ld[0][1] * (ld[1][0] * ld[2][2] - ld[2][0] * ld[1][2]) +
                             ld[0][2] * (ld[1][0] * ld[2][1] - ld[2][0] * ld[1][1]);
         wave_lds_sync();
         // values (and r, s, t derivatives of u, v, w, RT) at this lane's Gauss point
-        double u[CNV], du[4][3];
+        T u[CNV], du[4][3];
 #pragma unroll
-        for (int n = 0; n < CNV; ++n) u[n] = 0.0;
+        for (int n = 0; n < CNV; ++n) u[n] = T(0.0);
 #pragma unroll
         for (int n = 0; n < 4; ++n)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) du[n][c] = 0.0;
+            for (int c = 0; c < 3; ++c) du[n][c] = T(0.0);
         if (g < G) {
 #pragma unroll
             for (int k = 0; k < TM; ++k) {
-                const double pk = sops[D::OP_P + k * 32 + g];
+                const T pk = sops[D::OP_P + k * 32 + g];
 #pragma unroll
                 for (int n = 0; n < CNV; ++n) u[n] += pk * suh[hw][n][k];
                 if constexpr (FULL) {
-                    const double pr = sops[D::OP_PR + k * 32 + g], ps = sops[D::OP_PS + k * 32 + g],
+                    const T pr = sops[D::OP_PR + k * 32 + g], ps = sops[D::OP_PS + k * 32 + g],
                                  pt = sops[D::OP_PT + k * 32 + g];
 #pragma unroll
                     for (int n = 0; n < 4; ++n) {
@@ -492,35 +498,35 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
                 }
             }
         } else {
-            u[0] = 1.0;
-            u[4] = 1.0;  // keeps the lanes past the last Gauss point finite
+            u[0] = T(1.0);
+            u[4] = T(1.0);  // keeps the lanes past the last Gauss point finite
         }
         // convective, pressure and energy fluxes (rsdl.cpp:494-540)
-        double cv[CNV][3];
+        T cv[CNV][3];
         {
-            const double rho = (u[0] + a.atm) / u[4];
-            const double fx = rho * u[1], fy = rho * u[2], fz = rho * u[3];
+            const T rho = (u[0] + T(a.atm)) / u[4];
+            const T fx = rho * u[1], fy = rho * u[2], fz = rho * u[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) cv[0][c] = d[c][0] * fx + d[c][1] * fy + d[c][2] * fz;
 #pragma unroll
             for (int n = 1; n < 4; ++n)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) cv[n][c] = u[n] * cv[0][c] + d[c][n - 1] * u[0];
-            const double hh = gogm1 * u[4] + 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
+            const T hh = gogm1 * u[4] + T(0.5) * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) cv[4][c] = hh * cv[0][c];
         }
 #pragma unroll
         for (int n = 0; n < CNV; ++n)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? cv[n][c] : 0.0;
+            for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? cv[n][c] : T(0.0);
         wave_lds_sync();
         // lf_im = intgrtrst of the fluxes (rsdl.cpp:542-543); mode g / 3 lands in lane g
         double* lfo = a.lf + (size_t)tt * (2 * CNV * TM);
         const bool owner = valid && g < 3 * TM && g % 3 == 0;
         {
-            double li[CNV];
-            tet_intgrtrst<D, CNV, D::FROW>(sops, sf[hw], g, li);
+            T li[CNV];
+            tet_intgrtrst<D, CNV, D::FROW, T>(sops, sf[hw], g, li);
             if (owner)
 #pragma unroll
                 for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + n] = li[n];
@@ -528,27 +534,27 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
         if constexpr (FULL) {
             // viscous stress and heat flux (rsdl.cpp:584-755): visc(n,m)(a,b) applied to
             // du(m+1, b); energy: kcond tensor on d(RT) plus the viscous dissipation
-            const double mujcbi = a.mu / cjcb, kcjcbi = a.kcond / cjcb / a.R;
-            double df[CNV][3];
+            const T mujcbi = T(a.mu) / cjcb, kcjcbi = T(a.kcond) / cjcb / T(a.R);
+            T df[CNV][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) df[0][c] = 0.0;
+            for (int c = 0; c < 3; ++c) df[0][c] = T(0.0);
 #pragma unroll
             for (int n = 0; n < 3; ++n)
 #pragma unroll
                 for (int aa = 0; aa < 3; ++aa) {
-                    double s = 0.0;
+                    T s = T(0.0);
 #pragma unroll
                     for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
                         for (int m = 0; m < 3; ++m) {
-                            double vs;
+                            T vs;
                             if (n == m) {
-                                vs = 4. / 3. * d[aa][n] * d[bb][n];
+                                vs = T(4.) / T(3.) * d[aa][n] * d[bb][n];
 #pragma unroll
                                 for (int c = 0; c < 3; ++c)
                                     if (c != n) vs += d[aa][c] * d[bb][c];
                             } else {
-                                vs = d[aa][m] * d[bb][n] - 2. / 3. * d[aa][n] * d[bb][m];
+                                vs = d[aa][m] * d[bb][n] - T(2.) / T(3.) * d[aa][n] * d[bb][m];
                             }
                             s += (-mujcbi * vs) * du[m][bb];
                         }
@@ -556,9 +562,9 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
                 }
 #pragma unroll
             for (int aa = 0; aa < 3; ++aa) {
-                const double k0 = -kcjcbi * (d[aa][0] * d[0][0] + d[aa][1] * d[0][1] + d[aa][2] * d[0][2]);
-                const double k1 = -kcjcbi * (d[aa][0] * d[1][0] + d[aa][1] * d[1][1] + d[aa][2] * d[1][2]);
-                const double k2 = -kcjcbi * (d[aa][0] * d[2][0] + d[aa][1] * d[2][1] + d[aa][2] * d[2][2]);
+                const T k0 = -kcjcbi * (d[aa][0] * d[0][0] + d[aa][1] * d[0][1] + d[aa][2] * d[0][2]);
+                const T k1 = -kcjcbi * (d[aa][0] * d[1][0] + d[aa][1] * d[1][1] + d[aa][2] * d[1][2]);
+                const T k2 = -kcjcbi * (d[aa][0] * d[2][0] + d[aa][1] * d[2][1] + d[aa][2] * d[2][2]);
                 df[4][aa] = k0 * du[3][0] + k1 * du[3][1] + k2 * du[3][2];
                 df[4][aa] += df[1][aa] * u[1] + df[2][aa] * u[2] + df[3][aa] * u[3];
             }
@@ -566,39 +572,50 @@ __global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
 #pragma unroll
             for (int n = 0; n < CNV; ++n)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? cv[n][c] + df[n][c] : 0.0;
+                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? cv[n][c] + df[n][c] : T(0.0);
             wave_lds_sync();
             // strong residual: derivr/derivs/derivt of the total fluxes (rsdl.cpp:768-773)
-            double res[CNV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            if (g < G) tet_strong_residual<D, CNV>(sops, sf[hw], g, res);
+            T res[CNV] = {T(0.0), T(0.0), T(0.0), T(0.0), T(0.0)};
+            if (g < G) tet_strong_residual<D, CNV, T>(sops, sf[hw], g, res);
             // stabilisation with the element tau (rsdl.cpp:776-827)
-            double tres[CNV];
+            T tres[CNV];
 #pragma unroll
             for (int m = 0; m < CNV; ++m) {
-                double s = 0.0;
+                T s = T(0.0);
 #pragma unroll
                 for (int n = 0; n < CNV; ++n) s += stau[hw][5 * m + n] * res[n];
                 tres[m] = s;
             }
-            tetc_stabilise(u, tres, d, gogm1, a.atm, df);
+            tetc_stabilise<T>(u, tres, d, gogm1, T(a.atm), df);
             wave_lds_sync();  // the half-wave is done reading the total fluxes
 #pragma unroll
             for (int n = 0; n < CNV; ++n)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? df[n][c] : 0.0;
+                for (int c = 0; c < 3; ++c) sf[hw][3 * n + c][g] = g < G ? df[n][c] : T(0.0);
             wave_lds_sync();
             // lf_re = beta * intgrtrst (rsdl.cpp:829-835)
-            double lr[CNV];
-            tet_intgrtrst<D, CNV, D::FROW>(sops, sf[hw], g, lr);
+            T lr[CNV];
+            tet_intgrtrst<D, CNV, D::FROW, T>(sops, sf[hw], g, lr);
             if (owner)
 #pragma unroll
-                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = lr[n] * a.beta;
+                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = lr[n] * T(a.beta);
         } else {
             if (owner)
 #pragma unroll
-                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = 0.0;
+                for (int n = 0; n < CNV; ++n) lfo[(g / 3) * (2 * CNV) + CNV + n] = T(0.0);
         }
     }
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tetc(TetcElemArgs a, int noct) {
+    elem_tetc_body<double, P, FULL>(a, noct);
+}
+
+// FP32 arithmetic (the inner solve of the defect correction); FP64 state and records
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tetc32(TetcElemArgs32 a, int noct) {
+    elem_tetc_body<float, P, FULL>(a, noct);
 }
 
 // inflow face flux (tet_hp/cns/bdry.cpp:58-104, 129-150): pressure from the solution,

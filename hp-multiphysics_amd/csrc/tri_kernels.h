@@ -24,36 +24,47 @@ struct Dims {
     static constexpr int PACK = OFF_IS + INT;
 };
 
-struct ElemArgs {
+// Arguments of the element residual kernel: T = double (the FP64 residual) or float
+// (the FP32 residual of the defect-correction inner solve)
+template <typename T>
+struct ElemArgsT {
     const double* pnts;
     const int* tri_pnt;
     const int* tri_seg;
     const int* tri_sgn;
     const int* order;
     int begin, end;
-    const double* ugv;
-    const double* ugs;
-    const double* ugi;
-    double* resv;  // imaginary part (lf_im) target
-    double* ress;
-    double* resi;
-    double* rrv;   // real part (lf_re) target
-    double* rrs;
-    double* rri;
-    const double* ops;
+    const T* ugv;
+    const T* ugs;
+    const T* ugi;
+    T* resv;  // imaginary part (lf_im) target
+    T* ress;
+    T* resi;
+    T* rrv;   // real part (lf_re) target
+    T* rrs;
+    T* rri;
+    const T* ops;
     int sm0, im0;
-    double rho, mu, adis, beta;
+    T rho, mu, adis, beta;
     // first colour that touches each vertex / side: that colour initialises the
     // coefficient from the stage's carried real part instead of accumulating
     const unsigned char* fcv;
     const unsigned char* fcs;
     int colour;
-    double keep;  // 1 - beta(stage) for stage < nstage, 0 for the full residual
-    // optional: per-Gauss-point linearisation data (u, v, strong residual) in FP32,
-    // stored per 16-element group in the accumulator layout; group = gbase + wave task
+    T keep;  // 1 - beta(stage) for stage < nstage, 0 for the full residual
+    // optional (FP64): per-Gauss-point linearisation data (u, v, strong residual) in
+    // FP32, stored per 16-element group in the accumulator layout; group = gbase + wave task
     float* lin;
     int gbase;
+    // optional (FP32): constant source sfac[0] * src added on the first visit of every
+    // coefficient (the scaled FP64 residual of the defect correction)
+    const double* srcv;
+    const double* srcs;
+    const double* srci;
+    const double* sfac;
 };
+typedef ElemArgsT<double> ElemArgs;
+typedef ElemArgsT<float> ElemArgs32;
 
 // FP32 linearised residual J(u0) * delta of one colour (mixed-precision inner solve)
 struct LinArgs {

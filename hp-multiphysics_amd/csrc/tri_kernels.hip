@@ -17,9 +17,18 @@ namespace hpg {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// v_mfma_f32_16x16x4f32: accumulator register i of lane l holds D[4*(l/16) + i][l%16]
+__device__ __forceinline__ f4 mfma(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { typedef d4 type; };
+template <> struct Vec4<float> { typedef f4 type; };
 
 // ---------------------------------------------------------------------------------
 // tri_hp::ugtouht / lftog index map for mode k of element t (tri_hp/gtol.cpp:14-55,
@@ -122,11 +131,12 @@ __device__ __forceinline__ ElemSlots<P> elem_slots(const A& a, const ElemConn& c
 // residual values of the current group are requested before its contractions), so
 // the dependent gathers overlap the MFMA work of earlier groups.  The grid is a
 // multiple of 8 blocks of 4 waves.
-template <int P, bool FULL>
-__global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
+template <typename T, int P, bool FULL, bool EM = false>
+__device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restrict__ eres = nullptr) {
     using D = Dims<P>;
+    using V4 = typename Vec4<T>::type;
     constexpr int GT = D::GT, KT = D::KT;
-    __shared__ double sops[D::PACK];
+    __shared__ T sops[D::PACK];
     for (int k = threadIdx.x; k < D::PACK; k += blockDim.x) sops[k] = a.ops[k];
     __syncthreads();
 
@@ -151,19 +161,19 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         // old residual values of this group, consumed by the scatter at the end and
         // requested late (register pressure): before the last contraction of the full
         // residual, after the contractions of the inviscid one
-        double ro[NV][KT], io[NV][KT];
+        T ro[NV][KT], io[NV][KT];
         auto load_old = [&]() {
 #pragma unroll
             for (int s = 0; s < KT; ++s) {
                 const int k = 4 * s + q, kind = mode_kind<P>(k);
                 const bool live = S.t >= 0 && k < D::TM;
                 const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
-                const double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
-                const double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+                const T* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
+                const T* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
 #pragma unroll
                 for (int n = 0; n < NV; ++n) {
-                    ro[n][s] = (live && (FULL || a.keep != 0.0)) ? dr[off + n] : 0.0;
-                    io[n][s] = (live && !((S.first >> s) & 1)) ? di[off + n] : 0.0;
+                    ro[n][s] = (live && (FULL || a.keep != T(0))) ? dr[off + n] : T(0);
+                    io[n][s] = (live && !((S.first >> s) & 1)) ? di[off + n] : T(0);
                 }
             }
         };
@@ -173,30 +183,30 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         const int tn3 = elem_of_group(a, g + 3 * WX, gend, e);
 
         // modal coefficients of this group: K-step s of lane l is mode 4s + l/16
-        double b[NV][KT];
+        T b[NV][KT];
 #pragma unroll
         for (int s = 0; s < KT; ++s) {
             const int k = 4 * s + q, kind = mode_kind<P>(k);
             const bool live = S.t >= 0 && k < D::TM;
             const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
-            const double sg = S.off[s] < 0 ? -1.0 : 1.0;
-            const double* src = kind == 0 ? a.ugv : (kind == 1 ? a.ugs : a.ugi);
+            const T sg = S.off[s] < 0 ? T(-1) : T(1);
+            const T* src = kind == 0 ? a.ugv : (kind == 1 ? a.ugs : a.ugi);
 #pragma unroll
-            for (int n = 0; n < NV; ++n) b[n][s] = live ? sg * src[off + n] : 0.0;
+            for (int n = 0; n < NV; ++n) b[n][s] = live ? sg * src[off + n] : T(0);
         }
         const double x0 = S.x[0], y0 = S.y[0], x1 = S.x[1], y1 = S.y[1], x2 = S.x[2], y2 = S.y[2];
         // ldcrd(n,0) = dx_n/dr, ldcrd(n,1) = dx_n/ds (rsdl.cpp:44-47)
-        const double l00 = 0.5 * (x2 - x1), l01 = 0.5 * (x0 - x1);
-        const double l10 = 0.5 * (y2 - y1), l11 = 0.5 * (y0 - y1);
+        const T l00 = T(0.5 * (x2 - x1)), l01 = T(0.5 * (x0 - x1));
+        const T l10 = T(0.5 * (y2 - y1)), l11 = T(0.5 * (y0 - y1));
 
-        const d4 z = {0.0, 0.0, 0.0, 0.0};
-        d4 U[GT], V[GT], Pp[GT];
+        const V4 z = {T(0), T(0), T(0), T(0)};
+        V4 U[GT], V[GT], Pp[GT];
 #pragma unroll
         for (int mt = 0; mt < GT; ++mt) {
-            d4 cu = z, cv = z, cp = z;
+            V4 cu = z, cv = z, cp = z;
 #pragma unroll
             for (int s = 0; s < KT; ++s) {
-                const double av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
+                const T av = sops[D::OFF_P + (mt * KT + s) * 64 + lane];
                 cu = mfma(av, b[0][s], cu);
                 cv = mfma(av, b[1][s], cv);
                 cp = mfma(av, b[2][s], cp);
@@ -205,16 +215,16 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         }
 
         // inviscid fluxes (rsdl.cpp:262-291): continuity C, momentum A
-        d4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
-        const double rho = a.rho;
+        V4 C0[GT], C1[GT], A00[GT], A01[GT], A10[GT], A11[GT];
+        const T rho = a.rho;
 #pragma unroll
         for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const double u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
-                const double fx = rho * u, fy = rho * v;
-                const double c0 = +l11 * fx - l01 * fy;
-                const double c1 = -l10 * fx + l00 * fy;
+                const T u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
+                const T fx = rho * u, fy = rho * v;
+                const T c0 = +l11 * fx - l01 * fy;
+                const T c1 = -l10 * fx + l00 * fy;
                 C0[mt][i] = c0; C1[mt][i] = c1;
                 A00[mt][i] = u * c0 + l11 * pr;
                 A01[mt][i] = u * c1 - l10 * pr;
@@ -223,80 +233,80 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
             }
 
         // lf_im = intgrtrs of the inviscid fluxes (rsdl.cpp:292-294)
-        d4 LI0 = z, LI1 = z, LI2 = z;
+        V4 LI0 = z, LI1 = z, LI2 = z;
 #pragma unroll
         for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int kk = kt * 4 + i;
-                const double ar = sops[D::OFF_IR + kk * 64 + lane];
-                const double as = sops[D::OFF_IS + kk * 64 + lane];
+                const T ar = sops[D::OFF_IR + kk * 64 + lane];
+                const T as = sops[D::OFF_IS + kk * 64 + lane];
                 LI0 = mfma(ar, A00[kt][i], LI0); LI0 = mfma(as, A01[kt][i], LI0);
                 LI1 = mfma(ar, A10[kt][i], LI1); LI1 = mfma(as, A11[kt][i], LI1);
                 LI2 = mfma(ar, C0[kt][i], LI2);  LI2 = mfma(as, C1[kt][i], LI2);
             }
 
-        d4 LR0 = z, LR1 = z, LR2 = z;
+        V4 LR0 = z, LR1 = z, LR2 = z;
         if constexpr (FULL) {
             // derivatives of u, v at the Gauss points (proj with r/s derivatives)
-            d4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
+            V4 Ur[GT], Us[GT], Vr[GT], Vs[GT];
 #pragma unroll
             for (int mt = 0; mt < GT; ++mt) {
-                d4 c1 = z, c2 = z, c3 = z, c4 = z;
+                V4 c1 = z, c2 = z, c3 = z, c4 = z;
 #pragma unroll
                 for (int s = 0; s < KT; ++s) {
-                    const double ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
-                    const double as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
+                    const T ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
+                    const T as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
                     c1 = mfma(ar, b[0][s], c1); c2 = mfma(as, b[0][s], c2);
                     c3 = mfma(ar, b[1][s], c3); c4 = mfma(as, b[1][s], c4);
                 }
                 Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
             }
             // viscous stress tensor (rsdl.cpp:303-326); steady so rhobd0 = 0
-            const double cjcb = l00 * l11 - l10 * l01;
-            const double cjcbi = a.mu / cjcb;
-            const double v0000 = -cjcbi * (2. * l11 * l11 + l01 * l01);
-            const double v0011 = -cjcbi * (2. * l10 * l10 + l00 * l00);
-            const double v0001 = cjcbi * (2. * l11 * l10 + l01 * l00);
-            const double v1100 = -cjcbi * (l11 * l11 + 2. * l01 * l01);
-            const double v1111 = -cjcbi * (l10 * l10 + 2. * l00 * l00);
-            const double v1101 = cjcbi * (l11 * l10 + 2. * l01 * l00);
-            const double v0100 = cjcbi * l01 * l11;
-            const double v0111 = cjcbi * l00 * l10;
-            const double v0101 = -cjcbi * l01 * l10;
-            const double v0110 = -cjcbi * l00 * l11;
-            d4 F00[GT], F01[GT], F10[GT], F11[GT];
+            const T cjcb = l00 * l11 - l10 * l01;
+            const T cjcbi = a.mu / cjcb;
+            const T v0000 = -cjcbi * (T(2) * l11 * l11 + l01 * l01);
+            const T v0011 = -cjcbi * (T(2) * l10 * l10 + l00 * l00);
+            const T v0001 = cjcbi * (T(2) * l11 * l10 + l01 * l00);
+            const T v1100 = -cjcbi * (l11 * l11 + T(2) * l01 * l01);
+            const T v1111 = -cjcbi * (l10 * l10 + T(2) * l00 * l00);
+            const T v1101 = cjcbi * (l11 * l10 + T(2) * l01 * l00);
+            const T v0100 = cjcbi * l01 * l11;
+            const T v0111 = cjcbi * l00 * l10;
+            const T v0101 = -cjcbi * l01 * l10;
+            const T v0110 = -cjcbi * l00 * l11;
+            V4 F00[GT], F01[GT], F10[GT], F11[GT];
 #pragma unroll
             for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const double ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
-                    const double f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
-                    const double f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
-                    const double f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
-                    const double f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
+                    const T ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
+                    const T f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
+                    const T f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
+                    const T f10 = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
+                    const T f11 = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
                     F00[mt][i] = f00; F01[mt][i] = f01; F10[mt][i] = f10; F11[mt][i] = f11;
                     A00[mt][i] += f00; A01[mt][i] += f01; A10[mt][i] += f10; A11[mt][i] += f11;
                 }
             // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:369-375)
-            d4 R0[GT], R1[GT], R2[GT];
+            V4 R0[GT], R1[GT], R2[GT];
 #pragma unroll
             for (int mt = 0; mt < GT; ++mt) {
-                d4 r0 = z, r1 = z, r2 = z;
+                V4 r0 = z, r1 = z, r2 = z;
 #pragma unroll
                 for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int kk = (mt * GT + kt) * 4 + i;
-                        const double dr = sops[D::OFF_DR + kk * 64 + lane];
-                        const double ds = sops[D::OFF_DS + kk * 64 + lane];
+                        const T dr = sops[D::OFF_DR + kk * 64 + lane];
+                        const T ds = sops[D::OFF_DS + kk * 64 + lane];
                         r0 = mfma(dr, A00[kt][i], r0); r0 = mfma(ds, A01[kt][i], r0);
                         r1 = mfma(dr, A10[kt][i], r1); r1 = mfma(ds, A11[kt][i], r1);
                         r2 = mfma(dr, C0[kt][i], r2);  r2 = mfma(ds, C1[kt][i], r2);
                     }
                 R0[mt] = r0; R1[mt] = r1; R2[mt] = r2;
             }
-            if (a.lin) {
+            if (sizeof(T) == 8 && a.lin) {
                 // linearisation data of this 16-element group, coalesced per (field, tile, reg)
                 float* L = a.lin + (size_t)(a.gbase + g) * (5 * GT * 4 * 64);
 #pragma unroll
@@ -312,42 +322,42 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
                     }
             }
             // per-point tau (CALC_TAU2, rsdl.cpp:377-395) and the stabilisation (:397-421)
-            const double dx1 = x0 - x2, dy1 = y0 - y2, dx2 = x1 - x0, dy2 = y1 - y0;
-            const double ar2 = dx1 * dy2 - dy1 * dx2;
-            const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
-                                 sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
-            const double h = (ar2 / perim) / (0.25 * (P + 1) * (P + 1));
-            const double nu = a.mu / rho;
-            const double gv = 2. * nu / h;
-            const double gvis = gv * gv;
-            const double tfac = a.adis * h / cjcb;
+            const T dx1 = T(x0 - x2), dy1 = T(y0 - y2), dx2 = T(x1 - x0), dy2 = T(y1 - y0);
+            const T ar2 = dx1 * dy2 - dy1 * dx2;
+            const T perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
+                            sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
+            const T h = (ar2 / perim) / (T(0.25) * (P + 1) * (P + 1));
+            const T nu = a.mu / rho;
+            const T gv = T(2) * nu / h;
+            const T gvis = gv * gv;
+            const T tfac = a.adis * h / cjcb;
 #pragma unroll
             for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const double u = U[mt][i], v = V[mt][i];
-                    const double qq = u * u + v * v;
-                    double gam = 3.0 * qq + gvis;
-                    if (a.mu == 0.0) gam = fmax(gam, 0.1);
-                    const double tau0 = tfac / sqrt(gam);
-                    const double tau2 = sqrt(qq) * tau0;
-                    const double t0 = tau0 * R0[mt][i], t1 = tau0 * R1[mt][i], t2 = tau2 * R2[mt][i];
-                    F00[mt][i] -= (l11 * (2 * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
-                    F01[mt][i] -= (-l10 * (2 * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
-                    F10[mt][i] -= +l11 * v * t0 + (l11 * u - l01 * (2. * v)) * t1 - l01 * t2;
-                    F11[mt][i] -= -l10 * v * t0 + (-l10 * u + l00 * (2. * v)) * t1 + l00 * t2;
+                    const T u = U[mt][i], v = V[mt][i];
+                    const T qq = u * u + v * v;
+                    T gam = T(3) * qq + gvis;
+                    if (a.mu == T(0)) gam = fmax(gam, T(0.1));
+                    const T tau0 = tfac / sqrt(gam);
+                    const T tau2 = sqrt(qq) * tau0;
+                    const T t0 = tau0 * R0[mt][i], t1 = tau0 * R1[mt][i], t2 = tau2 * R2[mt][i];
+                    F00[mt][i] -= (l11 * (T(2) * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
+                    F01[mt][i] -= (-l10 * (T(2) * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
+                    F10[mt][i] -= +l11 * v * t0 + (l11 * u - l01 * (T(2) * v)) * t1 - l01 * t2;
+                    F11[mt][i] -= -l10 * v * t0 + (-l10 * u + l00 * (T(2) * v)) * t1 + l00 * t2;
                     C0[mt][i] = -(l11 * t0 - l01 * t1);
                     C1[mt][i] = -(-l10 * t0 + l00 * t1);
                 }
-            load_old();
+            if constexpr (!EM) load_old();
             // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
 #pragma unroll
             for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = kt * 4 + i;
-                    const double ar = sops[D::OFF_IR + kk * 64 + lane];
-                    const double as = sops[D::OFF_IS + kk * 64 + lane];
+                    const T ar = sops[D::OFF_IR + kk * 64 + lane];
+                    const T as = sops[D::OFF_IS + kk * 64 + lane];
                     LR0 = mfma(ar, F00[kt][i], LR0); LR0 = mfma(as, F01[kt][i], LR0);
                     LR1 = mfma(ar, F10[kt][i], LR1); LR1 = mfma(as, F11[kt][i], LR1);
                     LR2 = mfma(ar, C0[kt][i], LR2);  LR2 = mfma(as, C1[kt][i], LR2);
@@ -355,6 +365,25 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         }
 
 
+        if constexpr (EM) {
+            // element-major output: lane (e, q) owns mode q of element e; the vertex
+            // gather adds the carried real part and sums over the elements
+            if (S.t >= 0 && q < D::TM) {
+                T* o = eres + ((size_t)S.t * D::TM + q) * (2 * NV);
+#pragma unroll
+                for (int n = 0; n < NV; ++n) o[n] = (n == 0 ? LI0 : (n == 1 ? LI1 : LI2))[0];
+                if constexpr (FULL) {
+                    const T bt = a.beta;
+                    o[NV + 0] = LR0[0] * bt;
+                    o[NV + 1] = LR1[0] * bt;
+                    o[NV + 2] = LR2[0] * bt;
+                }
+            }
+            S = Sn;
+            cnext = cn2;
+            tnext2 = tn3;
+            continue;
+        }
         if constexpr (!FULL) load_old();
         // lftog: res = sum lf_im + res_r,  res_r = (1-beta) res_r + beta sum lf_re
         // (tri_hp/nstage.cpp:21-82).  The first colour that touches a coefficient
@@ -364,23 +393,29 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
             const int k = 4 * s + q, kind = mode_kind<P>(k);
             if (S.t < 0 || k >= D::TM) continue;
             const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
-            const double sg = S.off[s] < 0 ? -1.0 : 1.0;
-            double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
-            double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
-            const double li[3] = {sg * LI0[s], sg * LI1[s], sg * LI2[s]};
-            double lr[3] = {0.0, 0.0, 0.0};
+            const T sg = S.off[s] < 0 ? T(-1) : T(1);
+            T* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
+            T* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+            const T li[3] = {sg * LI0[s], sg * LI1[s], sg * LI2[s]};
+            T lr[3] = {T(0), T(0), T(0)};
             if constexpr (FULL) {
-                const double bt = a.beta;
+                const T bt = a.beta;
                 lr[0] = sg * (LR0[s] * bt);
                 lr[1] = sg * (LR1[s] * bt);
                 lr[2] = sg * (LR2[s] * bt);
             }
             if ((S.first >> s) & 1) {
+                // FP32 inner solve: the constant source sfac * R64(u0) enters once, on
+                // the first visit (null in the FP64 kernel)
+                const double* sr = kind == 0 ? a.srcv : (kind == 1 ? a.srcs : a.srci);
 #pragma unroll
                 for (int n = 0; n < NV; ++n) {
-                    const double r0 = a.keep == 0.0 ? 0.0 : a.keep * ro[n][s];
-                    if (FULL || a.keep != 1.0) dr[off + n] = r0 + lr[n];
-                    di[off + n] = r0 + lr[n] + li[n];
+                    const T r0 = a.keep == T(0) ? T(0) : a.keep * ro[n][s];
+                    if (FULL || a.keep != T(1)) dr[off + n] = r0 + lr[n];
+                    if (sizeof(T) == 4 && sr)
+                        di[off + n] = r0 + lr[n] + li[n] + T(sr[off + n] * a.sfac[0]);
+                    else
+                        di[off + n] = r0 + lr[n] + li[n];
                 }
             } else {
 #pragma unroll
@@ -394,6 +429,27 @@ __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
         cnext = cn2;
         tnext2 = tn3;
     }
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
+    elem_ins_body<double, P, FULL>(a);
+}
+
+// p = 1 in element-major form (no colouring): for vertex mode k of element t,
+// eres[(t*3 + k)*2*NV + n] = lf_im, eres[... + NV + n] = beta lf_re (full residual
+// only); k_vstage1 performs the lftog sums in a fixed order per vertex
+template <bool FULL>
+__global__ __launch_bounds__(256) void k_elem_ins1_em(ElemArgs a, double* eres) {
+    elem_ins_body<double, 1, FULL, true>(a, eres);
+}
+
+// FP32 instantiation (the inner solve of the FP32 defect correction): the same
+// residual in single precision on v_mfma_f32_16x16x4f32; its operator pack permutes
+// the output rows of Ir/Is so that register s of lane group q again holds mode 4s + q
+template <int P, bool FULL>
+__global__ __launch_bounds__(256, 2) void k_elem_ins32(ElemArgs32 a) {
+    elem_ins_body<float, P, FULL>(a);
 }
 
 // ---------------------------------------------------------------------------------
@@ -421,26 +477,32 @@ __global__ void k_modes(int op, long nent, int nm, int mstride, double* __restri
 }
 
 // edge-boundary flux residual, tri_hp/hp_boundary.cpp:973-1049 with bdry_ins::generic::flux
-struct EdgeArgs {
+// S: storage type of the state and residual (double, or float for the FP32 inner
+// solve of the defect correction); the flux arithmetic is FP64 in both
+template <typename S>
+struct EdgeArgsT {
     const int* eseg;
     int begin, end;
     const int* seg_pnt;
     const double* pnts;
-    const double* ugv;
-    const double* ugs;
-    double* resv;
-    double* ress;
+    const S* ugv;
+    const S* ugs;
+    S* resv;
+    S* ress;
     const double* g1;   // gpx x (sm+2)
     const double* dg1;
     const double* w1;
     int gpx, sm, sm0;
     double rho, bc_pressure;
 };
+typedef EdgeArgsT<double> EdgeArgs;
 
 // P fixes the edge's mode count (p + 1) and Gauss points (p + 1), so the per-lane
 // arrays stay in registers
-template <int P>
-__global__ __launch_bounds__(64) void k_edge_ins(EdgeArgs a) {
+// EM: element-major output eout[(idx*2 + k)*NV + n] for endpoint k of boundary edge
+// idx (p = 1, summed into the vertices by k_vstage1) instead of the RMW scatter
+template <int P, typename S, bool EM>
+__device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restrict__ eout) {
     constexpr int SM = P - 1, N1 = SM + 2, GPX = P + 1;
     const int idx = a.begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.end) return;
@@ -486,13 +548,30 @@ __global__ __launch_bounds__(64) void k_edge_ins(EdgeArgs a) {
             for (int k = 0; k < N1; ++k) lf[n][k] += g[k] * r;
         }
     }
+    if constexpr (EM) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            eout[(2 * idx) * NV + n] = (S)lf[n][0];
+            eout[(2 * idx + 1) * NV + n] = (S)lf[n][1];
+        }
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-        a.resv[v0 * NV + n] += lf[n][0];
-        a.resv[v1 * NV + n] += lf[n][1];
+        a.resv[v0 * NV + n] += (S)lf[n][0];
+        a.resv[v1 * NV + n] += (S)lf[n][1];
 #pragma unroll
-        for (int m = 0; m < SM; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += lf[n][m + 2];
+        for (int m = 0; m < SM; ++m) a.ress[(sind * a.sm0 + m) * NV + n] += (S)lf[n][m + 2];
     }
+}
+
+template <int P, typename S = double>
+__global__ __launch_bounds__(64) void k_edge_ins(EdgeArgsT<S> a) {
+    edge_ins_body<P, S, false>(a, nullptr);
+}
+
+__global__ __launch_bounds__(64) void k_edge_ins1_em(EdgeArgs a, double* eout) {
+    edge_ins_body<1, double, true>(a, eout);
 }
 
 // ---------------------------------------------------------------------------------
@@ -637,7 +716,11 @@ struct MinvArgsT {
     T* resi;
     const T* vprcn;
     const T* sprcn;
-    const T* tprcn;
+    // element scalings of the side rows (side assembly couplings) and of the interior
+    // rows; both are tprcn for the analytic preconditioner, the learned one scales them
+    // by mode type
+    const T* tprcn_s;
+    const T* tprcn_i;
     const T* lump;  // sfmv | ifmb | vfms | sdiag | sfms | bfmi | idiag
     int npnt, nseg, ntri, sm, im, bm, sm0, im0;
     int o_sfmv, o_ifmb, o_vfms, o_sdiag, o_sfms, o_bfmi, o_idiag;
@@ -693,6 +776,76 @@ __global__ __launch_bounds__(256) void k_minv_vrtx(MinvArgsT<T> a) {
     if (a.ugv) a.ugv[v * NVn + n] = u0 - a.cfa * x;
 }
 
+// p = 1 stage on the vertices, one lane per (vertex, variable): the lftog sums of the
+// element-major element and edge results in a fixed order (tri_hp/nstage.cpp:21-82,
+// hp_boundary.cpp:973-1049), the multigrid source (nstage.cpp:76-82, 113-128) and,
+// with `ug` set, the vertex minvrt (setup_preconditioner.cpp:151-152, 179-183) and the
+// stage update ug = ug0 - cfa * res (nstage.cpp:387-409).  At p = 1 every mode is a
+// vertex mode, so this is the whole stage after the element residual.
+struct VStageArgs {
+    int npnt;
+    const int* vt_ptr;
+    const int* vt_tri;
+    const int* vt_loc;
+    const int* vb_ptr;  // boundary-edge endpoints at each vertex: entries 2*idx + k of eout
+    const int* vb_ent;
+    const double* eres;  // (ntri, 3, 2, NV)
+    const double* eout;  // (neseg, 2, NV)
+    double* resv;
+    double* rrv;
+    double keep;
+    int full;           // the stage evaluates the real part (beta > 0)
+    int mg;             // 0 none, 1 res += dres, 2 dres = fadd r0 - res, res += dres
+    double* dres;
+    const double* r0;
+    double fadd;
+    const unsigned char* vess;
+    const double* vprcn;
+    double* ug;         // null: residual only
+    double* ug0;
+    double cfa;
+    int save_u0;        // first stage: ug0 = ug
+};
+
+template <int NVn>
+__global__ __launch_bounds__(256) void k_vstage1(VStageArgs a) {
+    int v, n;
+    if (!minv_lane<double, NVn>(a.npnt, v, n)) return;
+    const long o = (long)v * NVn + n;
+    const double keep = a.keep;
+    const double r0 = (keep == 0.0 || !(a.full || keep != 0.0)) ? 0.0 : keep * a.rrv[o];
+    double acc = r0, accr = r0;
+    for (int k = a.vt_ptr[v]; k < a.vt_ptr[v + 1]; ++k) {
+        const double* e = a.eres + ((long)a.vt_tri[k] * 3 + a.vt_loc[k]) * (2 * NVn);
+        const double li = e[n];
+        if (a.full) {
+            const double lr = e[NVn + n];
+            accr += lr;
+            acc = (acc + lr) + li;
+        } else {
+            acc = acc + li;
+        }
+    }
+    for (int k = a.vb_ptr[v]; k < a.vb_ptr[v + 1]; ++k) acc += a.eout[(long)a.vb_ent[k] * NVn + n];
+    if (a.mg == 1) {
+        acc += a.dres[o];
+    } else if (a.mg == 2) {
+        const double d = a.fadd * a.r0[o] - acc;
+        a.dres[o] = d;
+        acc += d;
+    }
+    if (a.full || keep != 1.0) a.rrv[o] = accr;
+    if (a.ug) {
+        const double x = (a.vess[v] >> n) & 1 ? 0.0 : acc * a.vprcn[o];
+        const double u = a.save_u0 ? a.ug[o] : a.ug0[o];
+        if (a.save_u0) a.ug0[o] = u;
+        a.resv[o] = x;
+        a.ug[o] = u - a.cfa * x;
+    } else {
+        a.resv[o] = acc;
+    }
+}
+
 // sides: subtract interior couplings (:137-148), then the vertex solution (:192-214)
 template <typename T, int P, int NVn>
 __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
@@ -727,7 +880,7 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
         const int sgn = a.tri_sgn[3 * t + i];
-        const T tp = a.tprcn[NVn * t + n];
+        const T tp = a.tprcn_s[NVn * t + n];
         T uh[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) uh[j] = a.resv[a.tri_pnt[3 * t + (i + j) % 3] * NVn + n] * tp;
@@ -765,7 +918,7 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
         const int t = a.st_tri[2 * s + k];
         if (t < 0) continue;
         const int i = a.st_loc[2 * s + k];
-        const T tp = a.tprcn[NVn * t + n];
+        const T tp = a.tprcn_s[NVn * t + n];
         T uh[3];
         int sign[3];
 #pragma unroll
@@ -847,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_minv_int(MinvArgsT<T> a) {
             ms *= sgn;
         }
     }
-    const T tp = a.tprcn[NVn * t + n];
+    const T tp = a.tprcn_i[NVn * t + n];
 #pragma unroll
     for (int k = 0; k < IM; ++k) {
         b[k] /= tp;
@@ -1000,6 +1153,12 @@ template __global__ void k_elem_ins<2, true>(ElemArgs);
 template __global__ void k_elem_ins<2, false>(ElemArgs);
 template __global__ void k_elem_ins<4, true>(ElemArgs);
 template __global__ void k_elem_ins<4, false>(ElemArgs);
+template __global__ void k_elem_ins32<1, true>(ElemArgs32);
+template __global__ void k_elem_ins32<1, false>(ElemArgs32);
+template __global__ void k_elem_ins32<2, true>(ElemArgs32);
+template __global__ void k_elem_ins32<2, false>(ElemArgs32);
+template __global__ void k_elem_ins32<4, true>(ElemArgs32);
+template __global__ void k_elem_ins32<4, false>(ElemArgs32);
 
 
 // grid-stride 16-byte copy (reference kernel for traffic counters)

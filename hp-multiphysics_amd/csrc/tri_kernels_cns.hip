@@ -36,12 +36,13 @@ struct CDims {
 
 // ---------------------------------------------------------------------------------
 // 4x4 helpers
-__device__ __forceinline__ void mm4(const double A[4][4], const double B[4][4], double C[4][4]) {
+template <typename T>
+__device__ __forceinline__ void mm4(const T A[4][4], const T B[4][4], T C[4][4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            double t = 0.0;
+            T t = T(0.0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) t += A[i][k] * B[k][j];
             C[i][j] = t;
@@ -51,15 +52,16 @@ __device__ __forceinline__ void mm4(const double A[4][4], const double B[4][4], 
 // Solves A X = B for the four columns of B (overwritten by X); Gaussian elimination
 // with partial pivoting (first row of largest |pivot|), A is destroyed.  Row swaps
 // are unrolled selects so that everything stays in registers.
-__device__ __forceinline__ void solve4(double a[4][4], double b[4][4]) {
+template <typename T>
+__device__ __forceinline__ void solve4(T a[4][4], T b[4][4]) {
     int piv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         int p = k;
-        double best = fabs(a[k][k]);
+        T best = fabs(a[k][k]);
 #pragma unroll
         for (int r = k + 1; r < 4; ++r) {
-            const double v = fabs(a[r][k]);
+            const T v = fabs(a[r][k]);
             if (v > best) { best = v; p = r; }
         }
         piv[k] = p;
@@ -67,14 +69,14 @@ __device__ __forceinline__ void solve4(double a[4][4], double b[4][4]) {
         for (int r = k + 1; r < 4; ++r)
             if (p == r) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) { const double tmp = a[k][c]; a[k][c] = a[r][c]; a[r][c] = tmp; }
+                for (int c = 0; c < 4; ++c) { const T tmp = a[k][c]; a[k][c] = a[r][c]; a[r][c] = tmp; }
             }
-        const double inv = 1.0 / a[k][k];
+        const T inv = T(1.0) / a[k][k];
 #pragma unroll
         for (int r = k + 1; r < 4; ++r) {
             a[r][k] *= inv;
-            const double f = a[r][k];
-            if (f != 0.0) {
+            const T f = a[r][k];
+            if (f != T(0.0)) {
 #pragma unroll
                 for (int c = k + 1; c < 4; ++c) a[r][c] -= f * a[k][c];
             }
@@ -82,12 +84,12 @@ __device__ __forceinline__ void solve4(double a[4][4], double b[4][4]) {
     }
 #pragma unroll
     for (int col = 0; col < 4; ++col) {
-        double x[4] = {b[0][col], b[1][col], b[2][col], b[3][col]};
+        T x[4] = {b[0][col], b[1][col], b[2][col], b[3][col]};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int r = k + 1; r < 4; ++r)
-                if (piv[k] == r) { const double tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
+                if (piv[k] == r) { const T tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
 #pragma unroll
         for (int r = 1; r < 4; ++r)
 #pragma unroll
@@ -104,24 +106,25 @@ __device__ __forceinline__ void solve4(double a[4][4], double b[4][4]) {
 }
 
 // utilities/matrix.cpp:16-37: 20 power iterations from x = 1 and the Rayleigh quotient
-__device__ __forceinline__ double spectral_radius4(const double A[4][4]) {
-    double x[4] = {1.0, 1.0, 1.0, 1.0}, t[4] = {0.0, 0.0, 0.0, 0.0};
+template <typename T>
+__device__ __forceinline__ T spectral_radius4(const T A[4][4]) {
+    T x[4] = {T(1.0), T(1.0), T(1.0), T(1.0)}, t[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
     for (int it = 0; it < 20; ++it) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            double s = 0.0;
+            T s = T(0.0);
 #pragma unroll
             for (int j = 0; j < 4; ++j) s += A[i][j] * x[j];
             t[i] = s;
         }
-        double s = 0.0;
+        T s = T(0.0);
 #pragma unroll
         for (int i = 0; i < 4; ++i) s += t[i] * t[i];
-        const double nrm = sqrt(s);
+        const T nrm = sqrt(s);
 #pragma unroll
         for (int i = 0; i < 4; ++i) x[i] = t[i] / nrm;
     }
-    double num = 0.0, den = 0.0;
+    T num = T(0.0), den = T(0.0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) num += x[i] * t[i];
 #pragma unroll
@@ -133,49 +136,50 @@ __device__ __forceinline__ double spectral_radius4(const double A[4][4]) {
 // (hdt = 0) and with alph = 0 as the reference forces it: Pinv (the inverse of the
 // low-Mach preconditioner), Tau = Tinv^-1 P with Tinv = 2(|A| + |B| + h S)/h, and,
 // when requested, timestep = 1/spectral_radius(Tinv).
-__device__ __forceinline__ void cns_tau(const CnsPhys& ph, double pr, double u, double v, double rt, double h,
-                                        double Pinv[4][4], double Tau[4][4], double* timestep) {
-    const double gam = ph.gamma, gm1 = gam - 1.0, gogm1 = gam / gm1;
-    const double rho = pr / rt, ke = 0.5 * (u * u + v * v), c2 = gam * rt, c = sqrt(c2);
-    const double nu = ph.mu / rho, cp = gogm1 * ph.Rgas, alpha = ph.kcond / (rho * cp);
-    const double umag = sqrt(u * u + v * v);
-    const double M = (umag / c > 1.0e-5) ? umag / c : 1.0e-5;
-    const double nuh = 4.0 * nu / (h * c);
-    const double alh = 2.0 * alpha / (h * c);
-    double b2 = 1.0;
-    if (M < 0.8) {
-        const double s = M * M / (1.0 - M * M) + 0.0 * 0.0 + nuh * nuh + alh * alh;
-        b2 = (s < 1.0) ? s : 1.0;
+template <typename T>
+__device__ __forceinline__ void cns_tau(const CnsPhys& ph, T pr, T u, T v, T rt, T h,
+                                        T Pinv[4][4], T Tau[4][4], T* timestep) {
+    const T gam = T(ph.gamma), gm1 = gam - T(1.0), gogm1 = gam / gm1;
+    const T rho = pr / rt, ke = T(0.5) * (u * u + v * v), c2 = gam * rt, c = sqrt(c2);
+    const T nu = T(ph.mu) / rho, cp = gogm1 * T(ph.Rgas), alpha = T(ph.kcond) / (rho * cp);
+    const T umag = sqrt(u * u + v * v);
+    const T M = (umag / c > T(1.0e-5)) ? umag / c : T(1.0e-5);
+    const T nuh = T(4.0) * nu / (h * c);
+    const T alh = T(2.0) * alpha / (h * c);
+    T b2 = T(1.0);
+    if (M < T(0.8)) {
+        const T s = M * M / (T(1.0) - M * M) + T(0.0) * T(0.0) + nuh * nuh + alh * alh;
+        b2 = (s < T(1.0)) ? s : T(1.0);
     }
     // alph = 0: P and Pinv have b2 on (0,0) and the (3,0) coupling only
-    double P[4][4] = {{b2, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {(b2 - 1.0) / (gogm1 * rho), 0, 0, 1}};
-    const double Pi[4][4] = {{1.0 / b2, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {-(b2 - 1.0) / (gogm1 * rho * b2), 0, 0, 1}};
+    T P[4][4] = {{b2, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {(b2 - T(1.0)) / (gogm1 * rho), 0, 0, 1}};
+    const T Pi[4][4] = {{T(1.0) / b2, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {-(b2 - T(1.0)) / (gogm1 * rho * b2), 0, 0, 1}};
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) Pinv[i][j] = Pi[i][j];
-    const double dpdc[4][4] = {{ke * gm1, -u * gm1, -v * gm1, gm1},
-                               {-u / rho, 1.0 / rho, 0, 0},
-                               {-v / rho, 0, 1.0 / rho, 0},
+    const T dpdc[4][4] = {{ke * gm1, -u * gm1, -v * gm1, gm1},
+                               {-u / rho, T(1.0) / rho, 0, 0},
+                               {-v / rho, 0, T(1.0) / rho, 0},
                                {(gm1 * ke - rt) / rho, -u * gm1 / rho, -v * gm1 / rho, gm1 / rho}};
     {
-        double t[4][4];
+        T t[4][4];
         mm4(P, dpdc, t);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) P[i][j] = t[i][j];
     }
-    double Tinv[4][4];
+    T Tinv[4][4];
     {
-        const double temp1 = sqrt(u * u * (1.0 - 2.0 * b2 + b2 * b2) + 4.0 * b2 * c2);
-        const double V[4][4] = {{0.5 * (u * (b2 - 1.0) + temp1) * rho, 0.5 * (u * (b2 - 1.0) - temp1) * rho, 0, 0},
+        const T temp1 = sqrt(u * u * (T(1.0) - T(2.0) * b2 + b2 * b2) + T(4.0) * b2 * c2);
+        const T V[4][4] = {{T(0.5) * (u * (b2 - T(1.0)) + temp1) * rho, T(0.5) * (u * (b2 - T(1.0)) - temp1) * rho, 0, 0},
                                 {1, 1, 0, 0},
                                 {0, 0, 1, 0},
-                                {0.5 * (u * (b2 - 1.0) * gm1 + gm1 * temp1) / gam, 0.5 * (u * (b2 - 1.0) * gm1 - gm1 * temp1) / gam, 0, 1}};
-        const double e[4] = {fabs(0.5 * (u + u * b2 + temp1)), fabs(0.5 * (u + u * b2 - temp1)), fabs(u), fabs(u)};
-        double W[4][4] = {{1.0 / (temp1 * rho), -0.5 * (u * (b2 - 1.0) - temp1) / temp1, 0, 0},
-                          {-1.0 / (temp1 * rho), 0.5 * (u * (b2 - 1.0) + temp1) / temp1, 0, 0},
+                                {T(0.5) * (u * (b2 - T(1.0)) * gm1 + gm1 * temp1) / gam, T(0.5) * (u * (b2 - T(1.0)) * gm1 - gm1 * temp1) / gam, 0, 1}};
+        const T e[4] = {fabs(T(0.5) * (u + u * b2 + temp1)), fabs(T(0.5) * (u + u * b2 - temp1)), fabs(u), fabs(u)};
+        T W[4][4] = {{T(1.0) / (temp1 * rho), -T(0.5) * (u * (b2 - T(1.0)) - temp1) / temp1, 0, 0},
+                          {-T(1.0) / (temp1 * rho), T(0.5) * (u * (b2 - T(1.0)) + temp1) / temp1, 0, 0},
                           {0, 0, 1, 0},
                           {-gm1 / (gam * rho), 0, 0, 1}};
 #pragma unroll
@@ -185,31 +189,31 @@ __device__ __forceinline__ void cns_tau(const CnsPhys& ph, double pr, double u, 
         mm4(V, W, Tinv);  // |A|
     }
     {
-        const double temp2 = sqrt(v * v * (1.0 - 2.0 * b2 + b2 * b2) + 4.0 * b2 * c2);
-        const double V[4][4] = {{0, 0, 0.5 * (v * (b2 - 1.0) + temp2) * rho, 0.5 * (v * (b2 - 1.0) - temp2) * rho},
+        const T temp2 = sqrt(v * v * (T(1.0) - T(2.0) * b2 + b2 * b2) + T(4.0) * b2 * c2);
+        const T V[4][4] = {{0, 0, T(0.5) * (v * (b2 - T(1.0)) + temp2) * rho, T(0.5) * (v * (b2 - T(1.0)) - temp2) * rho},
                                 {0, 1, 0, 0},
                                 {0, 0, 1, 1},
-                                {1, 0, 0.5 * (gm1 * v * (b2 - 1.0) + gm1 * temp2) / gam, 0.5 * (gm1 * v * (b2 - 1.0) - gm1 * temp2) / gam}};
-        const double e[4] = {fabs(v), fabs(v), fabs(0.5 * (v + v * b2 + temp2)), fabs(0.5 * (v + v * b2 - temp2))};
-        double W[4][4] = {{-gm1 / (gam * rho), 0, 0, 1},
+                                {1, 0, T(0.5) * (gm1 * v * (b2 - T(1.0)) + gm1 * temp2) / gam, T(0.5) * (gm1 * v * (b2 - T(1.0)) - gm1 * temp2) / gam}};
+        const T e[4] = {fabs(v), fabs(v), fabs(T(0.5) * (v + v * b2 + temp2)), fabs(T(0.5) * (v + v * b2 - temp2))};
+        T W[4][4] = {{-gm1 / (gam * rho), 0, 0, 1},
                           {0, 1, 0, 0},
-                          {1.0 / (temp2 * rho), 0, 0.5 * (-v * (b2 - 1.0) + temp2) / temp2, 0},
-                          {-1.0 / (rho * temp2), 0, 0.5 * (v * (b2 - 1.0) + temp2) / temp2, 0}};
+                          {T(1.0) / (temp2 * rho), 0, T(0.5) * (-v * (b2 - T(1.0)) + temp2) / temp2, 0},
+                          {-T(1.0) / (rho * temp2), 0, T(0.5) * (v * (b2 - T(1.0)) + temp2) / temp2, 0}};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) W[i][j] = e[i] * W[i][j];
-        double B[4][4];
+        T B[4][4];
         mm4(V, W, B);
-        const double Sd[4][4] = {{0, 0, 0, 0}, {0, nu / (h * h), 0, 0}, {0, 0, nu / (h * h), 0}, {0, 0, 0, alpha / (h * h)}};
-        double S[4][4];
+        const T Sd[4][4] = {{0, 0, 0, 0}, {0, nu / (h * h), 0, 0}, {0, 0, nu / (h * h), 0}, {0, 0, 0, alpha / (h * h)}};
+        T S[4][4];
         mm4(P, Sd, S);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Tinv[i][j] = 2.0 * (Tinv[i][j] + B[i][j] + h * S[i][j]) / h;
+            for (int j = 0; j < 4; ++j) Tinv[i][j] = T(2.0) * (Tinv[i][j] + B[i][j] + h * S[i][j]) / h;
     }
-    if (timestep) *timestep = 1.0 / spectral_radius4(Tinv);
+    if (timestep) *timestep = T(1.0) / spectral_radius4(Tinv);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -252,7 +256,10 @@ __device__ __forceinline__ void lower4(const double L[16], double r[4]) {
 // tri_hp_cns::element_rsdl, straight branch (tri_hp/cns/rsdl.cpp:312-530), steady,
 // source "zero", CALC_TAU2 (tau at every Gauss point), for the elements
 // order[begin:end) of one colour.
-struct CnsElemArgs {
+// T: arithmetic type of the kernel (double, or float in the FP32 inner solve of the
+// defect correction); state and residual arrays are FP64 in both
+template <typename T>
+struct CnsElemArgsT {
     const double* pnts;
     const int* tri_pnt;
     const int* tri_seg;
@@ -268,14 +275,22 @@ struct CnsElemArgs {
     double* rrv;
     double* rrs;
     double* rri;
-    const double* ops;  // CDims<P> pack
+    const T* ops;  // CDims<P> pack
     int sm0, im0;
     CnsPhys ph;
     double beta, keep;
     const unsigned char* fcv;
     const unsigned char* fcs;
     int colour;
+    // FP32 kernel: constant source sfac[0] * src added on the first visit of every
+    // coefficient (the scaled FP64 residual of the defect correction); null otherwise
+    const double* srcv;
+    const double* srcs;
+    const double* srci;
+    const double* sfac;
 };
+typedef CnsElemArgsT<double> CnsElemArgs;
+typedef CnsElemArgsT<float> CnsElemArgs32;
 
 // octet loop of the persistent element kernels: XCD x of the 8 walks the contiguous
 // octet range [x*per, (x+1)*per) with the gridDim/8 blocks it is given
@@ -295,17 +310,17 @@ __device__ __forceinline__ void wave_lds_sync() {
     const int oend_ = min((xcd_ + 1) * per_, (noct));                                      \
     for (int oc = xcd_ * per_ + jb_; oc < oend_; oc += nbx_)
 
-template <int P, bool FULL>
-__global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
+template <typename T, int P, bool FULL>
+__device__ __forceinline__ void elem_cns_body(const CnsElemArgsT<T>& a, int noct) {
     using D = CDims<P>;
     constexpr int TM = D::TM, G = D::G, NO = NVC * TM;
     constexpr int SMD = D::SM > 0 ? D::SM : 1;
-    __shared__ double sops[D::SIZE];
-    __shared__ double suh[8][NVC][16];
-    __shared__ double sf[8][8][32];
+    __shared__ T sops[D::SIZE];
+    __shared__ T suh[8][NVC][16];
+    __shared__ T sf[8][8][32];
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
-    const double gm1 = a.ph.gamma - 1.0, gogm1 = a.ph.gamma / gm1;
+    const T gm1 = T(a.ph.gamma) - T(1.0), gogm1 = T(a.ph.gamma) / gm1;
     __syncthreads();  // operators staged
 
     HPG_OCTET_LOOP(noct, oc) {
@@ -324,8 +339,8 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
         const double x1 = a.pnts[2 * vtx[1]], y1 = a.pnts[2 * vtx[1] + 1];
         const double x2 = a.pnts[2 * vtx[2]], y2 = a.pnts[2 * vtx[2] + 1];
         // ldcrd(n,0) = dx_n/dr, ldcrd(n,1) = dx_n/ds (rsdl.cpp:56-59)
-        const double l00 = 0.5 * (x2 - x1), l01 = 0.5 * (x0 - x1);
-        const double l10 = 0.5 * (y2 - y1), l11 = 0.5 * (y0 - y1);
+        const T l00 = T(T(0.5) * (x2 - x1)), l01 = T(T(0.5) * (x0 - x1));
+        const T l10 = T(T(0.5) * (y2 - y1)), l11 = T(T(0.5) * (y0 - y1));
 
         if (g < TM) {
             int kind;
@@ -334,40 +349,40 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
             const double* src = kind == 0 ? a.ugv : (kind == 1 ? a.ugs : a.ugi);
 #pragma unroll
             for (int n = 0; n < NVC; ++n)
-                suh[hw][n][g] = valid ? sg * src[off + n] : ((g < 3 && (n == 0 || n == 3)) ? 1.0 : 0.0);
+                suh[hw][n][g] = valid ? T(sg * src[off + n]) : ((g < 3 && (n == 0 || n == 3)) ? T(T(1.0)) : T(T(0.0)));
         }
         wave_lds_sync();
 
         // values (and derivatives of u, v, RT) at this lane's Gauss point
-        double u[NVC], ur[NVC], us[NVC];
+        T u[NVC], ur[NVC], us[NVC];
 #pragma unroll
-        for (int n = 0; n < NVC; ++n) { u[n] = 0.0; ur[n] = 0.0; us[n] = 0.0; }
+        for (int n = 0; n < NVC; ++n) { u[n] = T(0.0); ur[n] = T(0.0); us[n] = T(0.0); }
 #pragma unroll 3
         for (int k = 0; k < TM; ++k) {
-            const double pk = sops[D::OP_P + k * 32 + g];
+            const T pk = sops[D::OP_P + k * 32 + g];
 #pragma unroll
             for (int n = 0; n < NVC; ++n) u[n] += pk * suh[hw][n][k];
             if constexpr (FULL) {
-                const double prk = sops[D::OP_PR + k * 32 + g], psk = sops[D::OP_PS + k * 32 + g];
+                const T prk = sops[D::OP_PR + k * 32 + g], psk = sops[D::OP_PS + k * 32 + g];
 #pragma unroll
                 for (int n = 1; n < NVC; ++n) { ur[n] += prk * suh[hw][n][k]; us[n] += psk * suh[hw][n][k]; }
             }
         }
-        if (g >= G) { u[0] = 1.0; u[3] = 1.0; }  // padding lanes: keep the pointwise work finite
+        if (g >= G) { u[0] = T(1.0); u[3] = T(1.0); }  // padding lanes: keep the pointwise work finite
 
         // convective terms (rsdl.cpp:315-351)
-        double cv[NVC][2];
+        T cv[NVC][2];
         {
-            const double rho = u[0] / u[3];
-            const double fluxx = rho * (u[1] - 0.0), fluxy = rho * (u[2] - 0.0);
+            const T rho = u[0] / u[3];
+            const T fluxx = rho * (u[1] - T(0.0)), fluxy = rho * (u[2] - T(0.0));
             cv[0][0] = +l11 * fluxx - l01 * fluxy;
             cv[0][1] = -l10 * fluxx + l00 * fluxy;
             cv[1][0] = u[1] * cv[0][0] + l11 * u[0];
             cv[1][1] = u[1] * cv[0][1] - l10 * u[0];
             cv[2][0] = u[2] * cv[0][0] - l01 * u[0];
             cv[2][1] = u[2] * cv[0][1] + l00 * u[0];
-            const double E = u[3] / gm1 + 0.5 * (u[1] * u[1] + u[2] * u[2]);
-            const double nfx = E * fluxx + u[0] * u[1], nfy = E * fluxy + u[0] * u[2];
+            const T E = u[3] / gm1 + T(0.5) * (u[1] * u[1] + u[2] * u[2]);
+            const T nfx = E * fluxx + u[0] * u[1], nfy = E * fluxy + u[0] * u[2];
             cv[3][0] = +l11 * nfx - l01 * nfy;
             cv[3][1] = -l10 * nfx + l00 * nfy;
         }
@@ -375,40 +390,40 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
         for (int n = 0; n < NVC; ++n) { sf[hw][2 * n][g] = cv[n][0]; sf[hw][2 * n + 1][g] = cv[n][1]; }
         wave_lds_sync();
         // lf_im = intgrtrs of the convective fluxes; lane g owns outputs g and g + 32
-        double li[2] = {0.0, 0.0};
+        T li[2] = {T(0.0), T(0.0)};
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             const int o = g + 32 * r;
             if (o < NO) {
                 const int n = o / TM, k = o - n * TM;
-                double s = 0.0;
+                T s = T(0.0);
 #pragma unroll 5
                 for (int q = 0; q < G; ++q)
                     s += sops[D::OP_IR + q * 16 + k] * sf[hw][2 * n][q] + sops[D::OP_IS + q * 16 + k] * sf[hw][2 * n + 1][q];
                 li[r] = s;
             }
         }
-        double lr[2] = {0.0, 0.0};
+        T lr[2] = {T(0.0), T(0.0)};
         if constexpr (FULL) {
             // viscous stress and heat flux (rsdl.cpp:355-420), steady: rhorbd0 = 0
-            const double cjcb = l00 * l11 - l10 * l01;
-            const double mujcbi = a.ph.mu / cjcb;
-            const double kcjcbi = a.ph.kcond / cjcb / a.ph.Rgas;
-            const double v0000 = -mujcbi * (4. / 3. * l11 * l11 + l01 * l01);
-            const double v0011 = -mujcbi * (4. / 3. * l10 * l10 + l00 * l00);
-            const double v0001 = mujcbi * (4. / 3. * l11 * l10 + l01 * l00);
-            const double v1100 = -mujcbi * (l11 * l11 + 4. / 3. * l01 * l01);
-            const double v1111 = -mujcbi * (l10 * l10 + 4. / 3. * l00 * l00);
-            const double v1101 = mujcbi * (l11 * l10 + 4. / 3. * l01 * l00);
-            const double v0100 = mujcbi * 1. / 3. * l01 * l11;
-            const double v0111 = mujcbi * 1. / 3. * l00 * l10;
-            const double v0101 = -mujcbi * (l01 * l10 - 2. / 3. * l00 * l11);
-            const double v0110 = -mujcbi * (l00 * l11 - 2. / 3. * l01 * l10);
-            const double k00 = -kcjcbi * (l11 * l11 + l01 * l01);
-            const double k11 = -kcjcbi * (l10 * l10 + l00 * l00);
-            const double k01 = kcjcbi * (l11 * l10 + l01 * l00);
-            double df[NVC][2];
-            df[0][0] = 0.0; df[0][1] = 0.0;
+            const T cjcb = l00 * l11 - l10 * l01;
+            const T mujcbi = T(a.ph.mu) / cjcb;
+            const T kcjcbi = T(a.ph.kcond) / cjcb / T(a.ph.Rgas);
+            const T v0000 = -mujcbi * (T(4.) / T(3.) * l11 * l11 + l01 * l01);
+            const T v0011 = -mujcbi * (T(4.) / T(3.) * l10 * l10 + l00 * l00);
+            const T v0001 = mujcbi * (T(4.) / T(3.) * l11 * l10 + l01 * l00);
+            const T v1100 = -mujcbi * (l11 * l11 + T(4.) / T(3.) * l01 * l01);
+            const T v1111 = -mujcbi * (l10 * l10 + T(4.) / T(3.) * l00 * l00);
+            const T v1101 = mujcbi * (l11 * l10 + T(4.) / T(3.) * l01 * l00);
+            const T v0100 = mujcbi * T(1.) / T(3.) * l01 * l11;
+            const T v0111 = mujcbi * T(1.) / T(3.) * l00 * l10;
+            const T v0101 = -mujcbi * (l01 * l10 - T(2.) / T(3.) * l00 * l11);
+            const T v0110 = -mujcbi * (l00 * l11 - T(2.) / T(3.) * l01 * l10);
+            const T k00 = -kcjcbi * (l11 * l11 + l01 * l01);
+            const T k11 = -kcjcbi * (l10 * l10 + l00 * l00);
+            const T k01 = kcjcbi * (l11 * l10 + l01 * l00);
+            T df[NVC][2];
+            df[0][0] = T(0.0); df[0][1] = T(0.0);
             df[1][0] = v0000 * ur[1] + v0100 * ur[2] + v0001 * us[1] + v0101 * us[2];
             df[1][1] = v0001 * ur[1] + v0110 * ur[2] + v0011 * us[1] + v0111 * us[2];
             df[2][0] = v0100 * ur[1] + v1100 * ur[2] + v0110 * us[1] + v1101 * us[2];
@@ -425,43 +440,43 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
             }
             wave_lds_sync();
             // strong residual: derivr/derivs of the total fluxes (rsdl.cpp:425-429)
-            double res[NVC] = {0.0, 0.0, 0.0, 0.0};
+            T res[NVC] = {T(0.0), T(0.0), T(0.0), T(0.0)};
 #pragma unroll 5
             for (int q = 0; q < G; ++q) {
-                const double dr = sops[D::OP_DR + q * 32 + g], ds = sops[D::OP_DS + q * 32 + g];
+                const T dr = sops[D::OP_DR + q * 32 + g], ds = sops[D::OP_DS + q * 32 + g];
 #pragma unroll
                 for (int n = 0; n < NVC; ++n) res[n] += dr * sf[hw][2 * n][q] + ds * sf[hw][2 * n + 1][q];
             }
             // CALC_TAU2 (rsdl.cpp:431-452) and the stabilisation (:454-488)
-            const double dx1 = x0 - x2, dy1 = y0 - y2, dx2 = x1 - x0, dy2 = y1 - y0;
-            const double ar2 = dx1 * dy2 - dy1 * dx2;
-            const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
+            const T dx1 = T(x0 - x2), dy1 = T(y0 - y2), dx2 = T(x1 - x0), dy2 = T(y1 - y0);
+            const T ar2 = dx1 * dy2 - dy1 * dx2;
+            const T perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
                                  sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
-            const double hmax = 0.25 * ar2 / ((ar2 / perim) * (0.25 * (P + 1) * (P + 1)));
+            const T hmax = T(0.25) * ar2 / ((ar2 / perim) * (T(0.25) * (P + 1) * (P + 1)));
             {
-                const double pr = u[0], uv = u[1], vv = u[2], rt = u[3];
-                double Pi[4][4], tau[4][4];
-                cns_tau(a.ph, pr, uv, vv, rt, hmax, Pi, tau, nullptr);
-                const double tf = a.ph.adis / cjcb;
-                double tres[4];
+                const T pr = u[0], uv = u[1], vv = u[2], rt = u[3];
+                T Pi[4][4], tau[4][4];
+                cns_tau<T>(a.ph, pr, uv, vv, rt, hmax, Pi, tau, nullptr);
+                const T tf = T(a.ph.adis) / cjcb;
+                T tres[4];
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    double s = 0.0;
+                    T s = T(0.0);
 #pragma unroll
                     for (int n = 0; n < 4; ++n) s += (tau[m][n] * tf) * res[n];
                     tres[m] = s;
                 }
-                const double rho = pr / rt, ke = 0.5 * (uv * uv + vv * vv);
-                const double A[4][4] = {
-                    {uv / rt, rho, 0.0, -rho * uv / rt},
-                    {uv * uv / rt + 1.0, 2.0 * rho * uv, 0.0, -rho * uv * uv / rt},
+                const T rho = pr / rt, ke = T(0.5) * (uv * uv + vv * vv);
+                const T A[4][4] = {
+                    {uv / rt, rho, T(0.0), -rho * uv / rt},
+                    {uv * uv / rt + T(1.0), T(2.0) * rho * uv, T(0.0), -rho * uv * uv / rt},
                     {uv * vv / rt, rho * vv, rho * uv, -rho * uv * vv / rt},
                     {uv * (gogm1 * rt + ke) / rt, rho * (gogm1 * rt + ke) + rho * uv * uv, rho * uv * vv,
                      -rho * uv * (gogm1 * rt + ke) / rt + rho * uv * gogm1}};
-                const double B[4][4] = {
-                    {vv / rt, 0.0, rho, -rho * vv / rt},
+                const T B[4][4] = {
+                    {vv / rt, T(0.0), rho, -rho * vv / rt},
                     {uv * vv / rt, rho * vv, rho * uv, -rho * uv * vv / rt},
-                    {vv * vv / rt + 1.0, 0.0, 2.0 * rho * vv, -rho * vv * vv / rt},
+                    {vv * vv / rt + T(1.0), T(0.0), T(2.0) * rho * vv, -rho * vv * vv / rt},
                     {vv * (gogm1 * rt + ke) / rt, rho * uv * vv, rho * (gogm1 * rt + ke) + rho * vv * vv,
                      -rho * vv * (gogm1 * rt + ke) / rt + rho * vv * gogm1}};
 #pragma unroll
@@ -482,11 +497,11 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
                 const int o = g + 32 * r;
                 if (o < NO) {
                     const int n = o / TM, k = o - n * TM;
-                    double s = 0.0;
+                    T s = T(0.0);
 #pragma unroll
                     for (int q = 0; q < G; ++q)
                         s += sops[D::OP_IR + q * 16 + k] * sf[hw][2 * n][q] + sops[D::OP_IS + q * 16 + k] * sf[hw][2 * n + 1][q];
-                    lr[r] = s * a.beta;
+                    lr[r] = s * T(a.beta);
                 }
             }
         }
@@ -504,11 +519,16 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
                 double* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
                 double* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
                 const bool first = kind == 2 || (kind == 0 ? a.fcv[sel3(vtx, k)] : a.fcs[sel3(seg, (k - 3) / SMD)]) == a.colour;
-                const double vli = sg * li[r], vlr = sg * lr[r];
+                const T vli = T(sg) * li[r], vlr = T(sg) * lr[r];
                 if (first) {
                     const double r0 = a.keep == 0.0 ? 0.0 : a.keep * dr[off];
                     if (FULL || a.keep != 1.0) dr[off] = r0 + vlr;
-                    di[off] = r0 + vlr + vli;
+                    if constexpr (sizeof(T) == 4) {
+                        const double* sr = kind == 0 ? a.srcv : (kind == 1 ? a.srcs : a.srci);
+                        di[off] = r0 + vlr + vli + (sr ? sr[off] * a.sfac[0] : 0.0);
+                    } else {
+                        di[off] = r0 + vlr + vli;
+                    }
                 } else {
                     if constexpr (FULL) dr[off] += vlr;
                     di[off] += vlr + vli;
@@ -516,6 +536,17 @@ __global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
             }
         }
     }
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
+    elem_cns_body<double, P, FULL>(a, noct);
+}
+
+// FP32 arithmetic (the inner solve of the defect correction); FP64 state and residual
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_cns32(CnsElemArgs32 a, int noct) {
+    elem_cns_body<float, P, FULL>(a, noct);
 }
 
 // ---------------------------------------------------------------------------------

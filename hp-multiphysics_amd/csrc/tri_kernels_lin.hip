@@ -261,7 +261,9 @@ __global__ __launch_bounds__(256) void k_elem_lin(LinArgs a) {
                 if (f32_row(i, q) != a.probe_m) continue;
                 const float li[3] = {LI0[i], LI1[i], LI2[i]};
                 const float lr[3] = {LR0[i], LR1[i], LR2[i]};
-                a.diag[(size_t)t * NV + a.probe_n] += fabs((double)li[a.probe_n] + (double)lr[a.probe_n]);
+                // summed per mode type (0 vertex, 1 side, 2 interior): diag[t][type][n]
+                a.diag[((size_t)t * 3 + mode_kind<P>(a.probe_m)) * NV + a.probe_n] +=
+                    fabs((double)li[a.probe_n] + (double)lr[a.probe_n]);
             }
         } else if (valid) {
 #pragma unroll
@@ -391,10 +393,123 @@ __global__ __launch_bounds__(64) void k_edge_lin(EdgeLinArgs a) {
     }
 }
 
-// ug += delta (FP64 += FP32) over n values
-__global__ void k_add_delta(long n, double* __restrict__ ug, const float* __restrict__ d) {
+// ---- FP32 defect correction with the nonlinear FP32 residual ----------------------
+// The inner solve runs the multistage scheme on F(v) = R32(v) + (1/c - 1) R64(u0) from
+// v = b = fl32(u0): R32(b + dhat) - R64(u0) ~ J(u0) dhat, so dhat solves the scaled
+// defect equation and u0 += c dhat.  The scale c keeps |dhat| near `h` |u0| (see
+// k_fd_scale), where FP32 differences of the residual resolve J(u0) dhat to about
+// 1e-3 relative; without it corrections below FP32 resolution of u0 would be lost.
+
+// max of |x| over a block, one atomicMax of the float bits (non-negative floats order
+// like their bit patterns)
+__device__ __forceinline__ void block_absmax(float x, unsigned* out) {
+    __shared__ float red[4];
+    for (int off = 32; off > 0; off >>= 1) x = fmaxf(x, __shfl_down(x, off, 64));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmaxf(m, red[k]);
+        atomicMax(out, __float_as_uint(m));
+    }
+}
+
+// The reductions below are grid-stride loops over a grid capped at RED_BLOCKS blocks, so
+// at most that many atomics meet on the one result word.
+constexpr int RED_BLOCKS = 2048;
+
+// b = v = fl32(u), umax = max |b| (blocks of 256)
+__global__ __launch_bounds__(256) void k_base32(long n, const double* __restrict__ u, float* __restrict__ b,
+                                                float* __restrict__ v, unsigned* umax) {
+    float m = 0.f;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const float x = (float)u[k];
+        b[k] = x;
+        v[k] = x;
+        m = fmaxf(m, fabsf(x));
+    }
+    block_absmax(m, umax);
+}
+
+// dmax = max |v - b| (blocks of 256)
+__global__ __launch_bounds__(256) void k_diffmax32(long n, const float* __restrict__ v, const float* __restrict__ b,
+                                                   unsigned* dmax) {
+    float m = 0.f;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(v[k] - b[k]));
+    block_absmax(m, dmax);
+}
+
+// scale of this outer iteration from the last one: c = |delta_prev| / (h |u0|) with
+// |delta_prev| = c_prev max|dhat_prev|, capped at 1 (then the inner solve is the plain
+// FP32 nonlinear smoother); c = 1 after a reset.  Writes c, 1/c - 1 and clears the
+// |dhat| maximum for this iteration.
+__global__ void k_fd_scale(double* scal, unsigned* mx, int reset, double h) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double un = (double)__uint_as_float(mx[0]), dn = (double)__uint_as_float(mx[1]);
+    double c = 1.0;
+    if (!reset && dn > 0.0 && un > 0.0) c = fmin(1.0, scal[0] * dn / (h * un));
+    if (!(c > 1e-30)) c = 1.0;
+    scal[0] = c;
+    scal[1] = 1.0 / c - 1.0;
+    mx[0] = 0u;
+    mx[1] = 0u;
+}
+
+// y = fl32(x / c) with 1/c = scal[1] + 1: the inner residual at v = b, F(b) = R64(u0) / c
+__global__ void k_scale_f32(long n, const double* __restrict__ x, const double* __restrict__ scal,
+                            float* __restrict__ y) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) ug[k] += (double)d[k];
+    if (k < n) y[k] = (float)(x[k] * (scal[1] + 1.0));
+}
+
+// ug += c (v - b), FP64
+__global__ void k_add_scaled(long n, double* __restrict__ ug, const float* __restrict__ v, const float* __restrict__ b,
+                             const double* __restrict__ scal) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) ug[k] += scal[0] * ((double)v[k] - (double)b[k]);
+}
+
+// ---- FP32 defect correction with FP64 storage (cns, tet): the inner state v lives in
+// the FP64 state arrays, so the FP64 preconditioner, boundary and projection steps run
+// on it unchanged; only the element residual is evaluated in FP32
+
+// u = fl64(fl32(u)) in place (the base b = fl32(u0) of the inner solve), umax = max |b|
+__global__ __launch_bounds__(256) void k_round32(long n, double* __restrict__ u, unsigned* umax) {
+    float m = 0.f;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const float x = (float)u[k];
+        u[k] = (double)x;
+        m = fmaxf(m, fabsf(x));
+    }
+    block_absmax(m, umax);
+}
+
+// x *= 1/c (1/c = scal[1] + 1): F(b) = R64(u0) / c
+__global__ void k_scale64(long n, double* __restrict__ x, const double* __restrict__ scal) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] *= scal[1] + 1.0;
+}
+
+// dmax = max |v - fl32(u0)|, then v = u0 + c (v - fl32(u0)): the FP64 update
+__global__ __launch_bounds__(256) void k_fd_final64(long n, double* __restrict__ v, const double* __restrict__ u0,
+                                                    const double* __restrict__ scal, unsigned* dmax) {
+    float m = 0.f;
+    const double c = scal[0];
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const double b = (double)(float)u0[k];
+        const double dh = v[k] - b;
+        m = fmaxf(m, fabsf((float)dh));
+        v[k] = u0[k] + c * dh;
+    }
+    block_absmax(m, dmax);
+}
+
+// clear n unsigned values (a fill kernel rather than a memset node)
+__global__ void k_clear_u32(int n, unsigned* x) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = 0u;
 }
 
 // FP64 -> FP32 copy
@@ -436,12 +551,18 @@ __global__ void k_mlp_features(int ntri, const double* pnts, const int* tri_pnt,
     o[10] = (float)log(fabs(area) / fmax(tprcn[3 * t], 1e-300));
 }
 
-// tprcn *= exp(clamp(y, -2, 2)) per element and variable
-__global__ void k_mlp_apply(int ntri, const float* Y, double* tprcn) {
+// learned element scalings of the vertex, side and interior rows of the preconditioner:
+// t_type = tprcn * exp(clamp(y[type][n], -2, 2)), Y = ntri x 9 ([type][var], type 0
+// vertex, 1 side, 2 interior)
+__global__ void k_mlp_apply(int ntri, const float* Y, const double* tprcn, double* tv, double* ts, double* ti) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ntri * NV) return;
-    const float v = fminf(2.f, fmaxf(-2.f, Y[k]));
-    tprcn[k] *= exp((double)v);
+    const int t = k / NV, n = k - t * NV;
+    const float* y = Y + (size_t)t * 3 * NV;
+    const double tp = tprcn[k];
+    tv[k] = tp * exp((double)fminf(2.f, fmaxf(-2.f, y[n])));
+    ts[k] = tp * exp((double)fminf(2.f, fmaxf(-2.f, y[NV + n])));
+    ti[k] = tp * exp((double)fminf(2.f, fmaxf(-2.f, y[2 * NV + n])));
 }
 
 }  // namespace hpg

@@ -19,7 +19,7 @@ HPG_FP64 = 0
 HPG_FP32_DEFECT = 1
 HPG_PRCN_ANALYTIC = 0
 HPG_PRCN_LEARNED = 1
-HPG_MLP_NOALLOC_MAX_SAMPLES = 28864
+HPG_MLP_NOALLOC_MAX_SAMPLES = 27712
 
 # every entry point include/hpgpu.h declares
 EXPORTS = (
@@ -28,16 +28,17 @@ EXPORTS = (
     "hpg_update", "hpg_maxres", "hpg_resnorm", "hpg_mg_restrict", "hpg_mg_prolongate", "hpg_get_log2p",
     "hpg_set_precision", "hpg_set_preconditioner", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
     "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32", "hpg_selftest_mfma_bf16",
-    "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_element_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
+    "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_residual_fp32", "hpg_element_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
     "hpg_mlp_loss_grad_noalloc",
-    "hpg_set_mlp", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
-    "hpg_get_block_preconditioner", "hpg_link_fine",
+    "hpg_set_mlp", "hpg_set_prcn_log_scalings", "hpg_mlp_features", "hpg_measure_jacobian_diag", "hpg_selftest_copy",
+    "hpg_get_block_preconditioner", "hpg_link_fine", "hpg_mg_cycle",
     "hpg_tet_create", "hpg_tet_destroy", "hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual",
     "hpg_tet_set_residual", "hpg_tet_get_preconditioner", "hpg_tet_get_cns_blocks", "hpg_tet_rsdl", "hpg_tet_setup_preconditioner",
     "hpg_tet_minvrt", "hpg_tet_update", "hpg_tet_maxres", "hpg_tet_resnorm", "hpg_tet_mg_restrict",
     "hpg_tet_mg_prolongate", "hpg_tet_get_log2p", "hpg_tet_synchronize", "hpg_tet_last_error",
     "hpg_tet_kernel_timing", "hpg_tet_kernel_timing_read", "hpg_tet_element_jacobian",
+    "hpg_tet_set_precision", "hpg_tet_residual_fp32",
 )
 
 
@@ -93,6 +94,7 @@ def load():
     L.hpg_set_tprcn.argtypes = [vp, dp]
     L.hpg_get_block_preconditioner.argtypes = [vp, dp, dp, dp]
     L.hpg_link_fine.argtypes = [vp, vp, dp, dp, dp, dp]
+    L.hpg_mg_cycle.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.hpg_tet_create.argtypes = [vp, vp, vp]
     L.hpg_tet_destroy.argtypes = [vp]
     for nm in ("hpg_tet_set_state", "hpg_tet_get_state", "hpg_tet_get_residual", "hpg_tet_set_residual"):
@@ -104,6 +106,8 @@ def load():
                "hpg_tet_mg_prolongate", "hpg_tet_synchronize"):
         getattr(L, nm).argtypes = [vp]
     L.hpg_tet_maxres.argtypes = [vp, dp, dp]
+    L.hpg_tet_set_precision.argtypes = [vp, C.c_int, C.c_int]
+    L.hpg_tet_residual_fp32.argtypes = [vp, dp, dp]
     L.hpg_tet_resnorm.argtypes = [vp, dp]
     L.hpg_tet_get_log2p.argtypes = [vp, C.POINTER(C.c_int)]
     L.hpg_tet_last_error.argtypes = [vp]
@@ -130,6 +134,7 @@ def load():
     L.hpg_selftest_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.hpg_set_defect_inner.argtypes = [vp, C.c_int]
     L.hpg_apply_jacobian.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.hpg_residual_fp32.argtypes = [vp, dp, dp, dp]
     L.hpg_element_jacobian.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double, dp]
     L.hpg_tet_element_jacobian.argtypes = [vp, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double, dp]
     L.hpg_mlp_create.argtypes = [C.c_uint, C.POINTER(vp)]
@@ -142,6 +147,7 @@ def load():
     L.hpg_mlp_loss_grad_noalloc.argtypes = [vp, dp, dp, C.c_int, dp, dp]
     L.hpg_mlp_train.argtypes = [vp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
     L.hpg_set_mlp.argtypes = [vp, vp]
+    L.hpg_set_prcn_log_scalings.argtypes = [vp, dp]
     L.hpg_mlp_features.argtypes = [vp, dp]
     L.hpg_measure_jacobian_diag.argtypes = [vp, dp]
     L.hpg_kernel_timing.argtypes = [vp, C.c_int]

@@ -14,13 +14,29 @@ from __future__ import annotations
 from . import mesh as M
 
 
+def fused_available(grids) -> bool:
+    """True when every grid is a GPU tri context (hpg_mg_cycle runs the whole cycle)."""
+    from .tri_hp import TriHpIns
+    return all(isinstance(g, TriHpIns) for g in grids)
+
+
 def cycle(grids, extra_finest_levels: int, vw: int = 1, itercrsn: int = 1, iterrfne: int = 0,
-          lvl: int = 0, mglvls: int | None = None, evaluate_preconditioner: bool = True) -> float:
+          lvl: int = 0, mglvls: int | None = None, evaluate_preconditioner: bool = True,
+          fused: bool | None = None) -> float:
     """block::cycle (tri_mesh/blocks.cpp:1267-1348) without the error-control level.
 
     Returns the fine-level max residual measured after the first smoothing pass
     (``maxres`` on the entry level), like the reference; raises FloatingPointError on a
-    preconditioner error or a residual that is NaN or exactly zero (``throw 1``)."""
+    preconditioner error or a residual that is NaN or exactly zero (``throw 1``).
+
+    On GPU contexts (``fused`` None or True) the whole cycle is one call of
+    hpg_mg_cycle: the same kernels in the same order, with no host round trip inside the
+    cycle, so setup_preconditioner errors are reported after the cycle rather than at
+    the failing level.  ``fused=False`` runs the level recursion call by call."""
+    if lvl == 0 and mglvls is None and evaluate_preconditioner and fused is not False and fused_available(grids):
+        return grids[0].mg_cycle(grids, extra_finest_levels, vw, itercrsn, iterrfne)
+    if fused:
+        raise ValueError("the fused cycle needs GPU tri contexts, the entry level and evaluate_preconditioner")
     ngrid = len(grids)
     if mglvls is None:
         mglvls = ngrid + extra_finest_levels
@@ -41,7 +57,7 @@ def cycle(grids, extra_finest_levels: int, vw: int = 1, itercrsn: int = 1, iterr
             return error
         g.rsdl()
         gp.mg_restrict()
-        cycle(grids, extra_finest_levels, vw, itercrsn, iterrfne, lvl + 1, mglvls, evaluate_preconditioner)
+        cycle(grids, extra_finest_levels, vw, itercrsn, iterrfne, lvl + 1, mglvls, evaluate_preconditioner, fused=False)
         gp.mg_prolongate()
         if evaluate_preconditioner and iterrfne:
             if g.setup_preconditioner():

@@ -147,6 +147,17 @@ class _TetHp:
                                                     float(eps_r), float(eps_a), _lib.ptr(K)), "element_jacobian")
         return K
 
+    def set_precision(self, mode: int, ninner: int = 1):
+        """HPG_FP64 (plain) or HPG_FP32_DEFECT: FP64 defect correction around `ninner`
+        multistage iterations whose element residuals run in FP32 arithmetic."""
+        self._check(self.L.hpg_tet_set_precision(self.h, int(mode), int(ninner)), "tet_set_precision")
+
+    def residual_fp32(self):
+        """The full residual with the element residual in FP32 arithmetic (inner-solve kernels)."""
+        v, e = self.zeros()
+        self._check(self.L.hpg_tet_residual_fp32(self.h, _lib.ptr(v), _lib.ptr(e)), "tet_residual_fp32")
+        return v, e
+
     def synchronize(self):
         self._check(self.L.hpg_tet_synchronize(self.h), "synchronize")
 

@@ -188,8 +188,8 @@ class TriHpIns:
             self.mg_prolongate()
 
     def set_precision(self, mode: int, ninner: int | None = None):
-        """HPG_FP64 (plain) or HPG_FP32_DEFECT (FP64 defect correction around FP32
-        inner multistage iterations on the linearised residual)."""
+        """HPG_FP64 (plain) or HPG_FP32_DEFECT (FP64 defect correction around `ninner`
+        multistage iterations with the FP32 element residual), ins and cns."""
         self._check(self.L.hpg_set_precision(self.h, mode), "set_precision")
         if ninner is not None:
             self._check(self.L.hpg_set_defect_inner(self.h, int(ninner)), "set_defect_inner")
@@ -204,6 +204,27 @@ class TriHpIns:
         out = self.zeros()
         self._check(self.L.hpg_apply_jacobian(self.h, *(_lib.ptr(x) for x in a), *(_lib.ptr(x) for x in out)),
                     "apply_jacobian")
+        return out
+
+    def mg_cycle(self, grids, extra_finest_levels: int, vw: int = 1, itercrsn: int = 1, iterrfne: int = 0) -> float:
+        """One block::cycle over `grids` (this context first) in one library call
+        (hpg_mg_cycle); returns the entry level's maxres; FloatingPointError as
+        hpmp.multigrid.cycle."""
+        hs = (C.c_void_p * len(grids))(*[g.h.value for g in grids])
+        mx = C.c_double()
+        rc = self.L.hpg_mg_cycle(hs, len(grids), int(extra_finest_levels), int(vw), int(itercrsn), int(iterrfne),
+                                 C.byref(mx))
+        self._check(rc, "mg_cycle")
+        if rc > 0:
+            raise FloatingPointError("setup_preconditioner reported an error")
+        if not mx.value > 0.0:
+            raise FloatingPointError("maxres is not positive (NaN or exact zero)")
+        return float(mx.value)
+
+    def residual_fp32(self):
+        """The full residual evaluated in FP32 by the defect correction's inner-solve kernels."""
+        out = self.zeros()
+        self._check(self.L.hpg_residual_fp32(self.h, *(_lib.ptr(x) for x in out)), "residual_fp32")
         return out
 
     def element_jacobian(self, tinds, eps_r: float = 1e-8, eps_a: float = 1e-8):

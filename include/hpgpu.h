@@ -112,6 +112,19 @@ int hpg_update(hpg_ctx *ctx);
  * to the fine state (tri_hp/movfin.cpp:16-82).  The fine context must outlive it. */
 int hpg_link_fine(hpg_ctx *coarse, hpg_ctx *fine, const int *cc_tri, const double *cc_wt, const int *fc_tri,
                   const double *fc_wt);
+/* replaces block::cycle (tri_mesh/blocks.cpp:1267-1348) over a stack of grids: grids[0]
+ * the fine context at its finest p-level, grids[k] (k >= 1) the coarse contexts linked
+ * with hpg_link_fine(grids[k], grids[k-1], ...); multigrid level lvl runs on grid
+ * clamp(lvl - extra_finest_levels, 0, ngrid - 1) (:1274-1275), the first
+ * extra_finest_levels levels lowering p on the fine mesh.  vw cycles per level (2: W),
+ * itercrsn updates before and iterrfne after each coarse correction, each after a
+ * setup_preconditioner.  The whole cycle is enqueued without host round trips: the
+ * setup_preconditioner error counts accumulate on the device and the return value is
+ * the number of grids that reported an error during the cycle (0: none; the reference
+ * stops at the first, this form finishes the cycle); maxres (may be NULL) receives the
+ * entry level's maxres after its first smoothing, as block::cycle reads it. */
+int hpg_mg_cycle(hpg_ctx *const *grids, int ngrid, int extra_finest_levels, int vw, int itercrsn, int iterrfne,
+                 double *maxres);
 /* replaces tri_hp::maxres (tri_hp/allocate.cpp:496-525); per_var may be NULL */
 int hpg_maxres(hpg_ctx *ctx, double *per_var, double *out);
 /* raw-residual judge: max |res| over every vertex, side and interior coefficient
@@ -127,9 +140,13 @@ int hpg_get_log2p(hpg_ctx *ctx, int *log2p);
 /* precision and preconditioner selection (no reference counterpart: additions
  * behind the same update() entry point).  In HPG_FP32_DEFECT mode hpg_update on
  * the finest level is one defect-correction iteration: the FP64 full residual
- * R(u0) (left in the residual arrays), then `ninner` multistage iterations of the
- * reference's scheme in FP32 on R(u0) + J(u0) delta = 0 with the exact Jacobian of
- * the element and edge residuals, then u0 += delta.  Coarse p-levels stay FP64. */
+ * R64(u0) (left in the residual arrays), then `ninner` multistage iterations of the
+ * reference's scheme in FP32 (FP32 MFMA element residual, FP32 minvrt) on
+ * F(v) = R32(v) + (1/c - 1) R64(u0) from v = fl32(u0), then u0 += c (v - fl32(u0))
+ * in FP64.  F(fl32(u0) + d) ~ R64(u0)/c + J(u0) d, so the inner solve approximates
+ * the defect equation for the scaled correction d = delta / c; c keeps |d| near
+ * 1e-3 |u0|, where FP32 residual differences resolve J(u0) d.  Coarse p-levels stay
+ * FP64.  The FP64 residual stays the judge, so the converged solution is FP64's. */
 int hpg_set_precision(hpg_ctx *ctx, int mode);
 int hpg_set_preconditioner(hpg_ctx *ctx, int mode);
 int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
@@ -138,6 +155,12 @@ int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
  * the directional derivative of tri_hp::rsdl(nstage). */
 int hpg_apply_jacobian(hpg_ctx *ctx, const double *dv, const double *ds, const double *di,
                        double *outv, double *outs, double *outi);
+/* Test support, no reference counterpart: the full residual of tri_hp::rsdl(nstage)
+ * (tri_hp/nstage.cpp:12-110) evaluated by the kernels of the defect correction's inner
+ * solve (finest level): ins in FP32 at fl32(u) (element and edge kernels); cns with the
+ * element residual in FP32 arithmetic and the boundary fluxes in FP64 (the residual
+ * arrays are overwritten). */
+int hpg_residual_fp32(hpg_ctx *ctx, double *resv, double *ress, double *resi);
 /* Replaces tri_hp::element_jacobian (tri_hp/nstage.cpp:131-165, fixed mesh) for a batch
  * of `nel` distinct elements `tind` on the current level, ins or cns: forward
  * differences of element_rsdl(tind, 0) with dw(var) = eps_r * sum of |uht(var)| over
@@ -150,15 +173,19 @@ int hpg_apply_jacobian(hpg_ctx *ctx, const double *dv, const double *ds, const d
  * perturbed coefficient per batch element: set it again before further use. */
 int hpg_element_jacobian(hpg_ctx *ctx, const int *tind, int nel, double eps_r, double eps_a, double *K);
 
-/* Learned preconditioner (no reference counterpart).  An MLP 11 -> 64 -> 64 -> 64 -> 3
+/* Learned preconditioner (no reference counterpart).  An MLP 11 -> 64 -> 64 -> 64 -> 9
  * (GELU, bfloat16 weights, FP32 accumulation) maps element features (vertex offsets,
- * mean u, v, p, log viscous scale, log analytic pseudo-time step) to log scalings of
- * the analytic tprcn of tri_hp_ins::setup_preconditioner
- * (tri_hp/ins/setup_preconditioner.cpp:202-204); with HPG_PRCN_LEARNED and an MLP
- * attached, hpg_setup_preconditioner applies tprcn *= exp(clamp(y, -2, 2)) before the
- * vertex/side assembly.  Training data come from the method itself: the features
- * and the measured element-Jacobian diagonal (sum over modes of |J_e(mn, mn)| per
- * variable n).  The MLP must not be trained while a context that uses it runs. */
+ * mean u, v, p, log viscous scale, log analytic pseudo-time step) to log scalings
+ * y[type][var] of the analytic element scaling tprcn of
+ * tri_hp_ins::setup_preconditioner (tri_hp/ins/setup_preconditioner.cpp:202-219) for
+ * the vertex (type 0), side (1) and interior (2) rows of the diagonal preconditioner;
+ * with HPG_PRCN_LEARNED and an MLP attached, hpg_setup_preconditioner on the finest
+ * level assembles vprcn from tprcn * exp(clamp(y[0], -2, 2)), sprcn and the side
+ * couplings of minvrt from the type-1 scaling and the interior solve from the type-2
+ * scaling (coarse p-levels stay analytic).  Training data come from the method itself:
+ * the features and the measured element-Jacobian diagonal per mode type (sum over the
+ * type's modes of |J_e(mn, mn)| per variable n).  The MLP must not be trained while a
+ * context that uses it runs. */
 typedef struct hpg_mlp hpg_mlp;
 int hpg_mlp_create(unsigned seed, hpg_mlp **out);
 int hpg_mlp_destroy(hpg_mlp *m);
@@ -170,25 +197,30 @@ int hpg_mlp_loss_grad(hpg_mlp *m, const float *X, const float *T, int n, float *
 int hpg_mlp_train(hpg_mlp *m, const float *X, const float *T, int n, int epochs, int batch, float lr, float *last_loss);
 /* No counterpart in the reference: an allocation-free form of hpg_mlp_loss_grad for
  * callers that cannot allocate device memory (code under HIP graph capture, tight
- * training loops, harnesses that forbid hipMalloc).  X (n x 11), T (n x 3), loss (1)
+ * training loops, harnesses that forbid hipMalloc).  X (n x 11), T (n x 9), loss (1)
  * and grad (hpg_mlp_num_params()) are DEVICE pointers of the caller; the per-block
  * partial gradients live in a 16 MiB static device array that the library holds on
  * each GPU until the process ends.  Returns -2 with nothing enqueued when
  * n > HPG_MLP_NOALLOC_MAX_SAMPLES.  Loss and gradient are bitwise those of
  * hpg_mlp_loss_grad; the call returns after its work is done.  Calls that overlap on
  * one GPU (from several threads or MLPs) share the array and race. */
-#define HPG_MLP_NOALLOC_MAX_SAMPLES 28864
+#define HPG_MLP_NOALLOC_MAX_SAMPLES 27712
 int hpg_mlp_loss_grad_noalloc(hpg_mlp *m, const float *X, const float *T, int n, float *loss, float *grad);
 int hpg_set_mlp(hpg_ctx *ctx, hpg_mlp *m);
+/* study support: log scalings y (ntri x 9, [type][var]) used in place of the MLP's output
+ * while HPG_PRCN_LEARNED is selected (y = NULL returns to the MLP) */
+int hpg_set_prcn_log_scalings(hpg_ctx *ctx, const float *y);
 int hpg_mlp_features(hpg_ctx *ctx, float *out);             /* ntri x 11 */
-int hpg_measure_jacobian_diag(hpg_ctx *ctx, double *out);   /* ntri x 3, finest level */
+int hpg_measure_jacobian_diag(hpg_ctx *ctx, double *out);   /* ntri x 3 x 3 ([type][var]), finest level */
 
 /* blocks until the context's stream is idle */
 int hpg_synchronize(hpg_ctx *ctx);
 /* Measurement support, no reference counterpart: bracket every element-residual
  * kernel launch with HIP events on the context's stream (enable != 0; clears the
  * record), and read back the summed duration, launch count and elements processed
- * of the full (beta > 0) or inviscid-only (full = 0) launches since enabling. */
+ * since enabling of one kind of launch: `full` = 1 the FP64 full (beta > 0) residual,
+ * 0 the FP64 inviscid-only one, 3 / 2 the same for the FP32 kernels of the
+ * defect-correction inner solve. */
 int hpg_kernel_timing(hpg_ctx *ctx, int enable);
 int hpg_kernel_timing_read(hpg_ctx *ctx, int full, double *total_ms, long long *nlaunch, long long *nelem);
 /* text of the last error on this context (static storage, never NULL) */
@@ -288,6 +320,17 @@ int hpg_tet_element_jacobian(hpg_tet_ctx *ctx, const int *tind, int nel, double 
 int hpg_tet_maxres(hpg_tet_ctx *ctx, double *per_var, double *out);
 /* max |res| over vertex and edge modes, essential components excluded */
 int hpg_tet_resnorm(hpg_tet_ctx *ctx, double *out);
+/* FP32 defect correction of tet_hp::update (no reference counterpart; the form of
+ * hpg_set_precision for tri contexts): with HPG_FP32_DEFECT each hpg_tet_update on the
+ * finest level evaluates the FP64 residual R64(u0), runs `ninner` multistage iterations
+ * whose element residuals (stages after the first) are evaluated in FP32 arithmetic on
+ * F(v) = R(v) + (1/c - 1) R64(u0) from v = fl32(u0), with the FP64 preconditioner and
+ * stage update, then sets u = u0 + c (v - fl32(u0)); c keeps |v - fl32(u0)| near
+ * 1e-3 |u0|.  The FP64 residual stays the judge of convergence. */
+int hpg_tet_set_precision(hpg_tet_ctx *ctx, int mode, int ninner);
+/* test support: the full residual with the element residual in FP32 arithmetic (the
+ * inner-solve kernels); overwrites the residual arrays */
+int hpg_tet_residual_fp32(hpg_tet_ctx *ctx, double *resv, double *rese);
 /* replace tet_hp::mg_restrict / mg_prolongate, p-levels (tet_hp/mg_restrict.cpp:4-25,
  * tet_hp/mg_prolongate.cpp:7-14) */
 int hpg_tet_mg_restrict(hpg_tet_ctx *ctx);

@@ -1,16 +1,16 @@
 """Oracle (test infrastructure only): numpy forward/backward of the preconditioner MLP.
 
-Network: 11 -> 64 -> 64 -> 64 -> 3, GELU (tanh form) on the hidden layers,
+Network: 11 -> 64 -> 64 -> 64 -> 9, GELU (tanh form) on the hidden layers,
 weights rounded to bfloat16 (round to nearest even) for the matrix products,
 accumulation in float32.  Flat parameter order: W1[11x64], b1[64], W2[64x64], b2,
-W3[64x64], b3, W4[64x3], b4[3] (row-major, W[in][out]).  Loss: mean over samples
+W3[64x64], b3, W4[64x9], b4[9] (row-major, W[in][out]).  Loss: mean over samples
 and outputs of (y - t)^2.  Only tests/ may import this module.
 """
 from __future__ import annotations
 
 import numpy as np
 
-SIZES = (11, 64, 64, 64, 3)
+SIZES = (11, 64, 64, 64, 9)
 C_GELU = np.float32(np.sqrt(2.0 / np.pi))
 
 

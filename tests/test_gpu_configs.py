@@ -44,3 +44,99 @@ def test_workload_a_rsdl_and_100_cycles(oracle_mod):
     er = rel_err(G.get_residual(), O.get_res())
     print("after 100 cycles: state rel err", es, "rsdl rel err", er, "maxres rel err", worst_maxres, flush=True)
     assert es < 1e-12 and er < 1e-12 and worst_maxres < 1e-12
+
+
+def test_workload_e_learned_setup_and_guard(oracle_mod):
+    """Workload E on its own mesh: 512 x 512 squares (jitter 0.2h), p = 4, mu = 1e-3,
+    Fourier state.  (1) The element features computed on the device against the same
+    quantities formed on the host from the mesh, the state and the analytic tprcn.
+    (2) The learned setup_preconditioner's vertex and side preconditioners against the
+    assembly (tri_hp/setup_preconditioner.cpp:9-96) of the analytic element scaling
+    times exp(clip(y)) with y from oracle/mlp.py on the same features.  (3) The guard's
+    fallback to the analytic preconditioner from a network whose pseudo-time steps are
+    7.4x the analytic ones, on the W cycle (vwcycle 2, itercrsn 2, iterrfne 2)."""
+    _case("workload E: 512x512 p=4 mu=1e-3 learned setup vs oracle MLP, guard fallback")
+    import mlp as OM
+    from hpmp import _lib, mesh as M, multigrid as MG, workloads as W
+    from hpmp.learned import MLP, GuardedSolver, features, shift_output
+    from hpmp.tri_hp import TriHpIns
+    m = M.unit_square(512, 0.2, 1)
+    assert m.ntri == 524288
+    st = W.fourier_state(m, 4, seed=2)
+    mu = 1e-3
+    G = TriHpIns(m, log2p=2, mu=mu)
+    G.set_state(*st)
+    G.set_preconditioner_mode(_lib.HPG_PRCN_ANALYTIC)
+    assert G.setup_preconditioner() == 0
+    vpa, spa, tp = G.get_preconditioner()
+
+    # (1) features: vertex coordinates about the centroid over sqrt(area), element-mean
+    # u, v, p, log(mu / (rho sqrt(area))), log(area / tprcn_u)
+    X = features(G)
+    tri = m.tri_pnt.reshape(-1, 3) if m.tri_pnt.ndim == 1 else m.tri_pnt
+    P = m.pnts[tri]                                   # (ntri, 3, 2)
+    area = 0.5 * np.abs((P[:, 0, 0] - P[:, 2, 0]) * (P[:, 1, 1] - P[:, 0, 1])
+                        - (P[:, 0, 1] - P[:, 2, 1]) * (P[:, 1, 0] - P[:, 0, 0]))
+    L = np.sqrt(area)
+    Xh = np.zeros_like(X, dtype=np.float64)
+    Xh[:, 0:6] = ((P - P.mean(axis=1, keepdims=True)) / L[:, None, None]).reshape(-1, 6)
+    Xh[:, 6:9] = st[0][tri].mean(axis=1)
+    Xh[:, 9] = np.log(1e-12 + mu / L)
+    Xh[:, 10] = np.log(area / tp[:, 0])
+    ef = np.abs(X - Xh).max(axis=0) / np.maximum(1.0, np.abs(Xh).max(axis=0))
+    print("feature max err per column", ef, flush=True)
+    assert np.all(np.isfinite(X)) and ef.max() < 1e-5
+
+    # (2) learned assembly against the oracle network on the same features
+    net = MLP(5)
+    theta = net.weights()
+    G.L.hpg_set_mlp(G.h, net.h)
+    G.set_preconditioner_mode(_lib.HPG_PRCN_LEARNED)
+    assert G.setup_preconditioner() == 0
+    vpl, spl, _ = G.get_preconditioner()
+    assert np.all(np.isfinite(vpl)) and np.all(np.isfinite(spl)) and (vpl > 0).all() and (spl > 0).all()
+    Yo = OM.forward(theta, X)[0].astype(np.float64)
+    y = np.clip(Yo, -2.0, 2.0)
+    tv, ts = tp * np.exp(y[:, 0:3]), tp * np.exp(y[:, 3:6])
+    sa, sl = np.zeros_like(vpa), np.zeros_like(vpa)
+    for k in range(3):
+        np.add.at(sa, tri[:, k], tp)
+        np.add.at(sl, tri[:, k], tv)
+    vexp = vpa * sa / sl                              # 1 / (vdiag * sum tv)
+    st_tri = np.full((m.nseg, 2), -1)
+    seg = m.tri_seg.reshape(-1, 3) if m.tri_seg.ndim == 1 else m.tri_seg
+    cnt = np.zeros(m.nseg, dtype=int)
+    for t in range(3):
+        s_ = seg[:, t]
+        slot = cnt[s_]
+        st_tri[s_, slot] = np.arange(m.ntri)
+        cnt[s_] += 1
+    ssum = ts[st_tri[:, 0]] + np.where(st_tri[:, 1:2] >= 0, ts[np.maximum(st_tri[:, 1], 0)], 0.0)
+    sexp = 1.0 / ssum
+    ev = np.abs(vpl / vexp - 1.0)
+    es = np.abs(spl / sexp - 1.0)
+    print("vertex prcn rel err median/99.9%/max", np.median(ev), np.quantile(ev, 0.999), ev.max(),
+          "side", np.median(es), np.quantile(es, 0.999), es.max(), flush=True)
+    # the network's outputs agree with the oracle's to FP32 summation order except where
+    # a bfloat16 activation rounds across a boundary (one bfloat16 ulp; a few samples)
+    for e in (ev, es):
+        assert np.median(e) < 1e-5 and np.quantile(e, 0.999) < 1e-3 and e.max() < 3e-2
+
+    # (3) guard on the W cycle
+    ngrid = 5
+    meshes, links = MG.square_hierarchy(m, 512, ngrid)
+    grids = MG.build_stack(TriHpIns, G, meshes, links, mu=mu)
+    w = net.weights()
+    w[:] = 0
+    net.set_weights(w)
+    shift_output(net, [-2.0] * 9)
+    G.set_state(*st)
+    gs = GuardedSolver(G, net, check_every=5, grids=grids, cyc=(2, 2, 2))
+    it, hist = gs.solve(tol_rel=1e-12, max_cycles=30)
+    print("fell back at", gs.fell_back_at, "history", hist, flush=True)
+    assert gs.fell_back_at is not None
+    assert np.isfinite(hist[-1][1]) and hist[-1][1] < 0.5 * hist[0][1]
+    for g in grids[:0:-1]:
+        g.close()
+    G.close()
+    net.close()

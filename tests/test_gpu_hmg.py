@@ -112,3 +112,25 @@ def test_h_multigrid_convergence_gain():
     err = rel_diff_mod_pressure(out[4][1], out[1][1])
     print("solution rel diff (mod pressure)", err, flush=True)
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("physics,n,log2p,ngrid,vw,ic,ir", [("ins", 16, 2, 3, 2, 2, 2), ("cns", 8, 1, 2, 1, 1, 1)])
+def test_fused_cycle_is_the_call_by_call_cycle(oracle_mod, physics, n, log2p, ngrid, vw, ic, ir):
+    """hpg_mg_cycle enqueues the same kernels in the same order as the level-by-level
+    recursion: states and maxres bitwise equal; and it stays on the oracle's trajectory."""
+    _case(f"fused cycle vs call by call {physics} {n}x{n} log2p={log2p} ngrid={ngrid} vw={vw} {ic}/{ir}")
+    from hpmp import multigrid as MG
+    _, ga, og = _stacks(oracle_mod, physics, n, log2p, ngrid)
+    _, gb, _ = _stacks(oracle_mod, physics, n, log2p, ngrid)
+    for k in range(3):
+        ea = MG.cycle(ga, log2p, vw, ic, ir, fused=True)
+        eb = MG.cycle(gb, log2p, vw, ic, ir, fused=False)
+        eo = MG.cycle(og, log2p, vw, ic, ir)
+        print("cycle", k, "maxres fused", ea, "call by call", eb, "oracle", eo, flush=True)
+        assert ea == eb
+        assert abs(ea - eo) <= 1e-10 * abs(eo)
+    for x, y in zip(ga[0].get_state(), gb[0].get_state()):
+        assert np.array_equal(x, y)
+    err = rel_err(ga[0].get_state(), og[0].get_state())
+    print("state rel err vs oracle", err, flush=True)
+    assert err < 1e-11

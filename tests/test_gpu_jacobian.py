@@ -101,8 +101,13 @@ def test_element_jacobian_diagonal_matches_linearised_probe(oracle_mod):
     m, G, O = _ins_pair(oracle_mod, 2, n=6)
     K = G.element_jacobian(np.arange(m.ntri))
     tm = K.shape[1] // 3
-    dK = np.abs(np.diagonal(K, axis1=1, axis2=2)).reshape(m.ntri, tm, 3).sum(axis=1)
+    p = 4   # log2p = 2
+    assert tm == (p + 1) * (p + 2) // 2
+    dg = np.abs(np.diagonal(K, axis1=1, axis2=2)).reshape(m.ntri, tm, 3)
+    # per mode type: vertex modes 0-2, side modes 3 .. 3p-1, interior modes 3p ..
+    dK = np.concatenate([dg[:, :3].sum(axis=1), dg[:, 3:3 * p].sum(axis=1), dg[:, 3 * p:].sum(axis=1)], axis=1)
     D = learned.jacobian_diag(G)
+    assert D.shape == (m.ntri, 9)
     err = np.abs(dK - D).max() / np.abs(D).max()
     print("rel diff", err, flush=True)
     assert err < 1e-4

@@ -61,7 +61,15 @@ def test_mlp_forward_ragged_sizes(n):
     Y = net.forward(X)
     Yo, _, _ = OM.forward(th, X)
     assert np.all(np.isfinite(Y))
-    assert np.abs(Y - Yo).max() <= 2e-3 * np.abs(Yo).max()
+    # Both round every activation to bfloat16 the same way, but an FP32 sum that lies
+    # within summation-order noise of a bfloat16 rounding boundary rounds to the
+    # neighbouring value (one bfloat16 ulp, 2^-8 relative): over 100 003 x 192 hidden
+    # activations a few hundred samples carry such a flip.  Every other sample agrees to
+    # FP32 summation order; a flipped one stays within a few bfloat16 ulps.
+    err = np.abs(Y - Yo).max(axis=1)
+    scale = np.abs(Yo).max()
+    assert np.mean(err > 2e-3 * scale) <= 1e-2
+    assert err.max() <= 1e-2 * scale
 
 
 def test_mlp_gradient_matches_oracle():
@@ -71,7 +79,7 @@ def test_mlp_gradient_matches_oracle():
     th = _theta(3)
     rng = np.random.default_rng(4)
     X = rng.standard_normal((777, 11)).astype(np.float32)
-    T = rng.standard_normal((777, 3)).astype(np.float32)
+    T = rng.standard_normal((777, 9)).astype(np.float32)
     net = MLP(0)
     net.set_weights(th)
     loss, g = net.loss_grad(X, T)
@@ -86,7 +94,9 @@ def test_mlp_training_reduces_loss_and_is_reproducible():
     from hpmp.learned import MLP
     rng = np.random.default_rng(5)
     X = rng.uniform(-1, 1, (8192, 11)).astype(np.float32)
-    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], X[:, 2] ** 2 - 0.3, np.tanh(X[:, 3] + X[:, 4])], 1).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], X[:, 2] ** 2 - 0.3, np.tanh(X[:, 3] + X[:, 4]),
+                  X[:, 5] * X[:, 6], np.cos(X[:, 7]) - 0.7, 0.5 * X[:, 8], X[:, 9] ** 3, X[:, 10] - X[:, 0],
+                  np.sin(X[:, 1] + X[:, 2])], 1).astype(np.float32)
     losses = []
     ws = []
     for _ in range(2):
@@ -144,7 +154,7 @@ def test_guard_falls_back_from_an_unstable_network():
     w = net.weights()
     w[:] = 0
     net.set_weights(w)
-    shift_output(net, [-2.0, -2.0, -2.0])     # pseudo-time steps 7.4x the analytic ones
+    shift_output(net, [-2.0] * 9)     # pseudo-time steps 7.4x the analytic ones
     G.set_state(*st)
     gs = GuardedSolver(G, net, check_every=10)
     it, hist = gs.solve(tol_rel=1e-6, max_cycles=6000)

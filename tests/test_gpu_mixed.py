@@ -1,8 +1,10 @@
-"""Mixed-precision path: FP32 linearised residual and FP64 defect correction.
+"""Mixed-precision path: FP32 residual kernels and FP64 defect correction.
 
-The FP32 Jacobian-vector product is pinned against central differences of the
-oracle's FP64 residual; the defect-corrected solve must reach the FP64 judge's
-1e-10 and the oracle's converged solution to 1e-10.
+The FP32 residual of the inner solve is pinned against the oracle's FP64 residual at
+FP32 accuracy; the exact FP32 Jacobian-vector product (hpg_apply_jacobian, used by the
+learned preconditioner's diagonal probe) against central differences of the oracle's
+FP64 residual; the defect-corrected solve must reach the FP64 judge's 1e-10 and the
+oracle's converged solution to 1e-10.
 """
 import numpy as np
 import pytest
@@ -58,6 +60,30 @@ def test_jacobian_matches_oracle_central_difference(oracle_mod, log2p):
     err = rel_err(jd, fd)
     print("rel err", err, flush=True)
     assert err < 2e-5
+
+
+@pytest.mark.parametrize("log2p", [0, 1, 2])
+def test_fp32_residual_matches_oracle(oracle_mod, log2p):
+    """The FP32 element and edge kernels of the inner solve (FP32 MFMA layout, operator
+    rows permuted back to the FP64 mode order) against the oracle's FP64 residual."""
+    _case(f"FP32 residual vs oracle log2p={log2p}")
+    from hpmp import mesh as M, workloads as W
+    from hpmp.tri_hp import TriHpIns
+    p = 1 << log2p
+    m = M.unit_square(8, 0.2, 3)
+    st = W.fourier_state(m, p, seed=23)
+    G = TriHpIns(m, log2p=log2p, mu=1e-2)
+    G.set_state(*st)
+    r32 = G.residual_fp32()
+    O = oracle_mod.TriHpIns(m, log2p=log2p, mu=1e-2)
+    O.set_state(*st)
+    O.rsdl()
+    err = rel_err(r32, O.get_res())
+    print("rel err", err, flush=True)
+    # FP32 evaluation: rounding of the state and of every product, FP32 accumulation
+    assert err < 2e-5
+    G.rsdl()
+    assert rel_err(G.get_residual(), O.get_res()) < 1e-12
 
 
 def test_mixed_precision_reaches_fp64_solution(oracle_mod):

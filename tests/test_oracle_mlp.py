@@ -8,7 +8,7 @@ def test_backprop_matches_central_differences():
     rng = np.random.default_rng(0)
     theta = rng.standard_normal(mlp.n_params()) * 0.2
     X = rng.standard_normal((37, 11))
-    T = rng.standard_normal((37, 3))
+    T = rng.standard_normal((37, 9))
     _, g = mlp.loss_and_grad(theta, X, T, exact=True)
     idx = rng.choice(theta.size, 60, replace=False)
     h = 1e-6

@@ -55,7 +55,7 @@ def test_loss_grad_noalloc_matches_loss_grad():
     nmax = _lib.HPG_MLP_NOALLOC_MAX_SAMPLES
     rng = np.random.default_rng(5)
     X = rng.standard_normal((nmax, 11)).astype(np.float32)
-    T = rng.standard_normal((nmax, 3)).astype(np.float32)
+    T = rng.standard_normal((nmax, 9)).astype(np.float32)
     dX, dT = _Dev(hip, X.nbytes), _Dev(hip, T.nbytes)
     dL, dG = _Dev(hip, 4), _Dev(hip, net.nparam * 4)
     try:
