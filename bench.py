@@ -20,23 +20,29 @@ Also reported (same JSON line):
 * ``element_jacobian``: batched forward-difference element Jacobians of 32 768
   distinct elements of the 512 x 512 p = 4 mesh (elements per second, K copied to
   host included), with the oracle's rate on 1024 elements beside it;
-* ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual with the
-  h+p multigrid cycle on the same 512 x 512 p = 4 mesh family at mu = 1e-2, for FP64
-  plain, mixed, mixed + learned preconditioner and FP64 + learned (training time
-  apart), with each converged solution's difference from the FP64 plain one.  (At
-  mu = 1e-3 the reference's explicit relaxation does not reach 1e-10 on meshes of
-  128 x 128 and finer with this cycle.)
+* ``time_to_tol``: wall time and cycles to a 1e-10 relative FP64 residual on the
+  same 512 x 512 p = 4 workload at mu = 1e-3 with the W cycle of the reference's
+  convergence deck (vwcycle 2, itercrsn 2, iterrfne 2), for FP64 plain, mixed, and
+  mixed + learned preconditioner (its training -- 2000 seeded states, device
+  training, calibration -- timed apart), with each converged solution's difference
+  from the FP64 plain one; a second, labelled configuration at mu = 1e-2 with the V
+  cycle (FP64 and mixed).
 
 Prints one JSON line (see the driver contract).  The CPU baseline is the oracle
-(single thread) on one step of the same 512 x 512 workload.
+(built -march=native on this host, OpenMP over this process's threads, and one
+thread beside it) on one step of the same 512 x 512 workload; as the checker the
+oracle also evaluates its FP64 residual at every converged GPU state.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -60,6 +66,41 @@ TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 218.0e6
 # 12 g^3 MAC = 12 750 MAC (25 500 flop) plus about 700 flop of pointwise tau, flux
 # Jacobians and stabilisation at each of the 25 Gauss points
 FLOP_CNS_FULL_P4 = 43000.0
+
+
+MI355X_FP32_TFLOPS = 157.3
+
+
+def _mixed_leg(g, state, steps, timing, timing_read, unknowns, flop, kernel, fp64_ms):
+    """The same steps with the FP32 defect correction (one FP64 residual and one FP32
+    multistage pass per update) and the FP32 element kernel's rate (timing code 3)."""
+    from hpmp import _lib
+    g.set_precision(_lib.HPG_FP32_DEFECT, ninner=1)
+    g.set_state(*state)
+    g.setup_preconditioner()
+    g.update()
+    g.set_state(*state)
+    g.synchronize()
+    timing(g.h, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if g.setup_preconditioner():
+            raise SystemExit("setup_preconditioner error in the mixed leg")
+        g.update()
+    g.synchronize()
+    t1 = time.perf_counter()
+    tot = C.c_double(); nl = C.c_longlong(); ne = C.c_longlong()
+    timing_read(g.h, 3, C.byref(tot), C.byref(nl), C.byref(ne))
+    timing(g.h, 0)
+    g.set_precision(_lib.HPG_FP64)
+    ach = flop * ne.value / (tot.value * 1e-3) / 1e12 if tot.value > 0 else 0.0
+    ms = 1e3 * (t1 - t0) / steps
+    return {"precision": "FP32 element residual inside FP64 defect correction, ninner 1",
+            "ms_per_step": ms, "dof_evals_per_s": unknowns * 5 * steps / (t1 - t0),
+            "speedup_vs_fp64_step": fp64_ms / ms,
+            "fp32_kernel": {"kernel": kernel, "avg_launch_ms": tot.value / max(1, nl.value),
+                            "achieved": ach, "peak": MI355X_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": ach / MI355X_FP32_TFLOPS}}
 
 
 def cns_throughput(args) -> dict:
@@ -104,6 +145,9 @@ def cns_throughput(args) -> dict:
                         "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
                         "avg_launch_ms": tot.value / max(1, nl.value),
                         "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_CNS_FULL_P4}}
+    if not args.no_mixed:
+        out["mixed"] = _mixed_leg(g, st, args.cns_steps, g.L.hpg_kernel_timing, g.L.hpg_kernel_timing_read, unknowns,
+                                  FLOP_CNS_FULL_P4, "k_elem_cns32<4,full>", out["ms_per_step"])
     g.close()
     return out
 
@@ -157,6 +201,9 @@ def tet_throughput(args) -> dict:
                         "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
                         "avg_launch_ms": tot.value / max(1, nl.value),
                         "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_TET_FULL_P2}}
+    if not args.no_mixed:
+        out["mixed"] = _mixed_leg(g, st, args.tet_steps, g.L.hpg_tet_kernel_timing, g.L.hpg_tet_kernel_timing_read,
+                                  unknowns, FLOP_TET_FULL_P2, "k_elem_tet32<2,full>", out["ms_per_step"])
     g.close()
     return out
 
@@ -204,72 +251,224 @@ def tetc_throughput(args) -> dict:
                         "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
                         "avg_launch_ms": tot.value / max(1, nl.value),
                         "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_TETC_FULL_P2}}
+    if not args.no_mixed:
+        out["mixed"] = _mixed_leg(g, (v, e), args.tet_steps, g.L.hpg_tet_kernel_timing,
+                                  g.L.hpg_tet_kernel_timing_read, unknowns, FLOP_TETC_FULL_P2,
+                                  "k_elem_tetc32<2,full>", out["ms_per_step"])
     g.close()
     return out
 
 
-def time_to_tol(args) -> dict:
-    """Cycles and wall time to a 1e-10 relative FP64 residual with the h+p multigrid
-    cycle (block::cycle over p = 4 -> 2 -> 1 on the fine mesh, then every coarser mesh
-    down to 4 x 4 squares): FP64 plain, mixed, mixed + learned, FP64 + learned."""
-    from hpmp import _lib, mesh as M, multigrid as MG, workloads as W
-    from hpmp.learned import MLP, GuardedSolver, calibrate, training_data
-    from hpmp.tri_hp import TriHpIns, rel_diff_mod_pressure
-
-    n, mu = args.tts_n, args.tts_mu
-    ngrid = int(np.log2(n // 4)) + 1
+def _tts_problem(n, mu, ngrid):
+    from hpmp import mesh as M, multigrid as MG, workloads as W
+    from hpmp.tri_hp import TriHpIns
     mesh = M.unit_square(n, 0.2, 1)
     st = W.fourier_state(mesh, 4, seed=2)
     meshes, links = MG.square_hierarchy(mesh, n, ngrid)
     g = TriHpIns(mesh, log2p=2, mu=mu)
     grids = MG.build_stack(TriHpIns, g, meshes, links, mu=mu)
-    res = {"workload": f"tri_hp ins {n}x{n} squares ({mesh.ntri} triangles, jitter 0.2h), p=4, mu={mu}, "
-                       f"Fourier state; h+p multigrid cycle (p 4->2->1, then {ngrid - 1} coarser meshes to 4x4), "
-                       "one update before and one after each coarse correction, judge every 25 cycles",
-           "tol_rel": 1e-10}
-    ref = None
+    return mesh, st, g, grids
 
-    def run(tag, precision, net):
-        nonlocal ref
+
+def _tts_legs(args, res, g, grids, st, cyc, legs, sols):
+    """Run each (tag, precision, net) leg from `st` to a 1e-10 relative FP64 residual."""
+    from hpmp.learned import GuardedSolver
+    from hpmp.tri_hp import rel_diff_mod_pressure
+    ref = None
+    for tag, precision, net in legs:
         g.set_precision(precision, ninner=args.ninner)
         g.set_state(*st)
         g.synchronize()
         t0 = time.perf_counter()
-        gs = GuardedSolver(g, net, grids=grids)
-        cyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles)
+        gs = GuardedSolver(g, net, grids=grids, cyc=cyc)
+        ncyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles)
         g.synchronize()
         t1 = time.perf_counter()
         sol = g.get_state()
         if ref is None:
             ref = sol
-        res[tag] = {"cycles": cyc, "seconds": t1 - t0, "final_rel": hist[-1][1] / hist[0][1],
+        res[tag] = {"cycles": ncyc, "seconds": t1 - t0, "final_rel": hist[-1][1] / hist[0][1],
                     "converged": bool(hist[-1][1] <= 1e-10 * hist[0][1]),
                     "rel_diff_vs_fp64_plain_mod_pressure_constant": rel_diff_mod_pressure(sol, ref),
                     "fell_back_at": gs.fell_back_at}
-        print(f"# time_to_tol {tag}: {cyc} cycles, {t1 - t0:.1f} s", file=sys.stderr, flush=True)
+        sols[f"{res['name']}/{tag}"] = sol
+        print(f"# time_to_tol {res['name']} {tag}: {ncyc} cycles, {t1 - t0:.1f} s", file=sys.stderr, flush=True)
+    base = res[legs[0][0]]["seconds"]
+    for tag, _, _ in legs[1:]:
+        res[tag]["speedup_vs_fp64_plain"] = base / res[tag]["seconds"]
 
-    run("fp64_plain", _lib.HPG_FP64, None)
-    run("mixed", _lib.HPG_FP32_DEFECT, None)
-    t0 = time.perf_counter()
-    states = [W.fourier_state(mesh, 4, seed=s) for s in (11, 12, 13, 14)]
-    g.set_precision(_lib.HPG_FP64)
-    X, T, _ = training_data(g, states)
-    pick = np.random.default_rng(0).permutation(X.shape[0])[:args.tts_samples]
-    X, T = X[pick], T[pick]
-    net = MLP(11)
-    loss = net.train(X, T, epochs=200, batch=256, lr=2e-3)
-    scale, rates = calibrate(g, net, states[0], cycles=50, grids=grids)
-    res["learned_training"] = {"seconds": time.perf_counter() - t0, "samples": int(X.shape[0]),
-                               "final_loss": loss, "calibrated_scale": scale}
-    run("mixed_learned", _lib.HPG_FP32_DEFECT, net)
-    run("fp64_learned", _lib.HPG_FP64, net)
-    for k in ("mixed", "mixed_learned", "fp64_learned"):
-        res[k]["speedup_vs_fp64_plain"] = res["fp64_plain"]["seconds"] / res[k]["seconds"]
+
+def time_to_tol(args, sols) -> dict:
+    """Cycles and wall time to a 1e-10 relative FP64 residual on the 512 x 512 p = 4
+    workload at mu = 1e-3 with the reference convergence deck's multigrid cycle (W cycle,
+    two relaxations before and after each coarse correction: vwcycle 2, itercrsn 2,
+    iterrfne 2) over p = 4 -> 2 -> 1 on the fine mesh and the meshes down to 32 x 32:
+    FP64 plain, mixed (FP32 defect correction), and mixed + learned preconditioner
+    (training apart).  A second, labelled configuration keeps mu = 1e-2 with the V cycle
+    (one relaxation before and after, meshes down to 4 x 4), FP64 and mixed.  Converged
+    states go to `sols` for the FP64 oracle check."""
+    from hpmp import _lib
+    from hpmp.learned import train_learned
+
+    out = {}
+    n = args.tts_n
+    ngrid = max(1, int(np.log2(n // 32)) + 1)
+    mesh, st, g, grids = _tts_problem(n, 1e-3, ngrid)
+    res = {"name": "mu1e-3_W222",
+           "workload": f"tri_hp ins {n}x{n} squares ({mesh.ntri} triangles, jitter 0.2h), p=4, mu=1e-3, "
+                       f"Fourier state; h+p W cycle (vwcycle 2, itercrsn 2, iterrfne 2; p 4->2->1, then "
+                       f"{ngrid - 1} coarser meshes to 32x32), FP64 judge every 25 cycles",
+           "tol_rel": 1e-10}
+    legs = [("fp64_plain", _lib.HPG_FP64, None), ("mixed", _lib.HPG_FP32_DEFECT, None)]
+    net = None
+    if not args.no_learned:
+        t0 = time.perf_counter()
+        net, info = train_learned(1e-3, n, ntrain=args.tts_states, cyc=(2, 2, 2),
+                                  log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        info["wall_seconds"] = time.perf_counter() - t0
+        res["learned_training"] = info
+        legs.append(("mixed_learned", _lib.HPG_FP32_DEFECT, net))
+        if args.tts_fp64_learned:
+            legs.append(("fp64_learned", _lib.HPG_FP64, net))
+    _tts_legs(args, res, g, grids, st, (2, 2, 2), legs, sols)
+    if net is not None:
+        res["learned_training"]["training_below_fp64_solve"] = bool(
+            res["learned_training"]["wall_seconds"] < res["fp64_plain"]["seconds"])
+        net.close()
     for x in grids[:0:-1]:
         x.close()
     g.close()
-    net.close()
-    return res
+    out["primary"] = res
+    if args.tts_secondary:
+        ngrid2 = int(np.log2(n // 4)) + 1
+        mesh, st, g, grids = _tts_problem(n, 1e-2, ngrid2)
+        res2 = {"name": "mu1e-2_V111",
+                "workload": f"tri_hp ins {n}x{n} squares, p=4, mu=1e-2 (a second configuration beside the "
+                            f"mu=1e-3 one), Fourier state; h+p V cycle (one relaxation before and after each "
+                            f"coarse correction; p 4->2->1, then {ngrid2 - 1} coarser meshes to 4x4)",
+                "tol_rel": 1e-10}
+        _tts_legs(args, res2, g, grids, st, (1, 1, 1),
+                  [("fp64_plain", _lib.HPG_FP64, None), ("mixed", _lib.HPG_FP32_DEFECT, None)], sols)
+        for x in grids[:0:-1]:
+            x.close()
+        g.close()
+        out["secondary"] = res2
+    return out
+
+
+def host_cpu() -> tuple[str, str]:
+    """(model name, flags) of this host's CPU from /proc/cpuinfo."""
+    model, flags = "unknown", ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return model, flags
+
+
+def start_native_oracle_build():
+    """Start the -march=native build of the oracle for this host's CPU in the background,
+    so it compiles while the GPU legs run.  The build folder is named after the CPU model
+    and flags, so a build made on another machine is never loaded here.  Returns
+    (library path, process)."""
+    model, flags = host_cpu()
+    key = hashlib.sha1((model + "|" + flags).encode()).hexdigest()[:12]
+    d = os.path.join(ROOT, "oracle", "_build", "native-" + key)
+    err = tempfile.TemporaryFile()
+    proc = subprocess.Popen(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_DIR={d}"],
+                            stdout=subprocess.DEVNULL, stderr=err)
+    return os.path.join(d, "liboracle.so"), proc, err
+
+
+def cpu_baseline(args, native, jac, sols=None, tts=None) -> dict:
+    """The oracle (C++ restatement of the reference path) on the host cores: one
+    setup_preconditioner + update step of the 512 x 512 p = 4 workload with one thread
+    and with OpenMP over every thread of this process's CPU share (element loops; the
+    results are bitwise those of one thread).  As the checker it then evaluates its own
+    FP64 residual at every converged GPU state of time_to_tol (the error of each result
+    against the FP64 oracle: its residual there relative to its residual at the start)."""
+    lib, proc, err = native
+    march = "native"
+    try:
+        rc = proc.wait(timeout=600)
+    except subprocess.TimeoutExpired:
+        proc.kill()
+        rc = -1
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if rc == 0 and os.path.exists(lib):
+        oracle.LIB_PATH = lib
+    else:
+        err.seek(0)
+        march = "x86-64-v3 (the -march=native build failed: " + err.read().decode(errors="replace")[-200:] + ")"
+    from hpmp import mesh as M, workloads as W
+    model, _ = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+    cm = M.unit_square(args.cpu_n, 0.2, 1)
+    cst = W.fourier_state(cm, 4, seed=2)
+    cunk = 3 * (cm.npnt + cm.nseg * 3 + cm.ntri * 3)
+    rates = {}
+    for nth in (1, threads):
+        oracle.set_threads(nth)
+        o = oracle.TriHpIns(cm, log2p=2, mu=1e-3)
+        o.set_state(*cst)
+        c0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            o.setup_preconditioner()
+            o.update()
+        rates[nth] = (cunk * 5 * args.cpu_steps / (time.perf_counter() - c0), time.perf_counter() - c0)
+        del o
+    if jac is not None:
+        # the oracle's element_jacobian beside it, one thread, on a bounded sample of the batch
+        oracle.set_threads(1)
+        o = oracle.TriHpIns(cm, log2p=2, mu=1e-3)
+        o.set_state(*cst)
+        jb = np.random.default_rng(0).choice(cm.ntri, size=1024, replace=False)
+        k0 = time.perf_counter()
+        for t in jb:
+            o.element_jacobian(int(t))
+        k1 = time.perf_counter()
+        jac["cpu_elements_per_s"] = jb.size / (k1 - k0)
+        jac["cpu_sample"] = f"oracle, 1 thread, {jb.size} elements, {k1 - k0:.2f} s"
+    if sols and tts:
+        oracle.set_threads(threads)
+        for key, sol in sols.items():
+            name, tag = key.split("/")
+            cfg = next(c for c in tts.values() if c["name"] == name)
+            mu = 1e-3 if name.startswith("mu1e-3") else 1e-2
+            mesh = M.unit_square(args.tts_n, 0.2, 1)
+            o = oracle.TriHpIns(mesh, log2p=2, mu=mu)
+            bv, bs = mesh.boundary_vertex_mask(), mesh.boundary_segment_mask()
+
+            def judge(state):
+                o.set_state(*state)
+                o.rsdl()
+                v, sd, it = [np.array(a) for a in o.get_res()]
+                v[bv, :2] = 0.0
+                sd[bs, :, :2] = 0.0
+                return max(np.abs(v).max(), np.abs(sd).max(), np.abs(it).max())
+
+            r0 = judge(W.fourier_state(mesh, 4, seed=2))
+            cfg[tag]["oracle_fp64_residual_rel"] = judge(sol) / r0
+            del o
+    return {
+        "value": rates[threads][0],
+        "unit": "DOF-evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (C++ restatement, OpenMP over {threads} threads in the element loops, -march={march}) "
+                  f"on {args.cpu_n}x{args.cpu_n} squares ({cm.ntri} triangles) of the same workload, "
+                  f"{args.cpu_steps} step(s), {rates[threads][1]:.1f} s",
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
+        "march": march,
+        "single_thread": {"value": rates[1][0], "cores": 1, "seconds": rates[1][1]},
+    }
 
 
 def main() -> int:
@@ -282,12 +481,14 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--ninner", type=int, default=2)
+    ap.add_argument("--ninner", type=int, default=1)
     ap.add_argument("--no-tts", action="store_true")
     ap.add_argument("--tts-n", type=int, default=512)
-    ap.add_argument("--tts-mu", type=float, default=1e-2)
     ap.add_argument("--tts-max-cycles", type=int, default=3000)
-    ap.add_argument("--tts-samples", type=int, default=65536)
+    ap.add_argument("--tts-states", type=int, default=2000)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--tts-fp64-learned", action="store_true")
+    ap.add_argument("--tts-secondary", type=int, default=1)
     ap.add_argument("--no-cns", action="store_true")
     ap.add_argument("--cns-n", type=int, default=724)
     ap.add_argument("--cns-steps", type=int, default=3)
@@ -314,6 +515,7 @@ def main() -> int:
     from hpmp import mesh as M, workloads as W
     from hpmp.tri_hp import TriHpIns
 
+    native = None if args.no_cpu else start_native_oracle_build()
     n = args.n
     mesh = M.unit_square(n, 0.2, 1)
     st = W.fourier_state(mesh, 4, seed=2)
@@ -356,12 +558,19 @@ def main() -> int:
             g.update()
         g.set_state(*st)
         g.synchronize()
+        g.L.hpg_kernel_timing(g.h, 1)
         m0 = time.perf_counter()
         for _ in range(args.steps):
             g.setup_preconditioner()
             g.update()
         g.synchronize()
         m1 = time.perf_counter()
+        kt = {}
+        for code, tag in ((1, "fp64_full"), (3, "fp32_full"), (2, "fp32_inviscid")):
+            g.L.hpg_kernel_timing_read(g.h, code, C.byref(tot), C.byref(nl), C.byref(ne))
+            kt[tag] = {"avg_launch_ms": tot.value / max(1, nl.value), "launches": nl.value,
+                       "elements_per_launch": ne.value / max(1, nl.value)}
+        g.L.hpg_kernel_timing(g.h, 0)
         g.rsdl()
         mixed = {
             "ninner": args.ninner,
@@ -371,6 +580,7 @@ def main() -> int:
             "dof_evals_per_s": unknowns * (1 + 5 * args.ninner) * args.steps / (m1 - m0),
             "judge_after_steps": g.resnorm(),
             "fp64_plain_judge_after_steps": judge,
+            "element_kernels": kt,
         }
         g.set_precision(_lib.HPG_FP64)
     jac = None
@@ -440,8 +650,9 @@ def main() -> int:
         out["mixed"] = mixed
     if jac is not None:
         out["element_jacobian"] = jac
+    sols = {}
     if not args.no_tts:
-        out["time_to_tol"] = time_to_tol(args)
+        out["time_to_tol"] = time_to_tol(args, sols)
     if not args.no_cns:
         out["cns"] = cns_throughput(args)
     if not args.no_tet:
@@ -449,35 +660,8 @@ def main() -> int:
         out["tet_cns"] = tetc_throughput(args)
 
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        cm = M.unit_square(args.cpu_n, 0.2, 1)
-        cst = W.fourier_state(cm, 4, seed=2)
-        o = oracle.TriHpIns(cm, log2p=2, mu=1e-3)
-        o.set_state(*cst)
-        cunk = 3 * (cm.npnt + cm.nseg * 3 + cm.ntri * 3)
-        c0 = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            o.setup_preconditioner()
-            o.update()
-        c1 = time.perf_counter()
-        if jac is not None:
-            # the oracle's element_jacobian beside it, on a bounded sample of the batch
-            jb = np.random.default_rng(0).choice(cm.ntri, size=1024, replace=False)
-            k0 = time.perf_counter()
-            for t in jb:
-                o.element_jacobian(int(t))
-            k1 = time.perf_counter()
-            out["element_jacobian"]["cpu_elements_per_s"] = jb.size / (k1 - k0)
-            out["element_jacobian"]["cpu_sample"] = f"oracle, 1 thread, {jb.size} elements, {k1 - k0:.2f} s"
-        out["cpu_baseline"] = {
-            "value": cunk * 5 * args.cpu_steps / (c1 - c0),
-            "unit": "DOF-evals/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"oracle (C++ restatement, 1 thread) on {args.cpu_n}x{args.cpu_n} squares "
-                      f"({cm.ntri} triangles) of the same workload, {args.cpu_steps} steps, {c1 - c0:.1f} s",
-        }
+        out["cpu_baseline"] = cpu_baseline(args, native, out.get("element_jacobian"), sols,
+                                           out.get("time_to_tol"))
     print(json.dumps(out), flush=True)
     return 0
 
