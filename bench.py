@@ -273,15 +273,21 @@ def _tts_problem(n, mu, ngrid):
 def _tts_legs(args, res, g, grids, st, cyc, legs, sols):
     """Run each (tag, precision, net) leg from `st` to a 1e-10 relative FP64 residual."""
     from hpmp.learned import GuardedSolver
-    from hpmp.tri_hp import rel_diff_mod_pressure
+    from hpmp.tri_hp import CFL_DEFAULT, rel_diff_mod_pressure
     ref = None
-    for tag, precision, net in legs:
+    for leg in legs:
+        tag, precision, net = leg[:3]
+        cscale = leg[3] if len(leg) > 3 else 1.0
+        for x in grids:
+            x.set_cfl(list(CFL_DEFAULT))
         g.set_precision(precision, ninner=args.ninner)
         g.set_state(*st)
         g.synchronize()
         t0 = time.perf_counter()
-        gs = GuardedSolver(g, net, grids=grids, cyc=cyc)
-        ncyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles)
+        gs = GuardedSolver(g, net, grids=grids, cyc=cyc, cfl_scale=None if cscale == 1.0 else cscale)
+        ncyc, hist = gs.solve(tol_rel=1e-10, max_cycles=args.tts_max_cycles,
+                              progress=lambda it, r: print(f"# {tag} cycle {it} judge {r:.3e}", file=sys.stderr,
+                                                           flush=True))
         g.synchronize()
         t1 = time.perf_counter()
         sol = g.get_state()
@@ -291,20 +297,27 @@ def _tts_legs(args, res, g, grids, st, cyc, legs, sols):
                     "converged": bool(hist[-1][1] <= 1e-10 * hist[0][1]),
                     "rel_diff_vs_fp64_plain_mod_pressure_constant": rel_diff_mod_pressure(sol, ref),
                     "fell_back_at": gs.fell_back_at}
+        if cscale != 1.0:
+            res[tag]["cfl_scale"] = cscale
         sols[f"{res['name']}/{tag}"] = sol
         print(f"# time_to_tol {res['name']} {tag}: {ncyc} cycles, {t1 - t0:.1f} s", file=sys.stderr, flush=True)
+    for x in grids:
+        x.set_cfl(list(CFL_DEFAULT))
     base = res[legs[0][0]]["seconds"]
-    for tag, _, _ in legs[1:]:
-        res[tag]["speedup_vs_fp64_plain"] = base / res[tag]["seconds"]
+    for leg in legs[1:]:
+        res[leg[0]]["speedup_vs_fp64_plain"] = base / res[leg[0]]["seconds"]
 
 
 def time_to_tol(args, sols) -> dict:
     """Cycles and wall time to a 1e-10 relative FP64 residual on the 512 x 512 p = 4
     workload at mu = 1e-3 with the reference convergence deck's multigrid cycle (W cycle,
     two relaxations before and after each coarse correction: vwcycle 2, itercrsn 2,
-    iterrfne 2) over p = 4 -> 2 -> 1 on the fine mesh and the meshes down to 32 x 32:
-    FP64 plain, mixed (FP32 defect correction), and mixed + learned preconditioner
-    (training apart).  A second, labelled configuration keeps mu = 1e-2 with the V cycle
+    iterrfne 2) over p = 4 -> 2 -> 1 on the fine mesh and the coarser meshes down to
+    16 x 16 (the fastest hierarchy for FP64 plain: 160 cycles, against 230 down to
+    32 x 32 and 160 at a higher cost per cycle down to 8 x 8 or 4 x 4): FP64
+    plain, mixed (FP32 defect correction), mixed + learned preconditioner (the network
+    on every level; training apart), and the control: FP64 with the analytic
+    preconditioner and the CFL multiplier calibrated the same way.  A second, labelled configuration keeps mu = 1e-2 with the V cycle
     (one relaxation before and after, meshes down to 4 x 4), FP64 and mixed.  Converged
     states go to `sols` for the FP64 oracle check."""
     from hpmp import _lib
@@ -312,24 +325,27 @@ def time_to_tol(args, sols) -> dict:
 
     out = {}
     n = args.tts_n
-    ngrid = max(1, int(np.log2(n // 32)) + 1)
+    ngrid = args.tts_ngrid or max(1, int(np.log2(n // 16)) + 1)
     mesh, st, g, grids = _tts_problem(n, 1e-3, ngrid)
     res = {"name": "mu1e-3_W222",
            "workload": f"tri_hp ins {n}x{n} squares ({mesh.ntri} triangles, jitter 0.2h), p=4, mu=1e-3, "
                        f"Fourier state; h+p W cycle (vwcycle 2, itercrsn 2, iterrfne 2; p 4->2->1, then "
-                       f"{ngrid - 1} coarser meshes to 32x32), FP64 judge every 25 cycles",
+                       f"{ngrid - 1} coarser meshes to {n >> (ngrid - 1)}x{n >> (ngrid - 1)}), FP64 judge every 25 cycles",
            "tol_rel": 1e-10}
     legs = [("fp64_plain", _lib.HPG_FP64, None), ("mixed", _lib.HPG_FP32_DEFECT, None)]
     net = None
     if not args.no_learned:
         t0 = time.perf_counter()
-        net, info = train_learned(1e-3, n, ntrain=args.tts_states, cyc=(2, 2, 2),
+        net, info = train_learned(1e-3, n, ntrain=args.tts_states, cyc=(2, 2, 2), cfl_control=True,
                                   log=lambda m: print("# " + m, file=sys.stderr, flush=True))
         info["wall_seconds"] = time.perf_counter() - t0
         res["learned_training"] = info
         legs.append(("mixed_learned", _lib.HPG_FP32_DEFECT, net))
         if args.tts_fp64_learned:
             legs.append(("fp64_learned", _lib.HPG_FP64, net))
+        # control: what the same calibration gives the analytic preconditioner through
+        # one CFL multiplier on every level (the reference deck's C)
+        legs.append(("fp64_cfl_calibrated", _lib.HPG_FP64, None, info["cfl_control"]["scale"]))
     _tts_legs(args, res, g, grids, st, (2, 2, 2), legs, sols)
     if net is not None:
         res["learned_training"]["training_below_fp64_solve"] = bool(
@@ -486,6 +502,7 @@ def main() -> int:
     ap.add_argument("--tts-n", type=int, default=512)
     ap.add_argument("--tts-max-cycles", type=int, default=3000)
     ap.add_argument("--tts-states", type=int, default=2000)
+    ap.add_argument("--tts-ngrid", type=int, default=0, help="meshes in the mu = 1e-3 hierarchy (0: down to 16 x 16)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--tts-fp64-learned", action="store_true")
     ap.add_argument("--tts-secondary", type=int, default=1)

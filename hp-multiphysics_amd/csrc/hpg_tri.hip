@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -114,6 +115,7 @@ struct Level {
     int p = 0;
     std::unique_ptr<TriOps> ops;
     double* pack = nullptr;
+    double* op1 = nullptr;  // p = 1: dense P, Pr, Ps, Ir, Is, Dr, Ds (k_elem_ins1_valu layout)
     double* Pmat = nullptr;
     double* lump = nullptr;
     double* edge = nullptr;  // g1 | dg1 | w1
@@ -189,6 +191,7 @@ struct hpg_ctx {
     // p = 1 vertex path (ins): element-major element and boundary-edge results and the
     // boundary-edge endpoints at each vertex (entries 2*idx + k into eout)
     double *eres = nullptr, *eout = nullptr;
+    float* eres32 = nullptr;  // the FP32 inner solve's element-major residual
     int *vb_ptr = nullptr, *vb_ent = nullptr;
     int *vs_ptr, *vs_side, *vs_end, *vt_ptr, *vt_tri, *vt_loc, *st_tri, *st_loc;
     unsigned char *vess, *sess;
@@ -346,6 +349,50 @@ void launch_elem(hpg_ctx* c, const ElemArgs& a0, bool full, const std::vector<in
     }
 }
 
+// element-major residual over a0.order[begin:end) (null order: mesh order) in one
+// persistent launch (timing records 1 / 0 like the colour launches)
+template <int P>
+void launch_elem_em(hpg_ctx* c, const ElemArgs& a, bool full) {
+    const int nlogical = (a.end - a.begin + 63) / 64;
+    if (nlogical == 0) return;
+    const size_t r = c->recs.size();
+    if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+    if (full) {
+        static const int cap = resident_blocks((const void*)k_elem_ins_em<P, true>);
+        hipLaunchKernelGGL((k_elem_ins_em<P, true>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream,
+                           a, c->eres);
+    } else {
+        static const int cap = resident_blocks((const void*)k_elem_ins_em<P, false>);
+        hipLaunchKernelGGL((k_elem_ins_em<P, false>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream,
+                           a, c->eres);
+    }
+    if (c->timing) {
+        check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+        c->recs.push_back({full ? 1 : 0, a.end - a.begin});
+    }
+}
+
+template <int P>
+void launch_elem32_em(hpg_ctx* c, const ElemArgs32& a, bool full) {
+    const int nlogical = (a.end - a.begin + 63) / 64;
+    if (nlogical == 0) return;
+    const size_t r = c->recs.size();
+    if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
+    if (full) {
+        static const int cap = resident_blocks((const void*)k_elem_ins32_em<P, true>);
+        hipLaunchKernelGGL((k_elem_ins32_em<P, true>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0,
+                           c->stream, a, c->eres32);
+    } else {
+        static const int cap = resident_blocks((const void*)k_elem_ins32_em<P, false>);
+        hipLaunchKernelGGL((k_elem_ins32_em<P, false>), dim3(persistent_grid(nlogical, cap)), dim3(256), 0,
+                           c->stream, a, c->eres32);
+    }
+    if (c->timing) {
+        check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
+        c->recs.push_back({full ? 3 : 2, a.end - a.begin});
+    }
+}
+
 // FP32 element residual of the defect-correction inner solve, colour by colour (timing
 // records 3 / 2 like the linearised kernel)
 template <int P>
@@ -439,7 +486,7 @@ MinvArgsT<T> minv_args(hpg_ctx* c) {
 
 void do_rsdl_cns(hpg_ctx* c, int stage);
 
-// ins stage at p = 1 in three launches: element residuals and boundary-edge fluxes in
+// ins stage at p = 1 in two launches: element residuals and boundary-edge fluxes in
 // element-major form (no colouring), then one lane per (vertex, variable) sums them in
 // a fixed order, adds the multigrid source and, with cfa >= 0, applies the vertex
 // minvrt and the stage update (save_u0: first stage, ug0 = ug)
@@ -451,36 +498,33 @@ void vstage_p1(hpg_ctx* c, int stage, double cfa, bool save_u0) {
     const Level& L = c->lv[c->log2p];
     ElemArgs a{};
     a.pnts = c->pnts; a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
-    a.order = c->order; a.begin = 0; a.end = c->ntri;
+    a.order = nullptr; a.begin = 0; a.end = c->ntri;   // mesh order: no colouring needed
     a.ugv = c->ugv; a.ugs = c->ugs; a.ugi = c->ugi;
     a.ops = L.pack; a.sm0 = c->sm0; a.im0 = c->im0;
     a.rho = c->prm.rho; a.mu = c->prm.mu; a.adis = c->prm.adis; a.beta = beta;
     a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0; a.keep = f;
-    const int nlogical = (c->ntri + 63) / 64;
+    const LevelDims d(1);
+    EdgeArgs e{};
+    e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
+    e.resv = c->resv; e.ress = c->ress;
+    e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
+    e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
+    e.begin = 0; e.end = c->neseg;
+    // element blocks (one thread per element, mesh order) followed by the boundary-edge
+    // blocks
+    const int eg = (int)nblk(c->ntri, 256);
+    const int eblk = (int)nblk(c->neseg, 256);
     const size_t r = c->recs.size();
     if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
-    if (full) {
-        static const int cap = resident_blocks((const void*)k_elem_ins1_em<true>);
-        hipLaunchKernelGGL(k_elem_ins1_em<true>, dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream, a,
-                           c->eres);
-    } else {
-        static const int cap = resident_blocks((const void*)k_elem_ins1_em<false>);
-        hipLaunchKernelGGL(k_elem_ins1_em<false>, dim3(persistent_grid(nlogical, cap)), dim3(256), 0, c->stream, a,
-                           c->eres);
-    }
+    if (full)
+        hipLaunchKernelGGL(k_elem_ins1_valu<true>, dim3(eg + eblk), dim3(256), 0, c->stream, a, L.op1, c->eres, e,
+                           c->eout, eg);
+    else
+        hipLaunchKernelGGL(k_elem_ins1_valu<false>, dim3(eg + eblk), dim3(256), 0, c->stream, a, L.op1, c->eres, e,
+                           c->eout, eg);
     if (c->timing) {
         check(hipEventRecord(c->event(2 * r + 1), c->stream), "hipEventRecord");
         c->recs.push_back({full ? 1 : 0, c->ntri});
-    }
-    if (c->neseg > 0) {
-        const LevelDims d(1);
-        EdgeArgs e{};
-        e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
-        e.resv = c->resv; e.ress = c->ress;
-        e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
-        e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
-        e.begin = 0; e.end = c->neseg;
-        hipLaunchKernelGGL(k_edge_ins1_em, dim3(nblk(c->neseg, 64)), dim3(64), 0, c->stream, e, c->eout);
     }
     VStageArgs v{};
     v.npnt = c->npnt; v.vt_ptr = c->vt_ptr; v.vt_tri = c->vt_tri; v.vt_loc = c->vt_loc;
@@ -519,11 +563,30 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     a.lin = (store_lin && beta > 0.0) ? c->lin : nullptr;
     a.gbase = 0;
     const bool full = beta > 0.0;
-    switch (L.p) {
-        case 1: launch_elem<1>(c, a, full); break;
-        case 2: launch_elem<2>(c, a, full); break;
-        case 4: launch_elem<4>(c, a, full); break;
-        default: throw std::runtime_error("unsupported p");
+    if (!store_lin) {
+        // element-major over all elements in mesh order, then the fixed-order gather
+        a.order = nullptr; a.begin = 0; a.end = c->ntri;
+        switch (L.p) {
+            case 2: launch_elem_em<2>(c, a, full); break;
+            case 4: launch_elem_em<4>(c, a, full); break;
+            default: throw std::runtime_error("unsupported p");
+        }
+        const LevelDims dd(L.p);
+        GatherArgsT<double> ga{};
+        ga.npnt = c->npnt; ga.nseg = c->nseg; ga.sm = dd.sm; ga.sm0 = c->sm0; ga.bm = dd.bm;
+        ga.vt_ptr = c->vt_ptr; ga.vt_tri = c->vt_tri; ga.vt_loc = c->vt_loc;
+        ga.st_tri = c->st_tri; ga.st_loc = c->st_loc; ga.eres = c->eres;
+        ga.resv = c->resv; ga.rrv = c->rrv; ga.ress = c->ress; ga.rrs = c->rrs;
+        ga.keep = f; ga.full = full ? 1 : 0;
+        const long lanes = ((long)c->npnt + (long)c->nseg * dd.sm) * NV;
+        hipLaunchKernelGGL((k_gather_vs<double, NV>), dim3(nblk(lanes)), dim3(256), 0, c->stream, ga);
+    } else {
+        switch (L.p) {
+            case 1: launch_elem<1>(c, a, full); break;
+            case 2: launch_elem<2>(c, a, full); break;
+            case 4: launch_elem<4>(c, a, full); break;
+            default: throw std::runtime_error("unsupported p");
+        }
     }
     const LevelDims d(L.p);
     EdgeArgs e;
@@ -574,11 +637,24 @@ void do_rsdl32(hpg_ctx* c, int stage) {
     a.lin = nullptr; a.gbase = 0;
     a.srcv = c->resv; a.srcs = c->ress; a.srci = c->resi; a.sfac = c->fd_scal + 1;
     const bool full = beta > 0.0;
-    switch (L.p) {
-        case 1: launch_elem32<1>(c, a, full); break;
-        case 2: launch_elem32<2>(c, a, full); break;
-        case 4: launch_elem32<4>(c, a, full); break;
-        default: throw std::runtime_error("unsupported p");
+    if (L.p >= 2) {
+        // element-major over all elements in mesh order, then the fixed-order gather
+        // (which adds the source of the vertex and side modes)
+        a.order = nullptr; a.begin = 0; a.end = c->ntri;
+        if (L.p == 2) launch_elem32_em<2>(c, a, full);
+        else launch_elem32_em<4>(c, a, full);
+        const LevelDims dd(L.p);
+        GatherArgsT<float> ga{};
+        ga.npnt = c->npnt; ga.nseg = c->nseg; ga.sm = dd.sm; ga.sm0 = c->sm0; ga.bm = dd.bm;
+        ga.vt_ptr = c->vt_ptr; ga.vt_tri = c->vt_tri; ga.vt_loc = c->vt_loc;
+        ga.st_tri = c->st_tri; ga.st_loc = c->st_loc; ga.eres = c->eres32;
+        ga.resv = c->r32v; ga.rrv = c->rr32v; ga.ress = c->r32s; ga.rrs = c->rr32s;
+        ga.keep = (float)f; ga.full = full ? 1 : 0;
+        ga.srcv = c->resv; ga.srcs = c->ress; ga.sfac = c->fd_scal + 1;
+        const long lanes = ((long)c->npnt + (long)c->nseg * dd.sm) * NV;
+        hipLaunchKernelGGL((k_gather_vs<float, NV>), dim3(nblk(lanes)), dim3(256), 0, c->stream, ga);
+    } else {
+        launch_elem32<1>(c, a, full);
     }
     const LevelDims d(L.p);
     EdgeArgsT<float> e;
@@ -975,9 +1051,9 @@ int do_setup(hpg_ctx* c, bool sync = true) {
         case 2: hipLaunchKernelGGL(k_setup_elem<2>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
         default: hipLaunchKernelGGL(k_setup_elem<4>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
     }
-    // learned preconditioner on the finest level: element features -> MLP (or the
-    // caller's log scalings) -> separate scalings of the vertex, side and interior rows
-    c->typed_prcn = c->prcn_mode == HPG_PRCN_LEARNED && (c->mlp || c->user_y) && c->log2p == c->log2pmax;
+    // learned preconditioner on this level: element features -> MLP (or the caller's
+    // log scalings) -> separate scalings of the vertex, side and interior rows
+    c->typed_prcn = c->prcn_mode == HPG_PRCN_LEARNED && (c->mlp || c->user_y);
     if (c->typed_prcn) {
         if (!c->user_y) {
             hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->ntri)), dim3(256), 0, c->stream, c->ntri, c->pnts,
@@ -1304,7 +1380,9 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->mlpY = c->alloc<float>((size_t)nt * MLP_OUT);
         c->jdiag = c->alloc<double>((size_t)nt * 3 * NV);
         if (c->physics == HPG_PHYSICS_INS) {
-            c->eres = c->alloc<double>((size_t)nt * 3 * 2 * NV);
+            // element-major vertex and side modes of the finest level (3p per element)
+            c->eres = c->alloc<double>((size_t)nt * 3 * (1 << c->log2pmax) * 2 * NV);
+            c->eres32 = c->alloc<float>((size_t)nt * 3 * (1 << c->log2pmax) * 2 * NV);
             c->eout = c->alloc<double>((size_t)neseg * 2 * NV);
             c->tprcn_vx = c->alloc<double>((size_t)nt * NV);
             c->tprcn_sd = c->alloc<double>((size_t)nt * NV);
@@ -1320,6 +1398,19 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
             const TriOps& o = *L.ops;
             const std::vector<double> pk = build_pack(o, [](int i, int q) { return 4 * i + q; });
             L.pack = c->upload(pk.data(), pk.size());
+            if (L.p == 1) {
+                std::vector<double> d1(OP1_SIZE);
+                std::copy(o.P.begin(), o.P.end(), d1.begin() + OP1_P);
+                std::copy(o.Pr.begin(), o.Pr.end(), d1.begin() + OP1_PR);
+                std::copy(o.Ps.begin(), o.Ps.end(), d1.begin() + OP1_PS);
+                std::copy(o.Ir.begin(), o.Ir.end(), d1.begin() + OP1_IR);
+                std::copy(o.Is.begin(), o.Is.end(), d1.begin() + OP1_IS);
+                std::copy(o.Dr.begin(), o.Dr.end(), d1.begin() + OP1_DR);
+                std::copy(o.Ds.begin(), o.Ds.end(), d1.begin() + OP1_DS);
+                if (o.P.size() != 12 || o.Ir.size() != 12 || o.Dr.size() != 16)
+                    throw std::runtime_error("p = 1 operator sizes");
+                L.op1 = c->upload(d1.data(), d1.size());
+            }
             const std::vector<double> pk32d = build_pack(o, [](int i, int q) { return f32_row(i, q); });
             const std::vector<float> pk32(pk32d.begin(), pk32d.end());
             L.pack32 = c->upload(pk32.data(), pk32.size());
@@ -1991,6 +2082,14 @@ int hpg_set_preconditioner(hpg_ctx* c, int mode) {
     if (!c || (mode != HPG_PRCN_ANALYTIC && mode != HPG_PRCN_LEARNED)) return -2;
     if (c->physics != HPG_PHYSICS_INS && mode != HPG_PRCN_ANALYTIC) { c->err = "the learned preconditioner is built for ins"; return -2; }
     c->prcn_mode = mode;
+    return 0;
+}
+
+int hpg_set_cfl(hpg_ctx* c, const double* cfl, int n) {
+    if (!c || !cfl || n < 1 || n > 4) return -2;
+    for (int k = 0; k < n; ++k)
+        if (!(cfl[k] >= 0.0) || !std::isfinite(cfl[k])) return -2;
+    for (int k = 0; k < n; ++k) c->prm.cfl[k] = cfl[k];
     return 0;
 }
 

@@ -75,10 +75,13 @@ struct ElemSlots {
 template <int P>
 __device__ __forceinline__ int mode_kind(int k) { return k < 3 ? 0 : (k < Dims<P>::BM ? 1 : 2); }
 
+// element of lane e in 16-element group g; a null order walks the elements in their
+// own (mesh) order, which keeps neighbouring elements in one group
 template <class A>
 __device__ __forceinline__ int elem_of_group(const A& a, int g, int gend, int e) {
     const int idx = a.begin + g * 16 + e;
-    return (g < gend && idx < a.end) ? a.order[idx] : -1;
+    if (!(g < gend && idx < a.end)) return -1;
+    return a.order ? a.order[idx] : idx;
 }
 
 template <class A>
@@ -132,7 +135,8 @@ __device__ __forceinline__ ElemSlots<P> elem_slots(const A& a, const ElemConn& c
 // the dependent gathers overlap the MFMA work of earlier groups.  The grid is a
 // multiple of 8 blocks of 4 waves.
 template <typename T, int P, bool FULL, bool EM = false>
-__device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restrict__ eres = nullptr) {
+__device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restrict__ eres = nullptr,
+                                              int egrid = 0) {
     using D = Dims<P>;
     using V4 = typename Vec4<T>::type;
     constexpr int GT = D::GT, KT = D::KT;
@@ -144,7 +148,8 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
     const int e = lane & 15, q = lane >> 4;
     // XCD-aware split: hardware deals consecutive blocks round-robin to the 8 XCDs;
     // XCD x owns groups [x*gpx, (x+1)*gpx) so neighbouring elements share its L2
-    const int WX = (gridDim.x >> 3) * 4;
+    // egrid: blocks of the launch that walk elements (the rest do other work)
+    const int WX = ((egrid ? egrid : (int)gridDim.x) >> 3) * 4;
     const int rank = (blockIdx.x >> 3) * 4 + w;
     const int xcd = blockIdx.x & 7;
     const int ng = (a.end - a.begin + 15) >> 4;
@@ -170,10 +175,13 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
                 const int off = S.off[s] < 0 ? ~S.off[s] : S.off[s];
                 const T* di = kind == 0 ? a.resv : (kind == 1 ? a.ress : a.resi);
                 const T* dr = kind == 0 ? a.rrv : (kind == 1 ? a.rrs : a.rri);
+                // element-major output (EM) reads back only the interior modes, which
+                // this element owns; the gathers read the carried parts of the others
+                const bool own = !EM || kind == 2;
 #pragma unroll
                 for (int n = 0; n < NV; ++n) {
-                    ro[n][s] = (live && (FULL || a.keep != T(0))) ? dr[off + n] : T(0);
-                    io[n][s] = (live && !((S.first >> s) & 1)) ? di[off + n] : T(0);
+                    ro[n][s] = (live && own && (FULL || a.keep != T(0))) ? dr[off + n] : T(0);
+                    io[n][s] = (live && !EM && !((S.first >> s) & 1)) ? di[off + n] : T(0);
                 }
             }
         };
@@ -349,7 +357,7 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
                     C0[mt][i] = -(l11 * t0 - l01 * t1);
                     C1[mt][i] = -(-l10 * t0 + l00 * t1);
                 }
-            if constexpr (!EM) load_old();
+            load_old();
             // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
 #pragma unroll
             for (int kt = 0; kt < GT; ++kt)
@@ -365,18 +373,44 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
         }
 
 
+        if constexpr (!FULL) load_old();
         if constexpr (EM) {
-            // element-major output: lane (e, q) owns mode q of element e; the vertex
-            // gather adds the carried real part and sums over the elements
-            if (S.t >= 0 && q < D::TM) {
-                T* o = eres + ((size_t)S.t * D::TM + q) * (2 * NV);
+            // element-major output of the vertex and side modes, side sign applied:
+            // eres[(t*BM + k)*2*NV + n] = lf_im, [... + NV + n] = beta lf_re (full
+            // residual only); the gather kernels add the carried real part and sum over
+            // the elements.  Interior modes belong to this element: final here, started
+            // from keep * res_r as on a first visit.
 #pragma unroll
-                for (int n = 0; n < NV; ++n) o[n] = (n == 0 ? LI0 : (n == 1 ? LI1 : LI2))[0];
+            for (int s = 0; s < KT; ++s) {
+                const int k = 4 * s + q, kind = mode_kind<P>(k);
+                if (S.t < 0 || k >= D::TM) continue;
+                const T sg = S.off[s] < 0 ? T(-1) : T(1);
+                const T li[3] = {sg * LI0[s], sg * LI1[s], sg * LI2[s]};
+                T lr[3] = {T(0), T(0), T(0)};
                 if constexpr (FULL) {
                     const T bt = a.beta;
-                    o[NV + 0] = LR0[0] * bt;
-                    o[NV + 1] = LR1[0] * bt;
-                    o[NV + 2] = LR2[0] * bt;
+                    lr[0] = sg * (LR0[s] * bt);
+                    lr[1] = sg * (LR1[s] * bt);
+                    lr[2] = sg * (LR2[s] * bt);
+                }
+                if (kind < 2) {
+                    T* o = eres + ((size_t)S.t * D::BM + k) * (2 * NV);
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) {
+                        o[n] = li[n];
+                        if constexpr (FULL) o[NV + n] = lr[n];
+                    }
+                } else {
+                    const int off = S.off[s];
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) {
+                        const T r0 = a.keep == T(0) ? T(0) : a.keep * ro[n][s];
+                        if (FULL || a.keep != T(1)) a.rri[off + n] = r0 + lr[n];
+                        if (sizeof(T) == 4 && a.srci)
+                            a.resi[off + n] = r0 + lr[n] + li[n] + T(a.srci[off + n] * a.sfac[0]);
+                        else
+                            a.resi[off + n] = r0 + lr[n] + li[n];
+                    }
                 }
             }
             S = Sn;
@@ -384,7 +418,6 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
             tnext2 = tn3;
             continue;
         }
-        if constexpr (!FULL) load_old();
         // lftog: res = sum lf_im + res_r,  res_r = (1-beta) res_r + beta sum lf_re
         // (tri_hp/nstage.cpp:21-82).  The first colour that touches a coefficient
         // starts both sums from keep * res_r; interiors belong to one element.
@@ -434,14 +467,6 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
 template <int P, bool FULL>
 __global__ __launch_bounds__(256) void k_elem_ins(ElemArgs a) {
     elem_ins_body<double, P, FULL>(a);
-}
-
-// p = 1 in element-major form (no colouring): for vertex mode k of element t,
-// eres[(t*3 + k)*2*NV + n] = lf_im, eres[... + NV + n] = beta lf_re (full residual
-// only); k_vstage1 performs the lftog sums in a fixed order per vertex
-template <bool FULL>
-__global__ __launch_bounds__(256) void k_elem_ins1_em(ElemArgs a, double* eres) {
-    elem_ins_body<double, 1, FULL, true>(a, eres);
 }
 
 // FP32 instantiation (the inner solve of the FP32 defect correction): the same
@@ -502,9 +527,8 @@ typedef EdgeArgsT<double> EdgeArgs;
 // EM: element-major output eout[(idx*2 + k)*NV + n] for endpoint k of boundary edge
 // idx (p = 1, summed into the vertices by k_vstage1) instead of the RMW scatter
 template <int P, typename S, bool EM>
-__device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restrict__ eout) {
+__device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restrict__ eout, int idx) {
     constexpr int SM = P - 1, N1 = SM + 2, GPX = P + 1;
-    const int idx = a.begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.end) return;
     const int sind = a.eseg[idx];
     const int v0 = a.seg_pnt[2 * sind], v1 = a.seg_pnt[2 * sind + 1];
@@ -567,11 +591,271 @@ __device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restri
 
 template <int P, typename S = double>
 __global__ __launch_bounds__(64) void k_edge_ins(EdgeArgsT<S> a) {
-    edge_ins_body<P, S, false>(a, nullptr);
+    edge_ins_body<P, S, false>(a, nullptr, a.begin + blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ __launch_bounds__(64) void k_edge_ins1_em(EdgeArgs a, double* eout) {
-    edge_ins_body<1, double, true>(a, eout);
+// Element-major residual (no colouring, mesh order): the element body with its EM
+// output, persistent over all elements; k_gather_vs then sums the vertex and side
+// modes in a fixed order.  At p = 2 the bound of 256 registers keeps two waves per
+// SIMD without spilling (272 otherwise, one wave).
+template <int P, bool FULL>
+__global__ __launch_bounds__(256, P == 2 ? 2 : 1) void k_elem_ins_em(ElemArgs a, double* eres) {
+    elem_ins_body<double, P, FULL, true>(a, eres);
+}
+
+// the FP32 inner solve's element residual in element-major form (as k_elem_ins32)
+template <int P, bool FULL>
+__global__ __launch_bounds__(256, 2) void k_elem_ins32_em(ElemArgs32 a, float* eres) {
+    elem_ins_body<float, P, FULL, true>(a, eres);
+}
+
+// lftog of the element-major vertex and side modes (tri_hp/nstage.cpp:21-82): one lane
+// per (vertex, variable), then one per (side, mode, variable).  Each sum starts from
+// keep * res_r and adds the elements' (beta lf_re, lf_im) in a fixed order (ascending
+// element at a vertex, the side's two elements in ascending order), as the first
+// colour and the later colours of the scatter do.  FP32 (the defect correction's inner
+// solve): the constant source sfac[0] * src enters with the first element, as on the
+// scatter's first visit.
+template <typename T>
+struct GatherArgsT {
+    int npnt, nseg, sm, sm0, bm;
+    const int* vt_ptr;
+    const int* vt_tri;
+    const int* vt_loc;
+    const int* st_tri;
+    const int* st_loc;
+    const T* eres;
+    T* resv;
+    T* rrv;
+    T* ress;
+    T* rrs;
+    T keep;
+    int full;
+    const double* srcv;  // FP32 only, may be null
+    const double* srcs;
+    const double* sfac;
+};
+
+template <typename T, int NVn>
+__device__ __forceinline__ void gather_sum(const GatherArgsT<T>& a, const T* e, int n, T& acc, T& accr) {
+    const T li = e[n];
+    if (a.full) {
+        const T lr = e[NVn + n];
+        accr += lr;
+        acc = (acc + lr) + li;
+    } else {
+        acc = acc + li;
+    }
+}
+
+template <typename T, int NVn>
+__global__ __launch_bounds__(256) void k_gather_vs(GatherArgsT<T> a) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nvl = (long)a.npnt * NVn;
+    const T keep = a.keep;
+    if (g < nvl) {
+        const int v = (int)(g / NVn), n = (int)(g - (long)v * NVn);
+        const T r0 = keep == T(0) ? T(0) : keep * a.rrv[g];
+        T acc = r0, accr = r0;
+        for (int k = a.vt_ptr[v]; k < a.vt_ptr[v + 1]; ++k) {
+            gather_sum<T, NVn>(a, a.eres + ((long)a.vt_tri[k] * a.bm + a.vt_loc[k]) * (2 * NVn), n, acc, accr);
+            if (k == a.vt_ptr[v] && a.srcv) acc = acc + T(a.srcv[g] * a.sfac[0]);
+        }
+        if (a.full || keep != T(1)) a.rrv[g] = accr;
+        a.resv[g] = acc;
+        return;
+    }
+    const long h = g - nvl;
+    const long per = (long)a.sm * NVn;
+    if (h >= (long)a.nseg * per) return;
+    const int s = (int)(h / per);
+    const int r = (int)(h - (long)s * per);
+    const int m = r / NVn, n = r - m * NVn;
+    const long o = ((long)s * a.sm0 + m) * NVn + n;
+    const T r0 = keep == T(0) ? T(0) : keep * a.rrs[o];
+    T acc = r0, accr = r0;
+    bool first = true;
+    for (int slot = 0; slot < 2; ++slot) {
+        const int t = a.st_tri[2 * s + slot];
+        if (t < 0) continue;
+        const int mode = 3 + a.st_loc[2 * s + slot] * a.sm + m;
+        gather_sum<T, NVn>(a, a.eres + ((long)t * a.bm + mode) * (2 * NVn), n, acc, accr);
+        if (first && a.srcs) acc = acc + T(a.srcs[o] * a.sfac[0]);
+        first = false;
+    }
+    if (a.full || keep != T(1)) a.rrs[o] = accr;
+    a.ress[o] = acc;
+}
+
+// p = 1 element residual on the vector ALU, one thread per element (mesh order), the
+// element-major output of k_elem_ins_em (at p = 1, eres[(t*3 + k)*2*NV + n]).  At p = 1
+// the 16 x 16 x 4 MFMA tiles are
+// mostly padding (4 Gauss points, 3 modes: 80 matrix instructions per 16 elements for
+// about 1.2 kflop of work per element), so the residual is written out as scalar
+// loops over the dense operators (uniform across the wave: scalar loads).  The same
+// sequence as elem_ins_body: proj, inviscid fluxes, intgrtrs (tri_hp/ins/rsdl.cpp:
+// 262-294); derivatives, viscous tensor, strong residual, tau and stabilisation,
+// intgrtrs of the real part (:296-426).
+// Dense p = 1 operators: P, Pr, Ps [4][3] | Ir, Is [3][4] | Dr, Ds [4][4]
+constexpr int OP1_P = 0, OP1_PR = 12, OP1_PS = 24, OP1_IR = 36, OP1_IS = 48, OP1_DR = 60, OP1_DS = 76, OP1_SIZE = 92;
+
+template <bool FULL>
+__global__ __launch_bounds__(256) void k_elem_ins1_valu(ElemArgs a, const double* __restrict__ op,
+                                                        double* __restrict__ eres, EdgeArgs e,
+                                                        double* __restrict__ eout, int egrid) {
+    if ((int)blockIdx.x >= egrid) {
+        edge_ins_body<1, double, true>(e, eout, e.begin + ((int)blockIdx.x - egrid) * (int)blockDim.x + (int)threadIdx.x);
+        return;
+    }
+    const int t = a.begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= a.end) return;
+    constexpr int G = 4, M = 3;
+    int vtx[3];
+    double x[3], y[3], b[NV][M];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vtx[k] = a.tri_pnt[3 * t + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = a.pnts[2 * vtx[k]];
+        y[k] = a.pnts[2 * vtx[k] + 1];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) b[n][k] = a.ugv[vtx[k] * NV + n];
+    }
+    const double l00 = 0.5 * (x[2] - x[1]), l01 = 0.5 * (x[0] - x[1]);
+    const double l10 = 0.5 * (y[2] - y[1]), l11 = 0.5 * (y[0] - y[1]);
+    double U[G], V[G], Pp[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        double u = 0.0, v = 0.0, pr = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const double w = op[OP1_P + g * M + m];
+            u += w * b[0][m]; v += w * b[1][m]; pr += w * b[2][m];
+        }
+        U[g] = u; V[g] = v; Pp[g] = pr;
+    }
+    const double rho = a.rho;
+    double C0[G], C1[G], A00[G], A01[G], A10[G], A11[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const double fx = rho * U[g], fy = rho * V[g];
+        const double c0 = +l11 * fx - l01 * fy;
+        const double c1 = -l10 * fx + l00 * fy;
+        C0[g] = c0; C1[g] = c1;
+        A00[g] = U[g] * c0 + l11 * Pp[g];
+        A01[g] = U[g] * c1 - l10 * Pp[g];
+        A10[g] = V[g] * c0 - l01 * Pp[g];
+        A11[g] = V[g] * c1 + l00 * Pp[g];
+    }
+    double LI[NV][M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const double ar = op[OP1_IR + m * G + g], as = op[OP1_IS + m * G + g];
+            s0 += ar * A00[g] + as * A01[g];
+            s1 += ar * A10[g] + as * A11[g];
+            s2 += ar * C0[g] + as * C1[g];
+        }
+        LI[0][m] = s0; LI[1][m] = s1; LI[2][m] = s2;
+    }
+    double* o = eres + (size_t)t * M * (2 * NV);
+    if constexpr (FULL) {
+        double Ur[G], Us[G], Vr[G], Vs[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            double c1 = 0.0, c2 = 0.0, c3 = 0.0, c4 = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double ar = op[OP1_PR + g * M + m], as = op[OP1_PS + g * M + m];
+                c1 += ar * b[0][m]; c2 += as * b[0][m];
+                c3 += ar * b[1][m]; c4 += as * b[1][m];
+            }
+            Ur[g] = c1; Us[g] = c2; Vr[g] = c3; Vs[g] = c4;
+        }
+        const double cjcb = l00 * l11 - l10 * l01;
+        const double cjcbi = a.mu / cjcb;
+        const double v0000 = -cjcbi * (2.0 * l11 * l11 + l01 * l01);
+        const double v0011 = -cjcbi * (2.0 * l10 * l10 + l00 * l00);
+        const double v0001 = cjcbi * (2.0 * l11 * l10 + l01 * l00);
+        const double v1100 = -cjcbi * (l11 * l11 + 2.0 * l01 * l01);
+        const double v1111 = -cjcbi * (l10 * l10 + 2.0 * l00 * l00);
+        const double v1101 = cjcbi * (l11 * l10 + 2.0 * l01 * l00);
+        const double v0100 = cjcbi * l01 * l11;
+        const double v0111 = cjcbi * l00 * l10;
+        const double v0101 = -cjcbi * l01 * l10;
+        const double v0110 = -cjcbi * l00 * l11;
+        double F00[G], F01[G], F10[G], F11[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const double ur = Ur[g], us = Us[g], vr = Vr[g], vs = Vs[g];
+            F00[g] = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
+            F01[g] = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
+            F10[g] = v0100 * ur + v1100 * vr + v0110 * us + v1101 * vs;
+            F11[g] = v0101 * ur + v1101 * vr + v0111 * us + v1111 * vs;
+            A00[g] += F00[g]; A01[g] += F01[g]; A10[g] += F10[g]; A11[g] += F11[g];
+        }
+        double R0[G], R1[G], R2[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+#pragma unroll
+            for (int h = 0; h < G; ++h) {
+                const double dr = op[OP1_DR + g * G + h], ds = op[OP1_DS + g * G + h];
+                r0 += dr * A00[h] + ds * A01[h];
+                r1 += dr * A10[h] + ds * A11[h];
+                r2 += dr * C0[h] + ds * C1[h];
+            }
+            R0[g] = r0; R1[g] = r1; R2[g] = r2;
+        }
+        const double dx1 = x[0] - x[2], dy1 = y[0] - y[2], dx2 = x[1] - x[0], dy2 = y[1] - y[0];
+        const double ar2 = dx1 * dy2 - dy1 * dx2;
+        const double perim = sqrt(dx1 * dx1 + dy1 * dy1) + sqrt(dx2 * dx2 + dy2 * dy2) +
+                             sqrt((dx1 + dx2) * (dx1 + dx2) + (dy1 + dy2) * (dy1 + dy2));
+        const double h = (ar2 / perim) / (0.25 * 2 * 2);
+        const double nu = a.mu / rho;
+        const double gv = 2.0 * nu / h;
+        const double gvis = gv * gv;
+        const double tfac = a.adis * h / cjcb;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const double u = U[g], v = V[g];
+            const double qq = u * u + v * v;
+            double gam = 3.0 * qq + gvis;
+            if (a.mu == 0.0) gam = fmax(gam, 0.1);
+            const double tau0 = tfac / sqrt(gam);
+            const double tau2 = sqrt(qq) * tau0;
+            const double t0 = tau0 * R0[g], t1 = tau0 * R1[g], t2 = tau2 * R2[g];
+            F00[g] -= (l11 * (2.0 * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
+            F01[g] -= (-l10 * (2.0 * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
+            F10[g] -= +l11 * v * t0 + (l11 * u - l01 * (2.0 * v)) * t1 - l01 * t2;
+            F11[g] -= -l10 * v * t0 + (-l10 * u + l00 * (2.0 * v)) * t1 + l00 * t2;
+            C0[g] = -(l11 * t0 - l01 * t1);
+            C1[g] = -(-l10 * t0 + l00 * t1);
+        }
+        const double bt = a.beta;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const double ar = op[OP1_IR + m * G + g], as = op[OP1_IS + m * G + g];
+                s0 += ar * F00[g] + as * F01[g];
+                s1 += ar * F10[g] + as * F11[g];
+                s2 += ar * C0[g] + as * C1[g];
+            }
+            double* om = o + m * (2 * NV);
+            om[0] = LI[0][m]; om[1] = LI[1][m]; om[2] = LI[2][m];
+            om[NV + 0] = s0 * bt; om[NV + 1] = s1 * bt; om[NV + 2] = s2 * bt;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double* om = o + m * (2 * NV);
+            om[0] = LI[0][m]; om[1] = LI[1][m]; om[2] = LI[2][m];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------

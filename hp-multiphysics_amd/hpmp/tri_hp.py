@@ -194,6 +194,11 @@ class TriHpIns:
         if ninner is not None:
             self._check(self.L.hpg_set_defect_inner(self.h, int(ninner)), "set_defect_inner")
 
+    def set_cfl(self, cfl):
+        """Pseudo-time CFL per p-level (the reference's per-level ``cfl`` input)."""
+        a = np.ascontiguousarray(cfl, dtype=np.float64)
+        self._check(self.L.hpg_set_cfl(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), int(a.size)), "set_cfl")
+
     def set_preconditioner_mode(self, mode: int):
         """HPG_PRCN_ANALYTIC or HPG_PRCN_LEARNED (needs an MLP attached, hpmp.learned)."""
         self._check(self.L.hpg_set_preconditioner(self.h, mode), "set_preconditioner")

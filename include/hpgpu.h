@@ -150,6 +150,11 @@ int hpg_get_log2p(hpg_ctx *ctx, int *log2p);
 int hpg_set_precision(hpg_ctx *ctx, int mode);
 int hpg_set_preconditioner(hpg_ctx *ctx, int mode);
 int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
+/* Replaces the pseudo-time CFL numbers of the first n p-levels (cfl[log2p], n <= 4):
+ * the per-level "cfl" list of the reference's input map (tri_hp/allocate.cpp:275-276),
+ * changed between cycles instead of at construction.  Returns -2 for n outside 1..4 or
+ * a CFL that is negative or not finite. */
+int hpg_set_cfl(hpg_ctx *ctx, const double *cfl, int n);
 /* J(u) delta at the current state through the FP32 linearised kernels (finest
  * level; overwrites the residual arrays with R(u)).  No reference counterpart;
  * the directional derivative of tri_hp::rsdl(nstage). */
@@ -179,10 +184,12 @@ int hpg_element_jacobian(hpg_ctx *ctx, const int *tind, int nel, double eps_r, d
  * y[type][var] of the analytic element scaling tprcn of
  * tri_hp_ins::setup_preconditioner (tri_hp/ins/setup_preconditioner.cpp:202-219) for
  * the vertex (type 0), side (1) and interior (2) rows of the diagonal preconditioner;
- * with HPG_PRCN_LEARNED and an MLP attached, hpg_setup_preconditioner on the finest
- * level assembles vprcn from tprcn * exp(clamp(y[0], -2, 2)), sprcn and the side
- * couplings of minvrt from the type-1 scaling and the interior solve from the type-2
- * scaling (coarse p-levels stay analytic).  Training data come from the method itself:
+ * with HPG_PRCN_LEARNED and an MLP attached, hpg_setup_preconditioner on every p-level
+ * of the context assembles vprcn from tprcn * exp(clamp(y[0], -2, 2)), sprcn and the
+ * side couplings of minvrt from the type-1 scaling and the interior solve from the
+ * type-2 scaling (the features carry the level through the analytic pseudo-time step;
+ * each coarse-mesh context of a multigrid stack takes the MLP and the mode on its own).
+ * Training data come from the method itself:
  * the features and the measured element-Jacobian diagonal per mode type (sum over the
  * type's modes of |J_e(mn, mn)| per variable n).  The MLP must not be trained while a
  * context that uses it runs. */

@@ -51,6 +51,7 @@ def _guard(s):
     g = GuardedSolver.__new__(GuardedSolver)
     g.s, g.mlp, g.check_every, g.grow_tol, g.window, g.learned, g.fell_back_at = s, object(), 10, 10.0, 4, True, None
     g.grids = None
+    g.cfl_scale, g.cfl0 = None, [2.5, 1.5, 1.0, 0.5]
     s.set_preconditioner_mode(_lib.HPG_PRCN_LEARNED)
     return g
 
@@ -104,3 +105,39 @@ def test_exception_in_learned_cycle_falls_back():
     assert g.fell_back_at == 15 and hist[-1][1] <= 1e-8
     # the best check was at cycle 10: state 0.8^10
     assert abs(hist[-1][1] - 0.8 ** 10 * 0.9 ** (it - 15)) < 1e-12
+
+
+class ScriptedCflSolver(ScriptedSolver):
+    """Residual follows rate_learned per cycle while the CFL differs from the reference
+    one, rate_analytic otherwise."""
+
+    def __init__(self, rate_tuned, rate_analytic=0.9):
+        super().__init__(rate_tuned, rate_analytic)
+        self.cfl = [2.5, 1.5, 1.0, 0.5]
+        self.cfls = []
+
+    def set_cfl(self, cfl):
+        self.cfl = list(cfl)
+        self.cfls.append(self.cfl)
+
+    def cycle(self, **kw):
+        self.ncycle += 1
+        self.x *= self.rl if self.cfl[0] != 2.5 else self.ra
+
+
+def test_tuned_cfl_guard_restores_the_reference_cfl():
+    from hpmp.tri_hp import CFL_DEFAULT
+    s = ScriptedCflSolver(1.5)                      # the tuned CFL diverges
+    g = GuardedSolver(s, None, check_every=10, cfl_scale=2.5)
+    assert s.cfl == [c * 2.5 for c in CFL_DEFAULT]
+    it, hist = g.solve(tol_rel=1e-6, max_cycles=2000)
+    assert g.fell_back_at is not None
+    assert s.cfl == list(CFL_DEFAULT)
+    assert hist[-1][1] <= 1e-6 * hist[0][1]
+
+
+def test_tuned_cfl_guard_keeps_a_contracting_cfl():
+    s = ScriptedCflSolver(0.5)
+    g = GuardedSolver(s, None, check_every=10, cfl_scale=2.0)
+    it, hist = g.solve(tol_rel=1e-6, max_cycles=2000)
+    assert g.fell_back_at is None and it <= 30
