@@ -56,10 +56,10 @@ MI355X_FP64_MFMA_TFLOPS = 78.6
 # of the reference x 2 + pointwise): full residual (beta > 0) and inviscid-only stage
 FLOP_FULL_P4 = 19250.0
 FLOP_INV_P4 = 7250.0
-# HBM bytes per launch of k_elem_ins<4, full> on this workload (7 colours, 74 898
-# elements per launch on average): FETCH_SIZE and WRITE_SIZE counter passes scaled by
+# HBM bytes per launch of k_elem_ins_em<4, full> on this workload (one launch over all
+# 524 288 elements per full residual): FETCH_SIZE and WRITE_SIZE counter passes scaled by
 # the same counters over a plain 1 GiB copy; profiles/traffic_ins_p4_512x512_pmc.md
-TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 218.0e6
+TRAFFIC_FULL_P4_BYTES_PER_LAUNCH = 545.3e6
 
 
 # tri_hp cns p = 4 full residual, algorithmic flops per element: transforms 30 G T +
@@ -372,6 +372,33 @@ def time_to_tol(args, sols) -> dict:
     return out
 
 
+def cycles_only(args) -> int:
+    """Exactly `--cycles-only` W(2,2,2) cycles of the mu = 1e-3 workload (FP64 plain, reference
+    CFL, hierarchy down to 16 x 16), one JSON line with the time per cycle: the run the
+    W-cycle kernel profile is taken from."""
+    from hpmp import multigrid as MG
+    n = args.tts_n
+    ngrid = args.tts_ngrid or max(1, int(np.log2(n // 16)) + 1)
+    mesh, st, g, grids = _tts_problem(n, 1e-3, ngrid)
+    g.set_state(*st)
+    MG.cycle(grids, g.log2pmax, vw=2, itercrsn=2, iterrfne=2)
+    g.set_state(*st)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.cycles_only):
+        MG.cycle(grids, g.log2pmax, vw=2, itercrsn=2, iterrfne=2)
+        print(f"# cycle {k + 1}", file=sys.stderr, flush=True)
+    g.synchronize()
+    t1 = time.perf_counter()
+    g.rsdl()
+    print(json.dumps({"cycles": args.cycles_only, "ms_per_cycle": 1e3 * (t1 - t0) / args.cycles_only,
+                      "judge": g.resnorm(), "ngrid": ngrid}), flush=True)
+    for x in grids[:0:-1]:
+        x.close()
+    g.close()
+    return 0
+
+
 def host_cpu() -> tuple[str, str]:
     """(model name, flags) of this host's CPU from /proc/cpuinfo."""
     model, flags = "unknown", ""
@@ -451,6 +478,24 @@ def cpu_baseline(args, native, jac, sols=None, tts=None) -> dict:
         k1 = time.perf_counter()
         jac["cpu_elements_per_s"] = jb.size / (k1 - k0)
         jac["cpu_sample"] = f"oracle, 1 thread, {jb.size} elements, {k1 - k0:.2f} s"
+    # workload A (32 x 32 squares, no jitter, p = 2, mu = 1e-2, N(0, 2^-k) modes on a u = 1
+    # free stream): one rsdl and 100 setup_preconditioner + update cycles, one thread
+    oracle.set_threads(1)
+    am = M.unit_square(32, 0.0, 0)
+    ast_ = W.noisy_freestream(am, 2, seed=2024)
+    o = oracle.TriHpIns(am, log2p=1, rho=1.0, mu=1e-2)
+    o.set_state(*ast_)
+    a0 = time.perf_counter()
+    o.rsdl()
+    a1 = time.perf_counter()
+    for _ in range(100):
+        o.setup_preconditioner()
+        o.update()
+    a2 = time.perf_counter()
+    aunk = 3 * (am.npnt + am.nseg + 0 * am.ntri)
+    workload_a = {"rsdl_seconds": a1 - a0, "rsdl_dof_evals_per_s": aunk / (a1 - a0),
+                  "cycles100_seconds": a2 - a1, "final_maxres": float(o.maxres()[0]), "cores": 1}
+    del o
     if sols and tts:
         oracle.set_threads(threads)
         for key, sol in sols.items():
@@ -484,6 +529,7 @@ def cpu_baseline(args, native, jac, sols=None, tts=None) -> dict:
         "nproc": os.cpu_count(),
         "march": march,
         "single_thread": {"value": rates[1][0], "cores": 1, "seconds": rates[1][1]},
+        "workload_a": workload_a,
     }
 
 
@@ -514,12 +560,16 @@ def main() -> int:
     ap.add_argument("--tet-steps", type=int, default=3)
     ap.add_argument("--no-jac", action="store_true")
     ap.add_argument("--jac-batch", type=int, default=32768)
+    ap.add_argument("--cycles-only", type=int, default=0,
+                    help="only run this many W cycles of the mu = 1e-3 time-to-tolerance workload (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
         return 2
+    if args.cycles_only:
+        return cycles_only(args)
     if args.copy_probe:
         from hpmp import _lib
         ms = C.c_double()
@@ -650,7 +700,7 @@ def main() -> int:
         "dof_evals_per_s_without_nv": value / 3,
         "roofline": {
             "bound": "mfma",
-            "kernel": "k_elem_ins<4,full>",
+            "kernel": "k_elem_ins_em<4,full>",
             "achieved": achieved,
             "peak": MI355X_FP64_MFMA_TFLOPS,
             "unit": "TFLOP/s",
