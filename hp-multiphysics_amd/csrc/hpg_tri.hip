@@ -229,6 +229,12 @@ struct hpg_ctx {
     bool typed_prcn = false;
     bool user_y = false;  // mlpY holds log scalings set by the caller (hpg_set_prcn_log_scalings)
     unsigned long long* d_max;
+    // hpg_mg_cycle as a captured HIP graph (on the fine context of a stack): replayed
+    // while the signature of the settings that fix the enqueued work is unchanged
+    bool graphs = true;
+    std::vector<unsigned long long> cg_sig;
+    bool cg_seen = false;  // the signature was run eagerly once; capture at the next call
+    hipGraphExec_t cg_exec = nullptr;
     Level lv[3];
     std::string err;
     // optional per-launch timing of the element-residual kernel (HIP events on `stream`)
@@ -264,6 +270,7 @@ struct hpg_ctx {
     }
     ~hpg_ctx() {
         if (stream) hipStreamSynchronize(stream);
+        if (cg_exec) hipGraphExecDestroy(cg_exec);
         for (void* p : allocs) hipFree(p);
         for (hipEvent_t e : events) hipEventDestroy(e);
         if (xev) hipEventDestroy(xev);
@@ -288,6 +295,15 @@ static inline unsigned nblk(long n, int bs = 256) { return unsigned((n + bs - 1)
 static inline unsigned nblk8(long n, int bs = 256) { return ((nblk(n, bs) + 7u) / 8u) * 8u; }
 
 namespace {
+
+// device-to-device copy of `bytes` (a multiple of 4) as a kernel launch, so that a
+// captured cycle holds kernel nodes only
+void dcopy(hpg_ctx* c, void* dst, const void* src, size_t bytes) {
+    const long n = (long)(bytes / 4);
+    if (n <= 0) return;
+    const unsigned nb = std::min<unsigned>(nblk(n), 4096u);
+    hipLaunchKernelGGL(k_copy_words, dim3(nb), dim3(256), 0, c->stream, n, (const unsigned*)src, (unsigned*)dst);
+}
 
 void launch_modes(hpg_ctx* c, int op, long nent, int nm, int mstride, double* x, double* y, const double* z, double k) {
     if (nent <= 0 || nm <= 0) return;
@@ -982,9 +998,9 @@ void cns_boundary_and_projection(hpg_ctx* c) {
 void update_cns(hpg_ctx* c) {
     const size_t nv = (size_t)c->npnt * c->nv * 8, ns = (size_t)c->nseg * c->sm0 * c->nv * 8,
                  ni = (size_t)c->ntri * c->im0 * c->nv * 8;
-    check(hipMemcpyAsync(c->ug0v, c->ugv, nv, hipMemcpyDeviceToDevice, c->stream), "D2D");
-    if (ns) check(hipMemcpyAsync(c->ug0s, c->ugs, ns, hipMemcpyDeviceToDevice, c->stream), "D2D");
-    if (ni) check(hipMemcpyAsync(c->ug0i, c->ugi, ni, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    dcopy(c, c->ug0v, c->ugv, nv);
+    if (ns) dcopy(c, c->ug0s, c->ugs, ns);
+    if (ni) dcopy(c, c->ug0i, c->ugi, ni);
     for (int stage = 0; stage < c->prm.nstage; ++stage) {
         do_rsdl_cns(c, stage);
         do_minvrt(c);
@@ -1004,7 +1020,7 @@ void mixed_update_cns(hpg_ctx* c) {
     const long nv = (long)c->npnt * c->nv, ns = (long)c->nseg * c->sm0 * c->nv, ni = (long)c->ntri * c->im0 * c->nv;
     do_rsdl_cns(c, c->prm.nstage);
     auto copy = [&](double* dst, const double* src, long n) {
-        if (n) check(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        if (n) dcopy(c, dst, src, n * 8);
     };
     copy(c->srcv, c->resv, nv); copy(c->srcs, c->ress, ns); copy(c->srci, c->resi, ni);
     copy(c->u0cv, c->ugv, nv); copy(c->u0cs, c->ugs, ns); copy(c->u0ci, c->ugi, ni);
@@ -1568,9 +1584,9 @@ void mixed_update(hpg_ctx* c) {
     hipLaunchKernelGGL(k_fd_scale, dim3(1), dim3(1), 0, c->stream, c->fd_scal, c->fd_max, c->fd_reset ? 1 : 0, FD_H);
     c->fd_reset = false;
     for (int it = 0; it < c->ninner; ++it) {
-        check(hipMemcpyAsync(c->d0v, c->dlv, nv * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
-        if (nsz) check(hipMemcpyAsync(c->d0s, c->dls, nsz * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
-        if (ni) check(hipMemcpyAsync(c->d0i, c->dli, ni * 4, hipMemcpyDeviceToDevice, c->stream), "D2D");
+        dcopy(c, c->d0v, c->dlv, nv * 4);
+        if (nsz) dcopy(c, c->d0s, c->dls, nsz * 4);
+        if (ni) dcopy(c, c->d0i, c->dli, ni * 4);
         for (int stage = 0; stage < c->prm.nstage; ++stage) {
             if (it == 0 && stage == 0) {
                 hipLaunchKernelGGL(k_scale_f32, dim3(nblk(nv)), dim3(256), 0, c->stream, nv, c->resv, c->fd_scal, c->r32v);
@@ -1816,9 +1832,9 @@ void do_update(hpg_ctx* c) {
     }
     const size_t nv = (size_t)c->npnt * NV * 8, ns = (size_t)c->nseg * c->sm0 * NV * 8,
                  ni = (size_t)c->ntri * c->im0 * NV * 8;
-    check(hipMemcpyAsync(c->ug0v, c->ugv, nv, hipMemcpyDeviceToDevice, c->stream), "D2D");
-    if (ns) check(hipMemcpyAsync(c->ug0s, c->ugs, ns, hipMemcpyDeviceToDevice, c->stream), "D2D");
-    if (ni) check(hipMemcpyAsync(c->ug0i, c->ugi, ni, hipMemcpyDeviceToDevice, c->stream), "D2D");
+    dcopy(c, c->ug0v, c->ugv, nv);
+    if (ns) dcopy(c, c->ug0s, c->ugs, ns);
+    if (ni) dcopy(c, c->ug0i, c->ugi, ni);
     for (int stage = 0; stage < c->prm.nstage; ++stage) {
         do_rsdl(c, stage);
         do_minvrt(c, c->prm.alpha[stage] * c->prm.cfl[c->log2p]);
@@ -2019,9 +2035,40 @@ void cycle_level(const CycleRun& r, int lvl) {
         for (int k = 0; k < r.iterrfne; ++k) do_update(g);
     }
 }
+
+// the host-side settings that fix what a cycle enqueues (kernels and their arguments)
+std::vector<unsigned long long> cycle_signature(hpg_ctx* const* grids, int ngrid, int extra, int vw, int ic, int ir) {
+    std::vector<unsigned long long> sg = {(unsigned long long)ngrid, (unsigned long long)extra,
+                                          (unsigned long long)vw, (unsigned long long)ic, (unsigned long long)ir};
+    for (int k = 0; k < ngrid; ++k) {
+        const hpg_ctx* g = grids[k];
+        sg.push_back((unsigned long long)(uintptr_t)g);
+        sg.push_back((unsigned long long)(uintptr_t)g->mlp);
+        sg.push_back(((unsigned long long)g->precision << 40) | ((unsigned long long)g->ninner << 24) |
+                     ((unsigned long long)g->prcn_mode << 16) | ((unsigned long long)g->user_y << 12) |
+                     ((unsigned long long)g->fd_reset << 8) | ((unsigned long long)g->timing << 4) |
+                     (unsigned long long)g->log2p);
+        for (int l = 0; l < 4; ++l) {
+            unsigned long long b;
+            std::memcpy(&b, &g->prm.cfl[l], sizeof(b));
+            sg.push_back(b);
+        }
+    }
+    return sg;
+}
+
+// clears the error flags of up to 16 contexts, from the fine context's stream
+struct FlagList { unsigned* p[16]; int n; };
+__global__ void k_clear_flags(FlagList f) {
+    const int k = threadIdx.x;
+    if (k < f.n) f.p[k][0] = 0u;
+}
 }  // namespace
 
-// block::cycle over a stack of grids, enqueued without host round trips inside the cycle
+// block::cycle over a stack of grids, enqueued without host round trips inside the cycle.
+// The second call with the same settings captures the cycle into a HIP graph (the
+// coarse contexts' streams join through the restriction and prolongation events);
+// later calls with the same settings replay it.
 int hpg_mg_cycle(hpg_ctx* const* grids, int ngrid, int extra_finest_levels, int vw, int itercrsn, int iterrfne,
                  double* maxres) {
     if (!grids || ngrid < 1 || extra_finest_levels < 0 || vw < 1 || itercrsn < 0 || iterrfne < 0) return -2;
@@ -2029,12 +2076,47 @@ int hpg_mg_cycle(hpg_ctx* const* grids, int ngrid, int extra_finest_levels, int 
         if (!grids[k] || (k > 0 && (!grids[k]->coarse_level || grids[k]->fine != grids[k - 1]))) return -2;
     hpg_ctx* f = grids[0];
     if (f->log2p != f->log2pmax || extra_finest_levels > f->log2pmax) { f->err = "cycle starts on the finest p-level"; return -2; }
+    if (ngrid > 16) { f->err = "at most 16 meshes in a cycle"; return -2; }
     int nerr = 0;
     HPG_TRY(f, {
-        for (int k = 0; k < ngrid; ++k)
-            hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, grids[k]->stream, 1, (unsigned*)grids[k]->d_err);
+        FlagList fl{};
+        fl.n = ngrid;
+        for (int k = 0; k < ngrid; ++k) fl.p[k] = (unsigned*)grids[k]->d_err;
         CycleRun r{grids, ngrid, extra_finest_levels, ngrid + extra_finest_levels, vw, itercrsn, iterrfne};
-        cycle_level(r, 0);
+        bool use_graph = f->graphs;
+        for (int k = 0; k < ngrid; ++k) use_graph = use_graph && !grids[k]->timing;
+        const std::vector<unsigned long long> sg =
+            use_graph ? cycle_signature(grids, ngrid, extra_finest_levels, vw, itercrsn, iterrfne)
+                      : std::vector<unsigned long long>();
+        if (use_graph && f->cg_exec && sg == f->cg_sig) {
+            check(hipGraphLaunch(f->cg_exec, f->stream), "hipGraphLaunch");
+        } else if (use_graph && f->cg_seen && sg == f->cg_sig) {
+            // second call with these settings: capture the cycle, then run the graph
+            if (f->cg_exec) { check(hipGraphExecDestroy(f->cg_exec), "hipGraphExecDestroy"); f->cg_exec = nullptr; }
+            hipGraph_t graph = nullptr;
+            check(hipStreamBeginCapture(f->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+            try {
+                hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, f->stream, fl);
+                cycle_level(r, 0);
+            } catch (...) {
+                hipStreamEndCapture(f->stream, &graph);
+                if (graph) hipGraphDestroy(graph);
+                throw;
+            }
+            check(hipStreamEndCapture(f->stream, &graph), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&f->cg_exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            check(e, "hipGraphInstantiate");
+            check(hipGraphLaunch(f->cg_exec, f->stream), "hipGraphLaunch");
+        } else {
+            hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, f->stream, fl);
+            cycle_level(r, 0);
+            if (use_graph) {
+                if (f->cg_exec && sg != f->cg_sig) { check(hipGraphExecDestroy(f->cg_exec), "hipGraphExecDestroy"); f->cg_exec = nullptr; }
+                f->cg_sig = sg;
+                f->cg_seen = true;
+            }
+        }
         check(hipGetLastError(), "cycle launch");
         unsigned long long mx[NVC] = {0, 0, 0, 0};
         check(hipMemcpyAsync(mx, f->d_max, sizeof(mx), hipMemcpyDeviceToHost, f->stream), "D2H");
@@ -2053,6 +2135,13 @@ int hpg_mg_cycle(hpg_ctx* const* grids, int ngrid, int extra_finest_levels, int 
         if (maxres) *maxres = m;
     });
     return nerr;
+}
+
+int hpg_set_cycle_graph(hpg_ctx* c, int enable) {
+    if (!c) return -2;
+    c->graphs = enable != 0;
+    if (!c->graphs) c->cg_seen = false;
+    return 0;
 }
 
 int hpg_get_log2p(hpg_ctx* c, int* l) {

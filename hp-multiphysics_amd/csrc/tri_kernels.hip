@@ -1445,6 +1445,11 @@ template __global__ void k_elem_ins32<4, true>(ElemArgs32);
 template __global__ void k_elem_ins32<4, false>(ElemArgs32);
 
 
+// grid-stride copy of n 4-byte words (device-to-device copies inside the solver)
+__global__ void k_copy_words(long n, const unsigned* __restrict__ src, unsigned* __restrict__ dst) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) dst[k] = src[k];
+}
+
 // grid-stride 16-byte copy (reference kernel for traffic counters)
 __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n16) {
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (long long)gridDim.x * blockDim.x)

@@ -194,6 +194,10 @@ class TriHpIns:
         if ninner is not None:
             self._check(self.L.hpg_set_defect_inner(self.h, int(ninner)), "set_defect_inner")
 
+    def set_cycle_graph(self, enable: bool):
+        """Graph capture of mg_cycle on this (fine) context: on by default."""
+        self._check(self.L.hpg_set_cycle_graph(self.h, int(bool(enable))), "set_cycle_graph")
+
     def set_cfl(self, cfl):
         """Pseudo-time CFL per p-level (the reference's per-level ``cfl`` input)."""
         a = np.ascontiguousarray(cfl, dtype=np.float64)

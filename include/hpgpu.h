@@ -122,9 +122,16 @@ int hpg_link_fine(hpg_ctx *coarse, hpg_ctx *fine, const int *cc_tri, const doubl
  * setup_preconditioner error counts accumulate on the device and the return value is
  * the number of grids that reported an error during the cycle (0: none; the reference
  * stops at the first, this form finishes the cycle); maxres (may be NULL) receives the
- * entry level's maxres after its first smoothing, as block::cycle reads it. */
+ * entry level's maxres after its first smoothing, as block::cycle reads it.  At most 16
+ * grids.  The second call with unchanged settings (the stack, the cycle shape, and per
+ * grid the precision, preconditioner mode and MLP, the CFL list and the p-level)
+ * captures the cycle into a HIP graph on grids[0]'s stream; later such calls replay it,
+ * with the same kernels and arguments as the enqueued form. */
 int hpg_mg_cycle(hpg_ctx *const *grids, int ngrid, int extra_finest_levels, int vw, int itercrsn, int iterrfne,
                  double *maxres);
+/* No reference counterpart: enable (default) or disable the graph capture of
+ * hpg_mg_cycle on this fine context (disabled: every cycle is enqueued call by call). */
+int hpg_set_cycle_graph(hpg_ctx *ctx, int enable);
 /* replaces tri_hp::maxres (tri_hp/allocate.cpp:496-525); per_var may be NULL */
 int hpg_maxres(hpg_ctx *ctx, double *per_var, double *out);
 /* raw-residual judge: max |res| over every vertex, side and interior coefficient

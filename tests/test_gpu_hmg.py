@@ -134,3 +134,45 @@ def test_fused_cycle_is_the_call_by_call_cycle(oracle_mod, physics, n, log2p, ng
     err = rel_err(ga[0].get_state(), og[0].get_state())
     print("state rel err vs oracle", err, flush=True)
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("physics,form", [("ins", "fp64"), ("ins", "mixed"), ("ins", "learned"), ("cns", "fp64")])
+def test_cycle_graph_replays_the_enqueued_cycle(oracle_mod, physics, form):
+    """hpg_mg_cycle captures the cycle into a HIP graph at the second call with unchanged
+    settings and replays it afterwards: every replay must give the state and maxres of
+    the enqueued cycle bitwise, with eager work (a residual evaluation and a host read)
+    between the replays; a changed setting (the CFL) must not replay the old graph."""
+    _case(f"cycle graph replay vs enqueued cycle {physics} {form}")
+    from hpmp import _lib, multigrid as MG
+    n, log2p, ngrid = (16, 2, 3) if physics == "ins" else (8, 1, 2)
+    vw, ic, ir = (2, 2, 2) if physics == "ins" else (1, 1, 1)
+    _, ga, _ = _stacks(oracle_mod, physics, n, log2p, ngrid)
+    _, gb, _ = _stacks(oracle_mod, physics, n, log2p, ngrid)
+    ga[0].set_cycle_graph(False)
+    for g in (ga, gb):
+        if form == "mixed":
+            g[0].set_precision(_lib.HPG_FP32_DEFECT, ninner=1)
+        if form == "learned":
+            for x in g:
+                m = x.mesh
+                y = np.tile(np.log([0.5, 0.5, 0.5, 0.7, 0.7, 0.7, 1.0, 1.0, 1.0]), (m.ntri, 1)).astype(np.float32)
+                assert x.L.hpg_set_prcn_log_scalings(x.h, _lib.ptr(y)) == 0
+                x.set_preconditioner_mode(_lib.HPG_PRCN_LEARNED)
+    for k in range(6):
+        if k == 4:
+            # a new CFL list: the next call runs enqueued, the one after captures anew
+            new_cfl = [0.8 * c for c in (2.5, 1.5, 1.0, 0.5)] if physics == "ins" else [0.008] * 4
+            for g in (ga, gb):
+                for x in g:
+                    x.set_cfl(new_cfl)
+        ea = MG.cycle(ga, log2p, vw, ic, ir, fused=True)
+        eb = MG.cycle(gb, log2p, vw, ic, ir, fused=True)
+        ra = ga[0].get_state()
+        rb = gb[0].get_state()
+        ga[0].rsdl()
+        gb[0].rsdl()
+        print("cycle", k, "maxres enqueued", ea, "graph", eb, "judge", ga[0].resnorm(), gb[0].resnorm(), flush=True)
+        assert ea == eb
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x, y)
+        assert np.array_equal(ga[0].get_residual()[0], gb[0].get_residual()[0])
