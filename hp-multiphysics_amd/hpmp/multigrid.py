@@ -80,6 +80,16 @@ def square_hierarchy(fine_mesh, n_fine: int, ngrid: int):
     return meshes, links
 
 
+def mesh_hierarchy(fine_mesh, ngrid: int, n_fine: int | None = None):
+    """Meshes and connection tables of an h-multigrid stack: the structured coarsening
+    of ``unit_square(n_fine)`` when n_fine is given, the unstructured one of
+    hpmp.coarsen (vertex removal and Delaunay, as tri_mesh::coarsen) otherwise."""
+    if n_fine is not None:
+        return square_hierarchy(fine_mesh, n_fine, ngrid)
+    from .coarsen import hierarchy
+    return hierarchy(fine_mesh, ngrid)
+
+
 def build_stack(cls, fine, meshes, links, **params):
     """Coarse levels of the same physics as ``fine`` on ``meshes[1:]`` at p = 1, linked
     downwards.  ``cls`` is the level class (GPU or CPU); ``params`` are its physical
