@@ -970,8 +970,11 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
     __shared__ double sc[8][3][NVC][16];  // u, conservative correction, primitive boundary correction
     __shared__ double sf[8][NVC][32];
     __shared__ double so[8][NVC][IM];
+    __shared__ double sL[8][3][16];   // the vertices' 4x4 blocks, loaded once per element
+    __shared__ double sid[2 * (IM > 0 ? IM : 1)];
     for (int k = threadIdx.x; k < TM * 32; k += blockDim.x) sP[k] = a.ops[D::OP_P + k];
     for (int k = threadIdx.x; k < G * 16; k += blockDim.x) sI[k] = a.ops[D::OP_I + k];
+    for (int k = threadIdx.x; k < 2 * IM; k += blockDim.x) sid[k] = a.idiag[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     const double gm1 = a.gamma - 1.0;
     __syncthreads();  // operators staged
@@ -1009,6 +1012,7 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
                 sc[hw][2][n][k] = tt;
             }
         }
+        for (int e = g; e < 48; e += 32) sL[hw][e >> 4][e & 15] = a.vpc[16 * sel3(vtx, e >> 4) + (e & 15)];
         wave_lds_sync();
         double w[NVC], r[NVC], tp[NVC];
 #pragma unroll
@@ -1029,8 +1033,7 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
             const double b0 = sP[0 * 32 + g], b1 = sP[1 * 32 + g], b2 = sP[2 * 32 + g];
 #pragma unroll
             for (int k = 0; k < 16; ++k)
-                L[k] = (k % 5 == 0 && g >= G) ? 1.0
-                                              : a.vpc[16 * vtx[0] + k] * b0 + a.vpc[16 * vtx[1] + k] * b1 + a.vpc[16 * vtx[2] + k] * b2;
+                L[k] = (k % 5 == 0 && g >= G) ? 1.0 : sL[hw][0][k] * b0 + sL[hw][1][k] * b1 + sL[hw][2][k] * b2;
         }
         switch_vars(w, r, gm1);
         lower4(L, r);
@@ -1052,11 +1055,11 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
 #pragma unroll
             for (int m = 0; m < IM; ++m) x[m] = so[hw][n][m];
 #pragma unroll
-            for (int j = 0; j < IM; ++j) x[j] = (x[j] - (j ? a.idiag[j * 2] * x[j - 1] : 0.0)) / a.idiag[j * 2 + 1];
+            for (int j = 0; j < IM; ++j) x[j] = (x[j] - (j ? sid[j * 2] * x[j - 1] : 0.0)) / sid[j * 2 + 1];
 #pragma unroll
             for (int j = IM - 1; j >= 0; --j) {
-                x[j] /= a.idiag[j * 2 + 1];
-                if (j) x[j - 1] -= a.idiag[j * 2] * x[j];
+                x[j] /= sid[j * 2 + 1];
+                if (j) x[j - 1] -= sid[j * 2] * x[j];
             }
 #pragma unroll
             for (int m = 0; m < IM; ++m) a.resi[(t * a.im0 + m) * NVC + n] = -x[m];

@@ -160,3 +160,40 @@ def test_guard_falls_back_from_an_unstable_network():
     it, hist = gs.solve(tol_rel=1e-6, max_cycles=6000)
     print("fell back at", gs.fell_back_at, "cycles", it, flush=True)
     assert gs.fell_back_at is not None and hist[-1][1] <= 1e-6 * hist[0][1]
+
+
+def test_learned_on_every_level_cuts_w_cycles():
+    """The learned form as the time-to-tolerance bench runs it, at a size a test affords:
+    64 x 64 squares, p = 4, mu = 8e-3 (the cell Reynolds number of 512 x 512 at 1e-3),
+    W(2,2,2) cycle over p 4 -> 2 -> 1 and the meshes to 16 x 16.  train_learned (200
+    seeded states on 32 x 32, device training, calibration on this mesh size) must give
+    a network with which the guarded solve reaches 1e-10 in fewer cycles than the
+    analytic preconditioner, without falling back, at the same solution."""
+    _case("learned on every level vs analytic, 64x64 p=4 mu=8e-3 W(2,2,2)")
+    from hpmp import mesh as M, multigrid as MG, workloads as W
+    from hpmp.learned import GuardedSolver, train_learned
+    from hpmp.tri_hp import TriHpIns, rel_diff_mod_pressure
+    n, mu = 64, 8e-3
+    net, info = train_learned(mu, n, ntrain=200, calib_n=64, log=lambda m: print(m, flush=True))
+    m = M.unit_square(n, 0.2, 1)
+    st = W.fourier_state(m, 4, seed=2)
+    meshes, links = MG.square_hierarchy(m, n, 3)
+    g = TriHpIns(m, log2p=2, mu=mu)
+    grids = MG.build_stack(TriHpIns, g, meshes, links, mu=mu)
+    res = {}
+    for tag, mlp in (("analytic", None), ("learned", net)):
+        g.set_state(*st)
+        gs = GuardedSolver(g, mlp, grids=grids, cyc=(2, 2, 2), check_every=5)
+        it, hist = gs.solve(tol_rel=1e-10, max_cycles=600)
+        res[tag] = (it, hist[-1][1] / hist[0][1], gs.fell_back_at, g.get_state())
+        print(tag, "cycles", it, "final", res[tag][1], "fell back at", gs.fell_back_at, flush=True)
+    assert res["analytic"][1] <= 1e-10 and res["learned"][1] <= 1e-10
+    assert res["learned"][2] is None
+    assert res["learned"][0] < res["analytic"][0]
+    err = rel_diff_mod_pressure(res["learned"][3], res["analytic"][3])
+    print("solution rel diff (mod pressure constant)", err, "type factors", info["type_factors"], flush=True)
+    assert err < 1e-8
+    for x in grids[:0:-1]:
+        x.close()
+    g.close()
+    net.close()
