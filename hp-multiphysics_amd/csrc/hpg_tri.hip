@@ -188,6 +188,14 @@ struct hpg_ctx {
     int* eseg = nullptr;
     std::vector<int> eoff;
     int neseg = 0;
+    std::vector<int> h_eseg;  // eseg on the host (boundary side of each edge index)
+    // curved boundary sides (ins, p >= 2): the elements owning one (clist) and the rest
+    // in mesh order (slist), the elements' coordinate side modes (crv, ncurv x 3 x sm0
+    // x 2), each edge's row in scrd (ecrv) and the rows (the sides' coordinate modes)
+    int ncurv = 0;
+    int *clist = nullptr, *slist = nullptr, *ecrv = nullptr;
+    double *crv = nullptr, *scrd = nullptr;
+    double *PRmat[3] = {nullptr, nullptr, nullptr}, *PSmat[3] = {nullptr, nullptr, nullptr};
     // p = 1 vertex path (ins): element-major element and boundary-edge results and the
     // boundary-edge endpoints at each vertex (entries 2*idx + k into eout)
     double *eres = nullptr, *eout = nullptr;
@@ -580,12 +588,25 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     a.gbase = 0;
     const bool full = beta > 0.0;
     if (!store_lin) {
-        // element-major over all elements in mesh order, then the fixed-order gather
-        a.order = nullptr; a.begin = 0; a.end = c->ntri;
+        // element-major over all elements in mesh order (the straight ones, then the
+        // curved ones, when there are curved boundary sides), then the fixed-order gather
+        a.order = c->ncurv ? c->slist : nullptr; a.begin = 0; a.end = c->ntri - c->ncurv;
         switch (L.p) {
             case 2: launch_elem_em<2>(c, a, full); break;
             case 4: launch_elem_em<4>(c, a, full); break;
             default: throw std::runtime_error("unsupported p");
+        }
+        if (c->ncurv) {
+            ElemArgs b = a;
+            b.order = c->clist; b.end = c->ncurv; b.crv = c->crv;
+            const dim3 grid(persistent_grid((c->ncurv + 63) / 64, 1 << 20));
+            if (L.p == 2) {
+                if (full) hipLaunchKernelGGL((k_elem_ins_em_curved<2, true>), grid, dim3(256), 0, c->stream, b, c->eres);
+                else hipLaunchKernelGGL((k_elem_ins_em_curved<2, false>), grid, dim3(256), 0, c->stream, b, c->eres);
+            } else {
+                if (full) hipLaunchKernelGGL((k_elem_ins_em_curved<4, true>), grid, dim3(256), 0, c->stream, b, c->eres);
+                else hipLaunchKernelGGL((k_elem_ins_em_curved<4, false>), grid, dim3(256), 0, c->stream, b, c->eres);
+            }
         }
         const LevelDims dd(L.p);
         GatherArgsT<double> ga{};
@@ -605,11 +626,12 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
         }
     }
     const LevelDims d(L.p);
-    EdgeArgs e;
+    EdgeArgs e{};
     e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->ugv; e.ugs = c->ugs;
     e.resv = c->resv; e.ress = c->ress;
     e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
     e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
+    e.ecrv = c->ecrv; e.scrd = c->scrd;
     for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
         e.begin = c->eoff[col];
         e.end = c->eoff[col + 1];
@@ -656,9 +678,21 @@ void do_rsdl32(hpg_ctx* c, int stage) {
     if (L.p >= 2) {
         // element-major over all elements in mesh order, then the fixed-order gather
         // (which adds the source of the vertex and side modes)
-        a.order = nullptr; a.begin = 0; a.end = c->ntri;
+        a.order = c->ncurv ? c->slist : nullptr; a.begin = 0; a.end = c->ntri - c->ncurv;
         if (L.p == 2) launch_elem32_em<2>(c, a, full);
         else launch_elem32_em<4>(c, a, full);
+        if (c->ncurv) {
+            ElemArgs32 b = a;
+            b.order = c->clist; b.end = c->ncurv; b.crv = c->crv;
+            const dim3 grid(persistent_grid((c->ncurv + 63) / 64, 1 << 20));
+            if (L.p == 2) {
+                if (full) hipLaunchKernelGGL((k_elem_ins32_em_curved<2, true>), grid, dim3(256), 0, c->stream, b, c->eres32);
+                else hipLaunchKernelGGL((k_elem_ins32_em_curved<2, false>), grid, dim3(256), 0, c->stream, b, c->eres32);
+            } else {
+                if (full) hipLaunchKernelGGL((k_elem_ins32_em_curved<4, true>), grid, dim3(256), 0, c->stream, b, c->eres32);
+                else hipLaunchKernelGGL((k_elem_ins32_em_curved<4, false>), grid, dim3(256), 0, c->stream, b, c->eres32);
+            }
+        }
         const LevelDims dd(L.p);
         GatherArgsT<float> ga{};
         ga.npnt = c->npnt; ga.nseg = c->nseg; ga.sm = dd.sm; ga.sm0 = c->sm0; ga.bm = dd.bm;
@@ -673,11 +707,12 @@ void do_rsdl32(hpg_ctx* c, int stage) {
         launch_elem32<1>(c, a, full);
     }
     const LevelDims d(L.p);
-    EdgeArgsT<float> e;
+    EdgeArgsT<float> e{};
     e.eseg = c->eseg; e.seg_pnt = c->seg_pnt; e.pnts = c->pnts; e.ugv = c->dlv; e.ugs = c->dls;
     e.resv = c->r32v; e.ress = c->r32s;
     e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
     e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
+    e.ecrv = c->ecrv; e.scrd = c->scrd;
     for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
         e.begin = c->eoff[col];
         e.end = c->eoff[col + 1];
@@ -1057,7 +1092,7 @@ int do_setup(hpg_ctx* c, bool sync = true) {
     const Level& L = c->lv[c->log2p];
     const LevelDims d(L.p);
     if (sync) check(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream), "memset");
-    SetupArgs s;
+    SetupArgs s{};
     s.pnts = c->pnts; s.tri_pnt = c->tri_pnt; s.tri_seg = c->tri_seg; s.tri_sgn = c->tri_sgn;
     s.ugv = c->ugv; s.ugs = c->ugs; s.ugi = c->ugi; s.Pmat = L.Pmat; s.tprcn = c->tprcn; s.err = c->d_err;
     s.ntri = c->ntri; s.p = d.p; s.sm = d.sm; s.im = d.im; s.tm = d.tm; s.G = d.G; s.sm0 = c->sm0; s.im0 = c->im0;
@@ -1066,6 +1101,14 @@ int do_setup(hpg_ctx* c, bool sync = true) {
         case 1: hipLaunchKernelGGL(k_setup_elem<1>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
         case 2: hipLaunchKernelGGL(k_setup_elem<2>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
         default: hipLaunchKernelGGL(k_setup_elem<4>, dim3(nblk8(c->ntri, 128)), dim3(128), 0, c->stream, s); break;
+    }
+    if (c->ncurv && d.p >= 2) {
+        // the curved branch overwrites its elements' tprcn (the reference's setinfo marks
+        // curved elements only for p >= 2, tri_hp/metric.cpp:45-70)
+        s.clist = c->clist; s.ncurv = c->ncurv; s.crv = c->crv;
+        s.PRmat = c->PRmat[c->log2p]; s.PSmat = c->PSmat[c->log2p];
+        if (d.p == 2) hipLaunchKernelGGL(k_setup_elem_curved<2>, dim3(nblk(c->ncurv, 64)), dim3(64), 0, c->stream, s);
+        else hipLaunchKernelGGL(k_setup_elem_curved<4>, dim3(nblk(c->ncurv, 64)), dim3(64), 0, c->stream, s);
     }
     // learned preconditioner on this level: element features -> MLP (or the caller's
     // log scalings) -> separate scalings of the vertex, side and interior rows
@@ -1263,6 +1306,7 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         colour(neseg, 2, ev.data(), np, eorder, eoffv, ecol);
         for (int j = 0; j < neseg; ++j) eseg[j] = m->ebdry_seg[eorder[j]];
         c->eseg = c->upload(eseg.data(), eseg.size());
+        c->h_eseg = eseg;
         c->eoff = eoffv;
         c->neseg = neseg;
         {
@@ -1660,6 +1704,7 @@ int hpg_set_defect_inner(hpg_ctx* c, int ninner) {
 int hpg_apply_jacobian(hpg_ctx* c, const double* dv, const double* ds, const double* di, double* ov, double* os,
                        double* oi) {
     if (!c || !dv || !ov) return -2;
+    if (c->ncurv && c->lv[c->log2p].p >= 2) { c->err = "the linearised residual has no curved branch"; return -2; }
     if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     if (c->log2p != c->log2pmax) { c->err = "apply_jacobian runs on the finest level"; return -2; }
     HPG_TRY(c, {
@@ -1714,6 +1759,7 @@ int hpg_mlp_features(hpg_ctx* c, float* out) {
 
 int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
     if (!c || !out) return -2;
+    if (c->ncurv && c->lv[c->log2p].p >= 2) { c->err = "the linearised residual has no curved branch"; return -2; }
     if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     if (c->log2p != c->log2pmax) { c->err = "measure_jacobian_diag runs on the finest level"; return -2; }
     HPG_TRY(c, {
@@ -1751,6 +1797,7 @@ int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
 
 int hpg_element_jacobian(hpg_ctx* c, const int* tind, int nel, double eps_r, double eps_a, double* K) {
     if (!c || nel < 0) return -2;
+    if (c->ncurv && c->lv[c->log2p].p >= 2) { c->err = "element_jacobian: no curved branch"; return -2; }
     if (nel == 0) return 0;
     if (!tind || !K) return -2;
     HPG_TRY(c, {
@@ -2044,6 +2091,7 @@ std::vector<unsigned long long> cycle_signature(hpg_ctx* const* grids, int ngrid
         const hpg_ctx* g = grids[k];
         sg.push_back((unsigned long long)(uintptr_t)g);
         sg.push_back((unsigned long long)(uintptr_t)g->mlp);
+        sg.push_back((unsigned long long)(uintptr_t)g->crv);
         sg.push_back(((unsigned long long)g->precision << 40) | ((unsigned long long)g->ninner << 24) |
                      ((unsigned long long)g->prcn_mode << 16) | ((unsigned long long)g->user_y << 12) |
                      ((unsigned long long)g->fd_reset << 8) | ((unsigned long long)g->timing << 4) |
@@ -2179,6 +2227,70 @@ int hpg_set_cfl(hpg_ctx* c, const double* cfl, int n) {
     for (int k = 0; k < n; ++k)
         if (!(cfl[k] >= 0.0) || !std::isfinite(cfl[k])) return -2;
     for (int k = 0; k < n; ++k) c->prm.cfl[k] = cfl[k];
+    return 0;
+}
+
+int hpg_set_curved(hpg_ctx* c, int nside, const int* sides, const double* crds) {
+    if (!c || nside < 0 || (nside > 0 && (!sides || !crds))) return -2;
+    if (c->physics != HPG_PHYSICS_INS) { c->err = "curved sides: ins only"; return -2; }
+    HPG_TRY(c, {
+        std::vector<int> row(c->nseg, -1);
+        for (int j = 0; j < c->neseg; ++j) row[c->h_eseg[j]] = -2;  // boundary side, not curved
+        for (int r = 0; r < nside; ++r) {
+            const int s = sides[r];
+            if (s < 0 || s >= c->nseg) throw std::runtime_error("curved side index out of range");
+            if (row[s] == -1) throw std::runtime_error("curved side is not a boundary side");
+            if (row[s] >= 0) throw std::runtime_error("curved side listed twice");
+            row[s] = r;
+        }
+        for (int r = 0; r < nside * c->sm0 * 2; ++r)
+            if (!std::isfinite(crds[r])) throw std::runtime_error("curved side coefficients not finite");
+        check(hipStreamSynchronize(c->stream), "sync");
+        std::vector<int> tseg(3 * (size_t)c->ntri), tsgn(3 * (size_t)c->ntri), stri(2 * (size_t)c->nseg);
+        check(hipMemcpy(tseg.data(), c->tri_seg, tseg.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(tsgn.data(), c->tri_sgn, tsgn.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(stri.data(), c->st_tri, stri.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        // elements owning a curved side, ascending; their coordinate side modes with the
+        // side sign of ugtouht (+1 on the reference's boundary sides)
+        std::vector<int> cl, sl;
+        std::vector<double> cv;
+        const int sm0 = c->sm0;
+        for (int t = 0; t < c->ntri; ++t) {
+            bool curved = false;
+            for (int i = 0; i < 3; ++i) curved |= row[tseg[3 * t + i]] >= 0;
+            if (!curved) { sl.push_back(t); continue; }
+            cl.push_back(t);
+            for (int i = 0; i < 3; ++i) {
+                const int r = row[tseg[3 * t + i]];
+                for (int m = 0; m < sm0; ++m)
+                    for (int n = 0; n < 2; ++n) {
+                        const double sg = ((m & 1) && tsgn[3 * t + i] < 0) ? -1.0 : 1.0;
+                        cv.push_back(r >= 0 ? sg * crds[((size_t)r * sm0 + m) * 2 + n] : 0.0);
+                    }
+            }
+        }
+        std::vector<int> ec(c->neseg);
+        for (int j = 0; j < c->neseg; ++j) ec[j] = std::max(row[c->h_eseg[j]], -1);
+        c->ncurv = (int)cl.size();
+        if (c->ncurv > 0) {
+            c->clist = c->upload(cl.data(), cl.size());
+            c->slist = c->upload(sl.data(), sl.size());
+            c->crv = c->upload(cv.data(), cv.size());
+            c->ecrv = c->upload(ec.data(), ec.size());
+            c->scrd = c->upload(crds, (size_t)nside * sm0 * 2);
+            for (int l = 0; l <= c->log2pmax; ++l) {
+                Level& L = c->lv[l];
+                if (L.p >= 2 && !c->PRmat[l]) {
+                    c->PRmat[l] = c->upload(L.ops->Pr.data(), L.ops->Pr.size());
+                    c->PSmat[l] = c->upload(L.ops->Ps.data(), L.ops->Ps.size());
+                }
+            }
+        } else {
+            c->clist = c->slist = c->ecrv = nullptr;
+            c->crv = c->scrd = nullptr;
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
     return 0;
 }
 

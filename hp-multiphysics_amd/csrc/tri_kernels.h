@@ -62,6 +62,10 @@ struct ElemArgsT {
     const double* srcs;
     const double* srci;
     const double* sfac;
+    // curved elements (the curved launch only): per entry of order[begin:end), the
+    // side-mode coefficients of the element's coordinates, (3 sides, sm0 modes, x/y)
+    // with the side sign applied, zero on straight sides
+    const double* crv;
 };
 typedef ElemArgsT<double> ElemArgs;
 typedef ElemArgsT<float> ElemArgs32;

@@ -134,7 +134,11 @@ __device__ __forceinline__ ElemSlots<P> elem_slots(const A& a, const ElemConn& c
 // residual values of the current group are requested before its contractions), so
 // the dependent gathers overlap the MFMA work of earlier groups.  The grid is a
 // multiple of 8 blocks of 4 waves.
-template <typename T, int P, bool FULL, bool EM = false>
+// CURV: the curved branch (tri_hp/ins/rsdl.cpp:83-257) for the elements order[begin:end)
+// that own a curved boundary side: the coordinate derivatives dcrd are projected to
+// every Gauss point from the element's coordinate modes (tri_hp/metric.cpp:12-21:
+// crdtocht + proj_bdry), so the metric terms, the Jacobian and tau vary per point.
+template <typename T, int P, bool FULL, bool EM = false, bool CURV = false>
 __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restrict__ eres = nullptr,
                                               int egrid = 0) {
     using D = Dims<P>;
@@ -208,6 +212,53 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
         const T l10 = T(0.5 * (y2 - y1)), l11 = T(0.5 * (y0 - y1));
 
         const V4 z = {T(0), T(0), T(0), T(0)};
+        // curved: dx/dr, dx/ds, dy/dr, dy/ds at the Gauss points.  The coordinate modes
+        // are taken relative to vertex 1 (the derivatives of a constant vanish), which
+        // keeps the FP32 form free of the cancellation of absolute coordinates.
+        V4 Xr[CURV ? GT : 1], Xs[CURV ? GT : 1], Yr[CURV ? GT : 1], Ys[CURV ? GT : 1];
+        if constexpr (CURV) {
+            constexpr int SMD = D::SM > 0 ? D::SM : 1;
+            const int cidx = a.begin + g * 16 + e;
+            T bx[KT], by[KT];
+#pragma unroll
+            for (int s = 0; s < KT; ++s) {
+                const int k = 4 * s + q;
+                T cx = T(0), cy = T(0);
+                if (k == 0) { cx = T(x0 - x1); cy = T(y0 - y1); }
+                else if (k == 2) { cx = T(x2 - x1); cy = T(y2 - y1); }
+                else if (k >= 3 && k < D::BM && S.t >= 0) {
+                    const int i = (k - 3) / SMD, m = (k - 3) - i * SMD;
+                    const double* cr = a.crv + ((size_t)(cidx * 3 + i) * a.sm0 + m) * 2;
+                    cx = T(cr[0]); cy = T(cr[1]);
+                }
+                bx[s] = cx; by[s] = cy;
+            }
+#pragma unroll
+            for (int mt = 0; mt < GT; ++mt) {
+                V4 c1 = z, c2 = z, c3 = z, c4 = z;
+#pragma unroll
+                for (int s = 0; s < KT; ++s) {
+                    const T ar = sops[D::OFF_PR + (mt * KT + s) * 64 + lane];
+                    const T as = sops[D::OFF_PS + (mt * KT + s) * 64 + lane];
+                    c1 = mfma(ar, bx[s], c1); c2 = mfma(as, bx[s], c2);
+                    c3 = mfma(ar, by[s], c3); c4 = mfma(as, by[s], c4);
+                }
+                Xr[mt] = c1; Xs[mt] = c2; Yr[mt] = c3; Ys[mt] = c4;
+            }
+        }
+        // metric terms of Gauss point (mt, i): the element constants of a straight
+        // element, the projected derivatives of a curved one.  The padding rows of the
+        // tiles (points beyond the G Gauss points) have zero operator rows, hence a zero
+        // Jacobian; they keep the straight metric so that their values stay finite (the
+        // integrations weight them by zero, and 0 x inf would reach every mode).
+        auto metric = [&](int mt, int i, T& m00, T& m01, T& m10, T& m11) {
+            if constexpr (CURV) {
+                m00 = Xr[mt][i]; m01 = Xs[mt][i]; m10 = Yr[mt][i]; m11 = Ys[mt][i];
+                if (m00 * m11 - m10 * m01 == T(0)) { m00 = l00; m01 = l01; m10 = l10; m11 = l11; }
+            } else {
+                m00 = l00; m01 = l01; m10 = l10; m11 = l11;
+            }
+        };
         V4 U[GT], V[GT], Pp[GT];
 #pragma unroll
         for (int mt = 0; mt < GT; ++mt) {
@@ -229,15 +280,17 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
         for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
+                T m00, m01, m10, m11;
+                metric(mt, i, m00, m01, m10, m11);
                 const T u = U[mt][i], v = V[mt][i], pr = Pp[mt][i];
                 const T fx = rho * u, fy = rho * v;
-                const T c0 = +l11 * fx - l01 * fy;
-                const T c1 = -l10 * fx + l00 * fy;
+                const T c0 = +m11 * fx - m01 * fy;
+                const T c1 = -m10 * fx + m00 * fy;
                 C0[mt][i] = c0; C1[mt][i] = c1;
-                A00[mt][i] = u * c0 + l11 * pr;
-                A01[mt][i] = u * c1 - l10 * pr;
-                A10[mt][i] = v * c0 - l01 * pr;
-                A11[mt][i] = v * c1 + l00 * pr;
+                A00[mt][i] = u * c0 + m11 * pr;
+                A01[mt][i] = u * c1 - m10 * pr;
+                A10[mt][i] = v * c0 - m01 * pr;
+                A11[mt][i] = v * c1 + m00 * pr;
             }
 
         // lf_im = intgrtrs of the inviscid fluxes (rsdl.cpp:292-294)
@@ -270,24 +323,40 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
                 }
                 Ur[mt] = c1; Us[mt] = c2; Vr[mt] = c3; Vs[mt] = c4;
             }
-            // viscous stress tensor (rsdl.cpp:303-326); steady so rhobd0 = 0
+            // viscous stress tensor (rsdl.cpp:303-326, curved :139-159 per point); steady
+            // so rhobd0 = 0
             const T cjcb = l00 * l11 - l10 * l01;
             const T cjcbi = a.mu / cjcb;
-            const T v0000 = -cjcbi * (T(2) * l11 * l11 + l01 * l01);
-            const T v0011 = -cjcbi * (T(2) * l10 * l10 + l00 * l00);
-            const T v0001 = cjcbi * (T(2) * l11 * l10 + l01 * l00);
-            const T v1100 = -cjcbi * (l11 * l11 + T(2) * l01 * l01);
-            const T v1111 = -cjcbi * (l10 * l10 + T(2) * l00 * l00);
-            const T v1101 = cjcbi * (l11 * l10 + T(2) * l01 * l00);
-            const T v0100 = cjcbi * l01 * l11;
-            const T v0111 = cjcbi * l00 * l10;
-            const T v0101 = -cjcbi * l01 * l10;
-            const T v0110 = -cjcbi * l00 * l11;
+            T v0000 = -cjcbi * (T(2) * l11 * l11 + l01 * l01);
+            T v0011 = -cjcbi * (T(2) * l10 * l10 + l00 * l00);
+            T v0001 = cjcbi * (T(2) * l11 * l10 + l01 * l00);
+            T v1100 = -cjcbi * (l11 * l11 + T(2) * l01 * l01);
+            T v1111 = -cjcbi * (l10 * l10 + T(2) * l00 * l00);
+            T v1101 = cjcbi * (l11 * l10 + T(2) * l01 * l00);
+            T v0100 = cjcbi * l01 * l11;
+            T v0111 = cjcbi * l00 * l10;
+            T v0101 = -cjcbi * l01 * l10;
+            T v0110 = -cjcbi * l00 * l11;
             V4 F00[GT], F01[GT], F10[GT], F11[GT];
 #pragma unroll
             for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
+                    if constexpr (CURV) {
+                        T m00, m01, m10, m11;
+                        metric(mt, i, m00, m01, m10, m11);
+                        const T cji = a.mu / (m00 * m11 - m10 * m01);
+                        v0000 = -cji * (T(2) * m11 * m11 + m01 * m01);
+                        v0011 = -cji * (T(2) * m10 * m10 + m00 * m00);
+                        v0001 = cji * (T(2) * m11 * m10 + m01 * m00);
+                        v1100 = -cji * (m11 * m11 + T(2) * m01 * m01);
+                        v1111 = -cji * (m10 * m10 + T(2) * m00 * m00);
+                        v1101 = cji * (m11 * m10 + T(2) * m01 * m00);
+                        v0100 = cji * m01 * m11;
+                        v0111 = cji * m00 * m10;
+                        v0101 = -cji * m01 * m10;
+                        v0110 = -cji * m00 * m11;
+                    }
                     const T ur = Ur[mt][i], us = Us[mt][i], vr = Vr[mt][i], vs = Vs[mt][i];
                     const T f00 = v0000 * ur + v0100 * vr + v0001 * us + v0101 * vs;
                     const T f01 = v0001 * ur + v0110 * vr + v0011 * us + v0111 * vs;
@@ -343,19 +412,23 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
             for (int mt = 0; mt < GT; ++mt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
+                    T m00, m01, m10, m11;
+                    metric(mt, i, m00, m01, m10, m11);
                     const T u = U[mt][i], v = V[mt][i];
                     const T qq = u * u + v * v;
                     T gam = T(3) * qq + gvis;
                     if (a.mu == T(0)) gam = fmax(gam, T(0.1));
-                    const T tau0 = tfac / sqrt(gam);
+                    // curved: cjcb of this Gauss point (rsdl.cpp:385)
+                    const T tf = CURV ? a.adis * h / (m00 * m11 - m10 * m01) : tfac;
+                    const T tau0 = tf / sqrt(gam);
                     const T tau2 = sqrt(qq) * tau0;
                     const T t0 = tau0 * R0[mt][i], t1 = tau0 * R1[mt][i], t2 = tau2 * R2[mt][i];
-                    F00[mt][i] -= (l11 * (T(2) * u) - l01 * v) * t0 - l01 * u * t1 + l11 * t2;
-                    F01[mt][i] -= (-l10 * (T(2) * u) + l00 * v) * t0 + l00 * u * t1 - l10 * t2;
-                    F10[mt][i] -= +l11 * v * t0 + (l11 * u - l01 * (T(2) * v)) * t1 - l01 * t2;
-                    F11[mt][i] -= -l10 * v * t0 + (-l10 * u + l00 * (T(2) * v)) * t1 + l00 * t2;
-                    C0[mt][i] = -(l11 * t0 - l01 * t1);
-                    C1[mt][i] = -(-l10 * t0 + l00 * t1);
+                    F00[mt][i] -= (m11 * (T(2) * u) - m01 * v) * t0 - m01 * u * t1 + m11 * t2;
+                    F01[mt][i] -= (-m10 * (T(2) * u) + m00 * v) * t0 + m00 * u * t1 - m10 * t2;
+                    F10[mt][i] -= +m11 * v * t0 + (m11 * u - m01 * (T(2) * v)) * t1 - m01 * t2;
+                    F11[mt][i] -= -m10 * v * t0 + (-m10 * u + m00 * (T(2) * v)) * t1 + m00 * t2;
+                    C0[mt][i] = -(m11 * t0 - m01 * t1);
+                    C1[mt][i] = -(-m10 * t0 + m00 * t1);
                 }
             load_old();
             // lf_re = intgrtrs of viscous - stabilisation fluxes (rsdl.cpp:424-426)
@@ -519,6 +592,10 @@ struct EdgeArgsT {
     const double* w1;
     int gpx, sm, sm0;
     double rho, bc_pressure;
+    // curved boundary sides (may be null): row of each edge idx in scrd, -1 if straight;
+    // scrd rows hold the side's coordinate modes (sm0 modes, x/y) in the side's direction
+    const int* ecrv;
+    const double* scrd;
 };
 typedef EdgeArgsT<double> EdgeArgs;
 
@@ -541,6 +618,14 @@ __device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restri
         for (int m = 0; m < SM; ++m) uh[n][m + 2] = a.ugs[(sind * a.sm0 + m) * NV + n];
     }
     const double X0 = a.pnts[2 * v0], Y0 = a.pnts[2 * v0 + 1], X1 = a.pnts[2 * v1], Y1 = a.pnts[2 * v1 + 1];
+    // crdtocht1d (tri_hp/gtol.cpp:296-320): the side modes of a curved side's coordinates
+    double cx[SM > 0 ? SM : 1], cy[SM > 0 ? SM : 1];
+    const int crow = a.ecrv ? a.ecrv[idx] : -1;
+#pragma unroll
+    for (int m = 0; m < SM; ++m) {
+        cx[m] = crow >= 0 ? a.scrd[((size_t)crow * a.sm0 + m) * 2] : 0.0;
+        cy[m] = crow >= 0 ? a.scrd[((size_t)crow * a.sm0 + m) * 2 + 1] : 0.0;
+    }
     double lf[NV][N1];
 #pragma unroll
     for (int n = 0; n < NV; ++n)
@@ -550,8 +635,15 @@ __device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restri
     for (int ip = 0; ip < GPX; ++ip) {
         const double* g = a.g1 + ip * N1;
         const double* dg = a.dg1 + ip * N1;
-        const double dxc = X0 * dg[0] + X1 * dg[1];
-        const double dyc = Y0 * dg[0] + Y1 * dg[1];
+        double dxc = X0 * dg[0] + X1 * dg[1];
+        double dyc = Y0 * dg[0] + Y1 * dg[1];
+        if (crow >= 0) {
+#pragma unroll
+            for (int m = 0; m < SM; ++m) {
+                dxc += cx[m] * dg[m + 2];
+                dyc += cy[m] * dg[m + 2];
+            }
+        }
         double u[NV];
 #pragma unroll
         for (int n = 0; n < NV; ++n) {
@@ -607,6 +699,19 @@ __global__ __launch_bounds__(256, P == 2 ? 2 : 1) void k_elem_ins_em(ElemArgs a,
 template <int P, bool FULL>
 __global__ __launch_bounds__(256, 2) void k_elem_ins32_em(ElemArgs32 a, float* eres) {
     elem_ins_body<float, P, FULL, true>(a, eres);
+}
+
+// the curved elements (order[begin:end), a.crv per entry) in element-major form: a
+// launch of its own after the straight elements' launch, over the few elements that
+// own a curved boundary side
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_ins_em_curved(ElemArgs a, double* eres) {
+    elem_ins_body<double, P, FULL, true, true>(a, eres);
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_ins32_em_curved(ElemArgs32 a, float* eres) {
+    elem_ins_body<float, P, FULL, true, true>(a, eres);
 }
 
 // lftog of the element-major vertex and side modes (tri_hp/nstage.cpp:21-82): one lane
@@ -882,6 +987,13 @@ struct SetupArgs {
     int* err;
     int ntri, p, sm, im, tm, G, sm0, im0;
     double rho, mu, adis;
+    // curved elements (k_setup_elem_curved): list, coordinate side modes per entry
+    // (the layout of ElemArgsT::crv), dense d/dr and d/ds projections (G x tm)
+    const int* clist;
+    int ncurv;
+    const double* crv;
+    const double* PRmat;
+    const double* PSmat;
 };
 
 template <int P>
@@ -934,6 +1046,95 @@ __global__ __launch_bounds__(128) void k_setup_elem(SetupArgs a) {
     hmax = sqrt(hmax);
     const double pf = 0.25 * (a.p + 1) * (a.p + 1);
     const double h = 4. * jcb / (pf * hmax);
+    hmax = hmax / pf;
+    if (!(h > 0.0) || qmax != qmax) {
+        atomicOr(a.err, 1);
+        a.tprcn[3 * t + 0] = a.tprcn[3 * t + 1] = a.tprcn[3 * t + 2] = 1.0;
+        return;
+    }
+    const double nu = a.mu / a.rho;
+    double gam = 3.0 * qmax + (2. * nu / hmax) * (2. * nu / hmax);
+    if (a.mu == 0.0) gam = fmax(gam, 0.1);
+    const double qs = sqrt(qmax);
+    const double lam1 = qs + sqrt(qmax + gam);
+    jcb *= 2. * nu * (1. / (hmax * hmax) + 1. / (h * h)) + 3 * lam1 / h;
+    a.tprcn[3 * t + 0] = a.rho * jcb;
+    a.tprcn[3 * t + 1] = a.rho * jcb;
+    a.tprcn[3 * t + 2] = jcb / gam;
+}
+
+// the curved branch of the same (tri_hp/ins/setup_preconditioner.cpp:44-90) for the
+// elements clist[0:ncurv), after k_setup_elem: h from the smallest Jacobian over the
+// Gauss points and hmax from the longest projected side tangent, both through the
+// coordinate derivatives at every point; tprcn overwritten
+template <int P>
+__global__ __launch_bounds__(64) void k_setup_elem_curved(SetupArgs a) {
+    constexpr int TM = Dims<P>::TM, SM = Dims<P>::SM, BM = Dims<P>::BM, G = Dims<P>::G;
+    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= a.ncurv) return;
+    const int t = a.clist[ci];
+    int vtx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vtx[k] = a.tri_pnt[3 * t + k];
+    double uh[2][TM], cx[BM], cy[BM];
+#pragma unroll
+    for (int k = 0; k < TM; ++k) {
+        int off;
+        double sg = 1.0;
+        const double* src;
+        if (k < 3) { off = vtx[k] * NV; src = a.ugv; }
+        else if (k < BM) {
+            const int i = (k - 3) / SM, m = (k - 3) - i * SM;
+            const int sgn = a.tri_sgn[3 * t + i];
+            sg = ((m & 1) && sgn < 0) ? -1.0 : 1.0;
+            off = (a.tri_seg[3 * t + i] * a.sm0 + m) * NV;
+            src = a.ugs;
+        } else { off = (t * a.im0 + (k - BM)) * NV; src = a.ugi; }
+        uh[0][k] = sg * src[off];
+        uh[1][k] = sg * src[off + 1];
+    }
+    const double X[3] = {a.pnts[2 * vtx[0]], a.pnts[2 * vtx[1]], a.pnts[2 * vtx[2]]};
+    const double Y[3] = {a.pnts[2 * vtx[0] + 1], a.pnts[2 * vtx[1] + 1], a.pnts[2 * vtx[2] + 1]};
+    // coordinate modes relative to vertex 1 (crdtocht, tri_hp/gtol.cpp:204-245)
+    cx[0] = X[0] - X[1]; cy[0] = Y[0] - Y[1];
+    cx[1] = 0.0; cy[1] = 0.0;
+    cx[2] = X[2] - X[1]; cy[2] = Y[2] - Y[1];
+#pragma unroll
+    for (int k = 3; k < BM; ++k) {
+        const int i = (k - 3) / SM, m = (k - 3) - i * SM;
+        cx[k] = a.crv[(((size_t)ci * 3 + i) * a.sm0 + m) * 2];
+        cy[k] = a.crv[(((size_t)ci * 3 + i) * a.sm0 + m) * 2 + 1];
+    }
+    const double dx1 = X[0] - X[2], dy1 = Y[0] - Y[2], dx2 = X[1] - X[0], dy2 = Y[1] - Y[0];
+    double jcb = 0.25 * (dx1 * dy2 - dy1 * dx2);
+    double jcbmin = jcb, hmax = 0.0, qmax = 0.0;
+    for (int g = 0; g < G; ++g) {
+        double u = 0.0, v = 0.0, xr = 0.0, xs = 0.0, yr = 0.0, ys = 0.0;
+#pragma unroll
+        for (int k = 0; k < TM; ++k) {
+            const double pv = a.Pmat[g * TM + k];
+            u += pv * uh[0][k];
+            v += pv * uh[1][k];
+        }
+#pragma unroll
+        for (int k = 0; k < BM; ++k) {
+            const double pr = a.PRmat[g * TM + k], ps = a.PSmat[g * TM + k];
+            xr += pr * cx[k]; xs += ps * cx[k];
+            yr += pr * cy[k]; ys += ps * cy[k];
+        }
+        jcbmin = fmin(jcbmin, xr * ys - yr * xs);
+        double hh = xr * xr + yr * yr;
+        hmax = hh > hmax ? hh : hmax;
+        hh = xs * xs + ys * ys;
+        hmax = hh > hmax ? hh : hmax;
+        hh = (xs - xr) * (xs - xr) + (ys - yr) * (ys - yr);
+        hmax = hh > hmax ? hh : hmax;
+        const double q = u * u + v * v;
+        qmax = (qmax > q) ? qmax : q;
+    }
+    hmax = 2. * sqrt(hmax);
+    const double pf = 0.25 * (a.p + 1) * (a.p + 1);
+    const double h = 4. * jcbmin / (pf * hmax);
     hmax = hmax / pf;
     if (!(h > 0.0) || qmax != qmax) {
         atomicOr(a.err, 1);

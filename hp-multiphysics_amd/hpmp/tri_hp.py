@@ -203,6 +203,16 @@ class TriHpIns:
         a = np.ascontiguousarray(cfl, dtype=np.float64)
         self._check(self.L.hpg_set_cfl(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), int(a.size)), "set_cfl")
 
+    def set_curved(self, sides, crds):
+        """Curved boundary sides (ins): ``crds`` (nside, sm0, 2), the coordinate side modes
+        of each listed boundary side in its own direction, as the reference's
+        ``hp_edge_bdry::crds`` (hpmp.curved.side_modes computes them from a boundary
+        shape).  An empty list makes every side straight again."""
+        sd = np.ascontiguousarray(sides, dtype=np.int32).reshape(-1)
+        cr = np.ascontiguousarray(crds, dtype=np.float64).reshape(sd.size, self.sm0, 2)
+        self._check(self.L.hpg_set_curved(self.h, int(sd.size), sd.ctypes.data_as(C.POINTER(C.c_int)),
+                                          cr.ctypes.data_as(C.POINTER(C.c_double))), "set_curved")
+
     def set_preconditioner_mode(self, mode: int):
         """HPG_PRCN_ANALYTIC or HPG_PRCN_LEARNED (needs an MLP attached, hpmp.learned)."""
         self._check(self.L.hpg_set_preconditioner(self.h, mode), "set_preconditioner")

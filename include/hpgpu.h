@@ -162,6 +162,19 @@ int hpg_set_defect_inner(hpg_ctx *ctx, int ninner);
  * changed between cycles instead of at construction.  Returns -2 for n outside 1..4 or
  * a CFL that is negative or not finite. */
 int hpg_set_cfl(hpg_ctx *ctx, const double *cfl, int n);
+/* Curved boundary sides (ins): the side-mode coefficients of the coordinates of each
+ * listed boundary side, crds (nside, sm0, 2) in the side's direction, the array
+ * hp_edge_bdry::crds that the reference's curv_init fills (tri_hp/hp_boundary.cpp:
+ * 621-675) and crdtocht / crdtocht1d read (tri_hp/gtol.cpp:204-320).  The elements that
+ * own one then take the curved branch of element_rsdl (tri_hp/ins/rsdl.cpp:83-257) and
+ * of setup_preconditioner (tri_hp/ins/setup_preconditioner.cpp:44-90), and their
+ * boundary edges the curved normals (tri_hp/hp_boundary.cpp:973-1015), on every level
+ * with p >= 2 (coarser p-levels use the leading modes; at p = 1 every element is
+ * straight, as tri_hp::metric::setinfo, tri_hp/metric.cpp:45-70).  nside = 0 makes every
+ * side straight again.  Returns -2 for a side that is not a boundary side, a side listed
+ * twice or non-finite coefficients; the element-Jacobian and linearised entry points
+ * return -2 while curved sides are set at p >= 2. */
+int hpg_set_curved(hpg_ctx *ctx, int nside, const int *sides, const double *crds);
 /* J(u) delta at the current state through the FP32 linearised kernels (finest
  * level; overwrites the residual arrays with R(u)).  No reference counterpart;
  * the directional derivative of tri_hp::rsdl(nstage). */

@@ -75,6 +75,7 @@ def lib():
         L.otri_element_rsdl.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.otri_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.otri_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
+        L.otri_set_curved.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
         L.ocns_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
         L.otri_get_mg.argtypes = [C.c_void_p, _dp, _dp]
         L.ocns_get_mg.argtypes = [C.c_void_p, _dp, _dp]
@@ -248,6 +249,15 @@ class TriHpIns:
         v, s, i = self._alloc()
         lib().otri_get_res(self.h, v, self._ptr(s), self._ptr(i))
         return v, s, i
+
+    def set_curved(self, sides, crds):
+        """Curved boundary sides: crds (nside, sm0, 2), the coordinate side modes of each
+        listed side in its own direction (hp_edge_bdry::crds)."""
+        sides = np.ascontiguousarray(sides, dtype=np.int32).reshape(-1)
+        crds = np.ascontiguousarray(crds, dtype=np.float64).reshape(sides.size, self.sm0, 2)
+        self._curved = (sides, crds)
+        lib().otri_set_curved(self.h, sides.size, sides if sides.size else np.zeros(1, np.int32),
+                              crds.reshape(-1) if crds.size else np.zeros(1))
 
     def set_res(self, v, s, i):
         lib().otri_set_res(self.h, np.ascontiguousarray(v, dtype=np.float64),

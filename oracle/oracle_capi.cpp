@@ -167,6 +167,9 @@ void otri_element_rsdl(void* h, int tind, int stage, const double* uh, double* l
 }
 void otri_element_jacobian(void* h, int tind, double eps_r, double eps_a, double* K) { T(h).element_jacobian(tind, eps_r, eps_a, K); }
 
+// curved boundary sides (ins): crds (nside, smf, 2) as hp_edge_bdry::crds
+void otri_set_curved(void* h, int nside, const int* sides, const double* crds) { T(h).set_curved(nside, sides, crds); }
+
 // h-multigrid link of a coarse level to its fine neighbour (ins and cns)
 void otri_get_mg(void* h, double* res0v, double* vug) { auto& x = T(h); std::memcpy(res0v, x.res0v.data(), x.m.npnt * 3 * 8); if (!x.vug_frst.empty()) std::memcpy(vug, x.vug_frst.data(), x.vug_frst.size() * 8); }
 void otri_link_fine(void* coarse, void* fine, const int* cc_tri, const double* cc_wt, const int* fc_tri, const double* fc_wt) {

@@ -45,6 +45,53 @@ double TriHp<NV>::inscribedradius(int t) const {
     return ar / perim;
 }
 
+// curved boundary sides; an element is curved when it owns one and the finest p > 1
+template <int NV>
+void TriHp<NV>::set_curved(int nside, const int* sides, const double* crds) {
+    seg_crv.assign(m.nseg, -1);
+    scrd.assign(crds, crds + (size_t)nside * smf * 2);
+    for (int r = 0; r < nside; ++r) seg_crv[sides[r]] = r;
+    tri_crv.assign(m.ntri, 0);
+    if (prm.log2pmax > 0)
+        for (int t = 0; t < m.ntri; ++t)
+            for (int i = 0; i < 3; ++i)
+                if (seg_crv[m.tri_seg[3 * t + i]] >= 0) tri_crv[t] = 1;
+}
+
+// tri_hp/gtol.cpp:204-245: vertex coordinates, the side modes of curved boundary
+// sides (as stored, no side sign), zero elsewhere
+template <int NV>
+void TriHp<NV>::crdtocht(int tind, double cht[2][15]) const {
+    const TriBasis& B = b();
+    for (int n = 0; n < 2; ++n)
+        for (int k = 0; k < 15; ++k) cht[n][k] = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int n = 0; n < 2; ++n) cht[n][i] = m.pnts[2 * m.tri_pnt[3 * tind + i] + n];
+    if (B.sm == 0) return;
+    int cnt = 3;
+    for (int i = 0; i < 3; ++i) {
+        const int sind = m.tri_seg[3 * tind + i];
+        const int r = seg_crv.empty() ? -1 : seg_crv[sind];
+        for (int mm = 0; mm < B.sm; ++mm) {
+            for (int n = 0; n < 2; ++n) cht[n][cnt] = r >= 0 ? scrd[((size_t)r * smf + mm) * 2 + n] : 0.0;
+            ++cnt;
+        }
+    }
+}
+
+// tri_hp/gtol.cpp:296-320
+template <int NV>
+void TriHp<NV>::crdtocht1d(int sind, double cht[2][8]) const {
+    const TriBasis& B = b();
+    const int v0 = m.seg_pnt[2 * sind], v1 = m.seg_pnt[2 * sind + 1];
+    const int r = seg_crv.empty() ? -1 : seg_crv[sind];
+    for (int n = 0; n < 2; ++n) {
+        cht[n][0] = m.pnts[2 * v0 + n];
+        cht[n][1] = m.pnts[2 * v1 + n];
+        for (int mm = 0; mm < B.sm; ++mm) cht[n][mm + 2] = r >= 0 ? scrd[((size_t)r * smf + mm) * 2 + n] : 0.0;
+    }
+}
+
 // tri_hp/gtol.cpp:14-55
 template <int NV>
 void TriHp<NV>::ugtouht(int tind) { ugtouht_to(tind, uht); }

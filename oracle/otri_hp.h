@@ -1,6 +1,6 @@
 // Oracle (test infrastructure only): CPU restatement of the reference's generic
-// tri_hp residual/relaxation machinery for straight-sided steady problems with the
-// diagonal preconditioner, shared by the ins and cns restatements.
+// tri_hp residual/relaxation machinery for steady problems with the diagonal
+// preconditioner, shared by the ins and cns restatements.
 //
 //   rsdl(stage)              tri_hp/nstage.cpp:12-110
 //   mg_source                tri_hp/nstage.cpp:113-129
@@ -15,6 +15,7 @@
 //   element_jacobian         tri_hp/nstage.cpp:131-165 (fixed mesh)
 //   cycle                    tri_mesh/blocks.cpp:1267-1348 (one grid, extra_finest_levels)
 //   area, inscribedradius    tri_mesh/tprims.cpp:66-80, 224-242
+//   crdtocht / crdtocht1d    tri_hp/gtol.cpp:204-245, 296-320 (curved boundary sides)
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it.
 #pragma once
@@ -73,6 +74,17 @@ class TriHp {
     std::vector<int> cc_tri, fc_tri;
     std::vector<double> cc_wt, fc_wt, vug_frst;
     void link_fine(TriHp* f, const int* cctri, const double* ccwt, const int* fctri, const double* fcwt);
+    // curved boundary sides: hp_edge_bdry::crds (tri_hp/hp_boundary.cpp:621-675), the
+    // row of each side in scrd (-1: straight), rows of smf modes x 2 in the side's
+    // direction; tri_crv marks the elements that own one when log2pmax > 0
+    // (tri_hp::metric::setinfo, tri_hp/metric.cpp:45-70)
+    std::vector<int> seg_crv;
+    std::vector<double> scrd;
+    std::vector<char> tri_crv;
+    void set_curved(int nside, const int* sides, const double* crds);
+    bool curved(int t) const { return !tri_crv.empty() && tri_crv[t]; }
+    void crdtocht(int tind, double cht[2][15]) const;
+    void crdtocht1d(int sind, double cht[2][8]) const;
 
     // solution and work arrays, reference layouts v(npnt,NV), s(nseg,sm,NV), i(ntri,im,NV)
     std::vector<double> ugv, ugs, ugi;
