@@ -189,6 +189,11 @@ struct hpg_ctx {
     std::vector<int> eoff;
     int neseg = 0;
     std::vector<int> h_eseg;  // eseg on the host (boundary side of each edge index)
+    std::vector<int> h_ebid;  // edge boundary of each edge index
+    // boundary flux type of each edge index (null: all generic) and the freestream
+    // velocity of the euler / characteristic fluxes (ins)
+    unsigned char* etype = nullptr;
+    double ibc_u[2] = {0.0, 0.0};
     // curved boundary sides (ins, p >= 2): the elements owning one (clist) and the rest
     // in mesh order (slist), the elements' coordinate side modes (crv, ncurv x 3 x sm0
     // x 2), each edge's row in scrd (ecrv) and the rows (the sides' coordinate modes)
@@ -533,6 +538,7 @@ void vstage_p1(hpg_ctx* c, int stage, double cfa, bool save_u0) {
     e.resv = c->resv; e.ress = c->ress;
     e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
     e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
+    e.etype = c->etype; e.ub0 = c->ibc_u[0]; e.ub1 = c->ibc_u[1]; e.mu = c->prm.mu;
     e.begin = 0; e.end = c->neseg;
     // element blocks (one thread per element, mesh order) followed by the boundary-edge
     // blocks
@@ -632,6 +638,7 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
     e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
     e.ecrv = c->ecrv; e.scrd = c->scrd;
+    e.etype = c->etype; e.ub0 = c->ibc_u[0]; e.ub1 = c->ibc_u[1]; e.mu = c->prm.mu;
     for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
         e.begin = c->eoff[col];
         e.end = c->eoff[col + 1];
@@ -713,6 +720,7 @@ void do_rsdl32(hpg_ctx* c, int stage) {
     e.g1 = L.edge; e.dg1 = L.edge + (d.p + 1) * (d.sm + 2); e.w1 = L.edge + 2 * (d.p + 1) * (d.sm + 2);
     e.gpx = d.p + 1; e.sm = d.sm; e.sm0 = c->sm0; e.rho = c->prm.rho; e.bc_pressure = c->prm.bc_pressure;
     e.ecrv = c->ecrv; e.scrd = c->scrd;
+    e.etype = c->etype; e.ub0 = c->ibc_u[0]; e.ub1 = c->ibc_u[1]; e.mu = c->prm.mu;
     for (size_t col = 0; col + 1 < c->eoff.size(); ++col) {
         e.begin = c->eoff[col];
         e.end = c->eoff[col + 1];
@@ -1307,6 +1315,9 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         for (int j = 0; j < neseg; ++j) eseg[j] = m->ebdry_seg[eorder[j]];
         c->eseg = c->upload(eseg.data(), eseg.size());
         c->h_eseg = eseg;
+        c->h_ebid.assign(neseg, 0);
+        for (int j = 0; j < neseg; ++j)
+            while (c->h_ebid[j] + 1 < m->nebd && m->ebdry_start[c->h_ebid[j] + 1] <= eorder[j]) ++c->h_ebid[j];
         c->eoff = eoffv;
         c->neseg = neseg;
         {
@@ -2092,6 +2103,12 @@ std::vector<unsigned long long> cycle_signature(hpg_ctx* const* grids, int ngrid
         sg.push_back((unsigned long long)(uintptr_t)g);
         sg.push_back((unsigned long long)(uintptr_t)g->mlp);
         sg.push_back((unsigned long long)(uintptr_t)g->crv);
+        sg.push_back((unsigned long long)(uintptr_t)g->etype);
+        for (double x : {g->ibc_u[0], g->ibc_u[1], g->prm.bc_pressure}) {
+            unsigned long long b;
+            std::memcpy(&b, &x, sizeof(b));
+            sg.push_back(b);
+        }
         sg.push_back(((unsigned long long)g->precision << 40) | ((unsigned long long)g->ninner << 24) |
                      ((unsigned long long)g->prcn_mode << 16) | ((unsigned long long)g->user_y << 12) |
                      ((unsigned long long)g->fd_reset << 8) | ((unsigned long long)g->timing << 4) |
@@ -2227,6 +2244,32 @@ int hpg_set_cfl(hpg_ctx* c, const double* cfl, int n) {
     for (int k = 0; k < n; ++k)
         if (!(cfl[k] >= 0.0) || !std::isfinite(cfl[k])) return -2;
     for (int k = 0; k < n; ++k) c->prm.cfl[k] = cfl[k];
+    return 0;
+}
+
+int hpg_set_edge_flux(hpg_ctx* c, int nebd, const int* types, const double* ibc) {
+    if (!c || nebd < 0 || (nebd > 0 && !types) || !ibc) return -2;
+    if (c->physics != HPG_PHYSICS_INS) { c->err = "edge flux types: ins only"; return -2; }
+    const int nb = c->h_ebid.empty() ? 0 : *std::max_element(c->h_ebid.begin(), c->h_ebid.end()) + 1;
+    if (nebd < nb) { c->err = "edge flux types: one per edge boundary"; return -2; }
+    for (int b = 0; b < nebd; ++b)
+        if (types[b] < HPG_EDGE_GENERIC || types[b] > HPG_EDGE_CHARACTERISTIC) { c->err = "edge flux type"; return -2; }
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(ibc[k])) { c->err = "freestream state not finite"; return -2; }
+    HPG_TRY(c, {
+        std::vector<unsigned char> et(c->neseg);
+        bool any = false;
+        for (int j = 0; j < c->neseg; ++j) {
+            et[j] = (unsigned char)types[c->h_ebid[j]];
+            any |= et[j] != HPG_EDGE_GENERIC;
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+        c->etype = any ? c->upload(et.data(), et.size()) : nullptr;
+        c->ibc_u[0] = ibc[0];
+        c->ibc_u[1] = ibc[1];
+        c->prm.bc_pressure = ibc[2];
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
     return 0;
 }
 

@@ -596,8 +596,62 @@ struct EdgeArgsT {
     // scrd rows hold the side's coordinate modes (sm0 modes, x/y) in the side's direction
     const int* ecrv;
     const double* scrd;
+    // flux type of each edge idx (may be null: all generic), the freestream velocity of
+    // the euler and characteristic fluxes (the freestream pressure is bc_pressure), mu
+    // for the characteristic's gam
+    const unsigned char* etype;
+    double ub0, ub1, mu;
 };
 typedef EdgeArgsT<double> EdgeArgs;
+
+enum EdgeFlux : unsigned char { EDGE_GENERIC = 0, EDGE_EULER = 1, EDGE_CHARACTERISTIC = 2 };
+
+// boundary flux of tri_hp_ins at one Gauss point (steady, mesh velocity zero), unit
+// normal (nx, ny), jcb the point's length factor |dx/dxi|:
+//   generic (inflow, outflow, symmetry), bdry_ins.h:34-49: mass flux with the interior
+//     velocity, the freestream pressure pb on the momentum;
+//   euler, bdry_ins.h:96-113: mass flux and momentum with the freestream velocity, the
+//     interior pressure;
+//   characteristic, tri_hp/ins/bdry.cpp:245-320: the boundary state from the
+//     characteristic swap between the interior and the freestream (its natural fluxes
+//     zero), then the euler-form flux of that state.
+template <int P>
+__device__ __forceinline__ void edge_flux_ins(int type, const double* u, double nx, double ny, double jcb,
+                                              double rho, double mu, double ub0, double ub1, double pb,
+                                              double* fl) {
+    if (type == EDGE_GENERIC) {
+        const double f2 = rho * (u[0] * nx + u[1] * ny);
+        fl[0] = f2 * u[0] + pb * nx;
+        fl[1] = f2 * u[1] + pb * ny;
+        fl[2] = f2;
+        return;
+    }
+    double b0 = ub0, b1 = ub1, b2 = u[2];
+    if (type == EDGE_CHARACTERISTIC) {
+        const double rhoi = 1. / rho, nu = mu / rho;
+        const double hmax = jcb * 2.0 / (0.25 * (P + 1) * (P + 1));
+        const double qmax = u[0] * u[0] + u[1] * u[1];
+        const double gv = 2. * nu / hmax;
+        const double gam = 3.0 * qmax + gv * gv;
+        const double ul = u[0] * nx + u[1] * ny, vl = -u[0] * ny + u[1] * nx, pl = u[2];
+        const double ur = ub0 * nx + ub1 * ny, vr = -ub0 * ny + ub1 * nx, pr = pb;
+        const double cl = sqrt(ul * ul + gam), cr = sqrt(ur * ur + gam);
+        const double c = 0.5 * (cl + cr), sn = 0.5 * (ul + ur), vt = 0.5 * (vl + vr);
+        const double den = 1. / (2 * c);
+        const double lam0 = sn, lam1 = sn + c, lam2 = sn - c;
+        const double w0 = ((pl - pr) * rhoi + (ul * lam1 - ur * lam2)) * den;
+        const double w1 = lam0 > 0.0 ? vt * ((pr - pl) * rhoi + lam2 * (ur - ul)) * den / (lam0 - lam2) + vl
+                                     : vt * ((pr - pl) * rhoi + lam1 * (ur - ul)) * den / (lam0 - lam1) + vr;
+        const double w2 = (rho * (ul - ur) * gam - lam2 * pl + lam1 * pr) * den;
+        b0 = w0 * nx - w1 * ny;
+        b1 = w0 * ny + w1 * nx;
+        b2 = w2;
+    }
+    const double f2 = rho * (b0 * nx + b1 * ny);
+    fl[0] = f2 * b0 + b2 * nx;
+    fl[1] = f2 * b1 + b2 * ny;
+    fl[2] = f2;
+}
 
 // P fixes the edge's mode count (p + 1) and Gauss points (p + 1), so the per-lane
 // arrays stay in registers
@@ -621,6 +675,7 @@ __device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restri
     // crdtocht1d (tri_hp/gtol.cpp:296-320): the side modes of a curved side's coordinates
     double cx[SM > 0 ? SM : 1], cy[SM > 0 ? SM : 1];
     const int crow = a.ecrv ? a.ecrv[idx] : -1;
+    const int etype = a.etype ? a.etype[idx] : EDGE_GENERIC;
 #pragma unroll
     for (int m = 0; m < SM; ++m) {
         cx[m] = crow >= 0 ? a.scrd[((size_t)crow * a.sm0 + m) * 2] : 0.0;
@@ -655,8 +710,8 @@ __device__ __forceinline__ void edge_ins_body(const EdgeArgsT<S>& a, S* __restri
         double nx = dyc, ny = -dxc;
         const double jcb = sqrt(nx * nx + ny * ny);
         nx /= jcb; ny /= jcb;
-        const double f2 = a.rho * (u[0] * nx + u[1] * ny);
-        const double fl[NV] = {f2 * u[0] + a.bc_pressure * nx, f2 * u[1] + a.bc_pressure * ny, f2};
+        double fl[NV];
+        edge_flux_ins<P>(etype, u, nx, ny, jcb, a.rho, a.mu, a.ub0, a.ub1, a.bc_pressure, fl);
 #pragma unroll
         for (int n = 0; n < NV; ++n) {
             const double r = fl[n] * jcb * a.w1[ip];

@@ -203,6 +203,15 @@ class TriHpIns:
         a = np.ascontiguousarray(cfl, dtype=np.float64)
         self._check(self.L.hpg_set_cfl(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), int(a.size)), "set_cfl")
 
+    def set_edge_flux(self, types, ibc=(0.0, 0.0, 0.0)):
+        """Boundary flux of each edge boundary (ins): HPG_EDGE_GENERIC (inflow, outflow,
+        symmetry), HPG_EDGE_EULER or HPG_EDGE_CHARACTERISTIC, and the freestream state
+        ``ibc`` = (u, v, p) of the initial/boundary-condition function."""
+        t = np.ascontiguousarray(types, dtype=np.int32).reshape(-1)
+        b = np.ascontiguousarray(ibc, dtype=np.float64).reshape(3)
+        self._check(self.L.hpg_set_edge_flux(self.h, int(t.size), t.ctypes.data_as(C.POINTER(C.c_int)),
+                                             b.ctypes.data_as(C.POINTER(C.c_double))), "set_edge_flux")
+
     def set_curved(self, sides, crds):
         """Curved boundary sides (ins): ``crds`` (nside, sm0, 2), the coordinate side modes
         of each listed boundary side in its own direction, as the reference's

@@ -175,6 +175,20 @@ int hpg_set_cfl(hpg_ctx *ctx, const double *cfl, int n);
  * twice or non-finite coefficients; the element-Jacobian and linearised entry points
  * return -2 while curved sides are set at p >= 2. */
 int hpg_set_curved(hpg_ctx *ctx, int nside, const int *sides, const double *crds);
+/* Boundary flux of each edge boundary (ins), the ebdry objects that
+ * tri_hp_ins::getnewedgeobject creates (tri_hp/ins/getnewbdry.cpp:70-110):
+ *   HPG_EDGE_GENERIC        inflow, outflow, symmetry (bdry_ins::generic::flux,
+ *                           tri_hp/ins/bdry_ins.h:34-49; the three differ only in their
+ *                           essential masks: u and v, none, the direction `dir`);
+ *   HPG_EDGE_EULER          bdry_ins::euler (bdry_ins.h:96-113);
+ *   HPG_EDGE_CHARACTERISTIC bdry_ins::characteristic (tri_hp/ins/bdry.cpp:245-320).
+ * types[b] for edge boundary b (mesh order); ibc = the freestream state (u, v, p) of
+ * the initial/boundary-condition function (ibc_ins::freestream, tri_hp/ins/getnewibc.cpp:
+ * 16-51: (flowspeed cos a, flowspeed sin a, 0)); its pressure replaces
+ * hpg_params.bc_pressure.  Default: every boundary generic, velocity (0, 0).  Returns
+ * -2 for an unknown type, fewer types than edge boundaries or a non-finite state. */
+enum { HPG_EDGE_GENERIC = 0, HPG_EDGE_EULER = 1, HPG_EDGE_CHARACTERISTIC = 2 };
+int hpg_set_edge_flux(hpg_ctx *ctx, int nebd, const int *types, const double *ibc);
 /* J(u) delta at the current state through the FP32 linearised kernels (finest
  * level; overwrites the residual arrays with R(u)).  No reference counterpart;
  * the directional derivative of tri_hp::rsdl(nstage). */

@@ -76,6 +76,8 @@ def lib():
         L.otri_element_jacobian.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.otri_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
         L.otri_set_curved.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.otri_set_edge_flux.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.otri_edge_flux.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.ocns_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
         L.otri_get_mg.argtypes = [C.c_void_p, _dp, _dp]
         L.ocns_get_mg.argtypes = [C.c_void_p, _dp, _dp]
@@ -249,6 +251,12 @@ class TriHpIns:
         v, s, i = self._alloc()
         lib().otri_get_res(self.h, v, self._ptr(s), self._ptr(i))
         return v, s, i
+
+    def set_edge_flux(self, types, ibc):
+        """Flux type of each edge boundary (0 generic, 1 euler, 2 characteristic) and the
+        freestream state (u, v, p)."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        lib().otri_set_edge_flux(self.h, t.size, t, np.ascontiguousarray(ibc, dtype=np.float64))
 
     def set_curved(self, sides, crds):
         """Curved boundary sides: crds (nside, sm0, 2), the coordinate side modes of each
@@ -481,3 +489,11 @@ class TriHpCns:
 
 def spectral_radius4(A):
     return lib().ocns_spectral_radius(np.ascontiguousarray(A, dtype=np.float64).ravel())
+
+
+def edge_flux(ftype, u, nrm, side_length, p, rho, mu, ub):
+    """One boundary point's flux of tri_hp_ins (generic 0, euler 1, characteristic 2)."""
+    out = np.zeros(3)
+    lib().otri_edge_flux(int(ftype), np.ascontiguousarray(u, dtype=np.float64), np.ascontiguousarray(nrm, dtype=np.float64),
+                         float(side_length), int(p), float(rho), float(mu), np.ascontiguousarray(ub, dtype=np.float64), out)
+    return out

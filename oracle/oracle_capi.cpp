@@ -170,6 +170,20 @@ void otri_element_jacobian(void* h, int tind, double eps_r, double eps_a, double
 // curved boundary sides (ins): crds (nside, smf, 2) as hp_edge_bdry::crds
 void otri_set_curved(void* h, int nside, const int* sides, const double* crds) { T(h).set_curved(nside, sides, crds); }
 
+// boundary flux types per edge boundary and the freestream state (u, v, p)
+void otri_set_edge_flux(void* h, int nebd, const int* types, const double* ibc) {
+    TriHpIns& x = T(h);
+    x.ebdry_flux.assign(types, types + nebd);
+    x.ibc_u[0] = ibc[0];
+    x.ibc_u[1] = ibc[1];
+    x.prm.bc_pressure = ibc[2];
+}
+// one boundary point's flux (test support)
+void otri_edge_flux(int type, const double* u, const double* nrm, double side_length, int p, double rho, double mu,
+                    const double* ub, double* flx) {
+    ins_edge_flux(type, u, nrm, side_length, p, rho, mu, ub, flx);
+}
+
 // h-multigrid link of a coarse level to its fine neighbour (ins and cns)
 void otri_get_mg(void* h, double* res0v, double* vug) { auto& x = T(h); std::memcpy(res0v, x.res0v.data(), x.m.npnt * 3 * 8); if (!x.vug_frst.empty()) std::memcpy(vug, x.vug_frst.data(), x.vug_frst.size() * 8); }
 void otri_link_fine(void* coarse, void* fine, const int* cc_tri, const double* cc_wt, const int* fc_tri, const double* fc_wt) {

@@ -226,7 +226,56 @@ void TriHpIns::element_rsdl_curved(int tind, int stage, const double uh[NV][15],
     }
 }
 
-// tri_hp/hp_boundary.cpp:973-1049 with the bdry_ins::generic flux (bdry_ins.h:34-49)
+void ins_edge_flux(int type, const double u[3], const double nrm[2], double side_length, int p, double rho,
+                   double mu, const double ub_in[3], double flx[3]) {
+    const double mv[2] = {0.0, 0.0};
+    if (type == EDGE_GENERIC) {
+        // bdry_ins::generic::flux: the ibc pressure on the momentum
+        flx[2] = rho * ((u[0] - mv[0]) * nrm[0] + (u[1] - mv[1]) * nrm[1]);
+        for (int n = 0; n < 2; ++n) flx[n] = flx[2] * u[n] + ub_in[2] * nrm[n];
+        return;
+    }
+    if (type == EDGE_EULER) {
+        // bdry_ins::euler::flux: ibc velocity, interior pressure
+        flx[2] = rho * ((ub_in[0] - mv[0]) * nrm[0] + (ub_in[1] - mv[1]) * nrm[1]);
+        for (int n = 0; n < 2; ++n) flx[n] = flx[2] * ub_in[n] + u[2] * nrm[n];
+        return;
+    }
+    // bdry_ins::characteristic::flux (natural fluxes zero, bd(0) = 0)
+    for (int n = 0; n < 3; ++n) flx[n] = 0.0;
+    const double bd0 = 0.0;
+    const double rhoi = 1. / rho, nu = mu / rho;
+    const double hmax = side_length * 2.0 / (0.25 * (p + 1) * (p + 1));
+    const double qmax = std::pow(u[0] - 0.5 * mv[0], 2.0) + std::pow(u[1] - 0.5 * mv[1], 2.0);
+    const double gam = 3.0 * qmax + (0.5 * hmax * bd0 + 2. * nu / hmax) * (0.5 * hmax * bd0 + 2. * nu / hmax);
+    const double ul = u[0] * nrm[0] + u[1] * nrm[1];
+    const double vl = -u[0] * nrm[1] + u[1] * nrm[0];
+    const double pl = u[2];
+    double ub[3] = {ub_in[0], ub_in[1], ub_in[2]}, uvp[3];
+    const double ur = ub[0] * nrm[0] + ub[1] * nrm[1];
+    const double vr = -ub[0] * nrm[1] + ub[1] * nrm[0];
+    const double pr = ub[2];
+    const double um = mv[0] * nrm[0] + mv[1] * nrm[1];
+    const double cl = std::sqrt((ul - .5 * um) * (ul - .5 * um) + gam);
+    const double cr = std::sqrt((ur - .5 * um) * (ur - .5 * um) + gam);
+    const double c = 0.5 * (cl + cr), s = 0.5 * (ul + ur), v = 0.5 * (vl + vr);
+    const double den = 1. / (2 * c);
+    const double lam0 = s - um, lam1 = s - .5 * um + c, lam2 = s - .5 * um - c;
+    uvp[0] = ((pl - pr) * rhoi + (ul * lam1 - ur * lam2)) * den;
+    if (lam0 > 0.0)
+        uvp[1] = v * ((pr - pl) * rhoi + lam2 * (ur - ul)) * den / (lam0 - lam2) + vl;
+    else
+        uvp[1] = v * ((pr - pl) * rhoi + lam1 * (ur - ul)) * den / (lam0 - lam1) + vr;
+    uvp[2] = (rho * (ul - ur) * gam - lam2 * pl + lam1 * pr) * den;
+    ub[0] = uvp[0] * nrm[0] - uvp[1] * nrm[1];
+    ub[1] = uvp[0] * nrm[1] + uvp[1] * nrm[0];
+    ub[2] = uvp[2];
+    const double f2 = rho * ((ub[0] - mv[0]) * nrm[0] + (ub[1] - mv[1]) * nrm[1]);
+    for (int n = 0; n < 2; ++n) flx[n] += f2 * ub[n] + ub[2] * nrm[n];
+    flx[2] = f2;
+}
+
+// tri_hp/hp_boundary.cpp:973-1049 with the boundary's flux (ins_edge_flux)
 void TriHpIns::edge_rsdl() {
     const TriBasis& B = b();
     const int nebd = int(m.ebdry_start.size()) - 1;
@@ -247,11 +296,11 @@ void TriHpIns::edge_rsdl() {
                 double nrm0 = dcrd[1][k], nrm1 = -dcrd[0][k];
                 const double jcb = std::sqrt(nrm0 * nrm0 + nrm1 * nrm1);
                 nrm0 /= jcb; nrm1 /= jcb;
-                const double mvx = 0.0, mvy = 0.0;
                 double flx[NV];
-                flx[NV - 1] = prm.rho * ((uu[0][k] - mvx) * nrm0 + (uu[1][k] - mvy) * nrm1);
-                flx[0] = flx[NV - 1] * uu[0][k] + prm.bc_pressure * nrm0;
-                flx[1] = flx[NV - 1] * uu[1][k] + prm.bc_pressure * nrm1;
+                const double uk[3] = {uu[0][k], uu[1][k], uu[2][k]}, nk[2] = {nrm0, nrm1};
+                const double ub[3] = {ibc_u[0], ibc_u[1], prm.bc_pressure};
+                ins_edge_flux(ebdry_flux.empty() ? EDGE_GENERIC : ebdry_flux[e], uk, nk, jcb, B.p, prm.rho, prm.mu,
+                              ub, flx);
                 for (int n = 0; n < NV; ++n) r[n][k] = flx[n] * jcb;
             }
             for (int n = 0; n < NV; ++n) B.intgrt1d(lfl[n], r[n]);
