@@ -190,6 +190,16 @@ struct hpg_ctx {
     int neseg = 0;
     std::vector<int> h_eseg;  // eseg on the host (boundary side of each edge index)
     std::vector<int> h_ebid;  // edge boundary of each edge index
+    std::vector<int> h_ebdry_start, h_ebdry_seg, h_ebdry_ess;  // the mesh's edge boundaries
+    // periodic boundaries (ins): vertex copies in groups (pg_ptr/pg_v, ascending), side
+    // pairs (ps_a, ps_b and the relative direction ps_o = +-1); the vertex and side sweeps
+    // of setup and minvrt read the union adjacency of every copy (vtu_*, vsu_*; the own
+    // lists when nothing is periodic)
+    bool periodic = false;
+    int npg = 0, nps = 0;
+    int *pg_ptr = nullptr, *pg_v = nullptr, *ps_a = nullptr, *ps_b = nullptr, *ps_o = nullptr;
+    int *vtu_ptr = nullptr, *vtu_tri = nullptr, *vtu_loc = nullptr;
+    int *vsu_ptr = nullptr, *vsu_side = nullptr, *vsu_end = nullptr;
     // boundary flux type of each edge index (null: all generic) and the freestream
     // velocity of the euler / characteristic fluxes (ins)
     unsigned char* etype = nullptr;
@@ -486,8 +496,9 @@ MinvArgsT<T> minv_args(hpg_ctx* c) {
     const LevelDims d(L.p);
     MinvArgsT<T> a;
     a.tri_pnt = c->tri_pnt; a.tri_seg = c->tri_seg; a.tri_sgn = c->tri_sgn;
-    a.vs_ptr = c->vs_ptr; a.vs_side = c->vs_side; a.vs_end = c->vs_end;
-    a.vt_ptr = c->vt_ptr; a.vt_tri = c->vt_tri; a.vt_loc = c->vt_loc;
+    // the vertex sweep couples every side and element of a vertex's periodic copies
+    a.vs_ptr = c->vsu_ptr; a.vs_side = c->vsu_side; a.vs_end = c->vsu_end;
+    a.vt_ptr = c->vtu_ptr; a.vt_tri = c->vtu_tri; a.vt_loc = c->vtu_loc;
     a.st_tri = c->st_tri; a.st_loc = c->st_loc;
     a.vess = c->vess; a.sess = c->sess;
     if constexpr (sizeof(T) == 8) {
@@ -519,6 +530,39 @@ void do_rsdl_cns(hpg_ctx* c, int stage);
 // element-major form (no colouring), then one lane per (vertex, variable) sums them in
 // a fixed order, adds the multigrid source and, with cfa >= 0, applies the vertex
 // minvrt and the stage update (save_u0: first stage, ug0 = ug)
+// every copy of a periodic vertex takes the sum over its group (after the residual)
+template <typename T>
+void periodic_combine(hpg_ctx* c, T* resv) {
+    if (!c->periodic || c->npg == 0) return;
+    hipLaunchKernelGGL((k_periodic_combine<T, NV>), dim3(nblk((long)c->npg * NV)), dim3(256), 0, c->stream, c->npg,
+                       c->pg_ptr, c->pg_v, resv);
+}
+
+// the state's periodic copies from the first copy of each group and side a of each pair
+void periodic_sync(hpg_ctx* c) {
+    if (!c->periodic) return;
+    const long n = (long)c->npg * NV + (long)c->nps * std::max(c->sm0, 0) * NV;
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_periodic_sync<double, NV>), dim3(nblk(n)), dim3(256), 0, c->stream, c->npg, c->pg_ptr,
+                       c->pg_v, c->nps, c->ps_a, c->ps_b, c->ps_o, c->sm0, c->sm0, c->ugv, c->ugs);
+    check(hipGetLastError(), "periodic sync");
+}
+
+// multigrid source of a coarse p-level (tri_hp/nstage.cpp:113-129): the first rsdl
+// after mg_restrict sets dres = fadd r0 - res, later ones add dres
+void mg_source_vsi(hpg_ctx* c) {
+    if (!c->coarse_flag) return;
+    const int l = c->log2p;
+    if (c->isfrst) {
+        modes_vsi(c, 2, c->resv, c->ress, c->resi, c->dresv[l], c->dress[l], c->dresi[l], c->r0v, c->r0s, c->r0i,
+                  c->prm.fadd);
+        c->isfrst = false;
+    } else {
+        modes_vsi(c, 1, c->resv, c->ress, c->resi, c->dresv[l], c->dress[l], c->dresi[l], nullptr, nullptr, nullptr,
+                  0.0);
+    }
+}
+
 void vstage_p1(hpg_ctx* c, int stage, double cfa, bool save_u0) {
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
@@ -560,7 +604,7 @@ void vstage_p1(hpg_ctx* c, int stage, double cfa, bool save_u0) {
     v.npnt = c->npnt; v.vt_ptr = c->vt_ptr; v.vt_tri = c->vt_tri; v.vt_loc = c->vt_loc;
     v.vb_ptr = c->vb_ptr; v.vb_ent = c->vb_ent; v.eres = c->eres; v.eout = c->eout;
     v.resv = c->resv; v.rrv = c->rrv; v.keep = f; v.full = full ? 1 : 0;
-    if (c->coarse_flag) {
+    if (c->coarse_flag && !c->periodic) {
         const int l = c->log2p;
         v.mg = c->isfrst ? 2 : 1;
         v.dres = c->dresv[l]; v.r0 = c->r0v; v.fadd = c->prm.fadd;
@@ -576,7 +620,14 @@ void vstage_p1(hpg_ctx* c, int stage, double cfa, bool save_u0) {
 
 void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
     if (c->physics == HPG_PHYSICS_CNS) { do_rsdl_cns(c, stage); return; }
-    if (c->lv[c->log2p].p == 1 && !store_lin) { vstage_p1(c, stage, -1.0, false); return; }
+    if (c->lv[c->log2p].p == 1 && !store_lin) {
+        vstage_p1(c, stage, -1.0, false);
+        if (c->periodic) {
+            periodic_combine<double>(c, c->resv);
+            mg_source_vsi(c);
+        }
+        return;
+    }
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     const double f = stage < nst ? 1.0 - beta : 0.0;
@@ -649,17 +700,8 @@ void do_rsdl(hpg_ctx* c, int stage, bool store_lin = false) {
             else hipLaunchKernelGGL(k_edge_ins<4>, be, te, 0, c->stream, e);
         }
     }
-    if (c->coarse_flag) {
-        const int l = c->log2p;
-        if (c->isfrst) {
-            modes_vsi(c, 2, c->resv, c->ress, c->resi, c->dresv[l], c->dress[l], c->dresi[l], c->r0v, c->r0s, c->r0i,
-                      c->prm.fadd);
-            c->isfrst = false;
-        } else {
-            modes_vsi(c, 1, c->resv, c->ress, c->resi, c->dresv[l], c->dress[l], c->dresi[l], nullptr, nullptr,
-                      nullptr, 0.0);
-        }
-    }
+    periodic_combine<double>(c, c->resv);
+    mg_source_vsi(c);
     check(hipGetLastError(), "rsdl launch");
 }
 
@@ -680,7 +722,8 @@ void do_rsdl32(hpg_ctx* c, int stage) {
     a.rho = (float)c->prm.rho; a.mu = (float)c->prm.mu; a.adis = (float)c->prm.adis; a.beta = (float)beta;
     a.fcv = c->fcv; a.fcs = c->fcs; a.colour = 0; a.keep = (float)f;
     a.lin = nullptr; a.gbase = 0;
-    a.srcv = c->resv; a.srcs = c->ress; a.srci = c->resi; a.sfac = c->fd_scal + 1;
+    // periodic: the vertex source enters after the copies are summed (once per group)
+    a.srcv = c->periodic ? nullptr : c->resv; a.srcs = c->ress; a.srci = c->resi; a.sfac = c->fd_scal + 1;
     const bool full = beta > 0.0;
     if (L.p >= 2) {
         // element-major over all elements in mesh order, then the fixed-order gather
@@ -707,7 +750,8 @@ void do_rsdl32(hpg_ctx* c, int stage) {
         ga.st_tri = c->st_tri; ga.st_loc = c->st_loc; ga.eres = c->eres32;
         ga.resv = c->r32v; ga.rrv = c->rr32v; ga.ress = c->r32s; ga.rrs = c->rr32s;
         ga.keep = (float)f; ga.full = full ? 1 : 0;
-        ga.srcv = c->resv; ga.srcs = c->ress; ga.sfac = c->fd_scal + 1;
+        // periodic: the vertex source enters after the copies are summed (once per group)
+        ga.srcv = c->periodic ? nullptr : c->resv; ga.srcs = c->ress; ga.sfac = c->fd_scal + 1;
         const long lanes = ((long)c->npnt + (long)c->nseg * dd.sm) * NV;
         hipLaunchKernelGGL((k_gather_vs<float, NV>), dim3(nblk(lanes)), dim3(256), 0, c->stream, ga);
     } else {
@@ -730,6 +774,11 @@ void do_rsdl32(hpg_ctx* c, int stage) {
             else if (L.p == 2) hipLaunchKernelGGL((k_edge_ins<2, float>), be, te, 0, c->stream, e);
             else hipLaunchKernelGGL((k_edge_ins<4, float>), be, te, 0, c->stream, e);
         }
+    }
+    if (c->periodic) {
+        periodic_combine<float>(c, c->r32v);
+        hipLaunchKernelGGL(k_add_src32, dim3(nblk((long)c->npnt * NV)), dim3(256), 0, c->stream, (long)c->npnt * NV,
+                           c->resv, c->fd_scal + 1, c->r32v);
     }
     check(hipGetLastError(), "rsdl32 launch");
 }
@@ -1130,7 +1179,7 @@ int do_setup(hpg_ctx* c, bool sync = true) {
         hipLaunchKernelGGL(k_mlp_apply, dim3(nblk((long)c->ntri * NV)), dim3(256), 0, c->stream, c->ntri, c->mlpY,
                            c->tprcn, c->tprcn_vx, c->tprcn_sd, c->tprcn_in);
     }
-    hipLaunchKernelGGL(k_setup_vrtx, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vt_ptr, c->vt_tri,
+    hipLaunchKernelGGL(k_setup_vrtx, dim3(nblk(c->npnt)), dim3(256), 0, c->stream, c->npnt, c->vtu_ptr, c->vtu_tri,
                        c->typed_prcn ? c->tprcn_vx : c->tprcn, c->vprcn, L.ops->vdiag);
     if (d.sm > 0)
         hipLaunchKernelGGL(k_setup_side, dim3(nblk(c->nseg)), dim3(256), 0, c->stream, c->nseg, c->st_tri,
@@ -1232,6 +1281,40 @@ void jacobian_colour(hpg_ctx* c, const JacArgs& ja, int N) {
 
 extern "C" {
 
+namespace {
+// the boundary-edge lists of the flux kernels: `sides` (boundary sides, edge boundary
+// bid of each) coloured so that no two edges of a colour share a vertex, each edge's
+// edge boundary, and the edge endpoints at each vertex (entries 2*idx + k of eout)
+void build_edges(hpg_ctx* c, const int* seg_pnt, int np, const std::vector<int>& sides, const std::vector<int>& bid) {
+    const int neseg = (int)sides.size();
+    std::vector<int> ev(2 * (size_t)neseg);
+    for (int j = 0; j < neseg; ++j) {
+        ev[2 * j] = seg_pnt[2 * sides[j]];
+        ev[2 * j + 1] = seg_pnt[2 * sides[j] + 1];
+    }
+    std::vector<int> eorder, eoffv, ecol, eseg(neseg);
+    colour(neseg, 2, ev.data(), np, eorder, eoffv, ecol);
+    c->h_ebid.assign(neseg, 0);
+    for (int j = 0; j < neseg; ++j) {
+        eseg[j] = sides[eorder[j]];
+        c->h_ebid[j] = bid[eorder[j]];
+    }
+    c->eseg = c->upload(eseg.data(), eseg.size());
+    c->h_eseg = eseg;
+    c->eoff = eoffv;
+    c->neseg = neseg;
+    std::vector<int> vb_ptr(np + 1, 0), vb_ent(2 * (size_t)neseg);
+    for (int j = 0; j < neseg; ++j)
+        for (int k = 0; k < 2; ++k) vb_ptr[seg_pnt[2 * eseg[j] + k] + 1]++;
+    for (int v = 0; v < np; ++v) vb_ptr[v + 1] += vb_ptr[v];
+    std::vector<int> f(vb_ptr.begin(), vb_ptr.end() - 1);
+    for (int j = 0; j < neseg; ++j)
+        for (int k = 0; k < 2; ++k) vb_ent[f[seg_pnt[2 * eseg[j] + k]]++] = 2 * j + k;
+    c->vb_ptr = c->upload(vb_ptr.data(), vb_ptr.size());
+    c->vb_ent = c->upload(vb_ent.data(), vb_ent.size());
+}
+}  // namespace
+
 int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
     if (!m || !prm || !out) return -2;
     *out = nullptr;
@@ -1305,32 +1388,20 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->order = c->upload(order.data(), order.size());
         c->coff = off;
         c->tcol = tcol;
-        std::vector<int> ev(2 * (size_t)neseg);
-        for (int j = 0; j < neseg; ++j) {
-            ev[2 * j] = m->seg_pnt[2 * m->ebdry_seg[j]];
-            ev[2 * j + 1] = m->seg_pnt[2 * m->ebdry_seg[j] + 1];
-        }
-        std::vector<int> eorder, eoffv, ecol, eseg(neseg);
-        colour(neseg, 2, ev.data(), np, eorder, eoffv, ecol);
-        for (int j = 0; j < neseg; ++j) eseg[j] = m->ebdry_seg[eorder[j]];
-        c->eseg = c->upload(eseg.data(), eseg.size());
-        c->h_eseg = eseg;
-        c->h_ebid.assign(neseg, 0);
-        for (int j = 0; j < neseg; ++j)
-            while (c->h_ebid[j] + 1 < m->nebd && m->ebdry_start[c->h_ebid[j] + 1] <= eorder[j]) ++c->h_ebid[j];
-        c->eoff = eoffv;
-        c->neseg = neseg;
         {
-            // endpoints of the boundary edges at each vertex, in the edge list's order
-            std::vector<int> vb_ptr(np + 1, 0), vb_ent(2 * (size_t)neseg);
-            for (int j = 0; j < neseg; ++j)
-                for (int k = 0; k < 2; ++k) vb_ptr[m->seg_pnt[2 * eseg[j] + k] + 1]++;
-            for (int v = 0; v < np; ++v) vb_ptr[v + 1] += vb_ptr[v];
-            std::vector<int> f(vb_ptr.begin(), vb_ptr.end() - 1);
-            for (int j = 0; j < neseg; ++j)
-                for (int k = 0; k < 2; ++k) vb_ent[f[m->seg_pnt[2 * eseg[j] + k]]++] = 2 * j + k;
-            c->vb_ptr = c->upload(vb_ptr.data(), vb_ptr.size());
-            c->vb_ent = c->upload(vb_ent.data(), vb_ent.size());
+            std::vector<int> sides, bid(neseg, 0);
+            if (neseg > 0) sides.assign(m->ebdry_seg, m->ebdry_seg + neseg);
+            for (int b = 0; b < m->nebd; ++b)
+                for (int j = m->ebdry_start[b]; j < m->ebdry_start[b + 1]; ++j) bid[j] = b;
+            build_edges(c.get(), m->seg_pnt, np, sides, bid);
+        }
+        // a mesh without edge boundaries may pass null arrays (checked above)
+        if (m->nebd > 0) {
+            c->h_ebdry_start.assign(m->ebdry_start, m->ebdry_start + m->nebd + 1);
+            c->h_ebdry_ess.assign(m->ebdry_essential, m->ebdry_essential + m->nebd);
+            if (neseg > 0) c->h_ebdry_seg.assign(m->ebdry_seg, m->ebdry_seg + neseg);
+        } else {
+            c->h_ebdry_start.assign(1, 0);
         }
 
         // adjacency in ascending order (the reference's accumulation order)
@@ -1377,6 +1448,8 @@ int hpg_create(const hpg_tri_mesh* m, const hpg_params* prm, hpg_ctx** out) {
         c->vt_ptr = c->upload(vt_ptr.data(), vt_ptr.size());
         c->vt_tri = c->upload(vt_tri.data(), vt_tri.size());
         c->vt_loc = c->upload(vt_loc.data(), vt_loc.size());
+        c->vtu_ptr = c->vt_ptr; c->vtu_tri = c->vt_tri; c->vtu_loc = c->vt_loc;
+        c->vsu_ptr = c->vs_ptr; c->vsu_side = c->vs_side; c->vsu_end = c->vs_end;
         c->st_tri = c->upload(st_tri.data(), st_tri.size());
         c->st_loc = c->upload(st_loc.data(), st_loc.size());
         c->vess = c->upload(vess.data(), vess.size());
@@ -1546,7 +1619,9 @@ static int copy3(hpg_ctx* c, bool h2d, double* dv, double* ds, double* di, const
 int hpg_set_state(hpg_ctx* c, const double* v, const double* s, const double* i) {
     if (!c) return -2;
     c->fd_reset = true;
-    return copy3(c, true, c->ugv, c->ugs, c->ugi, v, s, i, nullptr, nullptr, nullptr);
+    const int rc = copy3(c, true, c->ugv, c->ugs, c->ugi, v, s, i, nullptr, nullptr, nullptr);
+    if (rc == 0 && c->periodic) HPG_TRY(c, periodic_sync(c));
+    return rc;
 }
 int hpg_get_state(hpg_ctx* c, double* v, double* s, double* i) {
     if (!c) return -2;
@@ -1716,6 +1791,7 @@ int hpg_apply_jacobian(hpg_ctx* c, const double* dv, const double* ds, const dou
                        double* oi) {
     if (!c || !dv || !ov) return -2;
     if (c->ncurv && c->lv[c->log2p].p >= 2) { c->err = "the linearised residual has no curved branch"; return -2; }
+    if (c->periodic) { c->err = "the linearised residual does not join periodic copies"; return -2; }
     if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     if (c->log2p != c->log2pmax) { c->err = "apply_jacobian runs on the finest level"; return -2; }
     HPG_TRY(c, {
@@ -1771,6 +1847,7 @@ int hpg_mlp_features(hpg_ctx* c, float* out) {
 int hpg_measure_jacobian_diag(hpg_ctx* c, double* out) {
     if (!c || !out) return -2;
     if (c->ncurv && c->lv[c->log2p].p >= 2) { c->err = "the linearised residual has no curved branch"; return -2; }
+    if (c->periodic) { c->err = "the linearised residual does not join periodic copies"; return -2; }
     if (c->physics != HPG_PHYSICS_INS) { c->err = "ins only"; return -2; }
     if (c->log2p != c->log2pmax) { c->err = "measure_jacobian_diag runs on the finest level"; return -2; }
     HPG_TRY(c, {
@@ -1882,7 +1959,7 @@ void do_update(hpg_ctx* c) {
         mixed_update(c);
         return;
     }
-    if (c->lv[c->log2p].p == 1) {
+    if (c->lv[c->log2p].p == 1 && !c->periodic) {
         // vertex modes only: every stage is one vstage_p1 (ug0 saved by the first)
         for (int stage = 0; stage < c->prm.nstage; ++stage)
             vstage_p1(c, stage, c->prm.alpha[stage] * c->prm.cfl[c->log2p], stage == 0);
@@ -1940,6 +2017,7 @@ int hpg_resnorm(hpg_ctx* c, double* out) {
 int hpg_link_fine(hpg_ctx* c, hpg_ctx* f, const int* cc_tri, const double* cc_wt, const int* fc_tri,
                   const double* fc_wt) {
     if (!c || !f || c == f || !cc_tri || !cc_wt || !fc_tri || !fc_wt) return -2;
+    if (c->periodic || f->periodic) { c->err = "h-multigrid levels of periodic meshes are not supported"; return -2; }
     if (c->nv != f->nv || c->physics != f->physics || c->log2pmax != 0) {
         c->err = "a coarse level has the fine level's physics and runs at p = 1";
         return -2;
@@ -2104,6 +2182,8 @@ std::vector<unsigned long long> cycle_signature(hpg_ctx* const* grids, int ngrid
         sg.push_back((unsigned long long)(uintptr_t)g->mlp);
         sg.push_back((unsigned long long)(uintptr_t)g->crv);
         sg.push_back((unsigned long long)(uintptr_t)g->etype);
+        sg.push_back((unsigned long long)(uintptr_t)g->st_tri);
+        sg.push_back((unsigned long long)(uintptr_t)g->eseg);
         for (double x : {g->ibc_u[0], g->ibc_u[1], g->prm.bc_pressure}) {
             unsigned long long b;
             std::memcpy(&b, &x, sizeof(b));
@@ -2244,6 +2324,170 @@ int hpg_set_cfl(hpg_ctx* c, const double* cfl, int n) {
     for (int k = 0; k < n; ++k)
         if (!(cfl[k] >= 0.0) || !std::isfinite(cfl[k])) return -2;
     for (int k = 0; k < n; ++k) c->prm.cfl[k] = cfl[k];
+    return 0;
+}
+
+int hpg_set_periodic(hpg_ctx* c, int npair, const int* pairs) {
+    if (!c || npair < 0 || (npair > 0 && !pairs)) return -2;
+    if (c->physics != HPG_PHYSICS_INS) { c->err = "periodic boundaries: ins only"; return -2; }
+    if (c->periodic) { c->err = "periodic boundaries are already set"; return -2; }
+    if (c->coarse_level || c->fine) { c->err = "periodic boundaries on an h-multigrid level"; return -2; }
+    if (c->etype || c->ecrv) { c->err = "set periodic boundaries before edge flux types and curved sides"; return -2; }
+    const int nebd = (int)c->h_ebdry_start.size() - 1;
+    for (int k = 0; k < 2 * npair; ++k)
+        if (pairs[k] < 0 || pairs[k] >= nebd) { c->err = "periodic pair: edge boundary out of range"; return -2; }
+    if (npair == 0) return 0;
+    HPG_TRY(c, {
+        const int np = c->npnt, ns = c->nseg, nt = c->ntri;
+        check(hipStreamSynchronize(c->stream), "sync");
+        std::vector<double> X(2 * (size_t)np);
+        std::vector<int> sp(2 * (size_t)ns), tp(3 * (size_t)nt), tsg(3 * (size_t)nt), st(2 * (size_t)ns),
+            sl(2 * (size_t)ns);
+        check(hipMemcpy(X.data(), c->pnts, X.size() * 8, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(sp.data(), c->seg_pnt, sp.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(tp.data(), c->tri_pnt, tp.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(tsg.data(), c->tri_seg, tsg.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(st.data(), c->st_tri, st.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(sl.data(), c->st_loc, sl.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        // pair the sides as the reference's comm boundaries do (tri_mesh/tri_boundary.cpp:
+        // 344-470): side j of the first boundary with side n-1-j of the second; the
+        // second must be the first moved by one translation
+        std::vector<int> parent(np);
+        for (int v = 0; v < np; ++v) parent[v] = v;
+        auto find = [&](int v) { while (parent[v] != v) v = parent[v] = parent[parent[v]]; return v; };
+        std::vector<char> pbdry(nebd, 0), pside(ns, 0);
+        std::vector<int> psa, psb, pso;
+        for (int k = 0; k < npair; ++k) {
+            const int b1 = pairs[2 * k], b2 = pairs[2 * k + 1];
+            if (b1 == b2 || pbdry[b1] || pbdry[b2]) throw std::runtime_error("periodic pair: an edge boundary twice");
+            pbdry[b1] = pbdry[b2] = 1;
+            const int o1 = c->h_ebdry_start[b1], o2 = c->h_ebdry_start[b2];
+            const int n = c->h_ebdry_start[b1 + 1] - o1;
+            if (n != c->h_ebdry_start[b2 + 1] - o2 || n == 0) throw std::runtime_error("periodic pair: side counts differ");
+            double T0[2] = {0.0, 0.0};
+            for (int j = 0; j < n; ++j) {
+                const int sa = c->h_ebdry_seg[o1 + j], sb = c->h_ebdry_seg[o2 + n - 1 - j];
+                const int a0 = sp[2 * sa], a1 = sp[2 * sa + 1], q0 = sp[2 * sb], q1 = sp[2 * sb + 1];
+                const double L = std::hypot(X[2 * a1] - X[2 * a0], X[2 * a1 + 1] - X[2 * a0 + 1]);
+                const double tol = 1e-8 * std::max(L, 1e-300);
+                auto d = [&](int p, int q, int r, int t) {
+                    return std::hypot((X[2 * p] - X[2 * q]) - (X[2 * r] - X[2 * t]),
+                                      (X[2 * p + 1] - X[2 * q + 1]) - (X[2 * r + 1] - X[2 * t + 1]));
+                };
+                int m0, m1, o;
+                if (d(q1, a0, q0, a1) <= tol) { m0 = q1; m1 = q0; o = -1; }
+                else if (d(q0, a0, q1, a1) <= tol) { m0 = q0; m1 = q1; o = 1; }
+                else throw std::runtime_error("periodic pair: sides are not translates of each other");
+                const double T[2] = {X[2 * m0] - X[2 * a0], X[2 * m0 + 1] - X[2 * a0 + 1]};
+                if (j == 0) { T0[0] = T[0]; T0[1] = T[1]; }
+                else if (std::hypot(T[0] - T0[0], T[1] - T0[1]) > tol)
+                    throw std::runtime_error("periodic pair: not one translation");
+                if (st[2 * sa + 1] >= 0 || st[2 * sb + 1] >= 0 || st[2 * sa] < 0 || st[2 * sb] < 0)
+                    throw std::runtime_error("periodic pair: not boundary sides");
+                psa.push_back(sa); psb.push_back(sb); pso.push_back(o);
+                pside[sa] = pside[sb] = 1;
+                parent[find(a0)] = find(m0);
+                parent[find(a1)] = find(m1);
+            }
+        }
+        // vertex groups, copies ascending
+        std::vector<std::vector<int>> members(np);
+        for (int v = 0; v < np; ++v) members[find(v)].push_back(v);
+        std::vector<int> grp_of(np, -1), pg_ptr(1, 0), pg_v;
+        for (int v = 0; v < np; ++v)
+            if (members[v].size() > 1) {
+                const int g = (int)pg_ptr.size() - 1;
+                for (int u : members[v]) { pg_v.push_back(u); grp_of[u] = g; }
+                pg_ptr.push_back((int)pg_v.size());
+            }
+        // own adjacency (ascending), as hpg_create builds it
+        std::vector<std::vector<std::pair<int, int>>> vt(np), vs(np);
+        for (int t = 0; t < nt; ++t)
+            for (int i = 0; i < 3; ++i) vt[tp[3 * t + i]].push_back({t, i});
+        for (int q = 0; q < ns; ++q)
+            for (int i = 0; i < 2; ++i) vs[sp[2 * q + i]].push_back({q, i});
+        // a slave side of a pair stands for its master: (sa, end of the same vertex in sa)
+        std::vector<int> master(ns, -1), morient(ns, 1);
+        for (size_t k = 0; k < psb.size(); ++k) { master[psb[k]] = psa[k]; morient[psb[k]] = pso[k]; }
+        std::vector<int> vtu_ptr(np + 1, 0), vtu_tri, vtu_loc, vsu_ptr(np + 1, 0), vsu_side, vsu_end;
+        for (int v = 0; v < np; ++v) {
+            std::vector<std::pair<int, int>> a = vt[v], b;
+            const int g = grp_of[v];
+            const std::vector<int> one(1, v);
+            const std::vector<int>& cp = g >= 0 ? members[find(v)] : one;
+            if (g >= 0) {
+                a.clear();
+                for (int u : cp) a.insert(a.end(), vt[u].begin(), vt[u].end());
+                std::sort(a.begin(), a.end());
+            }
+            for (int u : cp)
+                for (auto e : vs[u]) {
+                    int q = e.first, end = e.second;
+                    if (master[q] >= 0) { end = morient[q] > 0 ? end : 1 - end; q = master[q]; }
+                    b.push_back({q, end});
+                }
+            std::sort(b.begin(), b.end());
+            b.erase(std::unique(b.begin(), b.end()), b.end());
+            for (auto e : a) { vtu_tri.push_back(e.first); vtu_loc.push_back(e.second); }
+            for (auto e : b) { vsu_side.push_back(e.first); vsu_end.push_back(e.second); }
+            vtu_ptr[v + 1] = (int)vtu_tri.size();
+            vsu_ptr[v + 1] = (int)vsu_side.size();
+        }
+        // each side of a pair: both elements ascending; the partner's flagged when the
+        // two copies run opposite ways
+        for (size_t k = 0; k < psa.size(); ++k) {
+            const int sa = psa[k], sb = psb[k];
+            const int ta = st[2 * sa], la = sl[2 * sa], tb = st[2 * sb], lb = sl[2 * sb];
+            const int fl = pso[k] < 0 ? 4 : 0;
+            const bool a_first = ta < tb;
+            st[2 * sa] = a_first ? ta : tb; sl[2 * sa] = a_first ? la : (lb | fl);
+            st[2 * sa + 1] = a_first ? tb : ta; sl[2 * sa + 1] = a_first ? (lb | fl) : la;
+            st[2 * sb] = a_first ? ta : tb; sl[2 * sb] = a_first ? (la | fl) : lb;
+            st[2 * sb + 1] = a_first ? tb : ta; sl[2 * sb + 1] = a_first ? lb : (la | fl);
+        }
+        // Dirichlet masks of the other boundaries, shared by every copy; no flux on the
+        // periodic boundaries (their sides are inside the domain)
+        std::vector<unsigned char> vess(np, 0), sess(ns, 0);
+        std::vector<int> esides, ebid;
+        for (int b = 0; b < nebd; ++b) {
+            if (pbdry[b]) continue;
+            const unsigned char mk = (unsigned char)(c->h_ebdry_ess[b] & 7);
+            for (int j = c->h_ebdry_start[b]; j < c->h_ebdry_start[b + 1]; ++j) {
+                const int q = c->h_ebdry_seg[j];
+                sess[q] |= mk;
+                vess[sp[2 * q]] |= mk;
+                vess[sp[2 * q + 1]] |= mk;
+                esides.push_back(q);
+                ebid.push_back(b);
+            }
+        }
+        for (size_t g = 0; g + 1 < pg_ptr.size(); ++g) {
+            unsigned char mk = 0;
+            for (int k = pg_ptr[g]; k < pg_ptr[g + 1]; ++k) mk |= vess[pg_v[k]];
+            for (int k = pg_ptr[g]; k < pg_ptr[g + 1]; ++k) vess[pg_v[k]] = mk;
+        }
+        build_edges(c, sp.data(), np, esides, ebid);
+        c->st_tri = c->upload(st.data(), st.size());
+        c->st_loc = c->upload(sl.data(), sl.size());
+        c->vess = c->upload(vess.data(), vess.size());
+        c->sess = c->upload(sess.data(), sess.size());
+        c->vtu_ptr = c->upload(vtu_ptr.data(), vtu_ptr.size());
+        c->vtu_tri = c->upload(vtu_tri.data(), vtu_tri.size());
+        c->vtu_loc = c->upload(vtu_loc.data(), vtu_loc.size());
+        c->vsu_ptr = c->upload(vsu_ptr.data(), vsu_ptr.size());
+        c->vsu_side = c->upload(vsu_side.data(), vsu_side.size());
+        c->vsu_end = c->upload(vsu_end.data(), vsu_end.size());
+        c->npg = (int)pg_ptr.size() - 1;
+        c->pg_ptr = c->upload(pg_ptr.data(), pg_ptr.size());
+        c->pg_v = c->upload(pg_v.data(), pg_v.size());
+        c->nps = (int)psa.size();
+        c->ps_a = c->upload(psa.data(), psa.size());
+        c->ps_b = c->upload(psb.data(), psb.size());
+        c->ps_o = c->upload(pso.data(), pso.size());
+        c->periodic = true;
+        periodic_sync(c);
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
     return 0;
 }
 

@@ -809,6 +809,18 @@ __device__ __forceinline__ void gather_sum(const GatherArgsT<T>& a, const T* e, 
 }
 
 template <typename T, int NVn>
+__device__ __forceinline__ void gather_sum_neg(const GatherArgsT<T>& a, const T* e, int n, T& acc, T& accr) {
+    const T li = -e[n];
+    if (a.full) {
+        const T lr = -e[NVn + n];
+        accr += lr;
+        acc = (acc + lr) + li;
+    } else {
+        acc = acc + li;
+    }
+}
+
+template <typename T, int NVn>
 __global__ __launch_bounds__(256) void k_gather_vs(GatherArgsT<T> a) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nvl = (long)a.npnt * NVn;
@@ -838,8 +850,14 @@ __global__ __launch_bounds__(256) void k_gather_vs(GatherArgsT<T> a) {
     for (int slot = 0; slot < 2; ++slot) {
         const int t = a.st_tri[2 * s + slot];
         if (t < 0) continue;
-        const int mode = 3 + a.st_loc[2 * s + slot] * a.sm + m;
-        gather_sum<T, NVn>(a, a.eres + ((long)t * a.bm + mode) * (2 * NVn), n, acc, accr);
+        // st_loc bit 2: the element is a periodic partner's, whose side runs the other
+        // way: its odd modes change sign in this side's frame
+        const int sl = a.st_loc[2 * s + slot];
+        const int mode = 3 + (sl & 3) * a.sm + m;
+        if ((sl & 4) && (m & 1))
+            gather_sum_neg<T, NVn>(a, a.eres + ((long)t * a.bm + mode) * (2 * NVn), n, acc, accr);
+        else
+            gather_sum<T, NVn>(a, a.eres + ((long)t * a.bm + mode) * (2 * NVn), n, acc, accr);
         if (first && a.srcs) acc = acc + T(a.srcs[o] * a.sfac[0]);
         first = false;
     }
@@ -1235,6 +1253,53 @@ __global__ void k_setup_side(int nseg, const int* st_tri, const double* tprcn, d
 }
 
 // ---------------------------------------------------------------------------------
+// periodic boundaries.  The reference keeps a partial residual on each copy of a
+// periodic vertex or side and averages the preconditioned values across the pair
+// (tri_hp/setup_preconditioner.cpp:24-36, 171-176, 255-258, 331-334 with the comm of
+// tri_mesh/tri_boundary.h:113-148), with the preconditioner sums averaged too, which
+// equals assembling the pair as one set of unknowns.  Here every copy holds the
+// assembled values: the vertex residual of each copy is summed over the group after
+// the residual (k_periodic_combine), the side and vertex sweeps read the union of the
+// copies' adjacency, and the state copies are kept equal (k_periodic_sync).
+// Vertex groups: copies pg_v[pg_ptr[g]:pg_ptr[g+1]] (ascending); one lane per (group, n).
+template <typename T, int NVn>
+__global__ void k_periodic_combine(int ngrp, const int* __restrict__ ptr, const int* __restrict__ pv, T* res) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrp * NVn) return;
+    const int grp = g / NVn, n = g - grp * NVn;
+    T sum = T(0);
+    for (int k = ptr[grp]; k < ptr[grp + 1]; ++k) sum += res[pv[k] * NVn + n];
+    for (int k = ptr[grp]; k < ptr[grp + 1]; ++k) res[pv[k] * NVn + n] = sum;
+}
+
+// r += fl32(sfac[0] * src): the FP32 inner solve's constant source on the vertices of a
+// periodic mesh, added after the copies are summed
+__global__ void k_add_src32(long n, const double* __restrict__ src, const double* __restrict__ sfac, float* r) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = r[i] + float(src[i] * sfac[0]);
+}
+
+// state copies from the first copy of each vertex group and from side a of each pair
+// (side b = o^m side a); one lane per (group, n), then per (pair, mode, n)
+template <typename T, int NVn>
+__global__ void k_periodic_sync(int ngrp, const int* __restrict__ ptr, const int* __restrict__ pv, int npair,
+                                const int* __restrict__ sa, const int* __restrict__ sb, const int* __restrict__ so,
+                                int sm, int sm0, T* v, T* sd) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngrp * NVn) {
+        const int grp = g / NVn, n = g - grp * NVn;
+        const T x = v[pv[ptr[grp]] * NVn + n];
+        for (int k = ptr[grp] + 1; k < ptr[grp + 1]; ++k) v[pv[k] * NVn + n] = x;
+        return;
+    }
+    const int h = g - ngrp * NVn;
+    if (h >= npair * sm * NVn) return;
+    const int pr = h / (sm * NVn), r = h - pr * (sm * NVn), m = r / NVn, n = r - m * NVn;
+    const T x = sd[((long)sa[pr] * sm0 + m) * NVn + n];
+    sd[((long)sb[pr] * sm0 + m) * NVn + n] = (so[pr] < 0 && (m & 1)) ? -x : x;
+}
+
+// ---------------------------------------------------------------------------------
 // minvrt, tri_hp/setup_preconditioner.cpp:102-373 (diagonal preconditioner)
 template <typename T>
 struct MinvArgsT {
@@ -1399,8 +1464,9 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
         for (int k = 0; k < 2; ++k) {
             const int t = a.st_tri[2 * s + k];
             if (t < 0) continue;
-            const int i = a.st_loc[2 * s + k];
-            const int sgn = a.tri_sgn[3 * t + i];
+            // a periodic partner's element (st_loc bit 2) sees this side reversed
+            const int sl = a.st_loc[2 * s + k], i = sl & 3;
+            const int sgn = (sl & 4) ? -a.tri_sgn[3 * t + i] : a.tri_sgn[3 * t + i];
             T bi[IM];
 #pragma unroll
             for (int m = 0; m < IM; ++m) bi[m] = a.resi[(t * a.im0 + m) * NVn + n];
@@ -1418,8 +1484,8 @@ __global__ __launch_bounds__(256) void k_minv_side(MinvArgsT<T> a) {
     for (int k = 0; k < 2; ++k) {
         const int t = a.st_tri[2 * s + k];
         if (t < 0) continue;
-        const int i = a.st_loc[2 * s + k];
-        const int sgn = a.tri_sgn[3 * t + i];
+        const int sl = a.st_loc[2 * s + k], i = sl & 3;
+        const int sgn = (sl & 4) ? -a.tri_sgn[3 * t + i] : a.tri_sgn[3 * t + i];
         const T tp = a.tprcn_s[NVn * t + n];
         T uh[3];
 #pragma unroll
@@ -1457,7 +1523,7 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
     for (int k = 0; k < 2; ++k) {
         const int t = a.st_tri[2 * s + k];
         if (t < 0) continue;
-        const int i = a.st_loc[2 * s + k];
+        const int sl = a.st_loc[2 * s + k], i = sl & 3;
         const T tp = a.tprcn_s[NVn * t + n];
         T uh[3];
         int sign[3];
@@ -1469,7 +1535,10 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
             const T sg = (MODE % 2) ? T(sign[j]) : T(1);
             uh[j] = sg * side_scaled<T, NVn>(a, sj, MODE, n) * tp;
         }
-        T ms = ((MODE + 1) % 2) ? T(sign[0]) : T(1);
+        // into this side's frame: a periodic partner's element (st_loc bit 2) has its
+        // own copy of the side running the other way
+        const int s0 = (sl & 4) ? -sign[0] : sign[0];
+        T ms = ((MODE + 1) % 2) ? T(s0) : T(1);
 #pragma unroll
         for (int m = MODE + 1; m < SM; ++m) {
 #pragma unroll
@@ -1477,7 +1546,7 @@ __global__ __launch_bounds__(256) void k_minv_sub(MinvArgsT<T> a) {
                 const T c = ms * a.lump[a.o_sfms + (MODE * SM + m) * 3 + j];
                 r[m] -= c * uh[j];
             }
-            ms *= sign[0];
+            ms *= s0;
         }
     }
 #pragma unroll

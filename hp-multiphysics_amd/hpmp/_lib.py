@@ -29,7 +29,7 @@ EXPORTS = (
     "hpg_create", "hpg_destroy", "hpg_set_state", "hpg_get_state", "hpg_get_residual", "hpg_set_residual",
     "hpg_get_preconditioner", "hpg_set_tprcn", "hpg_rsdl", "hpg_setup_preconditioner", "hpg_minvrt",
     "hpg_update", "hpg_maxres", "hpg_resnorm", "hpg_mg_restrict", "hpg_mg_prolongate", "hpg_get_log2p",
-    "hpg_set_precision", "hpg_set_preconditioner", "hpg_set_cfl", "hpg_set_curved", "hpg_set_edge_flux", "hpg_set_cycle_graph", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
+    "hpg_set_precision", "hpg_set_preconditioner", "hpg_set_cfl", "hpg_set_curved", "hpg_set_edge_flux", "hpg_set_periodic", "hpg_set_cycle_graph", "hpg_synchronize", "hpg_last_error", "hpg_basis_operator",
     "hpg_selftest_mfma", "hpg_kernel_timing", "hpg_kernel_timing_read", "hpg_selftest_mfma_f32", "hpg_selftest_mfma_bf16",
     "hpg_set_defect_inner", "hpg_apply_jacobian", "hpg_residual_fp32", "hpg_element_jacobian", "hpg_mlp_create", "hpg_mlp_destroy", "hpg_mlp_num_params",
     "hpg_mlp_get_weights", "hpg_mlp_set_weights", "hpg_mlp_forward", "hpg_mlp_loss_grad", "hpg_mlp_train",
@@ -131,6 +131,7 @@ def load():
     L.hpg_set_cycle_graph.argtypes = [vp, C.c_int]
     L.hpg_set_curved.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.hpg_set_edge_flux.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.hpg_set_periodic.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.hpg_last_error.argtypes = [vp]
     L.hpg_last_error.restype = C.c_char_p
     L.hpg_selftest_mfma.argtypes = [dp, dp, dp]

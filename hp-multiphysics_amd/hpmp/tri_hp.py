@@ -203,6 +203,14 @@ class TriHpIns:
         a = np.ascontiguousarray(cfl, dtype=np.float64)
         self._check(self.L.hpg_set_cfl(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), int(a.size)), "set_cfl")
 
+    def set_periodic(self, pairs):
+        """Periodic boundaries (ins): ``pairs`` of edge boundaries (b1, b2), side j of b1
+        matched with side n-1-j of b2 (the reference's prdc pairing).  Call before
+        set_edge_flux and set_curved; makes the state's copies equal."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        self._check(self.L.hpg_set_periodic(self.h, int(pr.shape[0]), pr.ctypes.data_as(C.POINTER(C.c_int))),
+                    "set_periodic")
+
     def set_edge_flux(self, types, ibc=(0.0, 0.0, 0.0)):
         """Boundary flux of each edge boundary (ins): HPG_EDGE_GENERIC (inflow, outflow,
         symmetry), HPG_EDGE_EULER or HPG_EDGE_CHARACTERISTIC, and the freestream state

@@ -188,6 +188,21 @@ int hpg_set_curved(hpg_ctx *ctx, int nside, const int *sides, const double *crds
  * hpg_params.bc_pressure.  Default: every boundary generic, velocity (0, 0).  Returns
  * -2 for an unknown type, fewer types than edge boundaries or a non-finite state. */
 enum { HPG_EDGE_GENERIC = 0, HPG_EDGE_EULER = 1, HPG_EDGE_CHARACTERISTIC = 2 };
+/* Periodic boundaries (ins), the reference's prdc edge boundaries (tri_mesh/tri_boundary.h:
+ * 113-148, getnewbdry.cpp:144-146): pairs[2k], pairs[2k+1] are two edge boundaries, side j
+ * of the first matched with side n-1-j of the second as the comm boundaries pair them
+ * (tri_mesh/tri_boundary.cpp:344-470); the second must be the first moved by one
+ * translation.  The reference keeps a partial residual on each copy and averages the
+ * preconditioner sums and the solved vertex and side values across the pair
+ * (tri_hp/setup_preconditioner.cpp:24-36, 171-176, 255-258, 331-334); here every copy
+ * holds the assembled residual, which gives the same update.  Residuals read back hold
+ * the assembled value on every copy (the reference's are the partial sums of each side of
+ * the pair).  The state's copies are made equal (from the first boundary's sides) here
+ * and by every hpg_set_state.  The periodic boundaries carry no flux and no essential
+ * mask.  Call before hpg_set_edge_flux and hpg_set_curved, once; not on h-multigrid
+ * levels.  The linearised and Jacobian-probe entry points return -2 on periodic
+ * contexts.  Returns -2 for boundaries that do not pair (side counts, a translation). */
+int hpg_set_periodic(hpg_ctx *ctx, int npair, const int *pairs);
 int hpg_set_edge_flux(hpg_ctx *ctx, int nebd, const int *types, const double *ibc);
 /* J(u) delta at the current state through the FP32 linearised kernels (finest
  * level; overwrites the residual arrays with R(u)).  No reference counterpart;

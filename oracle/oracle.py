@@ -77,6 +77,7 @@ def lib():
         L.otri_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
         L.otri_set_curved.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
         L.otri_set_edge_flux.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.otri_set_periodic.argtypes = [C.c_void_p, C.c_int, _ip]
         L.otri_edge_flux.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.ocns_link_fine.argtypes = [C.c_void_p, C.c_void_p, _ip, _dp, _ip, _dp]
         L.otri_get_mg.argtypes = [C.c_void_p, _dp, _dp]
@@ -251,6 +252,12 @@ class TriHpIns:
         v, s, i = self._alloc()
         lib().otri_get_res(self.h, v, self._ptr(s), self._ptr(i))
         return v, s, i
+
+    def set_periodic(self, pairs):
+        """Periodic pairs of edge boundaries (the reference's prdc pairing)."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        if lib().otri_set_periodic(self.h, pr.size // 2, pr) != 0:
+            raise RuntimeError("otri_set_periodic failed")
 
     def set_edge_flux(self, types, ibc):
         """Flux type of each edge boundary (0 generic, 1 euler, 2 characteristic) and the

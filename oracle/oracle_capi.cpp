@@ -116,6 +116,7 @@ void otri_set_state(void* h, const double* v, const double* s, const double* i) 
     std::memcpy(x.ugv.data(), v, x.ugv.size() * 8);
     if (!x.ugs.empty()) std::memcpy(x.ugs.data(), s, x.ugs.size() * 8);
     if (!x.ugi.empty()) std::memcpy(x.ugi.data(), i, x.ugi.size() * 8);
+    x.prdc_sync();
 }
 void otri_get_state(void* h, double* v, double* s, double* i) {
     TriHpIns& x = T(h);
@@ -169,6 +170,16 @@ void otri_element_jacobian(void* h, int tind, double eps_r, double eps_a, double
 
 // curved boundary sides (ins): crds (nside, smf, 2) as hp_edge_bdry::crds
 void otri_set_curved(void* h, int nside, const int* sides, const double* crds) { T(h).set_curved(nside, sides, crds); }
+
+// periodic pairs of edge boundaries (ins)
+int otri_set_periodic(void* h, int npair, const int* pairs) {
+    try {
+        T(h).set_periodic(npair, pairs);
+    } catch (...) {
+        return -1;
+    }
+    return 0;
+}
 
 // boundary flux types per edge boundary and the freestream state (u, v, p)
 void otri_set_edge_flux(void* h, int nebd, const int* types, const double* ibc) {

@@ -92,6 +92,85 @@ void TriHp<NV>::crdtocht1d(int sind, double cht[2][8]) const {
     }
 }
 
+template <int NV>
+void TriHp<NV>::set_periodic(int npair, const int* pairs) {
+    const int nebd = int(m.ebdry_start.size()) - 1;
+    std::vector<int> parent(m.npnt);
+    for (int v = 0; v < m.npnt; ++v) parent[v] = v;
+    auto find = [&](int v) { while (parent[v] != v) v = parent[v] = parent[parent[v]]; return v; };
+    prdc_bdry.assign(nebd, 0);
+    for (int k = 0; k < npair; ++k) {
+        const int b1 = pairs[2 * k], b2 = pairs[2 * k + 1];
+        prdc_bdry[b1] = prdc_bdry[b2] = 1;
+        m.ebdry_essential[b1] = m.ebdry_essential[b2] = 0;
+        const int o1 = m.ebdry_start[b1], o2 = m.ebdry_start[b2], n = m.ebdry_start[b1 + 1] - o1;
+        if (n != m.ebdry_start[b2 + 1] - o2) throw std::runtime_error("periodic pair: side counts differ");
+        for (int j = 0; j < n; ++j) {
+            const int sa = m.ebdry_seg[o1 + j], sb = m.ebdry_seg[o2 + n - 1 - j];
+            const int a0 = m.seg_pnt[2 * sa], a1 = m.seg_pnt[2 * sa + 1];
+            const int q0 = m.seg_pnt[2 * sb], q1 = m.seg_pnt[2 * sb + 1];
+            auto dx = [&](int p, int q, int r, int t, int c) {
+                return (m.pnts[2 * p + c] - m.pnts[2 * q + c]) - (m.pnts[2 * r + c] - m.pnts[2 * t + c]);
+            };
+            const double L = std::hypot(m.pnts[2 * a1] - m.pnts[2 * a0], m.pnts[2 * a1 + 1] - m.pnts[2 * a0 + 1]);
+            int m0, m1, o;
+            if (std::hypot(dx(q1, a0, q0, a1, 0), dx(q1, a0, q0, a1, 1)) <= 1e-8 * L) { m0 = q1; m1 = q0; o = -1; }
+            else if (std::hypot(dx(q0, a0, q1, a1, 0), dx(q0, a0, q1, a1, 1)) <= 1e-8 * L) { m0 = q0; m1 = q1; o = 1; }
+            else throw std::runtime_error("periodic pair: sides are not translates of each other");
+            prdc_a.push_back(sa); prdc_b.push_back(sb); prdc_o.push_back(o);
+            parent[find(a0)] = find(m0);
+            parent[find(a1)] = find(m1);
+        }
+    }
+    std::vector<std::vector<int>> mem(m.npnt);
+    for (int v = 0; v < m.npnt; ++v) mem[find(v)].push_back(v);
+    prdc_groups.clear();
+    for (int v = 0; v < m.npnt; ++v)
+        if (mem[v].size() > 1) prdc_groups.push_back(mem[v]);
+    prdc_sync();
+}
+
+template <int NV>
+void TriHp<NV>::prdc_sync() {
+    for (const auto& g : prdc_groups)
+        for (size_t k = 1; k < g.size(); ++k)
+            for (int n = 0; n < NV; ++n) ugv[g[k] * NV + n] = ugv[g[0] * NV + n];
+    for (size_t k = 0; k < prdc_a.size(); ++k)
+        for (int mm = 0; mm < smf; ++mm)
+            for (int n = 0; n < NV; ++n)
+                ugs[(prdc_b[k] * smf + mm) * NV + n] =
+                    ((mm & 1) && prdc_o[k] < 0 ? -1.0 : 1.0) * ugs[(prdc_a[k] * smf + mm) * NV + n];
+}
+
+// comm_finish with boundary::average (tri_mesh/boundary.h:636-656): own value plus the
+// partners', times 1/matches, the same on every copy
+template <int NV>
+void TriHp<NV>::vexchange(double* data) {
+    for (const auto& g : prdc_groups)
+        for (int n = 0; n < NV; ++n) {
+            double sum = 0.0;
+            for (int v : g) sum += data[v * NV + n];
+            sum *= 1. / g.size();
+            for (int v : g) data[v * NV + n] = sum;
+        }
+}
+
+// side pairs: the partner's odd modes change sign when the two copies run opposite
+// ways (hp_edge_bdry::smatchsolution_rcv, tri_hp/hp_boundary.cpp:748-790)
+template <int NV>
+void TriHp<NV>::sexchange(double* data, int stride, int mode, bool sign) {
+    for (size_t k = 0; k < prdc_a.size(); ++k) {
+        const double sg = (sign && (mode & 1) && prdc_o[k] < 0) ? -1.0 : 1.0;
+        for (int n = 0; n < NV; ++n) {
+            double& a = data[(prdc_a[k] * stride + mode) * NV + n];
+            double& bb = data[(prdc_b[k] * stride + mode) * NV + n];
+            const double avg = (a + sg * bb) * 0.5;
+            a = avg;
+            bb = sg * avg;
+        }
+    }
+}
+
 // tri_hp/gtol.cpp:14-55
 template <int NV>
 void TriHp<NV>::ugtouht(int tind) { ugtouht_to(tind, uht); }
@@ -263,6 +342,9 @@ void TriHp<NV>::assemble_prcn(int ntri_done) {
                 for (int n = 0; n < NV; ++n) sprcn[side * NV + n] += tprcn[tind * NV + n];
             }
         }
+    // tri_hp/setup_preconditioner.cpp:24-36: the sums averaged over periodic copies
+    vexchange(vprcn.data());
+    if (log2p) sexchange(sprcn.data(), 1, 0, false);
     for (int i = 0; i < m.npnt * NV; ++i) vprcn[i] = 1.0 / (B.vdiag * vprcn[i]);
     if (log2p)
         for (int i = 0; i < m.nseg * NV; ++i) sprcn[i] = 1.0 / sprcn[i];
@@ -333,6 +415,7 @@ void TriHp<NV>::minvrt() {
             }
     }
     for (int i = 0; i < m.npnt * NV; ++i) resv[i] *= vprcn[i];
+    vexchange(resv.data());  // setup_preconditioner.cpp:171-176
     vdirichlet();
     if (sm == 0) return;
 
@@ -359,6 +442,7 @@ void TriHp<NV>::minvrt() {
     for (int mode = 0; mode < sm - 1; ++mode) {
         for (int s = 0; s < m.nseg; ++s)
             for (int n = 0; n < NV; ++n) ress[(s * smf + mode) * NV + n] *= sprcn[s * NV + n] * B.sdiag[mode];
+        sexchange(ress.data(), smf, mode, true);  // :255-258
         sdirichlet(mode);
         for (int tind = 0; tind < m.ntri; ++tind) {
             int side[3], sign[3];
@@ -385,6 +469,7 @@ void TriHp<NV>::minvrt() {
     const int mode = sm - 1;
     for (int s = 0; s < m.nseg; ++s)
         for (int n = 0; n < NV; ++n) ress[(s * smf + mode) * NV + n] *= sprcn[s * NV + n] * B.sdiag[mode];
+    sexchange(ress.data(), smf, mode, true);  // :331-334
     sdirichlet(mode);
     if (im == 0) return;
 

@@ -16,6 +16,8 @@
 //   cycle                    tri_mesh/blocks.cpp:1267-1348 (one grid, extra_finest_levels)
 //   area, inscribedradius    tri_mesh/tprims.cpp:66-80, 224-242
 //   crdtocht / crdtocht1d    tri_hp/gtol.cpp:204-245, 296-320 (curved boundary sides)
+//   periodic exchanges       tri_hp/setup_preconditioner.cpp:24-36, 171-176, 255-258, 331-334,
+//                            tri_mesh/tri_boundary.cpp:344-470, tri_mesh/boundary.h:594-660
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it.
 #pragma once
@@ -85,6 +87,20 @@ class TriHp {
     bool curved(int t) const { return !tri_crv.empty() && tri_crv[t]; }
     void crdtocht(int tind, double cht[2][15]) const;
     void crdtocht1d(int sind, double cht[2][8]) const;
+    // periodic boundaries (prdc, tri_mesh/tri_boundary.h:113-148): side j of the first
+    // edge boundary of a pair matched with side n-1-j of the second (the comm
+    // boundaries' reverse order, tri_mesh/tri_boundary.cpp:344-470); vertex copies and
+    // side pairs (a, b, relative direction).  Every copy keeps its partial residual; the
+    // preconditioner sums and the solved vertex and side values are averaged over the
+    // copies where tri_hp exchanges them (tri_hp/setup_preconditioner.cpp:24-36, 171-176,
+    // 255-258, 331-334); the periodic boundaries carry no flux and no essential mask.
+    std::vector<std::vector<int>> prdc_groups;
+    std::vector<int> prdc_a, prdc_b, prdc_o;
+    std::vector<char> prdc_bdry;
+    void set_periodic(int npair, const int* pairs);
+    void prdc_sync();                         // state copies from the first copy / side a
+    void vexchange(double* data);             // average over each vertex group, NV per vertex
+    void sexchange(double* data, int stride, int mode, bool sign);  // side pairs, one mode
 
     // solution and work arrays, reference layouts v(npnt,NV), s(nseg,sm,NV), i(ntri,im,NV)
     std::vector<double> ugv, ugs, ugi;

@@ -279,7 +279,9 @@ void ins_edge_flux(int type, const double u[3], const double nrm[2], double side
 void TriHpIns::edge_rsdl() {
     const TriBasis& B = b();
     const int nebd = int(m.ebdry_start.size()) - 1;
-    for (int e = 0; e < nebd; ++e)
+    for (int e = 0; e < nebd; ++e) {
+        // a periodic boundary is a comm boundary: no flux of its own
+        if (!prdc_bdry.empty() && prdc_bdry[e]) continue;
         for (int j = m.ebdry_start[e]; j < m.ebdry_start[e + 1]; ++j) {
             const int sind = m.ebdry_seg[j];
             const int v0 = m.seg_pnt[2 * sind], v1 = m.seg_pnt[2 * sind + 1];
@@ -309,6 +311,7 @@ void TriHpIns::edge_rsdl() {
             for (int k = 0; k < B.sm; ++k)
                 for (int n = 0; n < NV; ++n) ress[(sind * smf + k) * NV + n] += lfl[n][k + 2];
         }
+    }
 }
 
 // tri_hp/ins/setup_preconditioner.cpp:9-221 (diagonal, REFINED_WAY, curved and straight)
