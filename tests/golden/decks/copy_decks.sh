@@ -6,3 +6,4 @@ set -e
 R=/root/reference/Testing/tri_hp
 D=$(dirname "$0")
 cp "$R/ins/Convergence/Inputs/run_master.inpt" "$D/ins_convergence_run_master.inpt"
+cp "$R/ins/Restart Test/Inputs/run.inpt" "$D/ins_restart_test_run.inpt"
