@@ -117,7 +117,7 @@ __device__ __forceinline__ void elem_tet_body(const TetElemArgsT<T>& a, int noct
     using D = TDims<P>;
     constexpr int TM = D::TM, G = D::G;
     __shared__ __attribute__((aligned(16))) T sops[D::SIZE];
-    __shared__ T suh[8][TNV][16];
+    __shared__ T suh[8][TNV][17];  // padded: the two half-waves' rows fall on different LDS banks
     __shared__ __attribute__((aligned(16))) T sf[8][12][D::FROW];  // point flux rows, padded
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;

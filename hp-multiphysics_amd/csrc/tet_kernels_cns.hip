@@ -412,7 +412,7 @@ __device__ __forceinline__ void elem_tetc_body(const TetcElemArgsT<T>& a, int no
     using D = TDims<P>;
     constexpr int TM = D::TM, G = D::G;
     __shared__ __attribute__((aligned(16))) T sops[D::SIZE];
-    __shared__ T suh[8][CNV][16];
+    __shared__ T suh[8][CNV][17];  // padded: the two half-waves' rows fall on different LDS banks
     __shared__ __attribute__((aligned(16))) T sf[8][15][D::FROW];  // point flux rows, padded
     __shared__ T stau[8][25];
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];

@@ -316,8 +316,11 @@ __device__ __forceinline__ void elem_cns_body(const CnsElemArgsT<T>& a, int noct
     constexpr int TM = D::TM, G = D::G, NO = NVC * TM;
     constexpr int SMD = D::SM > 0 ? D::SM : 1;
     __shared__ T sops[D::SIZE];
-    __shared__ T suh[8][NVC][16];
-    __shared__ T sf[8][8][32];
+    // rows padded by one word: the lanes of a half-wave read different variables' rows at
+    // one Gauss point, and the two half-waves of a wave read their own buffers, which
+    // unpadded sit a multiple of the LDS bank period apart
+    __shared__ T suh[8][NVC][17];
+    __shared__ T sf[8][8][33];
     for (int k = threadIdx.x; k < D::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     const int hw = threadIdx.x >> 5, g = threadIdx.x & 31;
     const T gm1 = T(a.ph.gamma) - T(1.0), gogm1 = T(a.ph.gamma) / gm1;
@@ -967,8 +970,9 @@ __global__ __launch_bounds__(256) void k_pnv_int(CnsProjArgs a, int noct) {
     constexpr int TM = D::TM, G = D::G, IM = D::IM, BM = D::BM, SM = D::SM;
     __shared__ double sP[TM * 32];
     __shared__ double sI[G * 16];
-    __shared__ double sc[8][3][NVC][16];  // u, conservative correction, primitive boundary correction
-    __shared__ double sf[8][NVC][32];
+    // rows padded by one word (LDS banks: see elem_cns_body)
+    __shared__ double sc[8][3][NVC][17];  // u, conservative correction, primitive boundary correction
+    __shared__ double sf[8][NVC][33];
     __shared__ double so[8][NVC][IM];
     __shared__ double sL[8][3][16];   // the vertices' 4x4 blocks, loaded once per element
     __shared__ double sid[2 * (IM > 0 ? IM : 1)];
