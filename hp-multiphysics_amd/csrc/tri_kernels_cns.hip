@@ -541,8 +541,10 @@ __device__ __forceinline__ void elem_cns_body(const CnsElemArgsT<T>& a, int noct
     }
 }
 
+// two waves per SIMD: at p = 4 the full residual needs 295 registers unbounded (one wave
+// per SIMD, its LDS round trips unhidden); bounded to 256 it spills 172 bytes per lane
 template <int P, bool FULL>
-__global__ __launch_bounds__(256) void k_elem_cns(CnsElemArgs a, int noct) {
+__global__ __launch_bounds__(256, 2) void k_elem_cns(CnsElemArgs a, int noct) {
     elem_cns_body<double, P, FULL>(a, noct);
 }
 
