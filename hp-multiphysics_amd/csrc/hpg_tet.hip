@@ -12,6 +12,8 @@ namespace {
 struct TetLevel {
     int p = 0;
     double* ops = nullptr;   // TDims<p>::SIZE
+    double* opsm = nullptr;  // TMDims<p>::SIZE, the matrix-core kernel's operators (p = 2)
+    float* opsm32 = nullptr;  // the same for the FP32 matrix-core kernel (its K order)
     float* ops32 = nullptr;  // the same operators in FP32 (defect-correction inner solve)
     double* P2 = nullptr;    // face values GF x tm2d
     double* W2 = nullptr;    // face weights GF
@@ -37,6 +39,48 @@ std::vector<double> build_tet_pack(const TetOps& o) {
     for (int op = 0; op < 3; ++op)
         for (int q = 0; q < D::G; ++q)
             for (int g = 0; g < D::G; ++g) pk[D::OP_DR + op * D::G * D::DQ + g * D::DQ + q] = (*der[op])[g * D::G + q];
+    return pk;
+}
+
+// operators of k_elem_tet_mfma as lane-ready A blocks: block (tile t, K-step k) lane l
+// holds A[16 t + l % 16][K index of lane l].  The projections' K index is the mode
+// 4 k + l / 16; the derivatives' and integrations' K index is the Gauss point that
+// accumulator register k % 4 of tile k / 4 holds in lane l: 16 (k / 4) + 4 (k % 4) +
+// l / 16 for the f64 matrix cores, 16 (k / 4) + 4 (l / 16) + k % 4 for the f32 ones
+// (f32 = true).  Rows and K indices beyond the G points and TM modes are zero.
+template <int P>
+std::vector<double> build_tet_mfma_pack(const TetOps& o, bool f32 = false) {
+    using D = TDims<P>;
+    using M = TMDims<P>;
+    std::vector<double> pk(M::SIZE, 0.0);
+    auto kpoint = [f32](int k, int l) {
+        return f32 ? 16 * (k / 4) + 4 * (l / 16) + k % 4 : 16 * (k / 4) + 4 * (k % 4) + l / 16;
+    };
+    const std::vector<double>* proj[4] = {&o.P, &o.Pr, &o.Ps, &o.Pt};
+    for (int op = 0; op < 4; ++op)
+        for (int t = 0; t < M::GT; ++t)
+            for (int k = 0; k < M::KT; ++k)
+                for (int l = 0; l < 64; ++l) {
+                    const int g = 16 * t + l % 16, m = 4 * k + l / 16;
+                    if (g < D::G && m < D::TM)
+                        pk[M::OFF_PROJ + ((op * M::GT + t) * M::KT + k) * 64 + l] = (*proj[op])[g * D::TM + m];
+                }
+    const std::vector<double>* der[3] = {&o.Dr, &o.Ds, &o.Dt};
+    for (int op = 0; op < 3; ++op)
+        for (int t = 0; t < M::GT; ++t)
+            for (int k = 0; k < M::QK; ++k)
+                for (int l = 0; l < 64; ++l) {
+                    const int g = 16 * t + l % 16, h = kpoint(k, l);
+                    if (g < D::G && h < D::G)
+                        pk[M::OFF_DER + ((op * M::GT + t) * M::QK + k) * 64 + l] = (*der[op])[g * D::G + h];
+                }
+    const std::vector<double>* integ[3] = {&o.Ir, &o.Is, &o.It};
+    for (int op = 0; op < 3; ++op)
+        for (int k = 0; k < M::QK; ++k)
+            for (int l = 0; l < 64; ++l) {
+                const int m = l % 16, h = kpoint(k, l);
+                if (m < D::TM && h < D::G) pk[M::OFF_INT + (op * M::QK + k) * 64 + l] = (*integ[op])[m * D::G + h];
+            }
     return pk;
 }
 
@@ -162,7 +206,20 @@ void launch_elem_tet(hpg_tet_ctx* c, int stage, const int* elist = nullptr, int 
     const int noct = (count + 7) / 8;
     const size_t r = c->recs.size();
     if (c->timing) check(hipEventRecord(c->event(2 * r), c->stream), "hipEventRecord");
-    if (beta > 0.0) {
+    if (P == 2) {
+        // matrix-core form: groups of 16 elements, four groups per block
+        a.ops = c->lv[c->log2p].opsm;
+        const int ngrp = (count + 15) / 16;
+        if (beta > 0.0) {
+            static const int cap = resident_blocks((const void*)k_elem_tet_mfma<2, true>);
+            hipLaunchKernelGGL((k_elem_tet_mfma<2, true>), dim3(persistent_grid((ngrp + 3) / 4, cap)), dim3(256), 0,
+                               c->stream, a, ngrp);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_tet_mfma<2, false>);
+            hipLaunchKernelGGL((k_elem_tet_mfma<2, false>), dim3(persistent_grid((ngrp + 3) / 4, cap)), dim3(256), 0,
+                               c->stream, a, ngrp);
+        }
+    } else if (beta > 0.0) {
         static const int cap = resident_blocks((const void*)k_elem_tet<P, true>);
         hipLaunchKernelGGL((k_elem_tet<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
     } else {
@@ -200,7 +257,20 @@ void launch_elem_tet32(hpg_tet_ctx* c, int stage) {
         a.pnts = c->pnts; a.tet_pnt = c->tet_pnt; a.tet_seg = c->tet_seg; a.tet_sgn = nullptr;
         a.ntet = count; a.elist = nullptr; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
         a.ops = c->lv[c->log2p].ops32; a.rho = c->prm.rho; a.mu = c->prm.mu; a.beta = beta;
-        if (beta > 0.0) {
+        if (P == 2) {
+            // matrix-core form: groups of 16 elements, four groups per block
+            a.ops = c->lv[c->log2p].opsm32;
+            const int ngrp = (count + 15) / 16;
+            if (beta > 0.0) {
+                static const int cap = resident_blocks((const void*)k_elem_tet32_mfma<2, true>);
+                hipLaunchKernelGGL((k_elem_tet32_mfma<2, true>), dim3(persistent_grid((ngrp + 3) / 4, cap)), dim3(256),
+                                   0, c->stream, a, ngrp);
+            } else {
+                static const int cap = resident_blocks((const void*)k_elem_tet32_mfma<2, false>);
+                hipLaunchKernelGGL((k_elem_tet32_mfma<2, false>), dim3(persistent_grid((ngrp + 3) / 4, cap)), dim3(256),
+                                   0, c->stream, a, ngrp);
+            }
+        } else if (beta > 0.0) {
             static const int cap = resident_blocks((const void*)k_elem_tet32<P, true>);
             hipLaunchKernelGGL((k_elem_tet32<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
         } else {
@@ -553,6 +623,13 @@ int hpg_tet_create(const hpg_tet_mesh* m, const hpg_params* prm, hpg_tet_ctx** o
             L.ops = c->upload(pk.data(), pk.size());
             const std::vector<float> pk32(pk.begin(), pk.end());
             L.ops32 = c->upload(pk32.data(), pk32.size());
+            if (o.p == 2) {
+                const std::vector<double> pm = build_tet_mfma_pack<2>(o);
+                L.opsm = c->upload(pm.data(), pm.size());
+                const std::vector<double> pm32d = build_tet_mfma_pack<2>(o, true);
+                const std::vector<float> pm32(pm32d.begin(), pm32d.end());
+                L.opsm32 = c->upload(pm32.data(), pm32.size());
+            }
             L.P2 = c->upload(o.P2.data(), o.P2.size());
             L.W2 = c->upload(o.W2.data(), o.W2.size());
             L.Pg = c->upload(o.P.data(), o.P.size());

@@ -303,6 +303,283 @@ __global__ __launch_bounds__(256) void k_elem_tet32(TetElemArgs32 a, int noct) {
     elem_tet_body<float, P, FULL>(a, noct);
 }
 
+// ---------------------------------------------------------------------------------
+// Matrix-core form of the FP64 element residual (p = 2): 16 tetrahedra per wave64,
+// every transform a contraction on v_mfma_f64_16x16x4f64 with the layout of the tri
+// kernels: columns are the wave's 16 elements, rows Gauss points (27 in two 16-row
+// tiles) or modes (10 in one tile), A operands lane-ready from LDS, B operands the
+// accumulator registers of the previous contraction (register i of lane l holds row
+// 4 i + l / 16 of its tile, which is the B operand of K-step 4 tile + i when the
+// operator's K index is laid out that way).  Point fluxes are formed in the lane that
+// holds the point, right before the contraction that consumes them, so no field is
+// exchanged between lanes and nothing but the operators sits in LDS.
+template <int P>
+struct TMDims {
+    using D = TDims<P>;
+    static constexpr int GT = (D::G + 15) / 16;  // Gauss-point tiles
+    static constexpr int KT = (D::TM + 3) / 4;   // K-steps over the modes
+    static constexpr int QK = 4 * GT;            // K-steps over the Gauss points
+    // lane-ready A blocks of 64 doubles: P, Pr, Ps, Pt [op][tile][K-step];
+    // Dr, Ds, Dt [op][tile][K-step]; Ir, Is, It [op][K-step]
+    static constexpr int OFF_PROJ = 0;
+    static constexpr int OFF_DER = 4 * GT * KT * 64;
+    static constexpr int OFF_INT = OFF_DER + 3 * GT * QK * 64;
+    static constexpr int SIZE = OFF_INT + 3 * QK * 64;
+    static_assert(D::TM <= 16, "the modes fit one tile");
+};
+
+// tet_hp_ins::element_rsdl, straight branch (tet_hp/ins/rsdl.cpp:15-87, 442-785),
+// steady, tau frozen by setup_preconditioner; writes lf like elem_tet_body.  ngrp
+// groups of 16 elements; the grid is a multiple of 8 blocks and XCD x walks the
+// contiguous group range [x * gpx, (x + 1) * gpx) (neighbouring elements share its L2)
+template <typename T, int P, bool FULL>
+__device__ __forceinline__ void elem_tet_mfma_body(const TetElemArgsT<T>& a, int ngrp) {
+    using D = TDims<P>;
+    using M = TMDims<P>;
+    using V4 = typename Vec4<T>::type;
+    constexpr int GT = M::GT, KT = M::KT, QK = M::QK, TM = D::TM;
+    // output row of accumulator register i in lane l: 4 i + l / 16 (f64), 4 (l / 16) + i (f32)
+    constexpr bool F64 = sizeof(T) == 8;
+    __shared__ T sops[M::SIZE];
+    for (int k = threadIdx.x; k < M::SIZE; k += blockDim.x) sops[k] = a.ops[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = lane & 15, q = lane >> 4;
+    const int WX = ((int)gridDim.x >> 3) * 4;
+    const int rank = ((int)blockIdx.x >> 3) * 4 + w;
+    const int xcd = (int)blockIdx.x & 7;
+    const int gpx = (ngrp + 7) >> 3;
+    const int gend = min(ngrp, (xcd + 1) * gpx);
+    const V4 z = {T(0), T(0), T(0), T(0)};
+    for (int grp = xcd * gpx + rank; grp < gend; grp += WX) {
+        const int slot = grp * 16 + e;
+        const bool valid = slot < a.ntet;
+        const int tt = valid ? (a.elist ? a.elist[slot] : slot) : 0;
+        // modal coefficients: K-step s of lane l is mode 4 s + l / 16 (vertices 0-3,
+        // then edge mode 0 of edges 0-5, sign^0 = 1); padding elements are zero
+        T b[TNV][KT];
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+            const int k = 4 * s + q;
+            const double* src = nullptr;
+            if (valid && k < TM)
+                src = k < 4 ? a.ugv + (size_t)a.tet_pnt[4 * tt + k] * TNV
+                            : a.uge + (size_t)a.tet_seg[6 * tt + (k - 4)] * a.em0 * TNV;
+#pragma unroll
+            for (int n = 0; n < TNV; ++n) b[n][s] = src ? T(src[n]) : T(0);
+        }
+        // coordinate derivatives (rsdl.cpp:58-62) and cofactors (:449-458); padding
+        // elements take the reference tetrahedron
+        T X[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int v = a.tet_pnt[4 * tt + k];
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                X[k][n] = valid ? T(a.pnts[3 * v + n]) : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? T(1) : T(-1));
+        }
+        T ld[3][3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            ld[n][0] = T(0.5) * (X[3][n] - X[2][n]);
+            ld[n][1] = T(0.5) * (X[1][n] - X[2][n]);
+            ld[n][2] = T(0.5) * (X[0][n] - X[2][n]);
+        }
+        T d[3][3];
+        d[0][0] = ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1];
+        d[0][1] = -ld[0][1] * ld[2][2] + ld[0][2] * ld[2][1];
+        d[0][2] = ld[0][1] * ld[1][2] - ld[0][2] * ld[1][1];
+        d[1][0] = -ld[1][0] * ld[2][2] + ld[1][2] * ld[2][0];
+        d[1][1] = ld[0][0] * ld[2][2] - ld[0][2] * ld[2][0];
+        d[1][2] = -ld[0][0] * ld[1][2] + ld[0][2] * ld[1][0];
+        d[2][0] = ld[1][0] * ld[2][1] - ld[1][1] * ld[2][0];
+        d[2][1] = -ld[0][0] * ld[2][1] + ld[0][1] * ld[2][0];
+        d[2][2] = ld[0][0] * ld[1][1] - ld[0][1] * ld[1][0];
+        const T cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
+                            ld[0][1] * (ld[1][0] * ld[2][2] - ld[2][0] * ld[1][2]) +
+                            ld[0][2] * (ld[1][0] * ld[2][1] - ld[2][0] * ld[1][1]);
+        const T rho = T(a.rho);
+        // values at the Gauss points (proj)
+        V4 U[TNV][GT];
+#pragma unroll
+        for (int mt = 0; mt < GT; ++mt) {
+            V4 c0 = z, c1 = z, c2 = z, c3 = z;
+#pragma unroll
+            for (int s = 0; s < KT; ++s) {
+                const T av = sops[M::OFF_PROJ + (mt * KT + s) * 64 + lane];
+                c0 = mfma(av, b[0][s], c0); c1 = mfma(av, b[1][s], c1);
+                c2 = mfma(av, b[2][s], c2); c3 = mfma(av, b[3][s], c3);
+            }
+            U[0][mt] = c0; U[1][mt] = c1; U[2][mt] = c2; U[3][mt] = c3;
+        }
+        // convective and pressure fluxes at point (tile kt, register i) (rsdl.cpp:461-508)
+        auto conv = [&](int kt, int i, T cv[3][3], T c3[3]) {
+            const T u0 = U[0][kt][i], u1 = U[1][kt][i], u2 = U[2][kt][i], pr = U[3][kt][i];
+            const T fx = rho * u0, fy = rho * u1, fz = rho * u2;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) c3[c] = d[c][0] * fx + d[c][1] * fy + d[c][2] * fz;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cv[0][c] = u0 * c3[c] + d[c][0] * pr;
+                cv[1][c] = u1 * c3[c] + d[c][1] * pr;
+                cv[2][c] = u2 * c3[c] + d[c][2] * pr;
+            }
+        };
+        // lf_im = intgrtrst of the fluxes (rsdl.cpp:509-511)
+        {
+            V4 L0 = z, L1 = z, L2 = z, L3 = z;
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = 4 * kt + i;
+                    __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
+                    T cv[3][3], c3[3];
+                    conv(kt, i, cv, c3);
+#pragma unroll
+                    for (int dd = 0; dd < 3; ++dd) {
+                        const T ad = sops[M::OFF_INT + (dd * QK + kk) * 64 + lane];
+                        L0 = mfma(ad, cv[0][dd], L0); L1 = mfma(ad, cv[1][dd], L1);
+                        L2 = mfma(ad, cv[2][dd], L2); L3 = mfma(ad, c3[dd], L3);
+                    }
+                }
+            // register i holds mode 4 i + l / 16 (f64) or 4 (l / 16) + i (f32) of element e
+            double* lfo = a.lf + (size_t)tt * (2 * TNV * TM);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = F64 ? 4 * i + q : 4 * q + i;
+                if (valid && m < TM) {
+                    double* o = lfo + m * (2 * TNV);
+                    o[0] = double(L0[i]); o[1] = double(L1[i]); o[2] = double(L2[i]); o[3] = double(L3[i]);
+                    if constexpr (!FULL) { o[4] = 0.0; o[5] = 0.0; o[6] = 0.0; o[7] = 0.0; }
+                }
+            }
+        }
+        if constexpr (FULL) {
+            // viscous fluxes (rsdl.cpp:518-674): visc(n,m)(a,b) = -mu/cjcb (delta_nm
+            // sum_c d(a)(c) d(b)(c) + d(a)(m) d(b)(n)) applied to du(m,b), summed as
+            // -mu/cjcb (sum_b dd(a,b) du(n,b) + sum_m d(a)(m) sum_b d(b)(n) du(m,b))
+            const T cjcbi = T(a.mu) / cjcb;
+            T dd[3][3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) dd[x][y] = d[x][0] * d[y][0] + d[x][1] * d[y][1] + d[x][2] * d[y][2];
+            // one Gauss-point tile at a time: the velocity derivatives du(m, b) of its
+            // points (proj with r, s, t derivatives), the total fluxes of each point into
+            // the strong residual (derivr/derivs/derivt, rsdl.cpp:683-692) and the
+            // viscous fluxes into lf_re (intgrtrst, :768-776; the stabilisation part
+            // follows once the strong residual is complete)
+            V4 R[TNV][GT];
+#pragma unroll
+            for (int n = 0; n < TNV; ++n)
+#pragma unroll
+                for (int mt = 0; mt < GT; ++mt) R[n][mt] = z;
+            V4 L0 = z, L1 = z, L2 = z, L3 = z;
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt) {
+                V4 DU[3][3];
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb) {
+                    V4 c0 = z, c1 = z, c2 = z;
+#pragma unroll
+                    for (int s = 0; s < KT; ++s) {
+                        const T av = sops[M::OFF_PROJ + (((1 + bb) * GT + kt) * KT + s) * 64 + lane];
+                        c0 = mfma(av, b[0][s], c0); c1 = mfma(av, b[1][s], c1); c2 = mfma(av, b[2][s], c2);
+                    }
+                    DU[0][bb] = c0; DU[1][bb] = c1; DU[2][bb] = c2;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = 4 * kt + i;
+                    __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
+                    T cv[3][3], c3[3], df[3][3], gm[3][3];
+                    conv(kt, i, cv, c3);
+#pragma unroll
+                    for (int m = 0; m < 3; ++m)
+#pragma unroll
+                        for (int n = 0; n < 3; ++n)
+                            gm[m][n] = d[0][n] * DU[m][0][i] + d[1][n] * DU[m][1][i] + d[2][n] * DU[m][2][i];
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) {
+                            const T h = dd[x][0] * DU[n][0][i] + dd[x][1] * DU[n][1][i] + dd[x][2] * DU[n][2][i];
+                            df[n][x] = -cjcbi * (h + d[x][0] * gm[0][n] + d[x][1] * gm[1][n] + d[x][2] * gm[2][n]);
+                        }
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const T ai = sops[M::OFF_INT + (x * QK + kk) * 64 + lane];
+                        L0 = mfma(ai, df[0][x], L0); L1 = mfma(ai, df[1][x], L1); L2 = mfma(ai, df[2][x], L2);
+                    }
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) cv[n][c] += df[n][c];
+#pragma unroll
+                    for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) {
+                            const T ad = sops[M::OFF_DER + ((x * GT + mt) * QK + kk) * 64 + lane];
+                            R[0][mt] = mfma(ad, cv[0][x], R[0][mt]); R[1][mt] = mfma(ad, cv[1][x], R[1][mt]);
+                            R[2][mt] = mfma(ad, cv[2][x], R[2][mt]); R[3][mt] = mfma(ad, c3[x], R[3][mt]);
+                        }
+                }
+            }
+            // stabilisation (rsdl.cpp:695-767) into lf_re
+            const T tm = T(a.tau[4 * tt]), tc = T(a.tau[4 * tt + 3]);
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = 4 * kt + i;
+                    __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
+                    T sf[3][3], c3[3];
+                    const T u0 = U[0][kt][i], u1 = U[1][kt][i], u2 = U[2][kt][i];
+                    const T t0 = tm * R[0][kt][i], t1 = tm * R[1][kt][i], t2 = tm * R[2][kt][i],
+                                 t3 = tc * R[3][kt][i];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        sf[0][c] = -((d[c][0] * (T(2) * u0) + d[c][1] * u1 + d[c][2] * u2) * t0 + d[c][1] * u0 * t1 +
+                                     d[c][2] * u0 * t2 + d[c][0] * t3);
+                        sf[1][c] = -(d[c][0] * u1 * t0 + (d[c][0] * u0 + d[c][1] * (T(2) * u1) + d[c][2] * u2) * t1 +
+                                     d[c][2] * u1 * t2 + d[c][1] * t3);
+                        sf[2][c] = -(d[c][0] * u2 * t0 + d[c][1] * u2 * t1 +
+                                     (d[c][0] * u0 + d[c][1] * u1 + d[c][2] * (T(2) * u2)) * t2 + d[c][2] * t3);
+                        c3[c] = -d[c][0] * t0 - d[c][1] * t1 - d[c][2] * t2;
+                    }
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const T ad = sops[M::OFF_INT + (x * QK + kk) * 64 + lane];
+                        L0 = mfma(ad, sf[0][x], L0); L1 = mfma(ad, sf[1][x], L1);
+                        L2 = mfma(ad, sf[2][x], L2); L3 = mfma(ad, c3[x], L3);
+                    }
+                }
+            double* lfo = a.lf + (size_t)tt * (2 * TNV * TM);
+            const T bt = T(a.beta);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = F64 ? 4 * i + q : 4 * q + i;
+                if (valid && m < TM) {
+                    double* o = lfo + m * (2 * TNV) + TNV;
+                    o[0] = double(L0[i] * bt); o[1] = double(L1[i] * bt); o[2] = double(L2[i] * bt); o[3] = double(L3[i] * bt);
+                }
+            }
+        }
+    }
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tet_mfma(TetElemArgs a, int ngrp) {
+    elem_tet_mfma_body<double, P, FULL>(a, ngrp);
+}
+
+// FP32 arithmetic (the inner solve of the defect correction) on v_mfma_f32_16x16x4f32
+template <int P, bool FULL>
+__global__ __launch_bounds__(256, 2) void k_elem_tet32_mfma(TetElemArgs32 a, int ngrp) {
+    elem_tet_mfma_body<float, P, FULL>(a, ngrp);
+}
+
 // inflow face flux, continuity only (tet_hp/ins/bdry.cpp:59-102, bdry_ins.h:122-134):
 // flf[f][m] = integral over the face of phi_m rho u.n, n = dx/dr x dx/ds, which for a
 // straight face is (p2 - p1) x (p0 - p1) / 4 at every Gauss point
