@@ -247,14 +247,14 @@ def tetc_throughput(args) -> dict:
            "unknowns": unknowns, "steps": args.tet_steps,
            "ms_per_step": 1e3 * (t1 - t0) / args.tet_steps,
            "dof_evals_per_s": unknowns * 5 * args.tet_steps / (t1 - t0),
-           "roofline": {"bound": "fp64 valu", "kernel": "k_elem_tetc<2,full>", "achieved": ach,
+           "roofline": {"bound": "mfma", "kernel": "k_elem_tetc_mfma<2,full>", "achieved": ach,
                         "peak": MI355X_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": ach / MI355X_FP64_MFMA_TFLOPS,
                         "avg_launch_ms": tot.value / max(1, nl.value),
                         "elements_per_launch": ne.value / max(1, nl.value), "flop_per_element": FLOP_TETC_FULL_P2}}
     if not args.no_mixed:
         out["mixed"] = _mixed_leg(g, (v, e), args.tet_steps, g.L.hpg_tet_kernel_timing,
                                   g.L.hpg_tet_kernel_timing_read, unknowns, FLOP_TETC_FULL_P2,
-                                  "k_elem_tetc32<2,full>", out["ms_per_step"])
+                                  "k_elem_tetc32_mfma<2,full>", out["ms_per_step"])
     g.close()
     return out
 

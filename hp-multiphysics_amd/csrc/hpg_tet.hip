@@ -175,7 +175,20 @@ void launch_elem_tetc(hpg_tet_ctx* c, double beta, const int* elist, int count) 
     a.ops = c->lv[c->log2p].ops; a.mu = c->prm.mu; a.kcond = c->kcond; a.R = c->prm.Rgas; a.gamma = c->prm.gamma;
     a.atm = 0.0; a.beta = beta;
     const int noct = (count + 7) / 8;
-    if (beta > 0.0) {
+    if (P == 2) {
+        // matrix-core form: groups of 16 elements, four groups per block
+        a.ops = c->lv[c->log2p].opsm;
+        const int ngrp = (count + 15) / 16;
+        if (beta > 0.0) {
+            static const int cap = resident_blocks((const void*)k_elem_tetc_mfma<2, true>);
+            hipLaunchKernelGGL((k_elem_tetc_mfma<2, true>), dim3(persistent_grid((ngrp + 3) / 4, cap)), dim3(256), 0,
+                               c->stream, a, ngrp);
+        } else {
+            static const int cap = resident_blocks((const void*)k_elem_tetc_mfma<2, false>);
+            hipLaunchKernelGGL((k_elem_tetc_mfma<2, false>), dim3(persistent_grid((ngrp + 3) / 4, cap)), dim3(256), 0,
+                               c->stream, a, ngrp);
+        }
+    } else if (beta > 0.0) {
         static const int cap = resident_blocks((const void*)k_elem_tetc<P, true>);
         hipLaunchKernelGGL((k_elem_tetc<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
     } else {
@@ -245,7 +258,19 @@ void launch_elem_tet32(hpg_tet_ctx* c, int stage) {
         a.ntet = count; a.elist = nullptr; a.em0 = c->em0; a.ugv = c->ugv; a.uge = c->uge; a.tau = c->tau; a.lf = c->lf;
         a.ops = c->lv[c->log2p].ops32; a.mu = c->prm.mu; a.kcond = c->kcond; a.R = c->prm.Rgas;
         a.gamma = c->prm.gamma; a.atm = 0.0; a.beta = beta;
-        if (beta > 0.0) {
+        if (P == 2) {
+            a.ops = c->lv[c->log2p].opsm32;
+            const int ngrp = (count + 15) / 16;
+            if (beta > 0.0) {
+                static const int cap = resident_blocks((const void*)k_elem_tetc32_mfma<2, true>);
+                hipLaunchKernelGGL((k_elem_tetc32_mfma<2, true>), dim3(persistent_grid((ngrp + 3) / 4, cap)),
+                                   dim3(256), 0, c->stream, a, ngrp);
+            } else {
+                static const int cap = resident_blocks((const void*)k_elem_tetc32_mfma<2, false>);
+                hipLaunchKernelGGL((k_elem_tetc32_mfma<2, false>), dim3(persistent_grid((ngrp + 3) / 4, cap)),
+                                   dim3(256), 0, c->stream, a, ngrp);
+            }
+        } else if (beta > 0.0) {
             static const int cap = resident_blocks((const void*)k_elem_tetc32<P, true>);
             hipLaunchKernelGGL((k_elem_tetc32<P, true>), dim3(persistent_grid(noct, cap)), dim3(256), 0, c->stream, a, noct);
         } else {

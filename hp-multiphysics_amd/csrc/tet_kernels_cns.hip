@@ -618,6 +618,292 @@ __global__ __launch_bounds__(256) void k_elem_tetc32(TetcElemArgs32 a, int noct)
     elem_tetc_body<float, P, FULL>(a, noct);
 }
 
+// Matrix-core form of the cns element residual (p = 2), the layout of
+// elem_tet_mfma_body with NV = 5 (p, u, v, w, RT): 16 tetrahedra per wave, Gauss-point
+// rows in two 16-row tiles, accumulators fed back as B operands, point fluxes formed in
+// the lane that holds the point.  The element tau (5 x 5) sits in LDS per wave.
+// Padding points and padding elements carry p = RT = 1 so that rho = (p + atm) / RT
+// stays finite (a zero operator column does not cancel an inf in a matrix product).
+template <typename T, int P, bool FULL>
+__device__ __forceinline__ void elem_tetc_mfma_body(const TetcElemArgsT<T>& a, int ngrp) {
+    using D = TDims<P>;
+    using M = TMDims<P>;
+    using V4 = typename Vec4<T>::type;
+    constexpr int GT = M::GT, KT = M::KT, QK = M::QK, TM = D::TM, G = D::G;
+    constexpr bool F64 = sizeof(T) == 8;
+    __shared__ T sops[M::SIZE];
+    __shared__ T stau[4][16][25];
+    for (int k = threadIdx.x; k < M::SIZE; k += blockDim.x) sops[k] = a.ops[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = lane & 15, q = lane >> 4;
+    const int WX = ((int)gridDim.x >> 3) * 4;
+    const int rank = ((int)blockIdx.x >> 3) * 4 + w;
+    const int xcd = (int)blockIdx.x & 7;
+    const int gpx = (ngrp + 7) >> 3;
+    const int gend = min(ngrp, (xcd + 1) * gpx);
+    const V4 z = {T(0), T(0), T(0), T(0)};
+    const T gm1 = T(a.gamma) - T(1), gogm1 = T(a.gamma) / gm1, atm = T(a.atm);
+    // Gauss point of accumulator register i of tile kt in this lane
+    auto point = [&](int kt, int i) { return F64 ? 16 * kt + 4 * i + q : 16 * kt + 4 * q + i; };
+    for (int grp = xcd * gpx + rank; grp < gend; grp += WX) {
+        const int slot = grp * 16 + e;
+        const bool valid = slot < a.ntet;
+        const int tt = valid ? (a.elist ? a.elist[slot] : slot) : 0;
+        T b[CNV][KT];
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+            const int k = 4 * s + q;
+            const double* src = nullptr;
+            if (valid && k < TM)
+                src = k < 4 ? a.ugv + (size_t)a.tet_pnt[4 * tt + k] * CNV
+                            : a.uge + (size_t)a.tet_seg[6 * tt + (k - 4)] * a.em0 * CNV;
+#pragma unroll
+            for (int n = 0; n < CNV; ++n)
+                b[n][s] = src ? T(src[n]) : (!valid && k < 4 && (n == 0 || n == 4) ? T(1) : T(0));
+        }
+        if constexpr (FULL) {
+            wave_lds_sync();  // the previous group is done with this wave's tau rows
+            for (int k = q; k < 25; k += 4) stau[w][e][k] = valid ? T(a.tau[25L * tt + k]) : T(0);
+            wave_lds_sync();
+        }
+        T X[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int v = a.tet_pnt[4 * tt + k];
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                X[k][n] = valid ? T(a.pnts[3 * v + n]) : ((k == 0 && n == 2) || (k == 1 && n == 1) || (k == 3 && n == 0) ? T(1) : T(-1));
+        }
+        T ld[3][3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            ld[n][0] = T(0.5) * (X[3][n] - X[2][n]);
+            ld[n][1] = T(0.5) * (X[1][n] - X[2][n]);
+            ld[n][2] = T(0.5) * (X[0][n] - X[2][n]);
+        }
+        T d[3][3];
+        d[0][0] = ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1];
+        d[0][1] = -ld[0][1] * ld[2][2] + ld[0][2] * ld[2][1];
+        d[0][2] = ld[0][1] * ld[1][2] - ld[0][2] * ld[1][1];
+        d[1][0] = -ld[1][0] * ld[2][2] + ld[1][2] * ld[2][0];
+        d[1][1] = ld[0][0] * ld[2][2] - ld[0][2] * ld[2][0];
+        d[1][2] = -ld[0][0] * ld[1][2] + ld[0][2] * ld[1][0];
+        d[2][0] = ld[1][0] * ld[2][1] - ld[1][1] * ld[2][0];
+        d[2][1] = -ld[0][0] * ld[2][1] + ld[0][1] * ld[2][0];
+        d[2][2] = ld[0][0] * ld[1][1] - ld[0][1] * ld[1][0];
+        const T cjcb = ld[0][0] * (ld[1][1] * ld[2][2] - ld[1][2] * ld[2][1]) -
+                       ld[0][1] * (ld[1][0] * ld[2][2] - ld[2][0] * ld[1][2]) +
+                       ld[0][2] * (ld[1][0] * ld[2][1] - ld[2][0] * ld[1][1]);
+        // values on Gauss-point tile mt (proj); each pass projects the tile it walks,
+        // which keeps a single tile of values live
+        auto values = [&](int mt, V4 UT[CNV]) {
+#pragma unroll
+            for (int n = 0; n < CNV; ++n) {
+                V4 c = z;
+#pragma unroll
+                for (int s = 0; s < KT; ++s) c = mfma(sops[M::OFF_PROJ + (mt * KT + s) * 64 + lane], b[n][s], c);
+                UT[n] = c;
+            }
+        };
+        // the state at point (kt, i) of the tile values UT; p = RT = 1 past the last
+        // Gauss point
+        auto state = [&](const V4 UT[CNV], int kt, int i, T u[CNV]) {
+            const bool pad = point(kt, i) >= G;
+#pragma unroll
+            for (int n = 0; n < CNV; ++n) u[n] = pad ? ((n == 0 || n == 4) ? T(1) : T(0)) : UT[n][i];
+        };
+        // convective, pressure and energy fluxes (rsdl.cpp:494-540)
+        auto conv = [&](const T u[CNV], T cv[CNV][3]) {
+            const T rho = (u[0] + atm) / u[4];
+            const T fx = rho * u[1], fy = rho * u[2], fz = rho * u[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cv[0][c] = d[c][0] * fx + d[c][1] * fy + d[c][2] * fz;
+#pragma unroll
+            for (int n = 1; n < 4; ++n)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cv[n][c] = u[n] * cv[0][c] + d[c][n - 1] * u[0];
+            const T hh = gogm1 * u[4] + T(0.5) * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cv[4][c] = hh * cv[0][c];
+        };
+        // lf_im = intgrtrst of the fluxes (rsdl.cpp:542-543)
+        {
+            V4 L[CNV];
+#pragma unroll
+            for (int n = 0; n < CNV; ++n) L[n] = z;
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt) {
+                V4 UT[CNV];
+                values(kt, UT);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = 4 * kt + i;
+                    __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
+                    T u[CNV], cv[CNV][3];
+                    state(UT, kt, i, u);
+                    conv(u, cv);
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const T ai = sops[M::OFF_INT + (x * QK + kk) * 64 + lane];
+#pragma unroll
+                        for (int n = 0; n < CNV; ++n) L[n] = mfma(ai, cv[n][x], L[n]);
+                    }
+                }
+            }
+            double* lfo = a.lf + (size_t)tt * (2 * CNV * TM);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = F64 ? 4 * i + q : 4 * q + i;
+                if (valid && m < TM) {
+                    double* o = lfo + m * (2 * CNV);
+#pragma unroll
+                    for (int n = 0; n < CNV; ++n) o[n] = double(L[n][i]);
+                    if constexpr (!FULL)
+#pragma unroll
+                        for (int n = 0; n < CNV; ++n) o[CNV + n] = 0.0;
+                }
+            }
+        }
+        if constexpr (FULL) {
+            // viscous stress and heat flux (rsdl.cpp:584-755): the stress of velocity n
+            // in direction a, -mu/cjcb (sum_b dd(a,b) du(n,b) + sum_m d(a)(m) gm(m,n)
+            // - 2/3 d(a)(n) sum_m gm(m,m)) with gm(m,n) = sum_b d(b)(n) du(m,b); the
+            // energy row kcond/(R cjcb) dd(a,.) d(RT) plus the viscous dissipation
+            const T mujcbi = T(a.mu) / cjcb, kcjcbi = T(a.kcond) / cjcb / T(a.R);
+            T dd[3][3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) dd[x][y] = d[x][0] * d[y][0] + d[x][1] * d[y][1] + d[x][2] * d[y][2];
+            V4 R[CNV][GT];
+#pragma unroll
+            for (int n = 0; n < CNV; ++n)
+#pragma unroll
+                for (int mt = 0; mt < GT; ++mt) R[n][mt] = z;
+            V4 L[CNV];
+#pragma unroll
+            for (int n = 0; n < CNV; ++n) L[n] = z;
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt) {
+                // values and r, s, t derivatives of u, v, w, RT on this tile
+                V4 UT[CNV];
+                values(kt, UT);
+                V4 DU[4][3];
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) {
+                        V4 c = z;
+#pragma unroll
+                        for (int s = 0; s < KT; ++s)
+                            c = mfma(sops[M::OFF_PROJ + (((1 + bb) * GT + kt) * KT + s) * 64 + lane], b[n + 1][s], c);
+                        DU[n][bb] = c;
+                    }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = 4 * kt + i;
+                    __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
+                    T u[CNV], cv[CNV][3], df[CNV][3], gm[3][3];
+                    state(UT, kt, i, u);
+                    conv(u, cv);
+#pragma unroll
+                    for (int m = 0; m < 3; ++m)
+#pragma unroll
+                        for (int n = 0; n < 3; ++n)
+                            gm[m][n] = d[0][n] * DU[m][0][i] + d[1][n] * DU[m][1][i] + d[2][n] * DU[m][2][i];
+                    const T dv = T(2) / T(3) * (gm[0][0] + gm[1][1] + gm[2][2]);
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) df[0][x] = T(0);
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) {
+                            const T h = dd[x][0] * DU[n][0][i] + dd[x][1] * DU[n][1][i] + dd[x][2] * DU[n][2][i];
+                            df[n + 1][x] = -mujcbi * (h + d[x][0] * gm[0][n] + d[x][1] * gm[1][n] + d[x][2] * gm[2][n] - d[x][n] * dv);
+                        }
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        df[4][x] = -kcjcbi * (dd[x][0] * DU[3][0][i] + dd[x][1] * DU[3][1][i] + dd[x][2] * DU[3][2][i]);
+                        df[4][x] += df[1][x] * u[1] + df[2][x] * u[2] + df[3][x] * u[3];
+                    }
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const T ai = sops[M::OFF_INT + (x * QK + kk) * 64 + lane];
+#pragma unroll
+                        for (int n = 1; n < CNV; ++n) L[n] = mfma(ai, df[n][x], L[n]);
+                    }
+                    // strong residual: derivr/derivs/derivt of the total fluxes (rsdl.cpp:768-773)
+#pragma unroll
+                    for (int n = 0; n < CNV; ++n)
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) cv[n][x] += df[n][x];
+#pragma unroll
+                    for (int mt = 0; mt < GT; ++mt)
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) {
+                            const T ad = sops[M::OFF_DER + ((x * GT + mt) * QK + kk) * 64 + lane];
+#pragma unroll
+                            for (int n = 0; n < CNV; ++n) R[n][mt] = mfma(ad, cv[n][x], R[n][mt]);
+                        }
+                }
+            }
+            // stabilisation with the element tau (rsdl.cpp:776-827) into lf_re
+#pragma unroll
+            for (int kt = 0; kt < GT; ++kt) {
+                V4 UT[CNV];
+                values(kt, UT);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int kk = 4 * kt + i;
+                    __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
+                    T u[CNV], tres[CNV], sf[CNV][3];
+                    state(UT, kt, i, u);
+#pragma unroll
+                    for (int m = 0; m < CNV; ++m) {
+                        T s = T(0);
+#pragma unroll
+                        for (int n = 0; n < CNV; ++n) s += stau[w][e][5 * m + n] * R[n][kt][i];
+                        tres[m] = s;
+                    }
+#pragma unroll
+                    for (int n = 0; n < CNV; ++n)
+#pragma unroll
+                        for (int x = 0; x < 3; ++x) sf[n][x] = T(0);
+                    tetc_stabilise<T>(u, tres, d, gogm1, atm, sf);
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const T ai = sops[M::OFF_INT + (x * QK + kk) * 64 + lane];
+#pragma unroll
+                        for (int n = 0; n < CNV; ++n) L[n] = mfma(ai, sf[n][x], L[n]);
+                    }
+                }
+            }
+            double* lfo = a.lf + (size_t)tt * (2 * CNV * TM);
+            const T bt = T(a.beta);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = F64 ? 4 * i + q : 4 * q + i;
+                if (valid && m < TM) {
+                    double* o = lfo + m * (2 * CNV) + CNV;
+#pragma unroll
+                    for (int n = 0; n < CNV; ++n) o[n] = double(L[n][i] * bt);
+                }
+            }
+        }
+    }
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tetc_mfma(TetcElemArgs a, int ngrp) {
+    elem_tetc_mfma_body<double, P, FULL>(a, ngrp);
+}
+
+template <int P, bool FULL>
+__global__ __launch_bounds__(256) void k_elem_tetc32_mfma(TetcElemArgs32 a, int ngrp) {
+    elem_tetc_mfma_body<float, P, FULL>(a, ngrp);
+}
+
 // inflow face flux (tet_hp/cns/bdry.cpp:58-104, 129-150): pressure from the solution,
 // u, v, w, RT from the boundary values; n = (p2 - p1) x (p0 - p1) / 4 for a straight face.
 // flf[f][n][m]
