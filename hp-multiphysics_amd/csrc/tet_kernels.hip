@@ -1,12 +1,13 @@
 // HIP kernels of the tet_hp / tet_hp_ins residual and relaxation path (gfx950),
 // p in {1, 2}: vertex and edge modes, NV = 4 (u, v, w, p).
 //
-// Element residual: one element per half-wave, one Gauss point per lane (27 points
-// at p = 2, 8 at p = 1).  The transform operators sit in LDS, lane-contiguous; the
-// point fields a reduction needs are exchanged through a half-wave's private LDS
-// rows.  Element residuals go to an element-major buffer lf[t][m][im/re][n] and the
-// gather kernels sum them per vertex / edge over the incident elements in
-// ascending element order, the order of the reference's element loop, so the
+// Element residual at p = 2: k_elem_tet_mfma, 16 elements per wave, the transforms
+// on the matrix cores (below).  At p = 1 (8 Gauss points, 4 modes): k_elem_tet, one
+// element per half-wave, one Gauss point per lane; the transform operators sit in LDS,
+// lane-contiguous, and the point fields a reduction needs are exchanged through a
+// half-wave's private LDS rows.  Element residuals go to an element-major buffer
+// lf[t][m][im/re][n] and the gather kernels sum them per vertex / edge over the
+// incident elements in ascending element order, the order of the reference's element loop, so the
 // assembly is deterministic without colouring.
 #include <hip/hip_runtime.h>
 

@@ -1,7 +1,8 @@
 // HIP kernels of the tet_hp_cns residual/relaxation path (gfx950), p in {1, 2},
-// NV = 5 (p, u, v, w, RT).  Same shapes as the tet_hp_ins kernels: one element per
-// half-wave with a lane per Gauss point and the transform operators in LDS; element
-// residuals go to the element-major buffer lf[t][m][im/re][n] and are gathered per
+// NV = 5 (p, u, v, w, RT).  Same shapes as the tet_hp_ins kernels: at p = 2 the
+// matrix-core element residual (k_elem_tetc_mfma, 16 elements per wave), at p = 1 one
+// element per half-wave with a lane per Gauss point and the transform operators in
+// LDS (k_elem_tetc); element residuals go to the element-major buffer lf[t][m][im/re][n] and are gathered per
 // vertex / edge in ascending element order (k_gather_tet<5>).  The 5x5 algebra of the
 // Weiss-Smith preconditioner and the element tau runs one element per lane.
 #include <hip/hip_runtime.h>
