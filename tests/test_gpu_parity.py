@@ -1,6 +1,6 @@
 """GPU parity: libhpgpu.so (through the C-ABI) against the CPU oracle.
 
-Bar (north star): residuals to 1e-12 relative in max norm, converged solutions to
+Bar: residuals to 1e-12 relative in max norm, converged solutions to
 1e-10.  Every test prints the case it is about to run before touching the GPU.
 """
 import numpy as np
