@@ -341,6 +341,9 @@ __device__ __forceinline__ void elem_tet_mfma_body(const TetElemArgsT<T>& a, int
     constexpr int GT = M::GT, KT = M::KT, QK = M::QK, TM = D::TM;
     // output row of accumulator register i in lane l: 4 i + l / 16 (f64), 4 (l / 16) + i (f32)
     constexpr bool F64 = sizeof(T) == 8;
+    // first Gauss point of K-step (tile kt, register i): its four points are
+    // 16 kt + 4 i + l / 16 (f64) or 16 kt + 4 (l / 16) + i (f32)
+    auto kfirst = [](int kt, int i) { return F64 ? 16 * kt + 4 * i : 16 * kt + i; };
     __shared__ T sops[M::SIZE];
     for (int k = threadIdx.x; k < M::SIZE; k += blockDim.x) sops[k] = a.ops[k];
     __syncthreads();
@@ -434,6 +437,7 @@ __device__ __forceinline__ void elem_tet_mfma_body(const TetElemArgsT<T>& a, int
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = 4 * kt + i;
+                    if (kfirst(kt, i) >= D::G) continue;  // K-step of padding points only
                     __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
                     T cv[3][3], c3[3];
                     conv(kt, i, cv, c3);
@@ -493,6 +497,7 @@ __device__ __forceinline__ void elem_tet_mfma_body(const TetElemArgsT<T>& a, int
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = 4 * kt + i;
+                    if (kfirst(kt, i) >= D::G) continue;  // K-step of padding points only
                     __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
                     T cv[3][3], c3[3], df[3][3], gm[3][3];
                     conv(kt, i, cv, c3);
@@ -534,6 +539,7 @@ __device__ __forceinline__ void elem_tet_mfma_body(const TetElemArgsT<T>& a, int
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = 4 * kt + i;
+                    if (kfirst(kt, i) >= D::G) continue;  // K-step of padding points only
                     __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
                     T sf[3][3], c3[3];
                     const T u0 = U[0][kt][i], u1 = U[1][kt][i], u2 = U[2][kt][i];

@@ -632,6 +632,9 @@ __device__ __forceinline__ void elem_tetc_mfma_body(const TetcElemArgsT<T>& a, i
     using V4 = typename Vec4<T>::type;
     constexpr int GT = M::GT, KT = M::KT, QK = M::QK, TM = D::TM, G = D::G;
     constexpr bool F64 = sizeof(T) == 8;
+    // first Gauss point of K-step (tile kt, register i): its four points are
+    // 16 kt + 4 i + l / 16 (f64) or 16 kt + 4 (l / 16) + i (f32)
+    auto kfirst = [](int kt, int i) { return F64 ? 16 * kt + 4 * i : 16 * kt + i; };
     __shared__ T sops[M::SIZE];
     __shared__ T stau[4][16][25];
     for (int k = threadIdx.x; k < M::SIZE; k += blockDim.x) sops[k] = a.ops[k];
@@ -740,6 +743,7 @@ __device__ __forceinline__ void elem_tetc_mfma_body(const TetcElemArgsT<T>& a, i
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = 4 * kt + i;
+                    if (kfirst(kt, i) >= D::G) continue;  // K-step of padding points only
                     __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
                     T u[CNV], cv[CNV][3];
                     state(UT, kt, i, u);
@@ -804,6 +808,7 @@ __device__ __forceinline__ void elem_tetc_mfma_body(const TetcElemArgsT<T>& a, i
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = 4 * kt + i;
+                    if (kfirst(kt, i) >= D::G) continue;  // K-step of padding points only
                     __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
                     T u[CNV], cv[CNV][3], df[CNV][3], gm[3][3];
                     state(UT, kt, i, u);
@@ -857,6 +862,7 @@ __device__ __forceinline__ void elem_tetc_mfma_body(const TetcElemArgsT<T>& a, i
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int kk = 4 * kt + i;
+                    if (kfirst(kt, i) >= D::G) continue;  // K-step of padding points only
                     __builtin_amdgcn_sched_barrier(0);  // keep the K-steps' operand loads apart
                     T u[CNV], tres[CNV], sf[CNV][3];
                     state(UT, kt, i, u);
