@@ -299,6 +299,9 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
         for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
+                // f64 layout: K-step (tile kt, register i) holds points 16 kt + 4 i .. + 3;
+                // a step of padding points only (past the G Gauss points) adds nothing
+                if (sizeof(T) == 8 && 16 * kt + 4 * i >= D::G) continue;
                 const int kk = kt * 4 + i;
                 const T ar = sops[D::OFF_IR + kk * 64 + lane];
                 const T as = sops[D::OFF_IS + kk * 64 + lane];
@@ -374,6 +377,7 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
                 for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
+                        if (sizeof(T) == 8 && 16 * kt + 4 * i >= D::G) continue;  // padding points only
                         const int kk = (mt * GT + kt) * 4 + i;
                         const T dr = sops[D::OFF_DR + kk * 64 + lane];
                         const T ds = sops[D::OFF_DS + kk * 64 + lane];
@@ -436,6 +440,7 @@ __device__ __forceinline__ void elem_ins_body(const ElemArgsT<T>& a, T* __restri
             for (int kt = 0; kt < GT; ++kt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
+                    if (sizeof(T) == 8 && 16 * kt + 4 * i >= D::G) continue;  // padding points only
                     const int kk = kt * 4 + i;
                     const T ar = sops[D::OFF_IR + kk * 64 + lane];
                     const T as = sops[D::OFF_IS + kk * 64 + lane];
