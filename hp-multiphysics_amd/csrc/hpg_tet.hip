@@ -200,6 +200,7 @@ void launch_elem_tetc(hpg_tet_ctx* c, double beta, const int* elist, int count) 
 template <int P>
 void launch_elem_tet(hpg_tet_ctx* c, int stage, const int* elist = nullptr, int count = -1) {
     if (count < 0) count = c->ntet;
+    if (count == 0) return;  // an empty element list: no launch (a grid of zero blocks is invalid)
     const int nst = c->prm.nstage;
     const double beta = stage < nst ? c->prm.beta[stage] : 1.0;
     if (c->cns()) {
